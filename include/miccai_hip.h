@@ -1,0 +1,240 @@
+/*
+ * libmiccai_hip.so - C ABI of the gfx950 HIP implementation of the
+ * tubular-segmentation path (Frangi vesselness -> direction adjustment ->
+ * gradient vector flow -> anisotropic-TV continuous max-flow).
+ *
+ * Conventions
+ *  - Volumes are C-order float32 (D0, D1, D2), axis 2 fastest; dims[] holds
+ *    (D0, D1, D2) and every entry point requires D0*D1*D2 < 2^31.
+ *  - "AoS x k": k floats per voxel, interleaved. "SoA x k": k separate fields
+ *    of N floats, field f at ptr + f*N.
+ *  - Pointers named *_dev are device pointers (e.g. torch.Tensor.data_ptr()),
+ *    owned by the caller. Pointers named *_host are host memory.
+ *  - stream is a hipStream_t passed as void* (NULL = the null stream).
+ *    Every entry point only enqueues work unless its comment says it blocks.
+ *  - Return: MH_OK (0); < 0 invalid argument or HIP error (message in
+ *    mh_last_error); > 0 a numerical early stop (NaN criterion).
+ *
+ * Reference paths below are relative to the reference repository's src/.
+ */
+#ifndef MICCAI_HIP_H
+#define MICCAI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_ERR_ARG -1
+#define MH_ERR_HIP -2
+#define MH_STOPPED_NAN 1
+
+typedef struct mh_ctx mh_ctx;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces ClProgramLoader.__init__ (helpers/cl_helpers.py:93-114): binds a
+ * device and allocates the context's small reduction/control block. */
+int mh_create(int device, mh_ctx **out);
+int mh_destroy(mh_ctx *ctx);
+/* Last error message of this context (never NULL). */
+const char *mh_last_error(mh_ctx *ctx);
+/* ABI version (major*100 + minor). */
+int mh_version(void);
+
+/* ------------------------------------------------------- elementwise/util */
+
+/* Replaces helpers.cl to_zeros / to_value (clcode/helpers.cl:13-34). */
+int mh_fill(mh_ctx *ctx, float *x_dev, int64_t n, float value, void *stream);
+
+/* Replaces Reducer.reduce with sum_abs.cl / max.cl (helpers/cl_helpers.py:
+ * 232-298, clcode/sum_abs.cl:25-58, clcode/max.cl:25-58). The result is written
+ * to out_dev (1 float, device). Max ignores NaN (fmax); an all-NaN input gives
+ * NaN. */
+int mh_reduce_sum_abs(mh_ctx *ctx, const float *x_dev, int64_t n, float *out_dev, void *stream);
+int mh_reduce_max(mh_ctx *ctx, const float *x_dev, int64_t n, float *out_dev, void *stream);
+
+/* ------------------------------------------------------------------ Frangi */
+
+/* One separable 3-pass filter (axis 0, then 1, then 2) with mirror boundary
+ * of period 2(n-1) and true-convolution tap order. Replaces
+ * GaussianFilter3D.execute + convolve_naive.cl axis0/1/2 (processing/
+ * qfrangi_gpu.py:91-155, clcode/convolve_naive.cl:25-166). taps*_host hold
+ * 2*r+1 floats; result in out_dev; swap_dev is N floats of scratch. Every dim
+ * used with r > 0 must be >= 2. */
+int mh_gauss_filter(mh_ctx *ctx, const float *in_dev, float *out_dev, float *swap_dev,
+                    const int64_t dims[3], const float *taps0_host, int r0,
+                    const float *taps1_host, int r1, const float *taps2_host, int r2,
+                    void *stream);
+
+/* All six scale-normalised Gaussian-derivative Hessian entries at one scale,
+ * written AoS x6 [H00, H11, H22, H01, H02, H12]. Replaces Hessian.execute
+ * (processing/qfrangi_gpu.py:208-271). taps_host is [axis][order] packed:
+ * for axis a (0..2) and derivative order o (0..2) the 2*radius[a]+1 taps start
+ * at taps_host + (3*a + o) * (2*max_radius + 1), max_radius = max(radius).
+ * work_dev must hold mh_hessian_work_floats(dims) floats. */
+int64_t mh_hessian_work_floats(const int64_t dims[3]);
+int mh_gauss_hessian(mh_ctx *ctx, const float *img_dev, const int64_t dims[3],
+                     const float *taps_host, const int radius[3], float *hess6_dev,
+                     float *work_dev, void *stream);
+
+/* 3x3 symmetric Jacobi eigen-decomposition per voxel, in place allowed.
+ * in: AoS x6 Hessian; out: AoS x6 [l1, l2, l3, phi, theta, psi] sorted by
+ * ascending |l|, frame squeezed (eigenvectors in columns). NaN x6 if not
+ * converged within maxiter sweeps. If norm_max_dev != NULL, the maximum over
+ * voxels of ||lambda||_2 (NaN skipped) is max-accumulated into *norm_max_dev
+ * (initialise it to 0). Replaces Eigendecomposition.execute + qeigs.cl eigs
+ * and the calculateNorm/redMax part of VesselnessUpdate (processing/
+ * qfrangi_gpu.py:274-332, 429-433; clcode/qeigs.cl:146-256;
+ * clcode/frangi.cl:107-115). */
+int mh_eigs(mh_ctx *ctx, const float *hess6_dev, float *eig6_dev, int64_t n, int maxiter,
+            float *norm_max_dev, void *stream);
+
+/* max over voxels of ||lambda||_2 of an AoS x6 eigen buffer (NaN skipped;
+ * 0 if every voxel is NaN) into *norm_max_dev. Replaces calculateNorm + the
+ * redMax reduction (clcode/frangi.cl:107-115, clcode/max.cl:25-58). */
+int mh_eig_norm_max(mh_ctx *ctx, const float *eig6_dev, int64_t n, float *norm_max_dev,
+                    void *stream);
+
+/* Frangi response of one scale and running maximum with strict '>' carrying
+ * the winner's squeezed frame. c = c_value if c_value >= 0, else
+ * c = 0.5 * (*norm_max_dev). Replaces updateVesselness (clcode/frangi.cl:46-92,
+ * processing/qfrangi_gpu.py:383-439). */
+int mh_update_vesselness(mh_ctx *ctx, float *val_dev, float *quat3_dev, const float *eig6_dev,
+                         int64_t n, float alpha, float beta, float c_value,
+                         const float *norm_max_dev, int bright_objects, void *stream);
+
+/* -------------------------------------------------------------- directions */
+
+/* Ridge mask: resp > threshold and resp > max of 8 trilinear samples in the
+ * cross-section plane. threshold_f must be the largest float <= the double
+ * threshold (so that the float comparison equals the double one). Replaces
+ * RawRidgeDetector.execute + detectRidges (processing/ridge_detection.py:
+ * 12-77, clcode/detect_ridges.cl:34-102). */
+int mh_detect_ridges(mh_ctx *ctx, float *ridge_dev, const float *resp_dev,
+                     const float *qeig3_dev, const int64_t dims[3], float threshold_f,
+                     void *stream);
+
+/* v1 = column 0 of the squeezed frame, AoS x3, in place allowed. Replaces
+ * extractMainDirections (clcode/adjust_directions.cl:29-43). */
+int mh_main_dirs(mh_ctx *ctx, float *dirs3_dev, const float *qeig3_dev, int64_t n, void *stream);
+
+/* For every voxel, nearest ridge point by scaled squared distance (lowest
+ * index wins ties), then flip the voxel's direction if its dot product with
+ * that ridge direction is negative. coords_dev: uint16 R x 3, rdirs_dev:
+ * float R x 3. Replaces adjustNonRidgeSigns (clcode/adjust_directions.cl:
+ * 182-246, processing/adjust_directions.py:419-446). */
+int mh_nonridge_signs(mh_ctx *ctx, float *dirs3_dev, const int64_t dims[3],
+                      const uint16_t *coords_dev, const float *rdirs_dev, int64_t nridge,
+                      const float scaling_host[3], void *stream);
+
+/* Direction -> orthonormal frame (v, v2, v x v2) -> squeezed quaternion,
+ * AoS x3, in place allowed; a zero vector yields the identity frame.
+ * Replaces TransformationCompleter.complete + completeTransformations
+ * (processing/adjust_directions.py:486-504, clcode/adjust_directions.cl:68-100). */
+int mh_complete_frames(mh_ctx *ctx, float *out3_dev, const float *dirs3_dev, int64_t n,
+                       void *stream);
+
+/* Host (CPU) hysteresis: out = candidate voxels 26-connected through
+ * candidates to a seed voxel (seed = sure && cand). Replaces
+ * hysteresis_threshold (processing/ridge_detection.py:80-131). Blocks. */
+int mh_hysteresis_host(const uint8_t *cand_host, const uint8_t *sure_host, uint8_t *out_host,
+                       const int64_t dims[3]);
+
+/* --------------------------------------------------------------------- GVF */
+
+typedef struct {
+    float mu;      /* regularisation weight */
+    float dt;      /* time step, gradient_vector_flow.py:58-65 */
+    float rss[3];  /* 1 / voxel_size^2 per axis */
+} mh_gvf_params;
+
+/* r = |v|^2, v SoA x3. Replaces the response kernel (clcode/gvf.cl:246-253). */
+int mh_gvf_response(mh_ctx *ctx, float *r_dev, const float *v3_dev, int64_t n, void *stream);
+
+/* `iters` explicit Euler steps g <- g - dt (r (g - v) - mu lap(g)) with
+ * mirrored neighbours at the faces (gvf.cl:44-173), ping-ponging between
+ * g3_dev and swap3_dev (both SoA x3); the result always ends in g3_dev.
+ * If mag_dev != NULL, *mag_dev receives sum over voxels of |delta| of the
+ * last step (the caller divides by N); work_dev (mh_gvf_work_floats(dims)
+ * floats) then holds the partial sums. Every dim must be >= 2. Replaces one
+ * chunk of GvfGpu._execute (processing/gradient_vector_flow.py:179-232). */
+int64_t mh_gvf_work_floats(const int64_t dims[3]);
+int mh_gvf(mh_ctx *ctx, float *g3_dev, float *swap3_dev, const float *v3_dev, const float *r_dev,
+           const int64_t dims[3], const mh_gvf_params *params, int iters, float *mag_dev,
+           float *work_dev, void *stream);
+
+/* Normalise non-zero vectors of g (SoA x3) to unit length
+ * (clcode/gvf.cl:215-232). */
+int mh_gvf_normalize(mh_ctx *ctx, float *g3_dev, int64_t n, void *stream);
+
+/* ---------------------------------------------------------------- max-flow */
+
+typedef struct {
+    float rcc;        /* 1 / (cc * min_s^2) */
+    float rs[3];      /* 1 / s_k (voxel size s) */
+    float ss[3];      /* steps * min_s^2 / s_k */
+    float sqev[3];    /* sqrt of the constant eigenvalues (const_evals) */
+    int const_evals;  /* 1: eigs AoS x3 squeezed; 0: AoS x6 [sqrt(l) x3, squeezed x3] */
+    int num_iter;
+    int update;       /* criterion every `update` iterations, at i % update == 0 */
+    float error_bound;
+} mh_cmf_params;
+
+typedef struct {
+    int iterations;   /* update_u applications performed */
+    int stopped;      /* 0: ran num_iter; 1: NaN criterion; 2: criterion < error_bound */
+    int ncrit;        /* number of criterion values written */
+    float last_crit;
+} mh_cmf_result;
+
+/* Device workspace needed by mh_cmf_solve for a volume of n voxels. */
+int64_t mh_cmf_workspace_bytes(int64_t n);
+
+/* One ATV continuous max-flow solve. u_dev holds u0 on entry and the result
+ * on exit. labels_dev (uint8, 1 = foreground seed, 2 = background seed) may be
+ * NULL for the unsupervised cut. crit_host (optional, ceil(num_iter/update)
+ * floats) receives every criterion sum|err| / (N * rcc). Blocks until the
+ * solve has finished. Replaces CmfCutQAnisotropicGpu._execute with update_p,
+ * update_u, update_tmp and the sum_abs criterion (processing/
+ * continuous_cut_anisotropic_gpu.py:198-344, clcode-anisotropic/
+ * update_qp.cl:150-316). */
+int mh_cmf_solve(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const float *alpha_dev,
+                 const float *cs_dev, const float *ct_dev, const uint8_t *labels_dev,
+                 const int64_t dims[3], const mh_cmf_params *params, void *work_dev,
+                 int64_t work_bytes, mh_cmf_result *result_host, float *crit_host,
+                 void *stream);
+
+/* ------------------------------------------------------ batched max-flow */
+
+/* Many independent unsupervised constant-eigenvalue solves on volumes that
+ * share the frame field, the terminal capacities and |grad I|^2. Solve s uses
+ * alpha = w[s] * exp(-gms / decay[s]^2) (phantom_experiment.py:359), computed
+ * on the fly. Solve s owns u_dev + s*N (u0 on entry, result on exit).
+ * No counterpart in the reference, which runs these solves one by one
+ * (phantom_experiment.py:350-368); each solve equals mh_cmf_solve on the same
+ * inputs within FP32 rounding. Blocks. */
+int64_t mh_cmf_batched_workspace_bytes(int64_t n, int nsolves);
+int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_host,
+                         const float *decay_host, const float *eigs_dev, const float *gms_dev,
+                         const float *cs_dev, const float *ct_dev, const int64_t dims[3],
+                         const mh_cmf_params *params, void *work_dev, int64_t work_bytes,
+                         mh_cmf_result *results_host, void *stream);
+
+/* Packed ragged batch: volume v has dims vdims_host[3v..3v+2] and its fields
+ * start at element offset voff_host[v] in every packed array (u, eigs*3,
+ * alpha, cs, ct). No counterpart in the reference. Blocks. */
+int64_t mh_cmf_ragged_workspace_bytes(int64_t total_voxels, int nvol);
+int mh_cmf_solve_ragged(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const float *alpha_dev,
+                        const float *cs_dev, const float *ct_dev, int nvol,
+                        const int64_t *vdims_host, const int64_t *voff_host,
+                        const mh_cmf_params *params, void *work_dev, int64_t work_bytes,
+                        mh_cmf_result *results_host, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MICCAI_HIP_H */

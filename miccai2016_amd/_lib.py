@@ -1,0 +1,95 @@
+"""ctypes binding of libmiccai_hip.so (declared in include/miccai_hip.h).
+
+The library is the only compute path: if it is missing or fails to load, the
+import of any processing module raises - there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmiccai_hip.so")
+
+MH_OK = 0
+MH_ERR_ARG = -1
+MH_ERR_HIP = -2
+MH_STOPPED_NAN = 1
+
+_lib = None
+
+vp = C.c_void_p
+i64 = C.c_int64
+i32 = C.c_int
+f32 = C.c_float
+I64x3 = i64 * 3
+F32x3 = C.c_float * 3
+I32x3 = C.c_int * 3
+
+
+class GvfParams(C.Structure):
+    _fields_ = [("mu", f32), ("dt", f32), ("rss", f32 * 3)]
+
+
+class CmfParams(C.Structure):
+    _fields_ = [("rcc", f32), ("rs", f32 * 3), ("ss", f32 * 3), ("sqev", f32 * 3),
+                ("const_evals", i32), ("num_iter", i32), ("update", i32), ("error_bound", f32)]
+
+
+class CmfResult(C.Structure):
+    _fields_ = [("iterations", i32), ("stopped", i32), ("ncrit", i32), ("last_crit", f32)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/miccai_hip.h
+SIGNATURES = {
+    "mh_create": (i32, [i32, C.POINTER(vp)]),
+    "mh_destroy": (i32, [vp]),
+    "mh_last_error": (C.c_char_p, [vp]),
+    "mh_version": (i32, []),
+    "mh_fill": (i32, [vp, vp, i64, f32, vp]),
+    "mh_reduce_sum_abs": (i32, [vp, vp, i64, vp, vp]),
+    "mh_reduce_max": (i32, [vp, vp, i64, vp, vp]),
+    "mh_gauss_filter": (i32, [vp, vp, vp, vp, I64x3, vp, i32, vp, i32, vp, i32, vp]),
+    "mh_hessian_work_floats": (i64, [I64x3]),
+    "mh_gauss_hessian": (i32, [vp, vp, I64x3, vp, I32x3, vp, vp, vp]),
+    "mh_eigs": (i32, [vp, vp, vp, i64, i32, vp, vp]),
+    "mh_eig_norm_max": (i32, [vp, vp, i64, vp, vp]),
+    "mh_update_vesselness": (i32, [vp, vp, vp, vp, i64, f32, f32, f32, vp, i32, vp]),
+    "mh_detect_ridges": (i32, [vp, vp, vp, vp, I64x3, f32, vp]),
+    "mh_main_dirs": (i32, [vp, vp, vp, i64, vp]),
+    "mh_nonridge_signs": (i32, [vp, vp, I64x3, vp, vp, i64, F32x3, vp]),
+    "mh_complete_frames": (i32, [vp, vp, vp, i64, vp]),
+    "mh_hysteresis_host": (i32, [vp, vp, vp, I64x3]),
+    "mh_gvf_response": (i32, [vp, vp, vp, i64, vp]),
+    "mh_gvf_work_floats": (i64, [I64x3]),
+    "mh_gvf": (i32, [vp, vp, vp, vp, vp, I64x3, C.POINTER(GvfParams), i32, vp, vp, vp]),
+    "mh_gvf_normalize": (i32, [vp, vp, i64, vp]),
+    "mh_cmf_workspace_bytes": (i64, [i64]),
+    "mh_cmf_solve": (i32, [vp, vp, vp, vp, vp, vp, vp, I64x3, C.POINTER(CmfParams), vp, i64,
+                           C.POINTER(CmfResult), vp, vp]),
+    "mh_cmf_batched_workspace_bytes": (i64, [i64, i32]),
+    "mh_cmf_solve_batched": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, I64x3,
+                                   C.POINTER(CmfParams), vp, i64, C.POINTER(CmfResult), vp]),
+    "mh_cmf_ragged_workspace_bytes": (i64, [i64, i32]),
+    "mh_cmf_solve_ragged": (i32, [vp, vp, vp, vp, vp, vp, i32, vp, vp, C.POINTER(CmfParams), vp,
+                                  i64, C.POINTER(CmfResult), vp]),
+}
+
+
+def lib():
+    """Load libmiccai_hip.so once; raise if it is absent or incomplete."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libmiccai_hip.so is not built; run `python -m miccai2016_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)  # AttributeError names a missing export
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def dims3(shape):
+    return I64x3(*[int(s) for s in shape])

@@ -1,0 +1,50 @@
+"""Build libmiccai_hip.so (gfx950) in-tree with hipcc.
+
+``python -m miccai2016_amd.build`` compiles every HIP source under csrc/ into
+``miccai2016_amd/libmiccai_hip.so``.
+"""
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libmiccai_hip.so")
+ARCH = os.environ.get("MH_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are disabled for every HIP source (the compiler
+# would otherwise form v_pk_* from ordinary float code).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=fast", "-Wall", "-Wno-unused-result",
+]
+
+
+def sources():
+    return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+
+
+def build(verbose=False):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), *sources(), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [ln for ln in (r.stdout + r.stderr).splitlines()
+            if "packed-fp32-ops" not in ln and "not a recognized feature" not in ln]
+    if r.returncode != 0:
+        sys.stderr.write("\n".join(msgs) + "\n")
+        raise RuntimeError("hipcc failed building libmiccai_hip.so")
+    if verbose and msgs:
+        print("\n".join(msgs))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

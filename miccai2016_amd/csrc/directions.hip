@@ -1,0 +1,289 @@
+// Direction adjustment on gfx950: ridge detection in the vesselness cross
+// section, main-direction extraction, nearest-ridge sign propagation and
+// direction -> frame completion. Host-side hysteresis flood fill at the end.
+#include <vector>
+
+#include "mh_common.h"
+
+namespace {
+
+// A voxel is a ridge voxel when its response exceeds the threshold and the
+// maxima of 8 trilinear samples taken in the plane spanned by the frame's
+// second and third columns, at offsets a*v2 + b*v3 with (a, b) in
+// {-1, 0, 1}^2 minus the origin. A sample's unit-cube origin is the position
+// truncated toward zero; corners outside the volume contribute the value -1.
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_kernel(float *__restrict__ ridge, const float *__restrict__ resp,
+             const float *__restrict__ qeig3, int d0, int d1, int d2, float thr)
+{
+    const int64_t n = int64_t(d0) * d1 * d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int i2 = int(i % d2);
+    const int i1 = int((i / d2) % d1);
+    const int i0 = int(i / (int64_t(d2) * d1));
+    const mh::Quat q = mh::expand_squeezed(qeig3[3 * i], qeig3[3 * i + 1], qeig3[3 * i + 2]);
+    const mh::Mat3 R = mh::quat_to_mat(q);
+    const float v2[3] = {R.m[1], R.m[4], R.m[7]};
+    const float v3[3] = {R.m[2], R.m[5], R.m[8]};
+    const float pos[3] = {float(i0), float(i1), float(i2)};
+    const int dim[3] = {d0, d1, d2};
+    // (a, b) offsets in the order the 8 neighbours are visited
+    const int oa[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    const int ob[8] = {-1, 0, 1, -1, 1, 1, -1, 0};
+    float best = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float f[3];
+        int o[3];
+        for (int k = 0; k < 3; ++k) {
+            const float p = pos[k] + float(oa[j]) * v2[k] + float(ob[j]) * v3[k];
+            o[k] = int(p);  // truncation toward zero
+            f[k] = p - float(o[k]);
+        }
+        float acc = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cr[3] = {c >> 2, (c >> 1) & 1, c & 1};
+            int ca[3];
+            bool inside = true;
+            float w[3];
+            for (int k = 0; k < 3; ++k) {
+                ca[k] = o[k] + cr[k];
+                inside = inside && ca[k] >= 0 && ca[k] < dim[k];
+                w[k] = cr[k] ? f[k] : 1.f - f[k];
+            }
+            const float s = inside ? resp[ca[2] + int64_t(d2) * (ca[1] + int64_t(d1) * ca[0])] : -1.f;
+            acc += s * w[0] * w[1] * w[2];
+        }
+        best = acc > best ? acc : best;
+    }
+    const float r = resp[i];
+    ridge[i] = (r > thr && r > best) ? 1.f : 0.f;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+main_dir_kernel(float *dirs, const float *qeig3, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const mh::Quat q = mh::expand_squeezed(qeig3[3 * i], qeig3[3 * i + 1], qeig3[3 * i + 2]);
+    const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    const float c0 = 1.f - 2.f * (yy + zz);
+    const float c1 = 2.f * (q.x * q.y + q.w * q.z);
+    const float c2 = 2.f * (q.x * q.z - q.w * q.y);
+    (void)xx;
+    dirs[3 * i] = c0;
+    dirs[3 * i + 1] = c1;
+    dirs[3 * i + 2] = c2;
+}
+
+// Brute-force nearest ridge point per voxel, ridge tiles staged in LDS. The
+// scan visits ridge points in index order with a strict '<', so the lowest
+// index wins ties; then the voxel's direction is flipped if it points away
+// from that ridge point's direction.
+constexpr int kRidgeTile = 1024;
+
+__global__ void __launch_bounds__(mh::kBlock)
+nonridge_kernel(float *__restrict__ dirs, int d0, int d1, int d2,
+                const uint16_t *__restrict__ coords, const float *__restrict__ rdirs, int nr,
+                float s0, float s1, float s2)
+{
+    __shared__ uint16_t tc[kRidgeTile][3];
+    const int64_t n = int64_t(d0) * d1 * d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    int p0 = 0, p1 = 0, p2 = 0;
+    if (active) {
+        p2 = int(i % d2);
+        p1 = int((i / d2) % d1);
+        p0 = int(i / (int64_t(d2) * d1));
+    }
+    float best = INFINITY;
+    int bj = 0;
+    for (int t0 = 0; t0 < nr; t0 += kRidgeTile) {
+        const int cnt = min(kRidgeTile, nr - t0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt * 3; k += blockDim.x) tc[k / 3][k % 3] = coords[3 * int64_t(t0) + k];
+        __syncthreads();
+        if (active) {
+            for (int j = 0; j < cnt; ++j) {
+                const float a = float(abs(p0 - int(tc[j][0]))) * s0;
+                const float c = float(abs(p1 - int(tc[j][1]))) * s1;
+                const float e = float(abs(p2 - int(tc[j][2]))) * s2;
+                const float dsq = a * a + c * c + e * e;
+                if (dsq < best) {
+                    best = dsq;
+                    bj = t0 + j;
+                }
+            }
+        }
+    }
+    if (!active) return;
+    float *v = dirs + 3 * i;
+    const float *r = rdirs + 3 * int64_t(bj);
+    const float x = v[0], y = v[1], z = v[2];
+    if (x * r[0] + y * r[1] + z * r[2] < 0.f) {
+        v[0] = -x;
+        v[1] = -y;
+        v[2] = -z;
+    }
+}
+
+__device__ __forceinline__ void normalize3(float v[3])
+{
+    const float d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (d == 0.f) return;  // a zero vector stays zero
+    const float r = 1.f / sqrtf(d);
+    v[0] *= r;
+    v[1] *= r;
+    v[2] *= r;
+}
+
+// Quaternion (w, x, y, z) of a rotation matrix by the largest-diagonal
+// branch (trace > 0, else the largest of the three diagonal entries).
+__device__ __forceinline__ mh::Quat mat_to_quat(const float m[9])
+{
+    mh::Quat q;
+    const float tr = m[0] + m[4] + m[8];
+    if (tr > 0.f) {
+        const float s = 0.5f * rsqrtf(tr + 1.f);
+        q.w = 0.25f / s;
+        q.x = (m[7] - m[5]) * s;
+        q.y = (m[2] - m[6]) * s;
+        q.z = (m[3] - m[1]) * s;
+    } else if (m[0] > m[4] && m[0] > m[8]) {
+        const float s = 0.5f * rsqrtf(1.f + m[0] - m[4] - m[8]);
+        q.w = (m[7] - m[5]) * s;
+        q.x = 0.25f / s;
+        q.y = (m[1] + m[3]) * s;
+        q.z = (m[2] + m[6]) * s;
+    } else if (m[4] > m[8]) {
+        const float s = 0.5f * rsqrtf(1.f + m[4] - m[0] - m[8]);
+        q.w = (m[2] - m[6]) * s;
+        q.x = (m[1] + m[3]) * s;
+        q.y = 0.25f / s;
+        q.z = (m[5] + m[7]) * s;
+    } else {
+        const float s = 0.5f * rsqrtf(1.f + m[8] - m[0] - m[4]);
+        q.w = (m[3] - m[1]) * s;
+        q.x = (m[2] + m[6]) * s;
+        q.y = (m[5] + m[7]) * s;
+        q.z = 0.25f / s;
+    }
+    return q;
+}
+
+// v -> (e0, e1, e0 x e1) with e0 = v/|v| and e1 perpendicular to e0 in the
+// coordinate plane that excludes e0's larger of |x|, |y|; det = +1.
+__global__ void __launch_bounds__(mh::kBlock)
+frame_kernel(float *out3, const float *dirs3, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float e0[3] = {dirs3[3 * i], dirs3[3 * i + 1], dirs3[3 * i + 2]};
+    normalize3(e0);
+    float e1[3];
+    if (fabsf(e0[0]) > fabsf(e0[1])) {
+        e1[0] = -e0[2]; e1[1] = 0.f; e1[2] = e0[0];
+    } else {
+        e1[0] = 0.f; e1[1] = e0[2]; e1[2] = -e0[1];
+    }
+    normalize3(e1);
+    const float e2[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                         e0[0] * e1[1] - e0[1] * e1[0]};
+    const float m[9] = {e0[0], e1[0], e2[0], e0[1], e1[1], e2[1], e0[2], e1[2], e2[2]};
+    const mh::Quat q = mat_to_quat(m);
+    float s[3];
+    mh::squeeze_quat(q.w, q.x, q.y, q.z, s);
+    out3[3 * i] = s[0];
+    out3[3 * i + 1] = s[1];
+    out3[3 * i + 2] = s[2];
+}
+
+}  // namespace
+
+MH_EXPORT int mh_detect_ridges(mh_ctx *ctx, float *ridge, const float *resp, const float *qeig3,
+                               const int64_t dims[3], float thr, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, ridge && resp && qeig3 && ridge != resp, "bad pointers");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    hipLaunchKernelGGL(ridge_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, ridge, resp, qeig3, int(dims[0]), int(dims[1]),
+                       int(dims[2]), thr);
+    MH_CHECK_LAUNCH(ctx, "ridge_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int mh_main_dirs(mh_ctx *ctx, float *dirs, const float *qeig3, int64_t n, void *stream)
+{
+    MH_REQUIRE(ctx, dirs && qeig3 && n >= 0 && n < (int64_t(1) << 31), "bad arguments");
+    if (n == 0) return MH_OK;
+    hipLaunchKernelGGL(main_dir_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, dirs, qeig3, n);
+    MH_CHECK_LAUNCH(ctx, "main_dir_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int mh_nonridge_signs(mh_ctx *ctx, float *dirs, const int64_t dims[3],
+                                const uint16_t *coords, const float *rdirs, int64_t nr,
+                                const float scaling[3], void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, dirs && scaling && nr >= 0 && nr < (int64_t(1) << 31), "bad arguments");
+    MH_REQUIRE(ctx, nr == 0 || (coords && rdirs), "null ridge arrays");
+    if (nr == 0) return MH_OK;  // no ridge point: nothing to align with
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    hipLaunchKernelGGL(nonridge_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, dirs, int(dims[0]), int(dims[1]), int(dims[2]), coords,
+                       rdirs, int(nr), scaling[0], scaling[1], scaling[2]);
+    MH_CHECK_LAUNCH(ctx, "nonridge_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int mh_complete_frames(mh_ctx *ctx, float *out3, const float *dirs3, int64_t n,
+                                 void *stream)
+{
+    MH_REQUIRE(ctx, out3 && dirs3 && n >= 0 && n < (int64_t(1) << 31), "bad arguments");
+    if (n == 0) return MH_OK;
+    hipLaunchKernelGGL(frame_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, out3, dirs3, n);
+    MH_CHECK_LAUNCH(ctx, "frame_kernel");
+    return MH_OK;
+}
+
+// Breadth-first flood fill over the 26-neighbourhood, restricted to
+// candidate voxels, from every seed (seed = sure && cand).
+MH_EXPORT int mh_hysteresis_host(const uint8_t *cand, const uint8_t *sure, uint8_t *out,
+                                 const int64_t dims[3])
+{
+    if (!cand || !sure || !out || !mh::dims_ok(dims)) return MH_ERR_ARG;
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t n = d0 * d1 * d2;
+    std::vector<int64_t> queue;
+    for (int64_t i = 0; i < n; ++i) {
+        out[i] = 0;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (sure[i] && cand[i]) {
+            out[i] = 1;
+            queue.push_back(i);
+        }
+    for (size_t h = 0; h < queue.size(); ++h) {
+        const int64_t i = queue[h];
+        const int64_t i2 = i % d2, i1 = (i / d2) % d1, i0 = i / (d1 * d2);
+        for (int64_t a = std::max<int64_t>(0, i0 - 1); a <= std::min(d0 - 1, i0 + 1); ++a)
+            for (int64_t bb = std::max<int64_t>(0, i1 - 1); bb <= std::min(d1 - 1, i1 + 1); ++bb)
+                for (int64_t c = std::max<int64_t>(0, i2 - 1); c <= std::min(d2 - 1, i2 + 1); ++c) {
+                    const int64_t j = c + d2 * (bb + d1 * a);
+                    if (cand[j] && !out[j]) {
+                        out[j] = 1;
+                        queue.push_back(j);
+                    }
+                }
+    }
+    return MH_OK;
+}

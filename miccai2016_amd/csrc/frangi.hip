@@ -1,0 +1,411 @@
+// Frangi vesselness on gfx950: separable Gaussian-derivative passes, the
+// per-voxel 3x3 Jacobi eigensolver and the multi-scale vesselness update.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mh_common.h"
+#include "mh_reduce.h"
+
+namespace {
+
+// ------------------------------------------------------ separable passes ---
+
+constexpr int kMaxRadius = 63;
+constexpr int kMaxTaps = 2 * kMaxRadius + 1;
+
+struct ConvOut {
+    float *ptr;
+    int order;   // which tap set
+    int stride;  // elements between consecutive voxels (1 = plain field, 6 = AoS x6)
+    int offset;  // element offset of this component
+};
+
+struct ConvGroup {
+    const float *in;
+    int nout;
+    ConvOut out[3];
+};
+
+// One pass along one axis; each group reads one input field once per tap and
+// feeds up to three outputs (derivative orders), so intermediates shared by
+// several Hessian entries are computed once.
+struct ConvPass {
+    ConvGroup grp[6];
+    int ngroups;
+    int radius;
+    float taps[3][kMaxTaps];
+};
+
+template <int AXIS>
+__global__ void __launch_bounds__(mh::kBlock)
+conv_pass_kernel(const ConvPass P, int d0, int d1, int d2)
+{
+    const int64_t n = int64_t(d0) * d1 * d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int i2 = int(i % d2);
+    const int i1 = int((i / d2) % d1);
+    const int i0 = int(i / (int64_t(d2) * d1));
+    const int len = AXIS == 0 ? d0 : (AXIS == 1 ? d1 : d2);
+    const int ia = AXIS == 0 ? i0 : (AXIS == 1 ? i1 : i2);
+    const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : (AXIS == 1 ? d2 : 1);
+    const int64_t base = i - int64_t(ia) * sa;
+    const int R = P.radius;
+    const int K = 2 * R + 1;
+    const int period = 2 * (len - 1);
+
+    for (int g = 0; g < P.ngroups; ++g) {
+        const ConvGroup &G = P.grp[g];
+        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+        // position in the mirror-completed, periodically tiled axis
+        int t = 0;
+        if (period > 0) {
+            t = (ia - R - 1) % period;
+            t += t < 0 ? period : 0;
+        }
+        for (int j = 0; j < K; ++j) {
+            int m = 0;
+            if (period > 0) {
+                ++t;
+                t = t == period ? 0 : t;
+                m = t < len ? t : period - t;
+            }
+            const float x = G.in[base + int64_t(m) * sa];
+            const int k = K - 1 - j;
+            acc0 += P.taps[G.out[0].order][k] * x;
+            if (G.nout > 1) acc1 += P.taps[G.out[1].order][k] * x;
+            if (G.nout > 2) acc2 += P.taps[G.out[2].order][k] * x;
+        }
+        G.out[0].ptr[i * G.out[0].stride + G.out[0].offset] = acc0;
+        if (G.nout > 1) G.out[1].ptr[i * G.out[1].stride + G.out[1].offset] = acc1;
+        if (G.nout > 2) G.out[2].ptr[i * G.out[2].stride + G.out[2].offset] = acc2;
+    }
+}
+
+int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, hipStream_t s)
+{
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    const dim3 grid(mh::blocks_for(n)), block(mh::kBlock);
+    const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
+    if (axis == 0) hipLaunchKernelGGL(conv_pass_kernel<0>, grid, block, 0, s, P, d0, d1, d2);
+    else if (axis == 1) hipLaunchKernelGGL(conv_pass_kernel<1>, grid, block, 0, s, P, d0, d1, d2);
+    else hipLaunchKernelGGL(conv_pass_kernel<2>, grid, block, 0, s, P, d0, d1, d2);
+    MH_CHECK_LAUNCH(ctx, "conv_pass_kernel");
+    return MH_OK;
+}
+
+void set_taps(ConvPass &P, int slot, const float *taps, int r)
+{
+    // taps of a shorter arm are centred in the common radius P.radius
+    for (int k = 0; k < kMaxTaps; ++k) P.taps[slot][k] = 0.f;
+    const int shift = P.radius - r;
+    for (int k = 0; k < 2 * r + 1; ++k) P.taps[slot][k + shift] = taps[k];
+}
+
+// ------------------------------------------------------------ eigensolver ---
+
+// Rotations that re-order the eigenvector rows into ascending |lambda|
+// order, scalar last, indexed by (order[0] << 1) | (order[1] > order[2]).
+// Each is the signed permutation matrix with row k = e_order[k], negated as a
+// whole for odd permutations (det = +1), converted to a unit quaternion.
+#define MH_SQ1_2 0.70710678118654752440f
+__constant__ float kSorter[6][4] = {
+    {0.f, 0.f, 0.f, 1.f},
+    {0.f, -MH_SQ1_2, MH_SQ1_2, 0.f},
+    {-MH_SQ1_2, MH_SQ1_2, 0.f, 0.f},
+    {0.5f, 0.5f, 0.5f, -0.5f},
+    {0.5f, 0.5f, 0.5f, 0.5f},
+    {-MH_SQ1_2, 0.f, MH_SQ1_2, 0.f},
+};
+
+__device__ __forceinline__ void atomic_max_nonneg(float *addr, float v)
+{
+    // non-negative floats order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned *>(addr), __float_as_uint(v));
+}
+
+struct MaxNonNeg {
+    __device__ static float identity() { return 0.f; }
+    __device__ static float combine(float a, float b) { return fmaxf(a, b); }
+};
+
+// Cyclic Jacobi on one symmetric 3x3 matrix with the approximate Givens
+// rotation of McAdams et al. (UW-Madison TR 1690, 2011, algorithm 2):
+// gamma = 3 + 2 sqrt 2 decides between the exact half-angle pair and the
+// fixed (cos, sin)(pi/8). Convergence (sum |offdiag| <= max|input| * eps) is
+// tested before every sweep, so a lane that has converged stops rotating.
+__global__ void __launch_bounds__(mh::kBlock)
+eig_kernel(const float *in6, float *out6, int64_t n, int maxiter, float *norm_max)
+{
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    float nrm = 0.f;
+    if (i < n) {
+        const float *h = in6 + 6 * i;
+        float d[3] = {h[0], h[1], h[2]};
+        float o[3] = {h[5], h[4], h[3]};  // o[r]: element (p, q), r = 3 - p - q
+        float qv[3] = {0.f, 0.f, 0.f};
+        float qw = 1.f;
+        float tol = 0.f;
+        for (int k = 0; k < 3; ++k) tol = fmaxf(tol, fmaxf(fabsf(d[k]), fabsf(o[k])));
+        tol *= FLT_EPSILON;
+        const float kGamma = 5.82842712474619009760f;
+        const float kCos = 0.92387953251128675613f, kSin = 0.38268343236508977173f;
+        bool conv = false;
+        for (int it = 0; it < maxiter; ++it) {
+            if (fabsf(o[0]) + fabsf(o[1]) + fabsf(o[2]) <= tol) {
+                conv = true;
+                break;
+            }
+#pragma unroll
+            for (int pq = 0; pq < 3; ++pq) {
+                const int p = pq == 2 ? 1 : 0;
+                const int q = pq == 0 ? 1 : 2;
+                const int r = 3 - p - q;
+                float c = 2.f * (d[q] - d[p]);
+                float s = o[r];
+                const float s2 = s * s, c2 = c * c;
+                const bool exact = kGamma * s2 < c2;
+                const float w = rsqrtf(s2 + c2);
+                c = exact ? w * c : kCos;
+                s = exact ? w * s : kSin;
+                // accumulate the half-angle rotation into the quaternion
+                const float ap = c * qv[p] - s * qv[q];
+                const float aq = c * qv[q] + s * qv[p];
+                qv[p] = ap;
+                qv[q] = aq;
+                const float sg = r == 1 ? s : -s;
+                const float nw = c * qw + sg * qv[r];
+                const float nr = c * qv[r] - sg * qw;
+                qw = nw;
+                qv[r] = nr;
+                // full-angle rotation J M J^T of the matrix
+                const float cc = 1.f - 2.f * (s * s);
+                const float ss = 2.f * c * s;
+                const float t1 = o[r] * cc + d[p] * ss;
+                const float t2 = d[q] * cc + o[r] * ss;
+                d[p] = cc * (d[p] * cc - o[r] * ss) - ss * (o[r] * cc - d[q] * ss);
+                d[q] = cc * t2 + ss * t1;
+                o[r] = cc * t1 - ss * t2;
+                const float oq = o[q];
+                o[q] = cc * oq - ss * o[p];
+                o[p] = cc * o[p] + ss * oq;
+            }
+        }
+        float res[6];
+        if (conv) {
+            // compare-exchange (0,1), (1,2), (0,1) on |lambda|, tracking order
+            int ord0 = 0, ord1 = 1, ord2 = 2;
+            float l0 = d[0], l1 = d[1], l2 = d[2];
+            if (!(fabsf(l0) < fabsf(l1))) { float t = l0; l0 = l1; l1 = t; int u = ord0; ord0 = ord1; ord1 = u; }
+            if (!(fabsf(l1) < fabsf(l2))) { float t = l1; l1 = l2; l2 = t; int u = ord1; ord1 = ord2; ord2 = u; }
+            if (!(fabsf(l0) < fabsf(l1))) { float t = l0; l0 = l1; l1 = t; int u = ord0; ord0 = ord1; ord1 = u; }
+            const float *L = kSorter[(ord0 << 1) | (ord1 > ord2 ? 1 : 0)];
+            // Hamilton product L * q (scalar last), then transpose (negate scalar)
+            const float sw = L[3] * qw - (L[0] * qv[0] + L[1] * qv[1] + L[2] * qv[2]);
+            const float sx = L[3] * qv[0] + L[0] * qw + L[1] * qv[2] - L[2] * qv[1];
+            const float sy = L[3] * qv[1] - L[0] * qv[2] + L[1] * qw + L[2] * qv[0];
+            const float sz = L[3] * qv[2] + L[0] * qv[1] - L[1] * qv[0] + L[2] * qw;
+            float sq[3];
+            mh::squeeze_quat(-sw, sx, sy, sz, sq);
+            res[0] = l0; res[1] = l1; res[2] = l2;
+            res[3] = sq[0]; res[4] = sq[1]; res[5] = sq[2];
+            nrm = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+            if (!(nrm >= 0.f)) nrm = 0.f;  // NaN is neutral in the maximum
+        } else {
+            for (int k = 0; k < 6; ++k) res[k] = __builtin_nanf("");
+        }
+        float *e = out6 + 6 * i;
+        for (int k = 0; k < 6; ++k) e[k] = res[k];
+    }
+    if (norm_max) {
+        nrm = mh::block_reduce<MaxNonNeg>(nrm);
+        if (threadIdx.x == 0 && nrm > 0.f) atomic_max_nonneg(norm_max, nrm);
+    }
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+eig_norm_kernel(const float *__restrict__ e6, int64_t n, float *norm_max)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float nrm = 0.f;
+    if (i < n) {
+        const float *e = e6 + 6 * i;
+        nrm = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        if (!(nrm >= 0.f)) nrm = 0.f;
+    }
+    nrm = mh::block_reduce<MaxNonNeg>(nrm);
+    if (threadIdx.x == 0 && nrm > 0.f) atomic_max_nonneg(norm_max, nrm);
+}
+
+// ------------------------------------------------------------- vesselness ---
+
+__global__ void __launch_bounds__(mh::kBlock)
+vesselness_kernel(float *__restrict__ val, float *__restrict__ quat, const float *__restrict__ e6,
+                  int64_t n, float alpha, float beta, float c_value, const float *norm_max,
+                  int bright)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float c = c_value >= 0.f ? c_value : 0.5f * (*norm_max);
+    const float *e = e6 + 6 * i;
+    const float l1 = fabsf(e[0]);
+    float l2 = bright ? e[1] : -e[1];
+    float l3 = bright ? e[2] : -e[2];
+    const bool valid = l2 < 0.f && l3 < 0.f;
+    l2 = fabsf(l2);
+    l3 = fabsf(l3);
+    const float rB = l1 / (sqrtf(l2 * l3) + FLT_MIN);
+    const float rA = l2 / (l3 + FLT_MIN);
+    const float S2 = l1 * l1 + l2 * l2 + l3 * l3;
+    const float ra = rA / alpha, rb = rB / beta;
+    const float v = float(valid) * ((1.f - expf(-0.5f * (ra * ra))) * expf(-0.5f * (rb * rb)) *
+                                    (1.f - expf(-0.5f * S2 / (c * c))));
+    if (v > val[i]) {
+        val[i] = v;
+        quat[3 * i] = e[3];
+        quat[3 * i + 1] = e[4];
+        quat[3 * i + 2] = e[5];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI ---
+
+MH_EXPORT int64_t mh_hessian_work_floats(const int64_t dims[3])
+{
+    if (!mh::dims_ok(dims)) return -1;
+    return 9 * dims[0] * dims[1] * dims[2];
+}
+
+MH_EXPORT int mh_gauss_filter(mh_ctx *ctx, const float *in, float *out, float *swap,
+                              const int64_t dims[3], const float *taps0, int r0,
+                              const float *taps1, int r1, const float *taps2, int r2, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, in && out && swap && taps0 && taps1 && taps2, "null pointer");
+    const int r[3] = {r0, r1, r2};
+    const float *taps[3] = {taps0, taps1, taps2};
+    for (int a = 0; a < 3; ++a) MH_REQUIRE(ctx, r[a] >= 0 && r[a] <= kMaxRadius, "radius out of range");
+    // axis 0 -> swap? choose buffers so the result of the third pass lands in out:
+    // pass0: in -> out, pass1: out -> swap, pass2: swap -> out
+    const float *src[3] = {in, out, swap};
+    float *dst[3] = {out, swap, out};
+    for (int a = 0; a < 3; ++a) {
+        ConvPass P;
+        std::memset(&P, 0, sizeof(P));
+        P.radius = r[a];
+        set_taps(P, 0, taps[a], r[a]);
+        P.ngroups = 1;
+        P.grp[0].in = src[a];
+        P.grp[0].nout = 1;
+        P.grp[0].out[0] = ConvOut{dst[a], 0, 1, 0};
+        int rc = launch_pass(ctx, a, P, dims, (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return MH_OK;
+}
+
+MH_EXPORT int mh_gauss_hessian(mh_ctx *ctx, const float *img, const int64_t dims[3],
+                               const float *taps, const int radius[3], float *hess6, float *work,
+                               void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, img && taps && radius && hess6 && work, "null pointer");
+    int rmax = 0;
+    for (int a = 0; a < 3; ++a) {
+        MH_REQUIRE(ctx, radius[a] >= 0 && radius[a] <= kMaxRadius, "radius out of range");
+        rmax = std::max(rmax, radius[a]);
+    }
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    const int K = 2 * rmax + 1;
+    auto tapset = [&](int axis, int order) { return taps + (3 * axis + order) * K; };
+    float *A[3] = {work, work + n, work + 2 * n};  // axis-0 results for orders 0, 1, 2
+    float *B = work + 3 * n;                          // 6 axis-1 results
+    hipStream_t s = (hipStream_t)stream;
+
+    // pass along axis 0: image -> A0, A1, A2
+    {
+        ConvPass P;
+        std::memset(&P, 0, sizeof(P));
+        P.radius = radius[0];
+        for (int o = 0; o < 3; ++o) set_taps(P, o, tapset(0, o), radius[0]);
+        P.ngroups = 1;
+        P.grp[0] = ConvGroup{img, 3, {{A[0], 0, 1, 0}, {A[1], 1, 1, 0}, {A[2], 2, 1, 0}}};
+        int rc = launch_pass(ctx, 0, P, dims, s);
+        if (rc) return rc;
+    }
+    // pass along axis 1: B0..B5 = A0*o0, A0*o1, A0*o2, A1*o0, A1*o1, A2*o0
+    {
+        ConvPass P;
+        std::memset(&P, 0, sizeof(P));
+        P.radius = radius[1];
+        for (int o = 0; o < 3; ++o) set_taps(P, o, tapset(1, o), radius[1]);
+        P.ngroups = 3;
+        P.grp[0] = ConvGroup{A[0], 3, {{B, 0, 1, 0}, {B + n, 1, 1, 0}, {B + 2 * n, 2, 1, 0}}};
+        P.grp[1] = ConvGroup{A[1], 2, {{B + 3 * n, 0, 1, 0}, {B + 4 * n, 1, 1, 0}, {}}};
+        P.grp[2] = ConvGroup{A[2], 1, {{B + 5 * n, 0, 1, 0}, {}, {}}};
+        int rc = launch_pass(ctx, 1, P, dims, s);
+        if (rc) return rc;
+    }
+    // pass along axis 2 into AoS x6 [H00, H11, H22, H01, H02, H12]:
+    //   H00 = (o2,o0,o0) from B5, H11 = (o0,o2,o0) from B2, H22 = (o0,o0,o2) from B0,
+    //   H01 = (o1,o1,o0) from B4, H02 = (o1,o0,o1) from B3, H12 = (o0,o1,o1) from B1
+    {
+        ConvPass P;
+        std::memset(&P, 0, sizeof(P));
+        P.radius = radius[2];
+        for (int o = 0; o < 3; ++o) set_taps(P, o, tapset(2, o), radius[2]);
+        P.ngroups = 6;
+        P.grp[0] = ConvGroup{B + 5 * n, 1, {{hess6, 0, 6, 0}, {}, {}}};
+        P.grp[1] = ConvGroup{B + 2 * n, 1, {{hess6, 0, 6, 1}, {}, {}}};
+        P.grp[2] = ConvGroup{B, 1, {{hess6, 2, 6, 2}, {}, {}}};
+        P.grp[3] = ConvGroup{B + 4 * n, 1, {{hess6, 0, 6, 3}, {}, {}}};
+        P.grp[4] = ConvGroup{B + 3 * n, 1, {{hess6, 1, 6, 4}, {}, {}}};
+        P.grp[5] = ConvGroup{B + n, 1, {{hess6, 1, 6, 5}, {}, {}}};
+        int rc = launch_pass(ctx, 2, P, dims, s);
+        if (rc) return rc;
+    }
+    return MH_OK;
+}
+
+MH_EXPORT int mh_eigs(mh_ctx *ctx, const float *hess6, float *eig6, int64_t n, int maxiter,
+                      float *norm_max, void *stream)
+{
+    MH_REQUIRE(ctx, hess6 && eig6 && n >= 0 && n < (int64_t(1) << 31), "bad arguments");
+    MH_REQUIRE(ctx, maxiter >= 0, "negative maxiter");
+    if (n == 0) return MH_OK;
+    hipLaunchKernelGGL(eig_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, hess6, eig6, n, maxiter, norm_max);
+    MH_CHECK_LAUNCH(ctx, "eig_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int mh_update_vesselness(mh_ctx *ctx, float *val, float *quat, const float *eig6,
+                                   int64_t n, float alpha, float beta, float c_value,
+                                   const float *norm_max, int bright, void *stream)
+{
+    MH_REQUIRE(ctx, val && quat && eig6 && n >= 0, "bad arguments");
+    MH_REQUIRE(ctx, c_value >= 0.f || norm_max, "c < 0 needs norm_max_dev");
+    if (n == 0) return MH_OK;
+    hipLaunchKernelGGL(vesselness_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, val, quat, eig6, n, alpha, beta, c_value, norm_max,
+                       bright);
+    MH_CHECK_LAUNCH(ctx, "vesselness_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int mh_eig_norm_max(mh_ctx *ctx, const float *eig6, int64_t n, float *norm_max,
+                              void *stream)
+{
+    MH_REQUIRE(ctx, eig6 && norm_max && n >= 0 && n < (int64_t(1) << 31), "bad arguments");
+    int rc = mh_fill(ctx, norm_max, 1, 0.f, stream);
+    if (rc || n == 0) return rc;
+    hipLaunchKernelGGL(eig_norm_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
+                       (hipStream_t)stream, eig6, n, norm_max);
+    MH_CHECK_LAUNCH(ctx, "eig_norm_kernel");
+    return MH_OK;
+}

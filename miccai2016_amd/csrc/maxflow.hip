@@ -1,0 +1,848 @@
+// Anisotropic-TV continuous max-flow on gfx950.
+//
+// State per voxel: hp = H p (3 floats, H = R(q) diag(sqrt(lambda))), the sink
+// flow pt, the labelling u, and tmp = div(hp) - ps + pt - rcc u, the residual
+// the p-step differentiates. One iteration is two stencil launches:
+//   p-step: p = H^-1 hp + H^T (ss * grad+ tmp); project |p| <= alpha; hp = H p
+//   u-step: div hp; ps, pt projections; err; u -= err / rcc; tmp = err - rcc u
+// The reference's third launch (tmp -= rcc u) is folded into the u-step; on
+// criterion iterations the u-step also reduces sum |err| and the last block to
+// finish decides the stop, so the host never waits inside the loop. A stopped
+// solve turns every later launch into an immediate return.
+#include <vector>
+
+#include "mh_common.h"
+#include "mh_reduce.h"
+
+namespace {
+
+// control block (int32 words) at the head of the solver workspace
+enum : int { kStop = 0, kNcrit = 1, kCounter = 2, kCtrlWords = 64 };
+constexpr int kCritLogMax = 1 << 16;
+
+struct SumOp {
+    __device__ static float identity() { return 0.f; }
+    __device__ static float combine(float a, float b) { return a + b; }
+};
+
+struct Dims {
+    int d0, d1, d2;
+};
+
+// min() of the source algorithm's kernel language: y if y < x, else x, so a
+// NaN residual propagates into the criterion while a NaN capacity is dropped.
+__device__ __forceinline__ float kmin(float x, float y) { return y < x ? y : x; }
+
+__device__ __forceinline__ void coords(int64_t i, const Dims D, int &i0, int &i1, int &i2)
+{
+    i2 = int(i % D.d2);
+    const int64_t r = i / D.d2;
+    i1 = int(r % D.d1);
+    i0 = int(r / D.d1);
+}
+
+// ------------------------------------------------------------------ init ---
+
+// q4 = unit quaternion of the squeezed frame; pt = min(cs, ct); hp = 0;
+// tmp = -rcc u (the residual with ps = pt and div hp = 0).
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_init_kernel(float4 *__restrict__ q4, const float *__restrict__ eigs, int eig_stride,
+                int eig_offset, float *__restrict__ hp, float *__restrict__ pt,
+                float *__restrict__ tmp, const float *__restrict__ u, const float *__restrict__ cs,
+                const float *__restrict__ ct, int64_t n, float rcc)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *e = eigs + i * eig_stride + eig_offset;
+    const mh::Quat q = mh::expand_squeezed(e[0], e[1], e[2]);
+    q4[i] = make_float4(q.w, q.x, q.y, q.z);
+    hp[i] = 0.f;
+    hp[n + i] = 0.f;
+    hp[2 * n + i] = 0.f;
+    pt[i] = fminf(cs[i], ct[i]);
+    tmp[i] = -(u[i] * rcc);
+}
+
+// ---------------------------------------------------------------- p-step ---
+
+struct PParams {
+    float ss0, ss1, ss2;
+    float sq0, sq1, sq2;  // constant sqrt(lambda) (CONST_EV)
+};
+
+template <bool CONST_EV>
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp, const float4 *__restrict__ q4,
+             const float *__restrict__ eigs6, const float *__restrict__ alpha, const Dims D,
+             const PParams P, const int *__restrict__ ctrl)
+{
+    if (ctrl[kStop]) return;
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int i0, i1, i2;
+    coords(i, D, i0, i1, i2);
+    const int64_t s0 = int64_t(D.d1) * D.d2;
+    const float t = tmp[i];
+    const float g0 = i0 < D.d0 - 1 ? (tmp[i + s0] - t) * P.ss0 : 0.f;
+    const float g1 = i1 < D.d1 - 1 ? (tmp[i + D.d2] - t) * P.ss1 : 0.f;
+    const float g2 = i2 < D.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
+    float sq0 = P.sq0, sq1 = P.sq1, sq2 = P.sq2;
+    if (!CONST_EV) {
+        sq0 = eigs6[6 * i];
+        sq1 = eigs6[6 * i + 1];
+        sq2 = eigs6[6 * i + 2];
+    }
+    const float4 qq = q4[i];
+    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
+    const float h0 = hp[i], h1 = hp[n + i], h2 = hp[2 * n + i];
+    // p = diag(1/sq) R^T hp + diag(sq) R^T g
+    float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / sq0 + sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
+    float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / sq1 + sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
+    float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / sq2 + sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
+    // project onto the ball of radius alpha
+    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    const float a = alpha[i];
+    const float sc = nrm > a ? a / nrm : 1.f;
+    p0 *= sc * sq0;
+    p1 *= sc * sq1;
+    p2 *= sc * sq2;
+    hp[i] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
+    hp[n + i] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
+    hp[2 * n + i] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+}
+
+// ---------------------------------------------------------------- u-step ---
+
+struct UParams {
+    float rcc, inv_rcc;
+    float rs0, rs1, rs2;
+    float error_bound;
+    float inv_n_rcc;  // 1 / (N rcc)
+};
+
+template <bool SUP, bool CHECK>
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float *__restrict__ cs,
+             const float *__restrict__ ct, float *__restrict__ pt, float *__restrict__ u,
+             const uint8_t *__restrict__ labels, const Dims D, const UParams P,
+             int *__restrict__ ctrl, float *__restrict__ partials, float *__restrict__ crit_log)
+{
+    if (ctrl[kStop]) return;
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    float aerr = 0.f;
+    if (i < n) {
+        int i0, i1, i2;
+        coords(i, D, i0, i1, i2);
+        const int64_t s0 = int64_t(D.d1) * D.d2;
+        const float *hp0 = hp, *hp1 = hp + n, *hp2 = hp + 2 * n;
+        // negative adjoint of the forward difference
+        const float x0 = (i0 < D.d0 - 1 ? hp0[i] : 0.f) - (i0 > 0 ? hp0[i - s0] : 0.f);
+        const float x1 = (i1 < D.d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - D.d2] : 0.f);
+        const float x2 = (i2 < D.d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f);
+        const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
+        float ui = u[i];
+        float ub = 1.f, uf = 0.f;
+        if (SUP) {
+            const uint8_t l = labels[i];
+            uf = float(l & 1);
+            ub = (l & 2) ? 0.f : 1.f;
+        }
+        const float ps = kmin(fmaf(-P.rcc, ui - ub, div + pt[i]), cs[i]);
+        const float pti = kmin(fmaf(P.rcc, ui - uf, ps - div), ct[i]);
+        const float err = div - ps + pti;
+        ui = ui - err / P.rcc;
+        pt[i] = pti;
+        u[i] = ui;
+        tmp[i] = fmaf(-ui, P.rcc, err);
+        aerr = fabsf(err);
+    }
+    if (!CHECK) return;
+    aerr = mh::block_reduce<SumOp>(aerr);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = aerr;
+        __threadfence();
+        last = atomicAdd(reinterpret_cast<unsigned *>(ctrl + kCounter), 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    float s = 0.f;
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) s += partials[k];
+    s = mh::block_reduce<SumOp>(s);
+    if (threadIdx.x == 0) {
+        const float crit = s * P.inv_n_rcc;
+        const int k = ctrl[kNcrit];
+        if (k < kCritLogMax) crit_log[k] = crit;
+        ctrl[kNcrit] = k + 1;
+        if (crit != crit) ctrl[kStop] = 1;
+        else if (crit < P.error_bound) ctrl[kStop] = 2;
+        ctrl[kCounter] = 0;
+    }
+}
+
+struct Workspace {
+    int *ctrl;
+    float *crit_log;
+    float *partials;
+    float *hp;
+    float *pt;
+    float *tmp;
+    float4 *q4;
+};
+
+int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+int64_t layout(int64_t n, char *base, Workspace *w)
+{
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    Workspace ws;
+    ws.ctrl = reinterpret_cast<int *>(take(sizeof(int) * kCtrlWords));
+    ws.crit_log = reinterpret_cast<float *>(take(sizeof(float) * kCritLogMax));
+    ws.partials = reinterpret_cast<float *>(take(sizeof(float) * (mh::blocks_for(n) + 64)));
+    ws.hp = reinterpret_cast<float *>(take(sizeof(float) * 3 * n));
+    ws.pt = reinterpret_cast<float *>(take(sizeof(float) * n));
+    ws.tmp = reinterpret_cast<float *>(take(sizeof(float) * n));
+    ws.q4 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    if (w) *w = ws;
+    return off;
+}
+
+__global__ void zero_ints(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+MH_EXPORT int64_t mh_cmf_workspace_bytes(int64_t n)
+{
+    if (n <= 0 || n >= (int64_t(1) << 31)) return -1;
+    return layout(n, nullptr, nullptr);
+}
+
+MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float *alpha,
+                           const float *cs, const float *ct, const uint8_t *labels,
+                           const int64_t dims[3], const mh_cmf_params *prm, void *work,
+                           int64_t work_bytes, mh_cmf_result *res, float *crit_host, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, u && eigs && alpha && cs && ct && prm && work, "null pointer");
+    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1, "num_iter >= 0 and update >= 1 required");
+    MH_REQUIRE(ctx, prm->rcc > 0.f, "rcc must be positive");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    MH_REQUIRE(ctx, work_bytes >= layout(n, nullptr, nullptr), "workspace too small");
+    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(work) & 255) == 0, "workspace must be 256-byte aligned");
+    Workspace W;
+    layout(n, static_cast<char *>(work), &W);
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned nb = mh::blocks_for(n);
+    const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
+
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, W.ctrl, int(kCtrlWords));
+    MH_CHECK_LAUNCH(ctx, "zero_ints");
+    const int es = prm->const_evals ? 3 : 6, eo = prm->const_evals ? 0 : 3;
+    hipLaunchKernelGGL(cmf_init_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.q4, eigs, es, eo, W.hp,
+                       W.pt, W.tmp, u, cs, ct, n, prm->rcc);
+    MH_CHECK_LAUNCH(ctx, "cmf_init_kernel");
+
+    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
+    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
+                     float(1.0 / (double(n) * double(prm->rcc)))};
+    const bool sup = labels != nullptr;
+    for (int it = 0; it < prm->num_iter; ++it) {
+        if (prm->const_evals)
+            hipLaunchKernelGGL(cmf_p_kernel<true>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
+                               eigs, alpha, D, PP, W.ctrl);
+        else
+            hipLaunchKernelGGL(cmf_p_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
+                               eigs, alpha, D, PP, W.ctrl);
+        const bool chk = (it % prm->update) == 0;
+        if (sup && chk)
+            hipLaunchKernelGGL((cmf_u_kernel<true, true>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
+                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+        else if (sup)
+            hipLaunchKernelGGL((cmf_u_kernel<true, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
+                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+        else if (chk)
+            hipLaunchKernelGGL((cmf_u_kernel<false, true>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
+                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+        else
+            hipLaunchKernelGGL((cmf_u_kernel<false, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
+                               W.tmp, cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+        MH_CHECK_LAUNCH(ctx, "cmf kernels");
+    }
+    int ctrl[kCtrlWords];
+    MH_CHECK(ctx, hipMemcpyAsync(ctrl, W.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
+    MH_CHECK(ctx, hipStreamSynchronize(s));
+    const int ncrit = std::min(ctrl[kNcrit], kCritLogMax);
+    std::vector<float> log(std::max(ncrit, 1));
+    if (ncrit > 0)
+        MH_CHECK(ctx, hipMemcpy(log.data(), W.crit_log, sizeof(float) * ncrit, hipMemcpyDeviceToHost));
+    if (crit_host && ncrit > 0) std::copy(log.begin(), log.begin() + ncrit, crit_host);
+    if (res) {
+        res->stopped = ctrl[kStop];
+        res->ncrit = ncrit;
+        res->iterations = ctrl[kStop] ? (ctrl[kNcrit] - 1) * prm->update + 1 : prm->num_iter;
+        res->last_crit = ncrit > 0 ? log[ncrit - 1] : 0.f;
+    }
+    return ctrl[kStop] == 1 ? MH_STOPPED_NAN : MH_OK;
+}
+
+// =========================================================================
+// Batched solves: S unsupervised constant-eigenvalue solves on one volume
+// that share the frame field, cs, ct and |grad I|^2. A thread owns one voxel
+// and a group of SI solves: the shared fields and the rotation are loaded
+// once per group, and each solve's alpha = w exp(-gms / decay^2) is formed
+// in registers. Solve s keeps its own state (hp, pt, tmp, u) and its own
+// stop flag and criterion log.
+// =========================================================================
+namespace {
+
+constexpr int kSI = 8;  // solves per thread
+
+struct BatchCtrl {
+    int *stop;        // [S]
+    int *ncrit;       // [S]
+    unsigned *count;  // [groups]
+    float *crit_log;  // [S][maxcheck]
+    int maxcheck;
+};
+
+struct SolveParams {
+    float w[kSI];
+    float k[kSI];  // 1 / decay^2
+};
+
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_batch_init_kernel(float *__restrict__ hp, float *__restrict__ pt, float *__restrict__ tmp,
+                      const float *__restrict__ u, const float *__restrict__ cs,
+                      const float *__restrict__ ct, int64_t n, float rcc)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = blockIdx.y;
+    float *h = hp + 3 * n * s;
+    h[i] = 0.f;
+    h[n + i] = 0.f;
+    h[2 * n + i] = 0.f;
+    pt[n * s + i] = fminf(cs[i], ct[i]);
+    tmp[n * s + i] = -(u[n * s + i] * rcc);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+quat_kernel(float4 *__restrict__ q4, const float *__restrict__ eigs, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const mh::Quat q = mh::expand_squeezed(eigs[3 * i], eigs[3 * i + 1], eigs[3 * i + 2]);
+    q4[i] = make_float4(q.w, q.x, q.y, q.z);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
+                   const float4 *__restrict__ q4, const float *__restrict__ gms, const Dims D,
+                   const PParams P, const SolveParams SP, int s0, int ns, const int *__restrict__ stop)
+{
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int i0, i1, i2;
+    coords(i, D, i0, i1, i2);
+    const int64_t st0 = int64_t(D.d1) * D.d2;
+    const bool e0 = i0 < D.d0 - 1, e1 = i1 < D.d1 - 1, e2 = i2 < D.d2 - 1;
+    const float4 qq = q4[i];
+    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
+    const float gm = gms[i];
+#pragma unroll
+    for (int k = 0; k < kSI; ++k) {
+        const int s = s0 + k;
+        if (k >= ns || stop[s]) continue;  // uniform over the block
+        const float *ts = tmp + n * s;
+        float *h = hp + 3 * n * s;
+        const float t = ts[i];
+        const float g0 = e0 ? (ts[i + st0] - t) * P.ss0 : 0.f;
+        const float g1 = e1 ? (ts[i + D.d2] - t) * P.ss1 : 0.f;
+        const float g2 = e2 ? (ts[i + 1] - t) * P.ss2 : 0.f;
+        const float h0 = h[i], h1 = h[n + i], h2 = h[2 * n + i];
+        float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / P.sq0 + P.sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
+        float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / P.sq1 + P.sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
+        float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / P.sq2 + P.sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
+        const float a = SP.w[k] * expf(-gm * SP.k[k]);
+        const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+        const float sc = nrm > a ? a / nrm : 1.f;
+        p0 *= sc * P.sq0;
+        p1 *= sc * P.sq1;
+        p2 *= sc * P.sq2;
+        h[i] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
+        h[n + i] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
+        h[2 * n + i] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+    }
+}
+
+template <bool CHECK>
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
+                   const float *__restrict__ cs, const float *__restrict__ ct,
+                   float *__restrict__ pt, float *__restrict__ u, const Dims D, const UParams P,
+                   int s0, int ns, BatchCtrl C, float *__restrict__ partials, int group)
+{
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    float aerr[kSI];
+    for (int k = 0; k < kSI; ++k) aerr[k] = 0.f;
+    if (i < n) {
+        int i0, i1, i2;
+        coords(i, D, i0, i1, i2);
+        const int64_t st0 = int64_t(D.d1) * D.d2;
+        const bool e0 = i0 < D.d0 - 1, e1 = i1 < D.d1 - 1, e2 = i2 < D.d2 - 1;
+        const bool f0 = i0 > 0, f1 = i1 > 0, f2 = i2 > 0;
+        const float csi = cs[i], cti = ct[i];
+#pragma unroll
+        for (int k = 0; k < kSI; ++k) {
+            const int s = s0 + k;
+            if (k >= ns || C.stop[s]) continue;  // uniform over the block
+            const float *h = hp + 3 * n * s;
+            const float x0 = (e0 ? h[i] : 0.f) - (f0 ? h[i - st0] : 0.f);
+            const float x1 = (e1 ? h[n + i] : 0.f) - (f1 ? h[n + i - D.d2] : 0.f);
+            const float x2 = (e2 ? h[2 * n + i] : 0.f) - (f2 ? h[2 * n + i - 1] : 0.f);
+            const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
+            float ui = u[n * s + i];
+            const float ps = kmin(fmaf(-P.rcc, ui - 1.f, div + pt[n * s + i]), csi);
+            const float pti = kmin(fmaf(P.rcc, ui, ps - div), cti);
+            const float err = div - ps + pti;
+            ui = ui - err / P.rcc;
+            pt[n * s + i] = pti;
+            u[n * s + i] = ui;
+            tmp[n * s + i] = fmaf(-ui, P.rcc, err);
+            aerr[k] = fabsf(err);
+        }
+    }
+    if (!CHECK) return;
+    const unsigned nbx = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < kSI; ++k) {
+        if (k >= ns) continue;  // uniform
+        const float v = mh::block_reduce<SumOp>(aerr[k]);
+        if (threadIdx.x == 0) partials[int64_t(s0 + k) * nbx + blockIdx.x] = v;
+    }
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(C.count + group, 1u) == nbx - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int k = 0; k < ns; ++k) {
+        const int s = s0 + k;
+        float acc = 0.f;
+        for (unsigned q = threadIdx.x; q < nbx; q += blockDim.x) acc += partials[int64_t(s) * nbx + q];
+        acc = mh::block_reduce<SumOp>(acc);
+        if (threadIdx.x == 0 && !C.stop[s]) {
+            const float crit = acc * P.inv_n_rcc;
+            const int c = C.ncrit[s];
+            if (c < C.maxcheck) C.crit_log[int64_t(s) * C.maxcheck + c] = crit;
+            C.ncrit[s] = c + 1;
+            if (crit != crit) C.stop[s] = 1;
+            else if (crit < P.error_bound) C.stop[s] = 2;
+        }
+    }
+    if (threadIdx.x == 0) C.count[group] = 0u;
+}
+
+struct BatchWorkspace {
+    int *stop, *ncrit;
+    unsigned *count;
+    float *crit_log, *partials, *hp, *pt, *tmp;
+    float4 *q4;
+    int maxcheck;
+};
+
+int64_t batch_layout(int64_t n, int S, int maxcheck, char *base, BatchWorkspace *w)
+{
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    const int groups = (S + kSI - 1) / kSI;
+    BatchWorkspace ws;
+    ws.stop = reinterpret_cast<int *>(take(sizeof(int) * (2 * int64_t(S) + groups)));
+    ws.ncrit = ws.stop ? ws.stop + S : nullptr;
+    ws.count = ws.stop ? reinterpret_cast<unsigned *>(ws.stop + 2 * S) : nullptr;
+    ws.crit_log = reinterpret_cast<float *>(take(sizeof(float) * int64_t(S) * maxcheck));
+    ws.partials = reinterpret_cast<float *>(take(sizeof(float) * int64_t(S) * mh::blocks_for(n)));
+    ws.q4 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    ws.hp = reinterpret_cast<float *>(take(sizeof(float) * 3 * n * S));
+    ws.pt = reinterpret_cast<float *>(take(sizeof(float) * n * S));
+    ws.tmp = reinterpret_cast<float *>(take(sizeof(float) * n * S));
+    ws.maxcheck = maxcheck;
+    if (w) *w = ws;
+    return off;
+}
+
+int maxcheck_for(const mh_cmf_params *p)
+{
+    return p->update >= 1 ? std::max(1, (p->num_iter + p->update - 1) / p->update) : 1;
+}
+
+}  // namespace
+
+MH_EXPORT int64_t mh_cmf_batched_workspace_bytes(int64_t n, int nsolves)
+{
+    if (n <= 0 || n >= (int64_t(1) << 31) || nsolves <= 0) return -1;
+    // the criterion log is sized for the largest check count a call may use
+    return batch_layout(n, nsolves, kCritLogMax / 16, nullptr, nullptr);
+}
+
+MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_host,
+                                   const float *decay_host, const float *eigs, const float *gms,
+                                   const float *cs, const float *ct, const int64_t dims[3],
+                                   const mh_cmf_params *prm, void *work, int64_t work_bytes,
+                                   mh_cmf_result *res, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, u && w_host && decay_host && eigs && gms && cs && ct && prm && work, "null pointer");
+    MH_REQUIRE(ctx, S >= 1, "nsolves must be >= 1");
+    MH_REQUIRE(ctx, prm->const_evals, "batched solves use constant eigenvalues");
+    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1 && prm->rcc > 0.f, "bad params");
+    const int maxcheck = maxcheck_for(prm);
+    MH_REQUIRE(ctx, maxcheck <= kCritLogMax / 16, "too many criterion checks (num_iter / update)");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    MH_REQUIRE(ctx, work_bytes >= batch_layout(n, S, kCritLogMax / 16, nullptr, nullptr), "workspace too small");
+    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(work) & 255) == 0, "workspace must be 256-byte aligned");
+    BatchWorkspace W;
+    batch_layout(n, S, maxcheck, static_cast<char *>(work), &W);
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned nb = mh::blocks_for(n);
+    const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
+    const int groups = (S + kSI - 1) / kSI;
+
+    const int nctrl = 2 * S + groups;
+    hipLaunchKernelGGL(zero_ints, dim3((nctrl + 255) / 256), dim3(256), 0, s, W.stop, nctrl);
+    hipLaunchKernelGGL(quat_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.q4, eigs, n);
+    hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, S), dim3(mh::kBlock), 0, s, W.hp, W.pt, W.tmp,
+                       u, cs, ct, n, prm->rcc);
+    MH_CHECK_LAUNCH(ctx, "batched init");
+
+    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
+    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
+                     float(1.0 / (double(n) * double(prm->rcc)))};
+    std::vector<SolveParams> sp(groups);
+    for (int g = 0; g < groups; ++g)
+        for (int k = 0; k < kSI; ++k) {
+            const int j = std::min(g * kSI + k, S - 1);
+            sp[g].w[k] = w_host[j];
+            sp[g].k[k] = float(1.0 / (double(decay_host[j]) * double(decay_host[j])));
+        }
+    const BatchCtrl C{W.stop, W.ncrit, W.count, W.crit_log, maxcheck};
+    for (int it = 0; it < prm->num_iter; ++it) {
+        const bool chk = (it % prm->update) == 0;
+        for (int g = 0; g < groups; ++g) {
+            const int s0 = g * kSI, ns = std::min(kSI, S - s0);
+            hipLaunchKernelGGL(cmf_batch_p_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
+                               gms, D, PP, sp[g], s0, ns, W.stop);
+        }
+        for (int g = 0; g < groups; ++g) {
+            const int s0 = g * kSI, ns = std::min(kSI, S - s0);
+            if (chk)
+                hipLaunchKernelGGL(cmf_batch_u_kernel<true>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
+                                   cs, ct, W.pt, u, D, UP, s0, ns, C, W.partials, g);
+            else
+                hipLaunchKernelGGL(cmf_batch_u_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
+                                   W.tmp, cs, ct, W.pt, u, D, UP, s0, ns, C, W.partials, g);
+        }
+        MH_CHECK_LAUNCH(ctx, "batched cmf kernels");
+    }
+    std::vector<int> stop(2 * S);
+    MH_CHECK(ctx, hipMemcpyAsync(stop.data(), W.stop, sizeof(int) * 2 * S, hipMemcpyDeviceToHost, s));
+    std::vector<float> log(int64_t(S) * maxcheck);
+    MH_CHECK(ctx, hipMemcpyAsync(log.data(), W.crit_log, sizeof(float) * log.size(), hipMemcpyDeviceToHost, s));
+    MH_CHECK(ctx, hipStreamSynchronize(s));
+    int any_nan = 0;
+    for (int j = 0; j < S; ++j) {
+        const int st = stop[j], nc = std::min(stop[S + j], maxcheck);
+        any_nan |= st == 1;
+        if (res) {
+            res[j].stopped = st;
+            res[j].ncrit = nc;
+            res[j].iterations = st ? (stop[S + j] - 1) * prm->update + 1 : prm->num_iter;
+            res[j].last_crit = nc > 0 ? log[int64_t(j) * maxcheck + nc - 1] : 0.f;
+        }
+    }
+    return any_nan ? MH_STOPPED_NAN : MH_OK;
+}
+
+// =========================================================================
+// Ragged batch: V volumes of different dims packed one after another; every
+// field of volume v starts at element offset off[v]. The grid covers the
+// blocks of all volumes; block -> volume by binary search over the block
+// prefix table. One unsupervised constant-eigenvalue solve per volume.
+// =========================================================================
+namespace {
+
+struct Vol {
+    int d0, d1, d2, block0;
+    int64_t off;
+};
+
+__device__ __forceinline__ int find_vol(const Vol *__restrict__ vols, int nvol, unsigned b)
+{
+    int lo = 0, hi = nvol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (unsigned(vols[mid].block0) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+ragged_init_kernel(const Vol *__restrict__ vols, int nvol, float4 *__restrict__ q4,
+                   const float *__restrict__ eigs, float *__restrict__ hp, float *__restrict__ pt,
+                   float *__restrict__ tmp, const float *__restrict__ u, const float *__restrict__ cs,
+                   const float *__restrict__ ct, float rcc)
+{
+    const int v = find_vol(vols, nvol, blockIdx.x);
+    const Vol V = vols[v];
+    const int64_t n = int64_t(V.d0) * V.d1 * V.d2;
+    const int64_t li = int64_t(blockIdx.x - V.block0) * blockDim.x + threadIdx.x;
+    if (li >= n) return;
+    const int64_t i = V.off + li;
+    const mh::Quat q = mh::expand_squeezed(eigs[3 * i], eigs[3 * i + 1], eigs[3 * i + 2]);
+    q4[i] = make_float4(q.w, q.x, q.y, q.z);
+    float *h = hp + 3 * V.off;
+    h[li] = 0.f;
+    h[n + li] = 0.f;
+    h[2 * n + li] = 0.f;
+    pt[i] = fminf(cs[i], ct[i]);
+    tmp[i] = -(u[i] * rcc);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+ragged_p_kernel(const Vol *__restrict__ vols, int nvol, float *__restrict__ hp,
+                const float *__restrict__ tmp, const float4 *__restrict__ q4,
+                const float *__restrict__ alpha, const PParams P, const int *__restrict__ stop)
+{
+    const int v = find_vol(vols, nvol, blockIdx.x);
+    if (stop[v]) return;
+    const Vol V = vols[v];
+    const int64_t n = int64_t(V.d0) * V.d1 * V.d2;
+    const int64_t li = int64_t(blockIdx.x - V.block0) * blockDim.x + threadIdx.x;
+    if (li >= n) return;
+    const Dims D{V.d0, V.d1, V.d2};
+    int i0, i1, i2;
+    coords(li, D, i0, i1, i2);
+    const int64_t i = V.off + li;
+    const int64_t st0 = int64_t(V.d1) * V.d2;
+    const float t = tmp[i];
+    const float g0 = i0 < V.d0 - 1 ? (tmp[i + st0] - t) * P.ss0 : 0.f;
+    const float g1 = i1 < V.d1 - 1 ? (tmp[i + V.d2] - t) * P.ss1 : 0.f;
+    const float g2 = i2 < V.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
+    const float4 qq = q4[i];
+    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
+    float *h = hp + 3 * V.off;
+    const float h0 = h[li], h1 = h[n + li], h2 = h[2 * n + li];
+    float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / P.sq0 + P.sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
+    float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / P.sq1 + P.sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
+    float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / P.sq2 + P.sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
+    const float a = alpha[i];
+    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    const float sc = nrm > a ? a / nrm : 1.f;
+    p0 *= sc * P.sq0;
+    p1 *= sc * P.sq1;
+    p2 *= sc * P.sq2;
+    h[li] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
+    h[n + li] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
+    h[2 * n + li] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+}
+
+template <bool CHECK>
+__global__ void __launch_bounds__(mh::kBlock)
+ragged_u_kernel(const Vol *__restrict__ vols, int nvol, const float *__restrict__ hp,
+                float *__restrict__ tmp, const float *__restrict__ cs, const float *__restrict__ ct,
+                float *__restrict__ pt, float *__restrict__ u, const UParams P, BatchCtrl C,
+                float *__restrict__ partials)
+{
+    const int v = find_vol(vols, nvol, blockIdx.x);
+    if (C.stop[v]) return;  // uniform over the block
+    const Vol V = vols[v];
+    const int64_t n = int64_t(V.d0) * V.d1 * V.d2;
+    const int64_t li = int64_t(blockIdx.x - V.block0) * blockDim.x + threadIdx.x;
+    float aerr = 0.f;
+    if (li < n) {
+        const Dims D{V.d0, V.d1, V.d2};
+        int i0, i1, i2;
+        coords(li, D, i0, i1, i2);
+        const int64_t i = V.off + li;
+        const int64_t st0 = int64_t(V.d1) * V.d2;
+        const float *h = hp + 3 * V.off;
+        const float x0 = (i0 < V.d0 - 1 ? h[li] : 0.f) - (i0 > 0 ? h[li - st0] : 0.f);
+        const float x1 = (i1 < V.d1 - 1 ? h[n + li] : 0.f) - (i1 > 0 ? h[n + li - V.d2] : 0.f);
+        const float x2 = (i2 < V.d2 - 1 ? h[2 * n + li] : 0.f) - (i2 > 0 ? h[2 * n + li - 1] : 0.f);
+        const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
+        float ui = u[i];
+        const float ps = kmin(fmaf(-P.rcc, ui - 1.f, div + pt[i]), cs[i]);
+        const float pti = kmin(fmaf(P.rcc, ui, ps - div), ct[i]);
+        const float err = div - ps + pti;
+        ui = ui - err / P.rcc;
+        pt[i] = pti;
+        u[i] = ui;
+        tmp[i] = fmaf(-ui, P.rcc, err);
+        aerr = fabsf(err);
+    }
+    if (!CHECK) return;
+    aerr = mh::block_reduce<SumOp>(aerr);
+    const int nbv = int((n + blockDim.x - 1) / blockDim.x);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = aerr;
+        __threadfence();
+        last = atomicAdd(C.count + v, 1u) == unsigned(nbv - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    float acc = 0.f;
+    for (int q = threadIdx.x; q < nbv; q += blockDim.x) acc += partials[V.block0 + q];
+    acc = mh::block_reduce<SumOp>(acc);
+    if (threadIdx.x == 0) {
+        const float crit = acc / (float(n) * P.rcc);
+        const int c = C.ncrit[v];
+        if (c < C.maxcheck) C.crit_log[int64_t(v) * C.maxcheck + c] = crit;
+        C.ncrit[v] = c + 1;
+        if (crit != crit) C.stop[v] = 1;
+        else if (crit < P.error_bound) C.stop[v] = 2;
+        C.count[v] = 0u;
+    }
+}
+
+struct RaggedWorkspace {
+    Vol *vols;
+    int *stop, *ncrit;
+    unsigned *count;
+    float *crit_log, *partials, *hp, *pt, *tmp;
+    float4 *q4;
+};
+
+int64_t ragged_layout(int64_t total, int nvol, int64_t nblocks, int maxcheck, char *base,
+                      RaggedWorkspace *w)
+{
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    RaggedWorkspace ws;
+    ws.vols = reinterpret_cast<Vol *>(take(sizeof(Vol) * nvol));
+    ws.stop = reinterpret_cast<int *>(take(sizeof(int) * 3 * int64_t(nvol)));
+    ws.ncrit = ws.stop ? ws.stop + nvol : nullptr;
+    ws.count = ws.stop ? reinterpret_cast<unsigned *>(ws.stop + 2 * nvol) : nullptr;
+    ws.crit_log = reinterpret_cast<float *>(take(sizeof(float) * int64_t(nvol) * maxcheck));
+    ws.partials = reinterpret_cast<float *>(take(sizeof(float) * nblocks));
+    ws.q4 = reinterpret_cast<float4 *>(take(sizeof(float4) * total));
+    ws.hp = reinterpret_cast<float *>(take(sizeof(float) * 3 * total));
+    ws.pt = reinterpret_cast<float *>(take(sizeof(float) * total));
+    ws.tmp = reinterpret_cast<float *>(take(sizeof(float) * total));
+    if (w) *w = ws;
+    return off;
+}
+
+}  // namespace
+
+MH_EXPORT int64_t mh_cmf_ragged_workspace_bytes(int64_t total, int nvol)
+{
+    if (total <= 0 || nvol <= 0) return -1;
+    // every volume can add at most one partially filled block
+    const int64_t nblocks = mh::blocks_for(total) + nvol;
+    return ragged_layout(total, nvol, nblocks, kCritLogMax / 16, nullptr, nullptr);
+}
+
+MH_EXPORT int mh_cmf_solve_ragged(mh_ctx *ctx, float *u, const float *eigs, const float *alpha,
+                                  const float *cs, const float *ct, int nvol, const int64_t *vdims,
+                                  const int64_t *voff, const mh_cmf_params *prm, void *work,
+                                  int64_t work_bytes, mh_cmf_result *res, void *stream)
+{
+    MH_REQUIRE(ctx, u && eigs && alpha && cs && ct && vdims && voff && prm && work, "null pointer");
+    MH_REQUIRE(ctx, nvol >= 1, "nvol must be >= 1");
+    MH_REQUIRE(ctx, prm->const_evals, "ragged solves use constant eigenvalues");
+    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1 && prm->rcc > 0.f, "bad params");
+    const int maxcheck = maxcheck_for(prm);
+    MH_REQUIRE(ctx, maxcheck <= kCritLogMax / 16, "too many criterion checks (num_iter / update)");
+    std::vector<Vol> vols(nvol);
+    int64_t total = 0, nblocks = 0;
+    for (int v = 0; v < nvol; ++v) {
+        MH_REQUIRE(ctx, mh::dims_ok(vdims + 3 * v), "bad volume dims");
+        const int64_t n = vdims[3 * v] * vdims[3 * v + 1] * vdims[3 * v + 2];
+        MH_REQUIRE(ctx, voff[v] >= 0, "negative volume offset");
+        vols[v] = Vol{int(vdims[3 * v]), int(vdims[3 * v + 1]), int(vdims[3 * v + 2]), int(nblocks), voff[v]};
+        nblocks += mh::blocks_for(n);
+        total = std::max(total, voff[v] + n);
+    }
+    // volumes must not overlap
+    {
+        std::vector<std::pair<int64_t, int64_t>> iv(nvol);
+        for (int v = 0; v < nvol; ++v)
+            iv[v] = {voff[v], voff[v] + vdims[3 * v] * vdims[3 * v + 1] * vdims[3 * v + 2]};
+        std::sort(iv.begin(), iv.end());
+        for (int v = 1; v < nvol; ++v) MH_REQUIRE(ctx, iv[v].first >= iv[v - 1].second, "volumes overlap");
+    }
+    MH_REQUIRE(ctx, nblocks < (int64_t(1) << 31), "too many blocks");
+    MH_REQUIRE(ctx, work_bytes >= ragged_layout(total, nvol, nblocks, kCritLogMax / 16, nullptr, nullptr),
+               "workspace too small");
+    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(work) & 255) == 0, "workspace must be 256-byte aligned");
+    RaggedWorkspace W;
+    ragged_layout(total, nvol, nblocks, maxcheck, static_cast<char *>(work), &W);
+    hipStream_t s = (hipStream_t)stream;
+    MH_CHECK(ctx, hipMemcpyAsync(W.vols, vols.data(), sizeof(Vol) * nvol, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(zero_ints, dim3((3 * nvol + 255) / 256), dim3(256), 0, s, W.stop, 3 * nvol);
+    hipLaunchKernelGGL(ragged_init_kernel, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols, nvol,
+                       W.q4, eigs, W.hp, W.pt, W.tmp, u, cs, ct, prm->rcc);
+    MH_CHECK_LAUNCH(ctx, "ragged init");
+    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
+    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound, 0.f};
+    const BatchCtrl C{W.stop, W.ncrit, W.count, W.crit_log, maxcheck};
+    for (int it = 0; it < prm->num_iter; ++it) {
+        hipLaunchKernelGGL(ragged_p_kernel, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols, nvol,
+                           W.hp, W.tmp, W.q4, alpha, PP, W.stop);
+        if ((it % prm->update) == 0)
+            hipLaunchKernelGGL(ragged_u_kernel<true>, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols,
+                               nvol, W.hp, W.tmp, cs, ct, W.pt, u, UP, C, W.partials);
+        else
+            hipLaunchKernelGGL(ragged_u_kernel<false>, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols,
+                               nvol, W.hp, W.tmp, cs, ct, W.pt, u, UP, C, W.partials);
+        MH_CHECK_LAUNCH(ctx, "ragged cmf kernels");
+    }
+    std::vector<int> st(2 * nvol);
+    MH_CHECK(ctx, hipMemcpyAsync(st.data(), W.stop, sizeof(int) * 2 * nvol, hipMemcpyDeviceToHost, s));
+    std::vector<float> log(int64_t(nvol) * maxcheck);
+    MH_CHECK(ctx, hipMemcpyAsync(log.data(), W.crit_log, sizeof(float) * log.size(), hipMemcpyDeviceToHost, s));
+    MH_CHECK(ctx, hipStreamSynchronize(s));
+    int any_nan = 0;
+    for (int v = 0; v < nvol; ++v) {
+        const int sv = st[v], nc = std::min(st[nvol + v], maxcheck);
+        any_nan |= sv == 1;
+        if (res) {
+            res[v].stopped = sv;
+            res[v].ncrit = nc;
+            res[v].iterations = sv ? (st[nvol + v] - 1) * prm->update + 1 : prm->num_iter;
+            res[v].last_crit = nc > 0 ? log[int64_t(v) * maxcheck + nc - 1] : 0.f;
+        }
+    }
+    return any_nan ? MH_STOPPED_NAN : MH_OK;
+}

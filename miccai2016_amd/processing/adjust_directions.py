@@ -1,0 +1,237 @@
+"""Consistent signs for the vesselness main directions, and frame completion.
+
+Drop-in for the reference's processing/adjust_directions.py: ridge voxels are
+detected in the response (GPU), hysteresis-thresholded (host C++), their main
+directions aligned along a minimum spanning tree (host, scipy), the ridge
+optionally cleaned, and every voxel's direction flipped to agree with its
+nearest ridge voxel (GPU). ``TransformationCompleter`` turns directions back
+into squeezed frames (GPU).
+"""
+import numpy as np
+from scipy.ndimage import label
+from scipy.sparse.csgraph import minimum_spanning_tree
+from scipy.spatial.distance import pdist, squareform
+
+from .. import _lib
+from ..device import default_context
+from ..helpers.parameters import TreePrintable
+from .ridge_detection import RawRidgeDetector, hysteresis_threshold
+
+
+class AdvancedAdjuster(TreePrintable):
+    """See the module docstring. threshold: scalar, or (lower, upper) for
+    hysteresis; qeigs: squeezed frames (D0, D1, D2, 3); response: vesselness."""
+
+    def __init__(self, threshold, qeigs, response, voxel_size=1, ridge_detection_only=False,
+                 clean_ridge=True, loader=None):
+        self.__threshold = None
+        self.__qeigs = None
+        self.__response = None
+        self.__voxel_size = None
+        self.__ridge = None
+        self.__main_dirs_unadjusted = None
+        self._ridge_detector = None
+        self._hysteresis = None
+        self._loader = loader if loader is not None else default_context()
+        if threshold is not None:
+            self.threshold = threshold
+        if qeigs is not None:
+            self.qeigs = qeigs
+        if response is not None:
+            self.response = response
+        if voxel_size is not None:
+            self.voxel_size = voxel_size
+        self.ridge_detection_only = ridge_detection_only
+        self.clean_ridge = clean_ridge
+
+    @property
+    def threshold(self):
+        return self.__threshold
+
+    @threshold.setter
+    def threshold(self, value):
+        try:
+            len(value)
+            value = tuple(sorted(value))
+            self._hysteresis = True
+            lower = value[0]
+        except TypeError:
+            self._hysteresis = False
+            lower = value
+        if value != self.__threshold:
+            self.__threshold = value
+            self._ridge_detector = RawRidgeDetector(self._loader, lower)
+
+    @property
+    def qeigs(self):
+        return self.__qeigs
+
+    @qeigs.setter
+    def qeigs(self, value):
+        self.__qeigs = np.require(value, dtype=np.float32, requirements=["C", "A"])
+
+    @property
+    def response(self):
+        return self.__response
+
+    @response.setter
+    def response(self, value):
+        self.__response = np.require(value, dtype=np.float32, requirements=["C", "A"])
+
+    @property
+    def voxel_size(self):
+        return self.__voxel_size
+
+    @voxel_size.setter
+    def voxel_size(self, value):
+        self.__voxel_size = tuple(np.multiply(np.ones(3), value))
+
+    @property
+    def ridge(self):
+        """The detected (and, if requested, cleaned) ridge after execute()."""
+        return self.__ridge
+
+    @property
+    def main_dirs_unadjusted(self):
+        """The extracted main directions before any sign adjustment."""
+        return self.__main_dirs_unadjusted
+
+    def execute(self, verbose=True):
+        """Adjusted directions (D0, D1, D2, 3), or None if ridge_detection_only."""
+        assert self.response.shape == self.qeigs.shape[:3] and self.qeigs.shape[3] == 3
+        dev = self._loader
+        self._verbose = verbose
+        shape = self.response.shape
+        self._d_qeigs = dev.upload(self.qeigs)
+        self._d_resp = dev.upload(self.response)
+        labeled = self._label_ridge()
+        adjusted = None
+        if (not self.ridge_detection_only) or self.clean_ridge:
+            main_dirs = self._qeigs_to_main_dirs()
+            coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
+            if self.clean_ridge:
+                labeled = self._do_clean_ridge(labeled, coords, rdirs)
+                if not self.ridge_detection_only:
+                    coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
+            if not self.ridge_detection_only:
+                adjusted = self._adjust_all_directions(coords, rdirs, shape)
+        return adjusted
+
+    def _label_ridge(self):
+        dev = self._loader
+        shape = self.response.shape
+        d_ridge = dev.empty(int(np.prod(shape)))
+        self._ridge_detector.execute(d_ridge, self._d_resp, self._d_qeigs, shape)
+        ridge = dev.download(d_ridge).reshape(shape)
+        if self._hysteresis:
+            sure = np.logical_and(ridge, self.response >= self.threshold[1])
+            ridge[:] = hysteresis_threshold(ridge, sure.astype(np.float32))
+        self.__ridge = ridge
+        return ridge
+
+    def _qeigs_to_main_dirs(self):
+        """Column 0 of every frame; also kept on the device for the final pass."""
+        dev = self._loader
+        n = self.response.size
+        rc = dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(self._d_qeigs), n,
+                                  dev.stream)
+        dev.check(rc, "mh_main_dirs")
+        main_dirs = dev.download(self._d_qeigs).reshape(self.qeigs.shape)
+        self.__main_dirs_unadjusted = main_dirs.copy()
+        return main_dirs
+
+    def _adjust_ridge_directions(self, labeled_ridge, main_dirs):
+        """MST over the ridge voxels (scaled Euclidean distances); signs
+        propagated from ridge voxel 0 so that every tree edge has a
+        non-negative dot product. Returns (uint16 coords R x 3, float32 dirs)."""
+        coords = np.argwhere(labeled_ridge != 0)
+        n = len(coords)
+        adjusted = np.full((n, 3), np.nan, dtype=np.float32)
+        if n == 0:
+            return coords.astype(np.uint16), adjusted
+        dirs = main_dirs[tuple(coords.T)]
+        adjusted[0] = dirs[0]
+        if n > 1:
+            mst = minimum_spanning_tree(squareform(pdist(np.multiply(coords, self.voxel_size),
+                                                         "euclidean")), overwrite=True)
+            adj = (mst + mst.T).tocsr()
+            stack = [0]
+            while stack:
+                i = stack.pop()
+                ref = adjusted[i]
+                for j in adj.indices[adj.indptr[i]:adj.indptr[i + 1]]:
+                    if np.isnan(adjusted[j, 0]):
+                        d = dirs[j]
+                        adjusted[j] = d if np.dot(d, ref) >= 0 else -d
+                        stack.append(j)
+        return np.require(coords, np.uint16, ["C"]), adjusted
+
+    def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):
+        """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed
+        (voxel-size scaled) directions are longest."""
+        comps, ncomp = label(labeled_ridge, structure=np.ones((3, 3, 3), dtype=int))
+        if ncomp == 0:
+            self.__ridge = labeled_ridge
+            return labeled_ridge
+        scaled = ridge_dirs * np.asarray(self.voxel_size)
+        lab_of = comps[tuple(np.asarray(ridge_coords, dtype=np.intp).T)]
+        sums = np.zeros((ncomp + 1, 3))
+        for k in range(3):
+            sums[:, k] = np.bincount(lab_of, weights=scaled[:, k], minlength=ncomp + 1)
+        lengths = np.linalg.norm(sums[1:], axis=1)
+        order = np.argsort(-lengths, kind="stable") + 1
+        keep = int(np.ceil(3 * np.log(np.e + ncomp)))
+        if getattr(self, "_verbose", False):
+            print("Keeping %s of %s labels." % (min(keep, ncomp), ncomp))
+        cleaned = np.isin(comps, order[:keep]).astype(labeled_ridge.dtype)
+        self.__ridge = cleaned
+        return cleaned
+
+    def _adjust_all_directions(self, ridge_coords, ridge_dirs, shape):
+        dev = self._loader
+        n_r = len(ridge_coords)
+        if n_r > 0:
+            # uint16 coordinates travel as their int16 bit pattern (torch has no uint16 copy)
+            raw = np.ascontiguousarray(ridge_coords, dtype=np.uint16).view(np.int16)
+            d_coords = dev.torch.from_numpy(raw).to(dev.device)
+            d_rdirs = dev.upload(ridge_dirs)
+            scaling = _lib.F32x3(*[float(v) for v in self.voxel_size])
+            rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(self._d_qeigs), _lib.dims3(shape),
+                                           dev.ptr(d_coords), dev.ptr(d_rdirs), n_r, scaling,
+                                           dev.stream)
+            dev.check(rc, "mh_nonridge_signs")
+        return dev.download(self._d_qeigs).reshape(self.qeigs.shape)
+
+
+
+class TransformationCompleter(TreePrintable):
+    """Directions -> orthonormal frames (det +1) -> squeezed quaternions."""
+
+    def __init__(self, dirs, loader=None):
+        self.__dirs = None
+        self._loader = loader if loader is not None else default_context()
+        if dirs is not None:
+            self.dirs = dirs
+
+    @property
+    def dirs(self):
+        return self.__dirs
+
+    @dirs.setter
+    def dirs(self, value):
+        self.__dirs = np.require(value, dtype=np.float32, requirements=["C", "A"])
+
+    def complete(self):
+        """Squeezed frames with the same shape as dirs."""
+        dev = self._loader
+        d = dev.upload(self.dirs)
+        out = self.complete_device(d)
+        return dev.download(out).reshape(self.dirs.shape)
+
+    def complete_device(self, d_dirs, out=None):
+        dev = self._loader
+        out = d_dirs if out is None else out
+        rc = dev.lib.mh_complete_frames(dev.handle, dev.ptr(out), dev.ptr(d_dirs),
+                                        d_dirs.numel() // 3, dev.stream)
+        dev.check(rc, "mh_complete_frames")
+        return out

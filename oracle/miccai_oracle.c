@@ -1,0 +1,910 @@
+/*
+ * CPU oracle for the tubular-segmentation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is a plain-C restatement of the
+ * reference algorithm, used by tests/ as the parity checker and by bench.py as
+ * the timed CPU baseline. Nothing in the product (miccai2016_amd/) links,
+ * loads or calls it.
+ *
+ * Conventions followed (reference = spezold/miccai2016, src/):
+ *  - Arithmetic is FP32 except where the reference's OpenCL C source promotes
+ *    to double: unsuffixed literals and the -D macros (RCC, SS*, RS*, SQEVAL*,
+ *    MU, DT, RSS*, THRESHOLD) are double there, so the same expressions are
+ *    written with double operands here; C's usual arithmetic conversions then
+ *    reproduce the OpenCL promotion on a double-capable device.
+ *  - Build with -ffp-contract=off: every operation rounds on its own.
+ *  - Volumes are C-order (D0, D1, D2), axis 2 fastest; index
+ *    i = i2 + D2 * (i1 + D1 * i0), as in clcode-anisotropic/update_qp.cl:168.
+ *  - Entry points take plain pointers and int64 sizes (ctypes-friendly).
+ *
+ * Parity status of each function is stated in its header comment and checked
+ * by tests/test_oracle_*.py against closed forms, library routines (scipy,
+ * numpy) and invariants.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API __attribute__((visibility("default")))
+
+static inline int64_t lin3(int64_t i0, int64_t i1, int64_t i2, int64_t d1, int64_t d2)
+{
+    return i2 + d2 * (i1 + d1 * i0);
+}
+
+ORC_API int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+ORC_API void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* Quaternion helpers (quaternion_math.h:20-147, 257-340).                    */
+/* Scalar part first unless the name says "last".                             */
+/* ------------------------------------------------------------------------- */
+
+/* qexpand, quaternion_math.h:79-90. The "* 0.5" literal is double; the
+ * cos/sin of the theta half-angle therefore take a double argument (double
+ * overload), while the phi/psi half-angles are first stored to float and use
+ * the float overload. */
+static void q_expand(const float v[3], float q[4])
+{
+    const float pp = (float)((v[0] + v[2]) * 0.5);
+    const float pm = (float)((v[0] - v[2]) * 0.5);
+    const float ct = (float)cos(v[1] * 0.5);
+    const float st = (float)sin(v[1] * 0.5);
+    q[0] = cosf(pp) * ct;
+    q[1] = cosf(pm) * st;
+    q[2] = sinf(pm) * st;
+    q[3] = sinf(pp) * ct;
+}
+
+/* qrot, quaternion_math.h:114-147: v' = q (0, v) q^-1 in two Hamilton
+ * products. */
+static void q_rot(const float q[4], const float v[3], float o[3])
+{
+    float t0 = -(q[1] * v[0] + q[2] * v[1] + q[3] * v[2]);
+    float t1 = q[0] * v[0] + q[2] * v[2] - q[3] * v[1];
+    float t2 = q[0] * v[1] - q[1] * v[2] + q[3] * v[0];
+    float t3 = q[0] * v[2] + q[1] * v[1] - q[2] * v[0];
+    float o0 = -t0 * q[1] + t1 * q[0] - t2 * q[3] + t3 * q[2];
+    float o1 = -t0 * q[2] + t1 * q[3] + t2 * q[0] - t3 * q[1];
+    float o2 = -t0 * q[3] - t1 * q[2] + t2 * q[1] + t3 * q[0];
+    o[0] = o0; o[1] = o1; o[2] = o2;
+}
+
+static float safe_atan2f(float y, float x)
+{
+    return (y == 0.0f && x == 0.0f) ? 0.0f : atan2f(y, x);
+}
+
+static float clampf(float x, float lo, float hi)
+{
+    return fminf(fmaxf(x, lo), hi);
+}
+
+/* z-x-z Euler squeeze of a unit quaternion (quaternion_math.h:20-39 for
+ * scalar-first, :45-69 for scalar-last). w = scalar, (x, y, z) = vector. */
+static void q_squeeze_parts(float w, float x, float y, float z, float out[3])
+{
+    const float q1spq2s = x * x + y * y;
+    const int cond1 = q1spq2s < FLT_EPSILON;
+    const int cond2 = cond1 || (1 - q1spq2s) < FLT_EPSILON;
+    const float q1q3 = x * z, q2q0 = y * w, q1q0 = x * w, q2q3 = y * z;
+
+    const float phi = safe_atan2f(q1q3 + q2q0, q1q0 - q2q3);
+    const float theta = (float)((cond1 ? 0 : M_PI) * cond2 +
+                                acosf(clampf(1 - 2 * q1spq2s, -1.0f, 1.0f)) * (double)(!cond2));
+    float at2 = (x * y - w * z) * cond2 + (q1q3 - q2q0) * (!cond2);
+    /* the second argument is double in the reference ((.5 - ...) literal) and
+     * cast to float by safeatan2 */
+    at2 = safe_atan2f(at2, (float)((.5 - (y * y + z * z)) * cond2 + (q1q0 + q2q3) * (!cond2)));
+    const float psi = (-at2 + phi * (cond1 ? -1 : 1)) * cond2 + at2 * (!cond2);
+    out[0] = phi; out[1] = theta; out[2] = psi;
+}
+
+ORC_API void orc_qexpand(const float *v, float *q, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k) q_expand(v + 3 * k, q + 4 * k);
+}
+
+ORC_API void orc_qsqueeze(const float *q, float *v, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k)
+        q_squeeze_parts(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3], v + 3 * k);
+}
+
+ORC_API void orc_qrot(const float *q, const float *v, float *o, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k) q_rot(q + 4 * k, v + 3 * k, o + 3 * k);
+}
+
+/* quat2matFloat3 (quaternion_math.h:257-289): columns of R(q); m is row-major
+ * 3x3 (m[r*3+c]). */
+static void q_to_mat(const float q[4], float m[9])
+{
+    const float qx2 = q[1] * q[1], qy2 = q[2] * q[2], qz2 = q[3] * q[3];
+    const float wx = q[0] * q[1], wy = q[0] * q[2], wz = q[0] * q[3];
+    const float xy = q[1] * q[2], xz = q[1] * q[3], yz = q[2] * q[3];
+    m[0] = 1 - 2 * (qy2 + qz2); m[1] = 2 * (xy - wz);       m[2] = 2 * (xz + wy);
+    m[3] = 2 * (xy + wz);       m[4] = 1 - 2 * (qx2 + qz2); m[5] = 2 * (yz - wx);
+    m[6] = 2 * (xz - wy);       m[7] = 2 * (yz + wx);       m[8] = 1 - 2 * (qx2 + qy2);
+}
+
+ORC_API void orc_quat2mat(const float *q, float *m, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k) q_to_mat(q + 4 * k, m + 9 * k);
+}
+
+/* mat2quat, quaternion_math.h:301-340 (four-branch stable form). The rsqrt
+ * is a float function fed a double argument (s + 1.0), so the argument is
+ * rounded to float first; "* 0.5" and "0.25 / s" are double. */
+static float rsqrt_of_double(double x)
+{
+    return 1.0f / sqrtf((float)x);
+}
+
+static void m_to_quat(const float m[9], float q[4])
+{
+#define M(r, c) m[(r) * 3 + (c)]
+    float s = M(0, 0) + M(1, 1) + M(2, 2);
+    if (s > 0) {
+        s = (float)(rsqrt_of_double(s + 1.0) * 0.5);
+        q[0] = (float)(0.25 / s);
+        q[1] = (M(2, 1) - M(1, 2)) * s;
+        q[2] = (M(0, 2) - M(2, 0)) * s;
+        q[3] = (M(1, 0) - M(0, 1)) * s;
+    } else if (M(0, 0) > M(1, 1) && M(0, 0) > M(2, 2)) {
+        s = (float)(rsqrt_of_double(1.0 + M(0, 0) - M(1, 1) - M(2, 2)) * 0.5);
+        q[0] = (M(2, 1) - M(1, 2)) * s;
+        q[1] = (float)(0.25 / s);
+        q[2] = (M(0, 1) + M(1, 0)) * s;
+        q[3] = (M(0, 2) + M(2, 0)) * s;
+    } else if (M(1, 1) > M(2, 2)) {
+        s = (float)(rsqrt_of_double(1.0 + M(1, 1) - M(0, 0) - M(2, 2)) * 0.5);
+        q[0] = (M(0, 2) - M(2, 0)) * s;
+        q[1] = (M(0, 1) + M(1, 0)) * s;
+        q[2] = (float)(0.25 / s);
+        q[3] = (M(1, 2) + M(2, 1)) * s;
+    } else {
+        s = (float)(rsqrt_of_double(1.0 + M(2, 2) - M(0, 0) - M(1, 1)) * 0.5);
+        q[0] = (M(1, 0) - M(0, 1)) * s;
+        q[1] = (M(0, 2) + M(2, 0)) * s;
+        q[2] = (M(1, 2) + M(2, 1)) * s;
+        q[3] = (float)(0.25 / s);
+    }
+#undef M
+}
+
+ORC_API void orc_mat2quat(const float *m, float *q, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k) m_to_quat(m + 9 * k, q + 4 * k);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gaussian derivative taps and separable convolution                         */
+/* (qfrangi_gpu.py:158-185, convolve_naive.cl:25-166).                        */
+/* ------------------------------------------------------------------------- */
+
+/* Returns the arm length R; writes 2R+1 taps (float) if out != NULL.
+ * round_up != 0: R = ceil(n_sigmas*sigma); else R = int(n_sigmas*sigma+0.5)
+ * (qfrangi_gpu.py:178-179). Taps are computed in double and rounded once. */
+ORC_API int orc_gauss_taps(double sigma, int order, double scale, double n_sigmas,
+                           int round_up, float *out)
+{
+    const int R = round_up ? (int)ceil(n_sigmas * sigma) : (int)(n_sigmas * sigma + 0.5);
+    if (!out) return R;
+    const double s2 = sigma * sigma;
+    double *g = (double *)malloc(sizeof(double) * (2 * R + 1));
+    double sum = 0.0;
+    for (int k = 0; k <= 2 * R; ++k) {
+        const double x = (double)(k - R);
+        g[k] = exp(-.5 * (x * x) / s2);
+        sum += g[k];
+    }
+    for (int k = 0; k <= 2 * R; ++k) {
+        const double x = (double)(k - R);
+        const double gn = g[k] / sum;
+        double v;
+        switch (order) {
+            case 0: v = gn; break;
+            case 1: v = gn * (-x) / s2; break;
+            case 2: v = gn * (x * x - s2) / (s2 * s2); break;
+            default: v = gn * x * (3 * s2 - x * x) / (s2 * s2 * s2); break;
+        }
+        out[k] = (float)(scale * v);
+    }
+    free(g);
+    return R;
+}
+
+/* One separable pass along `axis` with mirror boundary of period 2(n-1),
+ * taps applied flipped (true convolution), FP32 accumulation from -R to +R
+ * (convolve_naive.cl:25-63 and its two siblings). Requires dims[axis] >= 2. */
+ORC_API void orc_convolve_axis(float *out, const float *in, const int64_t *dims, int axis,
+                               const float *taps, int radius)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t n = dims[axis];
+    const int64_t period = 2 * (n - 1);
+    const int K = 2 * radius + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const int64_t ia = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
+                int64_t tile = (ia - radius - 1) % period;
+                tile += tile < 0 ? period : 0;
+                float acc = 0;
+                for (int j = 0; j < K; ++j) {
+                    tile++;
+                    tile = tile == period ? 0 : tile;
+                    const int64_t m = tile < n ? tile : period - tile;
+                    int64_t src;
+                    if (axis == 0) src = lin3(m, i1, i2, d1, d2);
+                    else if (axis == 1) src = lin3(i0, m, i2, d1, d2);
+                    else src = lin3(i0, i1, m, d1, d2);
+                    acc += taps[K - j - 1] * in[src];
+                }
+                out[lin3(i0, i1, i2, d1, d2)] = acc;
+            }
+}
+
+/* Hessian at one scale (qfrangi_gpu.py:208-271 with GaussianFilter3D.execute
+ * :91-155): for each of the 6 unique entries, filter along axes 0, 1, 2 with
+ * derivative orders e_i + e_j, taps scaled by sigma^order (gamma = 1).
+ * Output AoS x6 [H00, H11, H22, H01, H02, H12] (qfrangi_gpu.py:247, 225-226). */
+ORC_API int orc_hessian(const float *img, const int64_t *dims, const double *sigma3,
+                        double n_sigmas, int round_up, float *out6)
+{
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    float *a = (float *)malloc(sizeof(float) * n);
+    float *b = (float *)malloc(sizeof(float) * n);
+    if (!a || !b) { free(a); free(b); return -1; }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j <= i; ++j) {
+            int order[3] = {0, 0, 0};
+            order[i] = 1;
+            order[j] += 1;
+            const int comp = (i == j) ? i : 2 + i + j;
+            const float *src = img;
+            float *dst = a;
+            for (int ax = 0; ax < 3; ++ax) {
+                const double s = sigma3[ax];
+                const double scale = pow(s, (double)order[ax]);
+                const int R = orc_gauss_taps(s, order[ax], scale, n_sigmas, round_up, NULL);
+                float *taps = (float *)malloc(sizeof(float) * (2 * R + 1));
+                orc_gauss_taps(s, order[ax], scale, n_sigmas, round_up, taps);
+                orc_convolve_axis(dst, src, dims, ax, taps, R);
+                free(taps);
+                src = dst;
+                dst = (dst == a) ? b : a;
+            }
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < n; ++k) out6[6 * k + comp] = src[k];
+        }
+    free(a);
+    free(b);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 3x3 symmetric Jacobi eigensolver (qeigs.cl:146-256, eigcmp redefined to     */
+/* ascending |lambda| by qfrangi_gpu.py:300-302).                             */
+/* ------------------------------------------------------------------------- */
+
+/* Rotation that re-orders the rows of the eigenvector matrix: the signed
+ * permutation whose row k is e_order[k], negated as a whole for odd
+ * permutations so that det = +1. Stored scalar-last and indexed by
+ * (order[0] << 1) | (order[1] > order[2]), as qeigs.cl:89 indexes its table.
+ * Built once from the permutation itself (see sorter_init). */
+static float g_sorter[6][4];
+static int g_sorter_ready = 0;
+
+static void sorter_init(void)
+{
+    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int p = 0; p < 6; ++p) {
+        const int *o = perms[p];
+        /* parity: number of inversions */
+        int inv = (o[0] > o[1]) + (o[0] > o[2]) + (o[1] > o[2]);
+        double sgn = (inv & 1) ? -1.0 : 1.0;
+        double m[3][3] = {{0}};
+        for (int r = 0; r < 3; ++r) m[r][o[r]] = sgn;
+        /* rotation matrix -> quaternion, in double, largest-component branch */
+        double w, x, y, z;
+        double tr = m[0][0] + m[1][1] + m[2][2];
+        if (tr > 0) {
+            double s = sqrt(tr + 1.0) * 2;
+            w = 0.25 * s; x = (m[2][1] - m[1][2]) / s; y = (m[0][2] - m[2][0]) / s; z = (m[1][0] - m[0][1]) / s;
+        } else if (m[0][0] > m[1][1] && m[0][0] > m[2][2]) {
+            double s = sqrt(1.0 + m[0][0] - m[1][1] - m[2][2]) * 2;
+            w = (m[2][1] - m[1][2]) / s; x = 0.25 * s; y = (m[0][1] + m[1][0]) / s; z = (m[0][2] + m[2][0]) / s;
+        } else if (m[1][1] > m[2][2]) {
+            double s = sqrt(1.0 + m[1][1] - m[0][0] - m[2][2]) * 2;
+            w = (m[0][2] - m[2][0]) / s; x = (m[0][1] + m[1][0]) / s; y = 0.25 * s; z = (m[1][2] + m[2][1]) / s;
+        } else {
+            double s = sqrt(1.0 + m[2][2] - m[0][0] - m[1][1]) * 2;
+            w = (m[1][0] - m[0][1]) / s; x = (m[0][2] + m[2][0]) / s; y = (m[1][2] + m[2][1]) / s; z = 0.25 * s;
+        }
+        const int idx = (o[0] << 1) | (o[1] > o[2]);
+        g_sorter[idx][0] = (float)x; g_sorter[idx][1] = (float)y;
+        g_sorter[idx][2] = (float)z; g_sorter[idx][3] = (float)w;
+    }
+    g_sorter_ready = 1;
+}
+
+ORC_API void orc_sorter_table(float *out24)
+{
+    if (!g_sorter_ready) sorter_init();
+    memcpy(out24, g_sorter, sizeof(g_sorter));
+}
+
+static inline int eig_before(float a, float b) { return fabsf(a) < fabsf(b); }
+
+/* Compare-exchange network (0,1), (1,2), (0,1) on ascending |lambda|, then
+ * left-multiply the (scalar-last) quaternion by the matching sorter
+ * (qeigs.cl:60-92, cqmult :35-41). */
+static void sort_eigs(float q[4], float v[3])
+{
+    int order[3];
+    const float iq[4] = {q[0], q[1], q[2], q[3]};
+    int c, ot;
+    float vt;
+    c = eig_before(v[0], v[1]);
+    vt = v[0];
+    v[0] = c ? v[0] : v[1]; order[0] = c ? 0 : 1;
+    v[1] = c ? v[1] : vt;   order[1] = c ? 1 : 0;
+    c = eig_before(v[1], v[2]);
+    vt = v[1]; ot = order[1];
+    v[1] = c ? v[1] : v[2]; order[1] = c ? order[1] : 2;
+    v[2] = c ? v[2] : vt;   order[2] = c ? 2 : ot;
+    c = eig_before(v[0], v[1]);
+    vt = v[0]; ot = order[0];
+    v[0] = c ? v[0] : v[1]; order[0] = c ? order[0] : order[1];
+    v[1] = c ? v[1] : vt;   order[1] = c ? order[1] : ot;
+
+    const float *L = g_sorter[(order[0] << 1) | (order[1] > order[2])];
+    q[3] = L[3] * iq[3] - (L[0] * iq[0] + L[1] * iq[1] + L[2] * iq[2]);
+    q[0] = L[3] * iq[0] + L[0] * iq[3] + L[1] * iq[2] - L[2] * iq[1];
+    q[1] = L[3] * iq[1] - L[0] * iq[2] + L[1] * iq[3] + L[2] * iq[0];
+    q[2] = L[3] * iq[2] + L[0] * iq[1] - L[1] * iq[0] + L[2] * iq[3];
+}
+
+/* One matrix: in6 = [H00,H11,H22,H01,H02,H12]; out6 = [l1,l2,l3,phi,theta,psi]
+ * with |l1| <= |l2| <= |l3|. Cyclic Jacobi with the approximate Givens
+ * rotation of McAdams et al. (tech. report 1690, 2011, algorithm 2):
+ * gamma = 3 + 2 sqrt 2, fallback (cos, sin)(pi/8). Convergence is tested
+ * before each sweep: sum |offdiag| <= max |input| * FLT_EPSILON. NaN output if
+ * not converged after maxiter sweeps. */
+static void eig_one(const float *in6, float *out6, int maxiter)
+{
+    const double GAMMA_ = 3.0 + 2.0 * sqrt(2.0);
+    const double CSTAR = cos(M_PI / 8.0), SSTAR = sin(M_PI / 8.0);
+    float on[3] = {in6[0], in6[1], in6[2]};
+    float off[3] = {in6[5], in6[4], in6[3]}; /* off[r] holds [p][q], r = 3 ^ (p|q) */
+    float q[4] = {0.f, 0.f, 0.f, 1.f};       /* scalar last */
+    float t1 = fabsf(on[0]), t2;
+    t2 = fabsf(on[1]);  t1 = t1 < t2 ? t2 : t1;
+    t2 = fabsf(on[2]);  t1 = t1 < t2 ? t2 : t1;
+    t2 = fabsf(off[0]); t1 = t1 < t2 ? t2 : t1;
+    t2 = fabsf(off[1]); t1 = t1 < t2 ? t2 : t1;
+    t2 = fabsf(off[2]); t1 = t1 < t2 ? t2 : t1;
+    const float tol = t1 * FLT_EPSILON;
+
+    for (int it = 0; it < maxiter; ++it) {
+        if ((fabsf(off[0]) + fabsf(off[1]) + fabsf(off[2])) <= tol) {
+            sort_eigs(q, on);
+            q[3] = -q[3];
+            float sq[3];
+            q_squeeze_parts(q[3], q[0], q[1], q[2], sq);
+            out6[0] = on[0]; out6[1] = on[1]; out6[2] = on[2];
+            out6[3] = sq[0]; out6[4] = sq[1]; out6[5] = sq[2];
+            return;
+        }
+        for (int p = 0; p < 2; ++p)
+            for (int qq = p + 1; qq < 3; ++qq) {
+                const int r = 3 ^ (p | qq);
+                float c = (float)(2. * (on[qq] - on[p]));
+                float s = off[r];
+                float a = s * s, b = c * c;
+                int cond = (GAMMA_ * a < b);
+                a = 1.0f / sqrtf(a + b);
+                c = cond ? a * c : (float)CSTAR;
+                s = cond ? a * s : (float)SSTAR;
+                a = c * q[p] - s * q[qq];
+                b = c * q[qq] + s * q[p];
+                q[p] = a; q[qq] = b;
+                cond = (r == 1);
+                a = c * q[3] + (cond ? s : -s) * q[r];
+                b = c * q[r] + (cond ? -s : s) * q[3];
+                q[3] = a; q[r] = b;
+                a = 1 - 2 * (s * s);
+                s = 2 * c * s;
+                c = a;
+                a = off[r] * c + on[p] * s;
+                b = on[qq] * c + off[r] * s;
+                on[p] = c * (on[p] * c - off[r] * s) - s * (off[r] * c - on[qq] * s);
+                on[qq] = c * b + s * a;
+                off[r] = c * a - s * b;
+                b = off[qq];
+                off[qq] = c * b - s * off[p];
+                off[p] = c * off[p] + s * b;
+            }
+    }
+    for (int k = 0; k < 6; ++k) out6[k] = NAN;
+}
+
+ORC_API void orc_eigs(const float *in6, float *out6, int64_t n, int maxiter)
+{
+    if (!g_sorter_ready) sorter_init();
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        float tmp[6];
+        eig_one(in6 + 6 * k, tmp, maxiter);
+        memcpy(out6 + 6 * k, tmp, sizeof(tmp));
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Vesselness (frangi.cl:46-115, max.cl:25-58 + host np.max).                 */
+/* ------------------------------------------------------------------------- */
+
+/* c = 0.5 * max_k ||lambda_k||_2, NaN neutral (fmax); NaN if all NaN. */
+ORC_API float orc_half_max_norm(const float *eig6, int64_t n)
+{
+    float m = NAN;
+    for (int64_t k = 0; k < n; ++k) {
+        const float *e = eig6 + 6 * k;
+        const float nrm = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        m = fmaxf(m, nrm);
+    }
+    return (float)(0.5 * (double)m);
+}
+
+ORC_API void orc_update_vesselness(float *val, float *quat, const float *eig6, int64_t n,
+                                   float alpha, float beta, float c, int bright)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        const float *e = eig6 + 6 * k;
+        const float l1 = fabsf(e[0]);
+        float l2 = bright ? e[1] : -e[1];
+        float l3 = bright ? e[2] : -e[2];
+        const int valid = l2 < 0 && l3 < 0;
+        l2 = fabsf(l2);
+        l3 = fabsf(l3);
+        const float rB = l1 / (sqrtf(l2 * l3) + FLT_MIN);
+        const float rA = l2 / (l3 + FLT_MIN);
+        const float S2 = l1 * l1 + l2 * l2 + l3 * l3;
+        const float ra = rA / alpha, rb = rB / beta;
+        const float v = (float)valid * ((1.0f - expf(-0.5f * (ra * ra))) * expf(-0.5f * (rb * rb)) *
+                                        (1.0f - expf(-0.5f * S2 / (c * c))));
+        if (v > val[k]) {
+            val[k] = v;
+            quat[3 * k] = e[3];
+            quat[3 * k + 1] = e[4];
+            quat[3 * k + 2] = e[5];
+        }
+    }
+}
+
+/* Full multi-scale vesselness (qfrangi_gpu.py:511-631 with FeatureFilter
+ * round_up=False, n_sigmas=4): val starts at -inf; for each scale row
+ * sigma3[3*s..], Hessian -> eigs -> c (if c_fixed is NaN) -> update. */
+ORC_API int orc_vesselness(const float *img, const int64_t *dims, const double *sigma3,
+                           int nscales, float alpha, float beta, float c_fixed, int bright,
+                           float *val, float *quat)
+{
+    if (!g_sorter_ready) sorter_init();
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    float *h = (float *)malloc(sizeof(float) * 6 * n);
+    if (!h) return -1;
+    for (int64_t k = 0; k < n; ++k) val[k] = -INFINITY;
+    for (int s = 0; s < nscales; ++s) {
+        if (orc_hessian(img, dims, sigma3 + 3 * s, 4.0, 0, h)) { free(h); return -1; }
+        orc_eigs(h, h, n, 50);
+        float c = c_fixed;
+        if (isnan(c)) c = orc_half_max_norm(h, n);
+        orc_update_vesselness(val, quat, h, n, alpha, beta, c, bright);
+    }
+    free(h);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ridges and directions (detect_ridges.cl:34-102, adjust_directions.cl).     */
+/* ------------------------------------------------------------------------- */
+
+/* Ridge iff resp > threshold (double compare) and resp > max of 8 trilinear
+ * samples at pos + a*v2 + b*v3, (a,b) in {-1,0,1}^2 \ (0,0); corner outside
+ * the volume counts -1; origin by truncation toward zero. */
+ORC_API void orc_detect_ridges(float *out, const float *resp, const float *qeig3,
+                               const int64_t *dims, double threshold)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const int64_t i = lin3(i0, i1, i2, d1, d2);
+                float q[4], m[9];
+                q_expand(qeig3 + 3 * i, q);
+                q_to_mat(q, m);
+                const float c1[3] = {m[1], m[4], m[7]};
+                const float c2[3] = {m[2], m[5], m[8]};
+                const float pos[3] = {(float)i0, (float)i1, (float)i2};
+                float maxv = -INFINITY;
+                for (int j = 0; j < 8; ++j) {
+                    const float a = (float)((-1) * (j < 3) + (j > 4));
+                    const float b = (float)((j % 3) - 1 + (j == 4));
+                    float pj[3];
+                    int oj[3];
+                    for (int d = 0; d < 3; ++d) {
+                        pj[d] = pos[d] + a * c1[d] + b * c2[d];
+                        oj[d] = (int)pj[d]; /* truncation toward zero */
+                        pj[d] = pj[d] - (float)oj[d];
+                    }
+                    float vj = 0;
+                    for (int k = 0; k < 8; ++k) {
+                        const int cr[3] = {k >> 2, (k >> 1) & 1, k & 1};
+                        const int ca[3] = {oj[0] + cr[0], oj[1] + cr[1], oj[2] + cr[2]};
+                        float w[3];
+                        for (int d = 0; d < 3; ++d) w[d] = cr[d] == 1 ? pj[d] : 1 - pj[d];
+                        const int outside = ca[0] < 0 || ca[1] < 0 || ca[2] < 0 ||
+                                            ca[0] >= (int)d0 || ca[1] >= (int)d1 || ca[2] >= (int)d2;
+                        const float s = outside ? -1.0f : resp[lin3(ca[0], ca[1], ca[2], d1, d2)];
+                        vj += s * w[0] * w[1] * w[2];
+                    }
+                    maxv = vj > maxv ? vj : maxv;
+                }
+                out[i] = (float)(resp[i] > threshold && resp[i] > maxv);
+            }
+}
+
+/* Column 0 of R(qexpand(v)) (adjust_directions.cl:29-43). */
+ORC_API void orc_main_dirs(const float *qeig3, float *dirs, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        float q[4], m[9];
+        q_expand(qeig3 + 3 * k, q);
+        q_to_mat(q, m);
+        dirs[3 * k] = m[0];
+        dirs[3 * k + 1] = m[3];
+        dirs[3 * k + 2] = m[6];
+    }
+}
+
+/* Nearest ridge point by squared scaled distance (strict <, first index wins)
+ * then flip if dot < 0 (adjust_directions.cl:182-246). coords: uint16 R x 3. */
+ORC_API void orc_adjust_nonridge_signs(float *dirs, const int64_t *dims, const uint16_t *coords,
+                                       const float *rdirs, int64_t nr, const float *scaling)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const uint32_t p[3] = {(uint32_t)i0, (uint32_t)i1, (uint32_t)i2};
+                float best = INFINITY;
+                int64_t bj = 0;
+                for (int64_t j = 0; j < nr; ++j) {
+                    float df[3];
+                    for (int d = 0; d < 3; ++d) {
+                        const uint32_t c = coords[3 * j + d];
+                        const uint32_t ad = p[d] > c ? p[d] - c : c - p[d];
+                        df[d] = (float)ad * scaling[d];
+                    }
+                    const float dsq = df[0] * df[0] + df[1] * df[1] + df[2] * df[2];
+                    if (dsq < best) { best = dsq; bj = j; }
+                }
+                float *v = dirs + 3 * lin3(i0, i1, i2, d1, d2);
+                const float *r = rdirs + 3 * bj;
+                if (nr > 0 && (v[0] * r[0] + v[1] * r[1] + v[2] * r[2]) < 0) {
+                    v[0] = -v[0]; v[1] = -v[1]; v[2] = -v[2];
+                }
+            }
+}
+
+/* OpenCL normalize(): zero stays zero, otherwise v / |v|. */
+static void normalize3(const float v[3], float o[3])
+{
+    const float d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (d == 0.0f) { o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; return; }
+    const float r = 1.0f / sqrtf(d);
+    o[0] = v[0] * r; o[1] = v[1] * r; o[2] = v[2] * r;
+}
+
+/* v -> orthonormal frame (v1, v2, v1 x v2) -> quaternion -> squeeze
+ * (adjust_directions.cl:68-100). */
+ORC_API void orc_complete_frames(const float *dirs, float *out3, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        float e0[3], e1[3], e2[3], t[3];
+        normalize3(dirs + 3 * k, e0);
+        if (fabsf(e0[0]) > fabsf(e0[1])) { t[0] = -e0[2]; t[1] = 0.0f; t[2] = e0[0]; }
+        else { t[0] = 0.0f; t[1] = e0[2]; t[2] = -e0[1]; }
+        normalize3(t, e1);
+        e2[0] = e0[1] * e1[2] - e0[2] * e1[1];
+        e2[1] = e0[2] * e1[0] - e0[0] * e1[2];
+        e2[2] = e0[0] * e1[1] - e0[1] * e1[0];
+        const float m[9] = {e0[0], e1[0], e2[0], e0[1], e1[1], e2[1], e0[2], e1[2], e2[2]};
+        float q[4], s[3];
+        m_to_quat(m, q);
+        q_squeeze_parts(q[0], q[1], q[2], q[3], s);
+        out3[3 * k] = s[0]; out3[3 * k + 1] = s[1]; out3[3 * k + 2] = s[2];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reference-style partial reduction (cl_helpers.py:187-298, sum_abs.cl).     */
+/* Passes of 2*1024 elements per group with a pairwise tree in float, then a  */
+/* float32 sum of the remaining partials (np.sum of float32 is pairwise too;  */
+/* the remaining count is at most 1024 so a pairwise restatement is used).    */
+/* ------------------------------------------------------------------------- */
+/* numpy's float32 pairwise summation (blocks of 128, 8 partial sums). */
+static float pairwise_sum(const float *a, int64_t n)
+{
+    if (n < 8) {
+        float s = 0.f;
+        for (int64_t k = 0; k < n; ++k) s += a[k];
+        return s;
+    }
+    if (n <= 128) {
+        float r[8];
+        for (int j = 0; j < 8; ++j) r[j] = a[j];
+        int64_t k = 8;
+        for (; k < n - (n % 8); k += 8)
+            for (int j = 0; j < 8; ++j) r[j] += a[k + j];
+        float s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; k < n; ++k) s += a[k];
+        return s;
+    }
+    int64_t h = n / 2;
+    h -= h % 8;
+    return pairwise_sum(a, h) + pairwise_sum(a + h, n - h);
+}
+
+ORC_API float orc_sum_abs_reduce(const float *x, int64_t n)
+{
+    const int WG = 1024;
+    float *buf = (float *)malloc(sizeof(float) * (n + 1));
+    for (int64_t k = 0; k < n; ++k) buf[k] = fabsf(x[k]);
+    int64_t size = n;
+    float scratch[1024];
+    while (size > WG) {
+        const int64_t outn = (int64_t)ceil(ceil(size / 2.0) / WG);
+        const int64_t gsz = outn * WG;
+        float *nb = (float *)malloc(sizeof(float) * outn);
+        for (int64_t g = 0; g < outn; ++g) {
+            for (int l = 0; l < WG; ++l) {
+                const int64_t ig = g * WG + l;
+                const int64_t ip = ig + gsz;
+                scratch[l] = (ig < size ? buf[ig] : 0.0f) + (ip < size ? buf[ip] : 0.0f);
+            }
+            for (int s = WG >> 1; s > 0; s >>= 1)
+                for (int l = 0; l < s; ++l) scratch[l] += scratch[l + s];
+            nb[g] = scratch[0];
+        }
+        free(buf);
+        buf = nb;
+        size = outn;
+    }
+    const float r = pairwise_sum(buf, size);
+    free(buf);
+    return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gradient vector flow (gvf.cl:44-253, gradient_vector_flow.py:58-253).      */
+/* ------------------------------------------------------------------------- */
+
+/* r = |v|^2 (gvf.cl:246-253); v is SoA (3 fields of n). */
+ORC_API void orc_gvf_response(float *r, const float *v0, const float *v1, const float *v2, int64_t n)
+{
+    for (int64_t k = 0; k < n; ++k) r[k] = v0[k] * v0[k] + v1[k] * v1[k] + v2[k] * v2[k];
+}
+
+/* maxiter explicit Euler steps from g = 0: delta = r (g - v) - MU lap(g)
+ * (7-point, mirrored neighbour at edges), g -= DT delta. MU, DT, RSS are
+ * double as the reference's -D macros. If mag_out != NULL, mag_out[k] holds
+ * the mean |delta| at iterations i = k*update (the reference's check), using
+ * the reference-style float reduction. Requires every dim >= 2. */
+ORC_API void orc_gvf(float *g0, float *g1, float *g2, const float *v0, const float *v1,
+                     const float *v2, const float *r, const int64_t *dims, double mu, double dt,
+                     const double *rss, int maxiter, int update, float *mag_out)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t n = d0 * d1 * d2;
+    float *dl[3];
+    for (int c = 0; c < 3; ++c) dl[c] = (float *)malloc(sizeof(float) * n);
+    float *g[3] = {g0, g1, g2};
+    const float *v[3] = {v0, v1, v2};
+    for (int c = 0; c < 3; ++c) memset(g[c], 0, sizeof(float) * n);
+    for (int it = 0; it < maxiter; ++it) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int64_t i0 = 0; i0 < d0; ++i0)
+            for (int64_t i1 = 0; i1 < d1; ++i1)
+                for (int64_t i2 = 0; i2 < d2; ++i2) {
+                    const int64_t i = lin3(i0, i1, i2, d1, d2);
+                    const int64_t s0 = d1 * d2, s1 = d2;
+                    const int64_t b0 = i + (i0 > 0 ? -1 : 1) * s0, a0 = i + (i0 < d0 - 1 ? 1 : -1) * s0;
+                    const int64_t b1 = i + (i1 > 0 ? -1 : 1) * s1, a1 = i + (i1 < d1 - 1 ? 1 : -1) * s1;
+                    const int64_t b2 = i + (i2 > 0 ? -1 : 1), a2 = i + (i2 < d2 - 1 ? 1 : -1);
+                    for (int c = 0; c < 3; ++c) {
+                        const float *gc = g[c];
+                        const float gi = gc[i];
+                        float lap = (float)((gc[b0] + gc[a0] - 2 * gi) * rss[0]);
+                        lap = (float)(lap + (gc[b1] + gc[a1] - 2 * gi) * rss[1]);
+                        lap = (float)(lap + (gc[b2] + gc[a2] - 2 * gi) * rss[2]);
+                        dl[c][i] = (float)(r[i] * (gi - v[c][i]) - mu * lap);
+                    }
+                }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) g[c][i] = (float)(g[c][i] - dt * dl[c][i]);
+        if (mag_out && update > 0 && (it % update) == 0) {
+            float *mg = (float *)malloc(sizeof(float) * n);
+            for (int64_t i = 0; i < n; ++i)
+                mg[i] = sqrtf(dl[0][i] * dl[0][i] + dl[1][i] * dl[1][i] + dl[2][i] * dl[2][i]);
+            mag_out[it / update] = (float)((double)orc_sum_abs_reduce(mg, n) / (double)n);
+            free(mg);
+        }
+    }
+    for (int c = 0; c < 3; ++c) free(dl[c]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* ATV continuous max-flow (update_qp.cl:150-316,                             */
+/* continuous_cut_anisotropic_gpu.py:80-344).                                 */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    double rcc;       /* 1 / (cc * min_s^2) */
+    double rs[3];     /* 1 / s_k */
+    double ss[3];     /* steps * min_s^2 / s_k */
+    double sqev[3];   /* sqrt of the constant eigenvalues (const_evals != 0) */
+    int const_evals;  /* 1: eigs AoS x3 squeezed; 0: AoS x6 [sqrt(l) x3, squeezed x3] */
+    int supervised;   /* labels given */
+} orc_cut_params;
+
+/* update_p: p = H^-1 hp + H^T (SS * grad+ tmp); project |p| <= alpha;
+ * hp = H p, with H = R(q) diag(sqrt(lambda)). */
+ORC_API void orc_update_p(float *hp0, float *hp1, float *hp2, const float *tmp, const float *eigs,
+                          const float *alpha, const int64_t *dims, const orc_cut_params *P)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const int64_t i = lin3(i0, i1, i2, d1, d2);
+                float g[3] = {
+                    i0 < d0 - 1 ? (float)((tmp[i + d2 * d1] - tmp[i]) * P->ss[0]) : 0.f,
+                    i1 < d1 - 1 ? (float)((tmp[i + d2] - tmp[i]) * P->ss[1]) : 0.f,
+                    i2 < d2 - 1 ? (float)((tmp[i + 1] - tmp[i]) * P->ss[2]) : 0.f};
+                const float *e = P->const_evals ? eigs + 3 * i : eigs + 6 * i + 3;
+                double sq[3];
+                if (P->const_evals) { sq[0] = P->sqev[0]; sq[1] = P->sqev[1]; sq[2] = P->sqev[2]; }
+                else { sq[0] = eigs[6 * i]; sq[1] = eigs[6 * i + 1]; sq[2] = eigs[6 * i + 2]; }
+                float q[4], qt[4];
+                q_expand(e, q);
+                qt[0] = -q[0]; qt[1] = q[1]; qt[2] = q[2]; qt[3] = q[3];
+                /* H^T g */
+                q_rot(qt, g, g);
+                for (int k = 0; k < 3; ++k) g[k] = (float)(g[k] * sq[k]);
+                /* H^-1 hp */
+                float p[3] = {hp0[i], hp1[i], hp2[i]};
+                q_rot(qt, p, p);
+                for (int k = 0; k < 3; ++k) p[k] = (float)(p[k] / sq[k]);
+                p[0] += g[0]; p[1] += g[1]; p[2] += g[2];
+                const float nrm = sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+                const float rad = alpha[i];
+                const float sc = nrm > rad ? rad / nrm : 1.f;
+                p[0] *= sc; p[1] *= sc; p[2] *= sc;
+                for (int k = 0; k < 3; ++k) p[k] = (float)(sq[k] * p[k]);
+                q_rot(q, p, p);
+                hp0[i] = p[0]; hp1[i] = p[1]; hp2[i] = p[2];
+            }
+}
+
+/* OpenCL C min() on floats: y if y < x, else x (a NaN in x propagates, a NaN
+ * in y is dropped). */
+static inline float ocl_min(float x, float y) { return y < x ? y : x; }
+
+/* update_u: div(hp) (negative adjoint of grad+), ps/pt projections, u step,
+ * tmp = err (update_qp.cl:241-300). labels may be NULL. */
+ORC_API void orc_update_u(const float *hp0, const float *hp1, const float *hp2, float *tmp,
+                          const float *cs, const float *ct, float *pt, float *u,
+                          const uint8_t *labels, const int64_t *dims, const orc_cut_params *P)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const int64_t i = lin3(i0, i1, i2, d1, d2);
+                const float divhp = (float)(
+                    ((i0 < d0 - 1 ? hp0[i] : 0.f) - (i0 > 0 ? hp0[i - d2 * d1] : 0.f)) * P->rs[0] +
+                    ((i1 < d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - d2] : 0.f)) * P->rs[1] +
+                    ((i2 < d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f)) * P->rs[2]);
+                float ui = u[i];
+                float ps, pti;
+                if (labels) {
+                    const float uf = (float)(labels[i] & 1);
+                    const float ub = (float)(!(labels[i] & 2));
+                    ps = ocl_min((float)(divhp + pt[i] - P->rcc * (ui - ub)), cs[i]);
+                    pti = ocl_min((float)(-divhp + ps + P->rcc * (ui - uf)), ct[i]);
+                } else {
+                    ps = ocl_min((float)(divhp + pt[i] - P->rcc * (ui - 1)), cs[i]);
+                    pti = ocl_min((float)(-divhp + ps + P->rcc * ui), ct[i]);
+                }
+                pt[i] = pti;
+                const float err = divhp - ps + pti;
+                ui = (float)(ui - err / P->rcc);
+                u[i] = ui;
+                tmp[i] = err;
+            }
+}
+
+ORC_API void orc_update_tmp(float *tmp, const float *u, int64_t n, double rcc)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) tmp[i] = (float)(tmp[i] - u[i] * rcc);
+}
+
+/* Whole solve with the reference's loop order (continuous_cut_anisotropic_gpu
+ * .py:276-343). u holds u0 on entry (img / (max + tiny), computed by the
+ * caller) and the result on exit. crit_out (optional, length
+ * ceil(num_iter/update)) receives each criterion Sum|tmp| / (N * RCC).
+ * Returns the number of update_u calls performed. */
+ORC_API int orc_cmf_solve(float *u, const float *eigs, const float *alpha, const float *cs,
+                          const float *ct, const uint8_t *labels, const int64_t *dims,
+                          const orc_cut_params *P, int num_iter, int update, double error_bound,
+                          float *crit_out, int *ncrit_out)
+{
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    float *hp0 = (float *)calloc(n, sizeof(float));
+    float *hp1 = (float *)calloc(n, sizeof(float));
+    float *hp2 = (float *)calloc(n, sizeof(float));
+    float *pt = (float *)malloc(sizeof(float) * n);
+    float *tmp = (float *)calloc(n, sizeof(float));
+    for (int64_t i = 0; i < n; ++i) pt[i] = fminf(cs[i], ct[i]);
+    orc_update_tmp(tmp, u, n, P->rcc);
+    int done = 0, ncrit = 0;
+    for (int it = 0; it < num_iter; ++it) {
+        orc_update_p(hp0, hp1, hp2, tmp, eigs, alpha, dims, P);
+        orc_update_u(hp0, hp1, hp2, tmp, cs, ct, pt, u, labels, dims, P);
+        done = it + 1;
+        if (update > 0 && (it % update) == 0) {
+            const double crit = (double)orc_sum_abs_reduce(tmp, n) / ((double)n * P->rcc);
+            if (crit_out) crit_out[ncrit] = (float)crit;
+            ncrit++;
+            if (isnan(crit) || crit < error_bound) break;
+        }
+        orc_update_tmp(tmp, u, n, P->rcc);
+    }
+    if (ncrit_out) *ncrit_out = ncrit;
+    free(hp0); free(hp1); free(hp2); free(pt); free(tmp);
+    return done;
+}

@@ -1,0 +1,374 @@
+"""GPU parity: every HIP stage, called through libmiccai_hip, against the
+oracle on the same seeded inputs.
+
+Tolerances (FP32 on both sides, different operation order):
+  Hessian / Gaussian passes   max |d| <= 2e-5 * max |ref|
+  eigenvalues                 max |d| <= 1e-5 * max |lambda|
+  eigenvectors                angle <= 1e-3 rad up to sign (relative eigen-gap > 1e-3)
+  vesselness                  max |d| <= 1e-5 * max(ref)
+  GVF                         max |d| <= 1e-5 * max |ref|
+  max-flow u                  max |d| <= 1e-4 after the same iteration count
+  cut (u >= 0.5)              Dice >= 0.999
+  ridges, nearest-ridge signs exact except ties closer than FP32 rounding
+"""
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+from scipy.spatial.transform import Rotation
+
+from oracle import fp64, ref32
+
+pytestmark = pytest.mark.gpu
+
+
+def _squeezed(rot):
+    return ref32.qsqueeze(rot.as_quat()[..., [3, 0, 1, 2]].astype(np.float32))
+
+
+def _smooth_img(rng, shape, s=1.5):
+    return (ndi.gaussian_filter(rng.normal(size=shape), s) * 10).astype(np.float32)
+
+
+# ------------------------------------------------------------------ Frangi --
+
+def test_gauss_filter_matches_oracle(dev, rng):
+    from miccai2016_amd.processing.qfrangi_gpu import GaussianFilter3D
+    img = rng.normal(size=(13, 17, 19)).astype(np.float32)
+    gf = GaussianFilter3D(dev, round_up=False)
+    a, b = dev.upload(img).reshape(-1), dev.empty(img.size)
+    res, _ = gf.execute(a, b, img.shape, (1.2, 2.0, 1.5), (1, 0, 2))
+    got = dev.download(res).reshape(img.shape)
+    taps = gf.taps_for((1.2, 2.0, 1.5), (1, 0, 2))
+    want = img
+    for ax in range(3):
+        want = ref32.convolve_axis(want, ax, taps[ax])
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (9, 20, 31), (40, 3, 7)])
+def test_hessian_matches_oracle(dev, rng, shape):
+    from miccai2016_amd.processing.qfrangi_gpu import GaussianFilter3D, Hessian
+    img = _smooth_img(rng, shape, 1.0)
+    sig = (1.5, 1.0, 2.0)
+    h = Hessian(GaussianFilter3D(dev, round_up=False))
+    out = dev.empty(6 * img.size)
+    h.execute(dev.upload(img).reshape(-1), None, None, shape, out, sig)
+    got = dev.download(out).reshape(shape + (6,))
+    want = ref32.hessian(img, sig)
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
+def test_eigs_match_oracle_and_eigh(dev, rng):
+    from miccai2016_amd.processing.qfrangi_gpu import Eigendecomposition
+    n = 50000
+    a = rng.normal(size=(n, 3, 3))
+    m = (a + np.swapaxes(a, 1, 2)) / 2
+    h6 = np.stack([m[:, 0, 0], m[:, 1, 1], m[:, 2, 2], m[:, 0, 1], m[:, 0, 2], m[:, 1, 2]], -1)
+    h6 = h6.astype(np.float32)
+    h6[:5] = 0                      # zero matrices
+    h6[5:10, 3:] = 0                # diagonal matrices
+    buf = dev.upload(h6).reshape(-1)
+    norm = dev.zeros(1)
+    Eigendecomposition(dev).execute(n, buf, norm_max=norm)
+    got = dev.download(buf).reshape(n, 6)
+    want = ref32.eigs(h6)
+    lam = want[:, :3]
+    assert np.abs(got[:, :3] - lam).max() <= 1e-5 * np.abs(lam).max()
+    # frames: compare eigenvectors with the oracle's and with eigh
+    Rg = fp64.squeezed_to_mat(got[:, 3:])
+    Rw = fp64.squeezed_to_mat(want[:, 3:])
+    lam64, vec = fp64.eig_sorted(h6)
+    srt = np.sort(np.abs(lam64), -1)
+    gap = np.min(np.diff(srt, axis=-1), -1) / np.maximum(srt[:, 2], 1e-30)
+    ok = gap > 1e-3
+    for k in range(3):
+        c1 = np.abs(np.sum(Rg[ok, :, k] * Rw[ok, :, k], -1))
+        c2 = np.abs(np.sum(Rg[ok, :, k] * vec[ok, :, k], -1))
+        assert np.arccos(np.clip(c1, -1, 1)).max() < 1e-3
+        assert np.arccos(np.clip(c2, -1, 1)).max() < 1e-3
+    np.testing.assert_array_equal(got[:10], want[:10])   # degenerate cases bit-exact
+    nm = float(dev.download(norm)[0])
+    assert abs(nm - 2 * ref32.half_max_norm(want)) <= 1e-6 * nm
+
+
+def test_eigs_nonconvergence_nan(dev, rng):
+    from miccai2016_amd.processing.qfrangi_gpu import Eigendecomposition
+    h6 = rng.normal(size=(100, 6)).astype(np.float32)
+    buf = dev.upload(h6).reshape(-1)
+    Eigendecomposition(dev, maxiter=0).execute(100, buf)
+    assert np.isnan(dev.download(buf)).all()
+
+
+def _angle_ok(qa, qb):
+    Ra, Rb = fp64.squeezed_to_mat(qa), fp64.squeezed_to_mat(qb)
+    c = np.abs(np.sum(Ra[..., :, 0] * Rb[..., :, 0], -1))
+    return np.arccos(np.clip(c, -1, 1))
+
+
+def test_feature_filter_matches_oracle(dev):
+    from miccai2016_amd import phantom as ph
+    from miccai2016_amd.processing.qfrangi_gpu import FeatureFilter, VesselnessCalculation
+    img, _ = ph.make_helix(ph.HelixSpec((32, 40, 36), 3, 9, 20, 0.4, seed=5))
+    vp = VesselnessCalculation.Parameters()
+    vp.bright_objects = True
+    vp.scales = (1.0, 2.0, 3)
+    val, dirs, slc = FeatureFilter(vp, dev).execute_for(img, None, (1, 1, 1))
+    assert slc is None and val.shape == img.shape and dirs.shape == img.shape + (3,)
+    wv, wq = ref32.vesselness(img, fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1)))
+    assert np.abs(val - wv).max() <= 1e-5 * wv.max()
+    sig = wv > 0.01 * wv.max()
+    ang = _angle_ok(dirs[sig], wq[sig])
+    assert np.mean(ang < 1e-3) >= 0.999
+
+
+def test_feature_filter_roi_crop(dev):
+    from miccai2016_amd import phantom as ph
+    from miccai2016_amd.processing.qfrangi_gpu import FeatureFilter, VesselnessCalculation
+    img, truth = ph.make_helix(ph.HelixSpec((30, 30, 30), 3, 8, 20, 0.3, seed=1))
+    vp = VesselnessCalculation.Parameters()
+    vp.bright_objects, vp.scales = True, (1.0, 1.5, 2)
+    roi = np.zeros(img.shape, bool)
+    roi[10:15, 12:14, 5:9] = True
+    val, dirs, slc = FeatureFilter(vp, dev).execute_for(img, roi, (1, 1, 1))
+    assert slc == (slice(4, 21, 1), slice(6, 20, 1), slice(0, 15, 1))
+    wv, _ = ref32.vesselness(img[slc], fp64.scales_voxels((1.0, 1.5, 2), (1, 1, 1)))
+    assert np.abs(val - wv).max() <= 1e-5 * wv.max()
+
+
+# -------------------------------------------------------------- directions --
+
+def test_ridges_and_main_dirs_match_oracle(dev, rng):
+    from miccai2016_amd.processing.ridge_detection import RawRidgeDetector
+    shape = (20, 24, 28)
+    resp = _smooth_img(rng, shape, 2.0)
+    resp = (resp - resp.min()) / np.ptp(resp)
+    q = _squeezed(Rotation.random(int(np.prod(shape)), random_state=4)).reshape(shape + (3,))
+    out = dev.empty(resp.size)
+    RawRidgeDetector(dev, 0.01).execute(out, dev.upload(resp).reshape(-1),
+                                        dev.upload(q).reshape(-1), shape)
+    got = dev.download(out).reshape(shape)
+    want = ref32.detect_ridges(resp, q, 0.01)
+    assert got.sum() > 20
+    assert np.mean(got == want) >= 0.9999
+    d = dev.upload(q).reshape(-1)
+    assert dev.lib.mh_main_dirs(dev.handle, dev.ptr(d), dev.ptr(d), resp.size, dev.stream) == 0
+    np.testing.assert_allclose(dev.download(d).reshape(q.shape), ref32.main_dirs(q), atol=2e-6)
+
+
+def test_nonridge_signs_match_oracle(dev, rng):
+    from miccai2016_amd import _lib
+    shape = (17, 19, 23)
+    dirs = rng.normal(size=shape + (3,)).astype(np.float32)
+    coords = np.unique(np.stack([rng.integers(0, s, 3000) for s in shape], -1), axis=0)
+    coords = coords[rng.permutation(len(coords))].astype(np.uint16)   # > one LDS tile
+    rdirs = rng.normal(size=(len(coords), 3)).astype(np.float32)
+    d = dev.upload(dirs).reshape(-1)
+    import torch
+    dc = torch.from_numpy(coords.view(np.int16)).to(dev.device)
+    dr = dev.upload(rdirs)
+    rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(d), _lib.dims3(shape), dev.ptr(dc),
+                                   dev.ptr(dr), len(coords), _lib.F32x3(1, 1, 1), dev.stream)
+    assert rc == 0
+    got = dev.download(d).reshape(dirs.shape)
+    want = ref32.adjust_nonridge_signs(dirs, coords, rdirs)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_complete_frames_match_oracle(dev, rng):
+    from miccai2016_amd.processing.adjust_directions import TransformationCompleter
+    v = rng.normal(size=(4000, 3)).astype(np.float32)
+    v[:3] = 0
+    v[3] = [1, 0, 0]
+    v[4] = [0, 0, -2]
+    got = TransformationCompleter(v, dev).complete()
+    want = ref32.complete_frames(v)
+    np.testing.assert_allclose(fp64.squeezed_to_mat(got), fp64.squeezed_to_mat(want), atol=3e-6)
+    np.testing.assert_array_equal(got[:3], 0)
+
+
+def test_adjuster_end_to_end_matches_oracle_steps(dev):
+    from miccai2016_amd import phantom as ph
+    from miccai2016_amd.helpers.misc import normalize
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    img, _ = ph.make_helix(ph.HelixSpec((32, 32, 32), 3, 8, 16, 0.3, seed=2))
+    wv, wq = ref32.vesselness(img, fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1)))
+    ves = normalize(wv)
+    adj = AdvancedAdjuster((0.01, 0.3), wq, ves, (1, 1, 1), loader=dev)
+    out = adj.execute(verbose=False)
+    # oracle restatement of the same steps
+    ridge = ref32.detect_ridges(ves.astype(np.float32), wq, 0.01)
+    sure = np.logical_and(ridge, ves.astype(np.float32) >= np.float32(0.3))
+    ridge = fp64.hysteresis(ridge.astype(bool), sure).astype(np.float32)
+    md = ref32.main_dirs(wq)
+    co = np.argwhere(ridge != 0)
+    rd = fp64.mst_signs(co, md[tuple(co.T)])
+    cleaned = adj._do_clean_ridge(ridge, co.astype(np.uint16), rd)
+    co2 = np.argwhere(cleaned != 0)
+    rd2 = fp64.mst_signs(co2, md[tuple(co2.T)])
+    want = ref32.adjust_nonridge_signs(md, co2.astype(np.uint16), rd2)
+    assert np.mean(np.all(np.abs(out - want) < 1e-5, -1)) >= 0.999
+    np.testing.assert_array_equal(adj.ridge, cleaned)
+
+
+# --------------------------------------------------------------------- GVF --
+
+@pytest.mark.parametrize("mu,iters", [(0.1, 200), (10 ** 0.5, 60)])
+def test_gvf_matches_oracle(dev, rng, mu, iters):
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    v = ndi.gaussian_filter(rng.normal(size=(18, 20, 22, 3)), (1.5, 1.5, 1.5, 0))
+    v = (v / np.abs(v).max() * 0.9).astype(np.float32)
+    p = GvfParameters()
+    p.mu, p.tol, p.maxiter, p.callback, p.update, p.normalize = mu, 0, iters, None, 20, False
+    got = GvfGpu(v, p, (1, 1, 1), cl_program_loader=dev).execute()
+    want, _ = ref32.gvf(v, mu, iters)
+    assert got.shape == v.shape
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_gvf_callback_checks_and_normalize(dev, rng):
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    v = rng.uniform(-0.5, 0.5, size=(8, 9, 10, 3)).astype(np.float32)
+    seen = []
+    p = GvfParameters()
+    p.mu, p.tol, p.maxiter, p.update, p.normalize = 0.2, 0, 23, 5, False
+    p.callback = lambda g, i, m: seen.append((i, m, g.copy()))
+    g = GvfGpu(v, p, cl_program_loader=dev).execute()
+    want, mags = ref32.gvf(v, 0.2, 23, update=5)
+    assert [s[0] for s in seen] == [0, 5, 10, 15, 20]
+    np.testing.assert_allclose([s[1] for s in seen], mags, rtol=1e-4)
+    np.testing.assert_allclose(g, want, atol=1e-5 * np.abs(want).max())
+    ref10, _ = ref32.gvf(v, 0.2, 11)
+    np.testing.assert_allclose(seen[2][2], ref10, atol=1e-5 * np.abs(ref10).max())
+    # tol stops at the first check below it; normalize gives unit vectors
+    p.callback, p.tol, p.normalize = None, 1e9, True
+    g2 = GvfGpu(v, p, cl_program_loader=dev).execute()
+    one, _ = ref32.gvf(v, 0.2, 1)
+    nrm = np.linalg.norm(one, axis=-1, keepdims=True)
+    np.testing.assert_allclose(g2, np.where(nrm > 0, one / nrm, one), atol=1e-5)
+
+
+# ---------------------------------------------------------------- max-flow --
+
+def _cut_case(rng, shape, sigma=0.8):
+    from miccai2016_amd import phantom as ph
+    spec = ph.HelixSpec(shape, 3, min(shape[1:]) / 4, 20, sigma, seed=int(rng.integers(1 << 30)))
+    img, truth = ph.make_helix(spec)
+    q = _squeezed(Rotation.random(int(np.prod(shape)), random_state=8)).reshape(shape + (3,))
+    cs, ct = ph.terminal_capacities(img, sigma)
+    gms = ph.grad_mag_sq(img)
+    alpha = ph.nonterminal_weight(gms, 10 ** -1.625, 10 ** 1.5)
+    return img, q, alpha.astype(np.float32), cs.astype(np.float32), ct.astype(np.float32), gms
+
+
+def _cutter(dev, iters, update=30, error_bound=0.0, steps=0.16):
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    p = CmfCutQAnisotropicGpu.CutParameters()
+    p.num_iter, p.update, p.error_bound, p.steps = iters, update, error_bound, steps
+    return CmfCutQAnisotropicGpu(p, dev)
+
+
+@pytest.mark.parametrize("shape,eigs", [((16, 16, 16), (1, 10 ** -1.5, 10 ** -1.5)),
+                                        ((20, 17, 23), (1, 1, 1)),
+                                        ((7, 64, 5), (1, 0.3, 0.1))])
+def test_cmf_matches_oracle(dev, rng, shape, eigs):
+    img, q, alpha, cs, ct, _ = _cut_case(rng, shape)
+    u = _cutter(dev, 500).execute(img, q, eigs, alpha, cs, ct, verbose=False)
+    want, done, crit = ref32.cmf_solve(img, q, eigs, alpha, cs, ct, steps=0.16, num_iter=500,
+                                       update=30, error_bound=0)
+    assert done == 500
+    assert np.abs(u - want).max() <= 1e-4
+    assert fp64.fast_dice(u, want >= 0.5) >= 0.999
+
+
+def test_cmf_64_helix_config(dev):
+    """The 64^3 plain workload: max-flow on the full pipeline's inputs."""
+    from miccai2016_amd import phantom as ph
+    from miccai2016_amd.pipeline import PipelineConfig, prepare
+    img, truth = ph.make_helix(ph.PLAIN_64)
+    cfg = PipelineConfig()
+    prep = prepare(img, ph.PLAIN_64.sigma, cfg, dev)
+    ntw = ph.nonterminal_weight(prep.gms, 10 ** -1.625, 10 ** 1.5).astype(np.float32)
+    eigs = (1, cfg.a_atv, cfg.a_atv)
+    u = _cutter(dev, 500).execute(img, prep.quats, eigs, ntw, prep.cs, prep.ct, verbose=False)
+    want, _, _ = ref32.cmf_solve(img.astype(np.float32), prep.quats, eigs, ntw, prep.cs, prep.ct,
+                                 steps=0.16, num_iter=500, update=30, error_bound=0)
+    assert np.abs(u - want).max() <= 1e-4
+    assert fp64.fast_dice(u, want >= 0.5) >= 0.999
+    assert ph.fast_dice(u, truth) > 0.5
+
+
+def test_cmf_pointwise_eigs_and_supervised(dev, rng):
+    img, q, alpha, cs, ct, _ = _cut_case(rng, (12, 14, 16))
+    ev = rng.uniform(0.05, 1.0, size=img.shape + (3,))
+    u = _cutter(dev, 300).execute(img, q, ev, alpha, cs, ct, verbose=False)
+    want, _, _ = ref32.cmf_solve(img, q, ev, alpha, cs, ct, steps=0.16, num_iter=300, update=30,
+                                 error_bound=0)
+    assert np.abs(u - want).max() <= 1e-4
+    labels = np.zeros(img.shape, np.uint8)
+    labels[6, 7, 2:5] = 1
+    labels[0, :3, :3] = 2
+    u = _cutter(dev, 300).execute(img, q, (1, 0.1, 0.1), alpha, cs, ct, labels=labels,
+                                  verbose=False)
+    want, _, _ = ref32.cmf_solve(img, q, (1, 0.1, 0.1), alpha, cs, ct, labels=labels, steps=0.16,
+                                 num_iter=300, update=30, error_bound=0)
+    assert np.abs(u - want).max() <= 1e-4
+
+
+def test_cmf_stops_like_the_reference(dev, rng):
+    img, q, alpha, cs, ct, _ = _cut_case(rng, (16, 16, 16))
+    c = _cutter(dev, 3000, update=7, error_bound=1e-3)
+    u = c.execute(img, q, (1, 1, 1), alpha, cs, ct, verbose=False)
+    want, done, crit = ref32.cmf_solve(img, q, (1, 1, 1), alpha, cs, ct, steps=0.16, num_iter=3000,
+                                       update=7, error_bound=1e-3)
+    assert c.last_result.iterations == done and c.last_result.stopped == 2
+    np.testing.assert_allclose(c.last_criteria, crit, rtol=1e-3)
+    assert np.abs(u - want).max() <= 1e-4
+    # NaN frame -> NaN criterion at the first check -> one iteration
+    q2 = q.copy()
+    q2[3, 4, 5, 1] = np.nan
+    c2 = _cutter(dev, 100, update=10)
+    c2.execute(img, q2, (1, 1, 1), alpha, cs, ct, verbose=False)
+    assert c2.last_result.stopped == 1 and c2.last_result.iterations == 1
+
+
+def test_batched_solves_equal_plain_solves(dev, rng):
+    from miccai2016_amd import phantom as ph
+    img, q, _, cs, ct, gms = _cut_case(rng, (16, 18, 20))
+    w = np.array([10 ** -2.5, 10 ** -1.625, 0.1, 0.5, 3.0, 0.02, 0.05, 0.3, 1.0, 0.01], np.float32)
+    d = np.array([0.1, 10 ** 1.5, 1.0, 10.0, 1000.0, 3.0, 0.5, 30.0, 100.0, 2.0], np.float32)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    c = _cutter(dev, 200)
+    ub = c.execute_batched(img, q, eigs, w, d, gms.astype(np.float32), cs, ct)
+    assert ub.shape == (10,) + img.shape
+    assert all(r.iterations == 200 for r in c.last_result)
+    for s in range(10):
+        alpha = ph.nonterminal_weight(gms.astype(np.float32), w[s], d[s]).astype(np.float32)
+        up = _cutter(dev, 200).execute(img, q, eigs, alpha, cs, ct, verbose=False)
+        assert np.abs(ub[s] - up).max() <= 1e-4
+        assert fp64.fast_dice(ub[s], up >= 0.5) >= 0.999
+
+
+def test_ragged_solves_equal_plain_solves(dev, rng):
+    shapes = [(9, 12, 15), (20, 7, 11), (16, 16, 16), (5, 33, 8)]
+    cases = [_cut_case(rng, s) for s in shapes]
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    c = _cutter(dev, 150)
+    us = c.execute_ragged([k[0] for k in cases], [k[1] for k in cases], eigs,
+                          [k[2] for k in cases], [k[3] for k in cases], [k[4] for k in cases])
+    for k, u in zip(cases, us):
+        up = _cutter(dev, 150).execute(k[0], k[1], eigs, k[2], k[3], k[4], verbose=False)
+        assert np.abs(u - up).max() <= 1e-4
+
+
+def test_fill_and_reductions(dev, rng):
+    x = rng.normal(size=100003).astype(np.float32)
+    x[17] = np.nan
+    t = dev.upload(x)
+    mx = float(dev.download(dev.max(t))[0])
+    assert mx == np.nanmax(x)
+    x[17] = 0
+    t = dev.upload(x)
+    s = float(dev.download(dev.sum_abs(t))[0])
+    assert abs(s - np.abs(x.astype(np.float64)).sum()) <= 1e-5 * s
+    dev.fill(t, 2.5)
+    assert (dev.download(t) == 2.5).all()
