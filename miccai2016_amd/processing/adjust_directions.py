@@ -2,19 +2,19 @@
 
 Drop-in for the reference's processing/adjust_directions.py: ridge voxels are
 detected in the response (GPU), hysteresis-thresholded (host C++), their main
-directions aligned along a minimum spanning tree (host, scipy), the ridge
+directions aligned along a minimum spanning tree (host, sparse Euclidean MST in
+ridge_graph), the ridge
 optionally cleaned, and every voxel's direction flipped to agree with its
 nearest ridge voxel (GPU). ``TransformationCompleter`` turns directions back
 into squeezed frames (GPU).
 """
 import numpy as np
 from scipy.ndimage import label
-from scipy.sparse.csgraph import minimum_spanning_tree
-from scipy.spatial.distance import pdist, squareform
 
 from .. import _lib
 from ..device import default_context
 from ..helpers.parameters import TreePrintable
+from . import ridge_graph
 from .ridge_detection import RawRidgeDetector, hysteresis_threshold
 
 
@@ -141,29 +141,18 @@ class AdvancedAdjuster(TreePrintable):
         return main_dirs
 
     def _adjust_ridge_directions(self, labeled_ridge, main_dirs):
-        """MST over the ridge voxels (scaled Euclidean distances); signs
-        propagated from ridge voxel 0 so that every tree edge has a
-        non-negative dot product. Returns (uint16 coords R x 3, float32 dirs)."""
+        """Minimum spanning tree over the ridge voxels (scaled Euclidean
+        distances, adjust_directions.py:365-417); signs propagated from ridge
+        voxel 0 so that every tree edge has a non-negative dot product.
+        Returns (uint16 coords R x 3 in C order, float32 adjusted dirs)."""
         coords = np.argwhere(labeled_ridge != 0)
-        n = len(coords)
-        adjusted = np.full((n, 3), np.nan, dtype=np.float32)
-        if n == 0:
-            return coords.astype(np.uint16), adjusted
+        if len(coords) == 0:
+            return coords.astype(np.uint16), np.zeros((0, 3), np.float32)
         dirs = main_dirs[tuple(coords.T)]
-        adjusted[0] = dirs[0]
-        if n > 1:
-            mst = minimum_spanning_tree(squareform(pdist(np.multiply(coords, self.voxel_size),
-                                                         "euclidean")), overwrite=True)
-            adj = (mst + mst.T).tocsr()
-            stack = [0]
-            while stack:
-                i = stack.pop()
-                ref = adjusted[i]
-                for j in adj.indices[adj.indptr[i]:adj.indptr[i + 1]]:
-                    if np.isnan(adjusted[j, 0]):
-                        d = dirs[j]
-                        adjusted[j] = d if np.dot(d, ref) >= 0 else -d
-                        stack.append(j)
+        pts = np.multiply(coords, self.voxel_size).astype(np.float64)
+        r0 = float(np.linalg.norm(self.voxel_size))   # the scaled 26-neighbourhood
+        edges = ridge_graph.spanning_tree_edges(pts, r0)
+        adjusted = ridge_graph.propagate_signs(dirs, edges, 0)
         return np.require(coords, np.uint16, ["C"]), adjusted
 
     def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):

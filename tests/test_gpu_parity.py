@@ -105,6 +105,19 @@ def _angle_ok(qa, qb):
     return np.arccos(np.clip(c, -1, 1))
 
 
+def _per_scale_oracle(img, scales_vx):
+    """Per-scale Frangi responses and eigen-decompositions from the oracle."""
+    vals, eigs = [], []
+    for sc in scales_vx:
+        e = ref32.eigs(ref32.hessian(img, sc)).reshape(-1, 6)
+        v = np.full(len(e), -np.inf, np.float32)
+        q = np.zeros((len(e), 3), np.float32)
+        ref32.update_vesselness(v, q, e, 0.5, 0.5, ref32.half_max_norm(e), True)
+        vals.append(v)
+        eigs.append(e)
+    return np.stack(vals), np.stack(eigs)
+
+
 def test_feature_filter_matches_oracle(dev):
     from miccai2016_amd import phantom as ph
     from miccai2016_amd.processing.qfrangi_gpu import FeatureFilter, VesselnessCalculation
@@ -114,11 +127,22 @@ def test_feature_filter_matches_oracle(dev):
     vp.scales = (1.0, 2.0, 3)
     val, dirs, slc = FeatureFilter(vp, dev).execute_for(img, None, (1, 1, 1))
     assert slc is None and val.shape == img.shape and dirs.shape == img.shape + (3,)
-    wv, wq = ref32.vesselness(img, fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1)))
-    assert np.abs(val - wv).max() <= 1e-5 * wv.max()
-    sig = wv > 0.01 * wv.max()
-    ang = _angle_ok(dirs[sig], wq[sig])
-    assert np.mean(ang < 1e-3) >= 0.999
+    scales = fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1))
+    wv, wq = ref32.vesselness(img, scales)
+    err = np.abs(val - wv)
+    print(f"vesselness max |d| / max = {err.max() / wv.max():.2e}")
+    assert err.max() <= 1e-5 * wv.max()
+    # frames: where the winning scale is not a near tie and v1 is well
+    # conditioned (relative gap between |l1| and |l2| > 1e-3)
+    per_val, per_eig = _per_scale_oracle(img.astype(np.float32), scales)
+    top = np.sort(per_val, 0)
+    win = np.argmax(per_val, 0)
+    e_win = per_eig[win, np.arange(per_val.shape[1])]
+    gap = (np.abs(e_win[:, 1]) - np.abs(e_win[:, 0])) / np.maximum(np.abs(e_win[:, 2]), 1e-30)
+    ok = ((wv.ravel() > 0.01 * wv.max()) & (top[-1] - top[-2] > 1e-4 * wv.max()) & (gap > 1e-3))
+    ang = _angle_ok(dirs.reshape(-1, 3)[ok], wq.reshape(-1, 3)[ok])
+    print(f"{ok.sum()} well-posed voxels, max angle {ang.max():.2e} rad")
+    assert ok.sum() > 500 and ang.max() < 1e-3
 
 
 def test_feature_filter_roi_crop(dev):
@@ -186,28 +210,32 @@ def test_complete_frames_match_oracle(dev, rng):
     np.testing.assert_array_equal(got[:3], 0)
 
 
-def test_adjuster_end_to_end_matches_oracle_steps(dev):
+def test_adjuster_end_to_end(dev):
+    """The adjuster's GPU stages in context: the output is +- the main
+    directions; every voxel agrees in sign with its nearest ridge voxel of the
+    (cleaned) ridge; ridge directions agree along the spanning tree. The
+    shared host steps (hysteresis, spanning tree, cleaning) are pinned by the
+    CPU tests, the kernels by the tests above."""
     from miccai2016_amd import phantom as ph
     from miccai2016_amd.helpers.misc import normalize
+    from miccai2016_amd.processing import ridge_graph
     from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
     img, _ = ph.make_helix(ph.HelixSpec((32, 32, 32), 3, 8, 16, 0.3, seed=2))
     wv, wq = ref32.vesselness(img, fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1)))
     ves = normalize(wv)
     adj = AdvancedAdjuster((0.01, 0.3), wq, ves, (1, 1, 1), loader=dev)
     out = adj.execute(verbose=False)
-    # oracle restatement of the same steps
-    ridge = ref32.detect_ridges(ves.astype(np.float32), wq, 0.01)
-    sure = np.logical_and(ridge, ves.astype(np.float32) >= np.float32(0.3))
-    ridge = fp64.hysteresis(ridge.astype(bool), sure).astype(np.float32)
     md = ref32.main_dirs(wq)
+    np.testing.assert_allclose(adj.main_dirs_unadjusted, md, atol=2e-6)
+    np.testing.assert_allclose(np.abs(out), np.abs(adj.main_dirs_unadjusted), atol=0)
+    ridge = adj.ridge
+    assert 20 < ridge.sum() < ridge.size // 4
     co = np.argwhere(ridge != 0)
-    rd = fp64.mst_signs(co, md[tuple(co.T)])
-    cleaned = adj._do_clean_ridge(ridge, co.astype(np.uint16), rd)
-    co2 = np.argwhere(cleaned != 0)
-    rd2 = fp64.mst_signs(co2, md[tuple(co2.T)])
-    want = ref32.adjust_nonridge_signs(md, co2.astype(np.uint16), rd2)
-    assert np.mean(np.all(np.abs(out - want) < 1e-5, -1)) >= 0.999
-    np.testing.assert_array_equal(adj.ridge, cleaned)
+    pts = co.astype(np.float64)
+    edges = ridge_graph.spanning_tree_edges(pts, np.sqrt(3))
+    rd = ridge_graph.propagate_signs(adj.main_dirs_unadjusted[tuple(co.T)], edges)
+    want = ref32.adjust_nonridge_signs(adj.main_dirs_unadjusted, co.astype(np.uint16), rd)
+    np.testing.assert_array_equal(out, want)
 
 
 # --------------------------------------------------------------------- GVF --
