@@ -25,35 +25,28 @@ struct MaxOp {  // NaN-neutral maximum (fmax semantics)
     __device__ static float combine(float a, float b) { return fmaxf(a, b); }
 };
 
-// Grid-stride partials, then the last block to finish combines the partials
-// in block order, so the result does not depend on scheduling.
+// Grid-stride partials per block, then one work-group combines them in block
+// order, so the result does not depend on scheduling.
 template <class Op>
 __global__ void __launch_bounds__(mh::kBlock)
-reduce_kernel(const float *__restrict__ x, int64_t n, float *__restrict__ partials,
-              unsigned *__restrict__ counter, float *__restrict__ out)
+reduce_kernel(const float *__restrict__ x, int64_t n, float *__restrict__ partials)
 {
     float acc = Op::identity();
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
         acc = Op::combine(acc, Op::load(x[i]));
     acc = mh::block_reduce<Op>(acc);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = acc;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(mh::kBlock)
+combine_kernel(const float *__restrict__ partials, int nparts, float *__restrict__ out)
+{
     float r = Op::identity();
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-        r = Op::combine(r, partials[b]);
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) r = Op::combine(r, partials[b]);
     r = mh::block_reduce<Op>(r);
-    if (threadIdx.x == 0) {
-        *out = r;
-        atomicExch(counter, 0u);
-    }
+    if (threadIdx.x == 0) *out = r;
 }
 
 template <class Op>
@@ -61,11 +54,11 @@ int reduce_impl(mh_ctx *ctx, const float *x, int64_t n, float *out, void *stream
 {
     MH_REQUIRE(ctx, ctx && x && out && n > 0, "null pointer or empty input");
     const unsigned nb = std::min<int64_t>(kReduceBlocks, mh::blocks_for(n));
-    unsigned char *s = static_cast<unsigned char *>(ctx->scratch);
-    unsigned *counter = reinterpret_cast<unsigned *>(s);
-    float *partials = reinterpret_cast<float *>(s + 4096);
+    float *partials = reinterpret_cast<float *>(static_cast<unsigned char *>(ctx->scratch) + 4096);
     hipLaunchKernelGGL(reduce_kernel<Op>, dim3(nb), dim3(mh::kBlock), 0, (hipStream_t)stream, x, n,
-                       partials, counter, out);
+                       partials);
+    hipLaunchKernelGGL(combine_kernel<Op>, dim3(1), dim3(mh::kBlock), 0, (hipStream_t)stream, partials,
+                       int(nb), out);
     MH_CHECK_LAUNCH(ctx, "reduce_kernel");
     return MH_OK;
 }

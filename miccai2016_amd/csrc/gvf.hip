@@ -12,13 +12,11 @@ struct SumOp {
 
 // g' = g - dt * (r (g - v) - mu * lap g), lap with the 7-point stencil and
 // the neighbour across a face replaced by the one on the inner side.
-// With `mag`, |delta| is reduced: per-block partials, then the last block
-// sums them in block order.
+// With `partials`, each block stores the sum of |delta| over its voxels.
 __global__ void __launch_bounds__(mh::kBlock)
 gvf_step_kernel(float *__restrict__ out, const float *__restrict__ g, const float *__restrict__ v,
                 const float *__restrict__ r, int d0, int d1, int d2, float mu, float dt, float rss0,
-                float rss1, float rss2, float *__restrict__ partials, unsigned *__restrict__ counter,
-                float *__restrict__ mag)
+                float rss1, float rss2, float *__restrict__ partials)
 {
     const int64_t n = int64_t(d0) * d1 * d2;
     const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
@@ -47,24 +45,19 @@ gvf_step_kernel(float *__restrict__ out, const float *__restrict__ g, const floa
         }
         dmag = sqrtf(dsq);
     }
-    if (!mag) return;
+    if (!partials) return;
     dmag = mh::block_reduce<SumOp>(dmag);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = dmag;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[blockIdx.x] = dmag;
+}
+
+// Sum of the per-block partials in block order (one work-group).
+__global__ void __launch_bounds__(mh::kBlock)
+sum_partials_kernel(const float *__restrict__ partials, int nparts, float *__restrict__ out)
+{
     float s = 0.f;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) s += partials[k];
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += partials[k];
     s = mh::block_reduce<SumOp>(s);
-    if (threadIdx.x == 0) {
-        *mag = s;
-        atomicExch(counter, 0u);
-    }
+    if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void __launch_bounds__(mh::kBlock)
@@ -119,19 +112,17 @@ MH_EXPORT int mh_gvf(mh_ctx *ctx, float *g3, float *swap3, const float *v3, cons
     const int64_t n = dims[0] * dims[1] * dims[2];
     const unsigned nb = mh::blocks_for(n);
     hipStream_t s = (hipStream_t)stream;
-    unsigned *counter = work ? reinterpret_cast<unsigned *>(work + nb) : nullptr;
-    if (mag) {
-        // zero the block counter with a kernel (graph-capture friendly)
-        int rc = mh_fill(ctx, work + nb, 1, 0.f, stream);
-        if (rc) return rc;
-    }
     float *src = g3, *dst = swap3;
     for (int it = 0; it < iters; ++it) {
-        float *m = (mag && it == iters - 1) ? mag : nullptr;
+        float *parts = (mag && it == iters - 1) ? work : nullptr;
         hipLaunchKernelGGL(gvf_step_kernel, dim3(nb), dim3(mh::kBlock), 0, s, dst, src, v3, r,
                            int(dims[0]), int(dims[1]), int(dims[2]), p->mu, p->dt, p->rss[0],
-                           p->rss[1], p->rss[2], work, counter, m);
+                           p->rss[1], p->rss[2], parts);
         MH_CHECK_LAUNCH(ctx, "gvf_step_kernel");
+        if (parts) {
+            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(mh::kBlock), 0, s, parts, int(nb), mag);
+            MH_CHECK_LAUNCH(ctx, "sum_partials_kernel");
+        }
         float *t = src;
         src = dst;
         dst = t;

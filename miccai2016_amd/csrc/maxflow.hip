@@ -6,9 +6,10 @@
 //   p-step: p = H^-1 hp + H^T (ss * grad+ tmp); project |p| <= alpha; hp = H p
 //   u-step: div hp; ps, pt projections; err; u -= err / rcc; tmp = err - rcc u
 // The reference's third launch (tmp -= rcc u) is folded into the u-step; on
-// criterion iterations the u-step also reduces sum |err| and the last block to
-// finish decides the stop, so the host never waits inside the loop. A stopped
-// solve turns every later launch into an immediate return.
+// criterion iterations the u-step also stores per-block sums of |err| and a
+// one-block kernel adds them in block order and decides the stop, so the host
+// never waits inside the loop. A stopped solve turns every later launch into
+// an immediate return.
 #include <vector>
 
 #include "mh_common.h"
@@ -162,26 +163,27 @@ cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float 
     }
     if (!CHECK) return;
     aerr = mh::block_reduce<SumOp>(aerr);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = aerr;
-        __threadfence();
-        last = atomicAdd(reinterpret_cast<unsigned *>(ctrl + kCounter), 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[blockIdx.x] = aerr;
+}
+
+// Criterion of one solve from the u-step's per-block partials (summed in
+// block order): log it and set the stop flag on NaN or convergence. One
+// work-group; launched after a criterion iteration's u-step.
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
+                float error_bound, int *__restrict__ ctrl, float *__restrict__ crit_log)
+{
+    if (ctrl[kStop]) return;
     float s = 0.f;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) s += partials[k];
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += partials[k];
     s = mh::block_reduce<SumOp>(s);
     if (threadIdx.x == 0) {
-        const float crit = s * P.inv_n_rcc;
+        const float crit = s * inv_n_rcc;
         const int k = ctrl[kNcrit];
         if (k < kCritLogMax) crit_log[k] = crit;
         ctrl[kNcrit] = k + 1;
         if (crit != crit) ctrl[kStop] = 1;
-        else if (crit < P.error_bound) ctrl[kStop] = 2;
-        ctrl[kCounter] = 0;
+        else if (crit < error_bound) ctrl[kStop] = 2;
     }
 }
 
@@ -280,6 +282,9 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
         else
             hipLaunchKernelGGL((cmf_u_kernel<false, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
                                W.tmp, cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+        if (chk)
+            hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(mh::kBlock), 0, s, W.partials, int(nb),
+                               UP.inv_n_rcc, UP.error_bound, W.ctrl, W.crit_log);
         MH_CHECK_LAUNCH(ctx, "cmf kernels");
     }
     int ctrl[kCtrlWords];
@@ -438,29 +443,26 @@ cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
         const float v = mh::block_reduce<SumOp>(aerr[k]);
         if (threadIdx.x == 0) partials[int64_t(s0 + k) * nbx + blockIdx.x] = v;
     }
-    __shared__ bool last;
+}
+
+// One work-group per solve: criterion from its partials, log, stop flag.
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_batch_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
+                      float error_bound, BatchCtrl C)
+{
+    const int s = blockIdx.x;
+    if (C.stop[s]) return;
+    float acc = 0.f;
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += partials[int64_t(s) * nparts + q];
+    acc = mh::block_reduce<SumOp>(acc);
     if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(C.count + group, 1u) == nbx - 1;
+        const float crit = acc * inv_n_rcc;
+        const int c = C.ncrit[s];
+        if (c < C.maxcheck) C.crit_log[int64_t(s) * C.maxcheck + c] = crit;
+        C.ncrit[s] = c + 1;
+        if (crit != crit) C.stop[s] = 1;
+        else if (crit < error_bound) C.stop[s] = 2;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int k = 0; k < ns; ++k) {
-        const int s = s0 + k;
-        float acc = 0.f;
-        for (unsigned q = threadIdx.x; q < nbx; q += blockDim.x) acc += partials[int64_t(s) * nbx + q];
-        acc = mh::block_reduce<SumOp>(acc);
-        if (threadIdx.x == 0 && !C.stop[s]) {
-            const float crit = acc * P.inv_n_rcc;
-            const int c = C.ncrit[s];
-            if (c < C.maxcheck) C.crit_log[int64_t(s) * C.maxcheck + c] = crit;
-            C.ncrit[s] = c + 1;
-            if (crit != crit) C.stop[s] = 1;
-            else if (crit < P.error_bound) C.stop[s] = 2;
-        }
-    }
-    if (threadIdx.x == 0) C.count[group] = 0u;
 }
 
 struct BatchWorkspace {
@@ -566,6 +568,9 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                 hipLaunchKernelGGL(cmf_batch_u_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
                                    W.tmp, cs, ct, W.pt, u, D, UP, s0, ns, C, W.partials, g);
         }
+        if (chk)
+            hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(S), dim3(mh::kBlock), 0, s, W.partials, int(nb),
+                               UP.inv_n_rcc, UP.error_bound, C);
         MH_CHECK_LAUNCH(ctx, "batched cmf kernels");
     }
     std::vector<int> stop(2 * S);
@@ -707,27 +712,29 @@ ragged_u_kernel(const Vol *__restrict__ vols, int nvol, const float *__restrict_
     }
     if (!CHECK) return;
     aerr = mh::block_reduce<SumOp>(aerr);
-    const int nbv = int((n + blockDim.x - 1) / blockDim.x);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = aerr;
-        __threadfence();
-        last = atomicAdd(C.count + v, 1u) == unsigned(nbv - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[blockIdx.x] = aerr;
+}
+
+// One work-group per volume: criterion from the partials of its blocks.
+__global__ void __launch_bounds__(mh::kBlock)
+ragged_crit_kernel(const Vol *__restrict__ vols, const float *__restrict__ partials,
+                   float rcc, float error_bound, BatchCtrl C)
+{
+    const int v = blockIdx.x;
+    if (C.stop[v]) return;
+    const Vol V = vols[v];
+    const int64_t n = int64_t(V.d0) * V.d1 * V.d2;
+    const int nbv = int((n + mh::kBlock - 1) / mh::kBlock);
     float acc = 0.f;
     for (int q = threadIdx.x; q < nbv; q += blockDim.x) acc += partials[V.block0 + q];
     acc = mh::block_reduce<SumOp>(acc);
     if (threadIdx.x == 0) {
-        const float crit = acc / (float(n) * P.rcc);
+        const float crit = acc / (float(n) * rcc);
         const int c = C.ncrit[v];
         if (c < C.maxcheck) C.crit_log[int64_t(v) * C.maxcheck + c] = crit;
         C.ncrit[v] = c + 1;
         if (crit != crit) C.stop[v] = 1;
-        else if (crit < P.error_bound) C.stop[v] = 2;
-        C.count[v] = 0u;
+        else if (crit < error_bound) C.stop[v] = 2;
     }
 }
 
@@ -820,10 +827,12 @@ MH_EXPORT int mh_cmf_solve_ragged(mh_ctx *ctx, float *u, const float *eigs, cons
     for (int it = 0; it < prm->num_iter; ++it) {
         hipLaunchKernelGGL(ragged_p_kernel, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols, nvol,
                            W.hp, W.tmp, W.q4, alpha, PP, W.stop);
-        if ((it % prm->update) == 0)
+        if ((it % prm->update) == 0) {
             hipLaunchKernelGGL(ragged_u_kernel<true>, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols,
                                nvol, W.hp, W.tmp, cs, ct, W.pt, u, UP, C, W.partials);
-        else
+            hipLaunchKernelGGL(ragged_crit_kernel, dim3(nvol), dim3(mh::kBlock), 0, s, W.vols, W.partials,
+                               prm->rcc, prm->error_bound, C);
+        } else
             hipLaunchKernelGGL(ragged_u_kernel<false>, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols,
                                nvol, W.hp, W.tmp, cs, ct, W.pt, u, UP, C, W.partials);
         MH_CHECK_LAUNCH(ctx, "ragged cmf kernels");
