@@ -169,14 +169,12 @@ cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float 
 // Criterion of one solve from the u-step's per-block partials (summed in
 // block order): log it and set the stop flag on NaN or convergence. One
 // work-group; launched after a criterion iteration's u-step.
-__global__ void __launch_bounds__(mh::kBlock)
+__global__ void __launch_bounds__(1024)
 cmf_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
                 float error_bound, int *__restrict__ ctrl, float *__restrict__ crit_log)
 {
     if (ctrl[kStop]) return;
-    float s = 0.f;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += partials[k];
-    s = mh::block_reduce<SumOp>(s);
+    const float s = mh::block_sum_array<SumOp>(partials, nparts);
     if (threadIdx.x == 0) {
         const float crit = s * inv_n_rcc;
         const int k = ctrl[kNcrit];
@@ -283,7 +281,7 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
             hipLaunchKernelGGL((cmf_u_kernel<false, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
                                W.tmp, cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
         if (chk)
-            hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(mh::kBlock), 0, s, W.partials, int(nb),
+            hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(1024), 0, s, W.partials, int(nb),
                                UP.inv_n_rcc, UP.error_bound, W.ctrl, W.crit_log);
         MH_CHECK_LAUNCH(ctx, "cmf kernels");
     }

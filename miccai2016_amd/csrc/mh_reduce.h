@@ -34,4 +34,23 @@ __device__ __forceinline__ float block_reduce(float v)
     return v;
 }
 
+// Sum of n values by one work-group: each thread keeps four independent
+// partial sums (more loads in flight), combined in a fixed order, so the
+// result depends only on n and the block size.
+template <class Op>
+__device__ __forceinline__ float block_sum_array(const float *__restrict__ x, int n)
+{
+    float a0 = Op::identity(), a1 = Op::identity(), a2 = Op::identity(), a3 = Op::identity();
+    const int step = blockDim.x;
+    int k = threadIdx.x;
+    for (; k + 3 * step < n; k += 4 * step) {
+        a0 = Op::combine(a0, x[k]);
+        a1 = Op::combine(a1, x[k + step]);
+        a2 = Op::combine(a2, x[k + 2 * step]);
+        a3 = Op::combine(a3, x[k + 3 * step]);
+    }
+    for (; k < n; k += step) a0 = Op::combine(a0, x[k]);
+    return block_reduce<Op>(Op::combine(Op::combine(a0, a1), Op::combine(a2, a3)));
+}
+
 }  // namespace mh
