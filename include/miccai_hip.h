@@ -124,11 +124,16 @@ int mh_main_dirs(mh_ctx *ctx, float *dirs3_dev, const float *qeig3_dev, int64_t 
 /* For every voxel, nearest ridge point by scaled squared distance (lowest
  * index wins ties), then flip the voxel's direction if its dot product with
  * that ridge direction is negative. coords_dev: uint16 R x 3, rdirs_dev:
- * float R x 3. Replaces adjustNonRidgeSigns (clcode/adjust_directions.cl:
- * 182-246, processing/adjust_directions.py:419-446). */
+ * float R x 3. With work_dev (mh_nonridge_work_bytes(dims) bytes), an
+ * isotropic voxel size and many ridge points, the nearest point comes from an
+ * exact separable distance transform (same result as the scan); otherwise
+ * every ridge point is scanned. Replaces adjustNonRidgeSigns
+ * (clcode/adjust_directions.cl:182-246, processing/adjust_directions.py:419-446). */
+int64_t mh_nonridge_work_bytes(const int64_t dims[3]);
 int mh_nonridge_signs(mh_ctx *ctx, float *dirs3_dev, const int64_t dims[3],
                       const uint16_t *coords_dev, const float *rdirs_dev, int64_t nridge,
-                      const float scaling_host[3], void *stream);
+                      const float scaling_host[3], void *work_dev, int64_t work_bytes,
+                      void *stream);
 
 /* Direction -> orthonormal frame (v, v2, v x v2) -> squeezed quaternion,
  * AoS x3, in place allowed; a zero vector yields the identity frame.
@@ -232,6 +237,37 @@ int mh_cmf_solve_ragged(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const 
                         const int64_t *vdims_host, const int64_t *voff_host,
                         const mh_cmf_params *params, void *work_dev, int64_t work_bytes,
                         mh_cmf_result *results_host, void *stream);
+
+/* ------------------------------------------- resident jobs (continuous batching) */
+
+/* One unsupervised constant-eigenvalue solve whose state stays on the device
+ * between calls, so jobs of different sizes can join and leave a running set
+ * at iteration-chunk boundaries. All pointers are device pointers owned by
+ * the caller; state_dev holds mh_cmf_job_state_bytes(n, params) bytes. No
+ * counterpart in the reference (each of its solves runs alone, start to
+ * finish, continuous_cut_anisotropic_gpu.py:276-343); a job run in chunks
+ * computes the same iterations as mh_cmf_solve. */
+typedef struct {
+    int64_t dims[3];
+    float *u_dev;            /* u0 before mh_cmf_jobs_init, the labelling after */
+    const float *eigs_dev;   /* squeezed frames AoS x3 */
+    const float *alpha_dev;
+    const float *cs_dev;
+    const float *ct_dev;
+    void *state_dev;
+    mh_cmf_params params;    /* const_evals must be 1 */
+    int iter;                /* iterations done so far (updated by mh_cmf_jobs_step) */
+    int stopped;             /* 0 running, 1 NaN criterion, 2 converged (updated) */
+} mh_cmf_job;
+
+int64_t mh_cmf_job_state_bytes(int64_t n, const mh_cmf_params *params);
+/* Initialise the state of njobs jobs (frames expanded, hp = 0, pt = min(cs, ct),
+ * tmp = -rcc u0); sets iter = 0. Enqueues only. */
+int mh_cmf_jobs_init(mh_ctx *ctx, mh_cmf_job *jobs_host, int njobs, void *stream);
+/* Advance every job by min(iters, num_iter - iter) iterations (a stopped job
+ * does nothing) in one launch sequence covering all jobs; updates iter and
+ * stopped. Blocks. At most 1024 jobs per call. */
+int mh_cmf_jobs_step(mh_ctx *ctx, mh_cmf_job *jobs_host, int njobs, int iters, void *stream);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,11 @@ class CmfResult(C.Structure):
     _fields_ = [("iterations", i32), ("stopped", i32), ("ncrit", i32), ("last_crit", f32)]
 
 
+class CmfJob(C.Structure):
+    _fields_ = [("dims", i64 * 3), ("u", vp), ("eigs", vp), ("alpha", vp), ("cs", vp), ("ct", vp),
+                ("state", vp), ("params", CmfParams), ("iter", i32), ("stopped", i32)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/miccai_hip.h
 SIGNATURES = {
     "mh_create": (i32, [i32, C.POINTER(vp)]),
@@ -55,7 +60,8 @@ SIGNATURES = {
     "mh_update_vesselness": (i32, [vp, vp, vp, vp, i64, f32, f32, f32, vp, i32, vp]),
     "mh_detect_ridges": (i32, [vp, vp, vp, vp, I64x3, f32, vp]),
     "mh_main_dirs": (i32, [vp, vp, vp, i64, vp]),
-    "mh_nonridge_signs": (i32, [vp, vp, I64x3, vp, vp, i64, F32x3, vp]),
+    "mh_nonridge_work_bytes": (i64, [I64x3]),
+    "mh_nonridge_signs": (i32, [vp, vp, I64x3, vp, vp, i64, F32x3, vp, i64, vp]),
     "mh_complete_frames": (i32, [vp, vp, vp, i64, vp]),
     "mh_hysteresis_host": (i32, [vp, vp, vp, I64x3]),
     "mh_gvf_response": (i32, [vp, vp, vp, i64, vp]),
@@ -68,6 +74,9 @@ SIGNATURES = {
     "mh_cmf_batched_workspace_bytes": (i64, [i64, i32]),
     "mh_cmf_solve_batched": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, I64x3,
                                    C.POINTER(CmfParams), vp, i64, C.POINTER(CmfResult), vp]),
+    "mh_cmf_job_state_bytes": (i64, [i64, C.POINTER(CmfParams)]),
+    "mh_cmf_jobs_init": (i32, [vp, C.POINTER(CmfJob), i32, vp]),
+    "mh_cmf_jobs_step": (i32, [vp, C.POINTER(CmfJob), i32, i32, vp]),
     "mh_cmf_ragged_workspace_bytes": (i64, [i64, i32]),
     "mh_cmf_solve_ragged": (i32, [vp, vp, vp, vp, vp, vp, i32, vp, vp, C.POINTER(CmfParams), vp,
                                   i64, C.POINTER(CmfResult), vp]),

@@ -1,0 +1,53 @@
+// Per-voxel arithmetic of one ATV continuous max-flow iteration, shared by
+// every solver variant (single, batched, ragged, resident jobs) so that all of
+// them compute bit-identical iterations.
+//
+// H = R(q) diag(sqrt(lambda)). The p-step takes the forward differences of the
+// residual tmp, scaled by ss, and the previous hp = H p:
+//   p = H^-1 hp + H^T g, projected onto |p| <= alpha, hp' = H p.
+// The u-step takes div = -(adjoint of the forward difference) of hp':
+//   ps = min(div + pt - rcc (u - ub), cs), pt' = min(ps - div + rcc (u - uf), ct),
+//   err = div - ps + pt', u' = u - err / rcc, tmp' = err - rcc u'
+// with min(x, y) = (y < x ? y : x) as in the source algorithm's kernel
+// language (a NaN residual propagates into the criterion, a NaN capacity is
+// dropped) and ub = 1, uf = 0 for the unsupervised cut.
+#pragma once
+
+#include "mh_common.h"
+
+namespace mh {
+
+__device__ __forceinline__ float kmin(float x, float y) { return y < x ? y : x; }
+
+__device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, float g0, float g1,
+                                            float g2, const float4 qq, float sq0, float sq1,
+                                            float sq2, float alpha)
+{
+    const Mat3 R = quat_to_mat(Quat{qq.x, qq.y, qq.z, qq.w});
+    const float *m = R.m;
+    float p0 = (m[0] * h0 + m[3] * h1 + m[6] * h2) / sq0 + sq0 * (m[0] * g0 + m[3] * g1 + m[6] * g2);
+    float p1 = (m[1] * h0 + m[4] * h1 + m[7] * h2) / sq1 + sq1 * (m[1] * g0 + m[4] * g1 + m[7] * g2);
+    float p2 = (m[2] * h0 + m[5] * h1 + m[8] * h2) / sq2 + sq2 * (m[2] * g0 + m[5] * g1 + m[8] * g2);
+    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    const float sc = nrm > alpha ? alpha / nrm : 1.f;
+    p0 *= sc * sq0;
+    p1 *= sc * sq1;
+    p2 *= sc * sq2;
+    h0 = m[0] * p0 + m[1] * p1 + m[2] * p2;
+    h1 = m[3] * p0 + m[4] * p1 + m[5] * p2;
+    h2 = m[6] * p0 + m[7] * p1 + m[8] * p2;
+}
+
+// Returns err; updates pt, u and writes the next residual to tmp.
+__device__ __forceinline__ float cmf_u_voxel(float div, float &pt, float &u, float &tmp, float cs,
+                                             float ct, float rcc, float ub, float uf)
+{
+    const float ps = kmin(fmaf(-rcc, u - ub, div + pt), cs);
+    pt = kmin(fmaf(rcc, u - uf, ps - div), ct);
+    const float err = div - ps + pt;
+    u = u - err / rcc;
+    tmp = fmaf(-u, rcc, err);
+    return err;
+}
+
+}  // namespace mh

@@ -97,6 +97,7 @@ MH_EXPORT int mh_destroy(mh_ctx *ctx)
     if (!ctx) return MH_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->desc) (void)hipFree(ctx->desc);
     delete ctx;
     return MH_OK;
 }

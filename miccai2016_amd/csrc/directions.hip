@@ -132,6 +132,129 @@ nonridge_kernel(float *__restrict__ dirs, int d0, int d1, int d2,
     }
 }
 
+// ---- nearest ridge point by an exact separable distance transform ---------
+// For an isotropic voxel size the squared distances are integers (times s^2),
+// so the nearest ridge point can be found axis by axis: after the pass along
+// axis a, every voxel holds the smallest squared distance to a ridge point
+// reachable by moving along the axes processed so far, and the lowest ridge
+// index among the points at that distance. The lowest index over a tie set
+// composes across passes (each candidate already carries the lowest index of
+// its own tie set), so the result equals the brute-force scan with strict '<'.
+constexpr unsigned kNoRidge = 0xffffffffu;
+
+__global__ void __launch_bounds__(mh::kBlock) fill_u32_kernel(unsigned *__restrict__ x, int64_t n, unsigned v)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = v;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_scatter_kernel(unsigned *__restrict__ ridx, const uint16_t *__restrict__ coords, int nr,
+                     int d0, int d1, int d2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nr) return;
+    const int c0 = coords[3 * j], c1 = coords[3 * j + 1], c2 = coords[3 * j + 2];
+    if (c0 >= d0 || c1 >= d1 || c2 >= d2) return;
+    atomicMin(&ridx[c2 + int64_t(d2) * (c1 + int64_t(d1) * c0)], unsigned(j));
+}
+
+// Pass along axis 2: two sweeps per line give the nearest ridge voxel on the
+// line (the lower index wins when the left and right ones are equidistant).
+__global__ void __launch_bounds__(mh::kBlock)
+edt_axis2_kernel(const unsigned *__restrict__ ridx, unsigned *__restrict__ dist,
+                 unsigned *__restrict__ best, int d0, int d1, int d2)
+{
+    const int64_t line = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (line >= int64_t(d0) * d1) return;
+    const int64_t base = line * d2;
+    int last = -1;
+    unsigned lastj = kNoRidge;
+    for (int x = 0; x < d2; ++x) {
+        const unsigned j = ridx[base + x];
+        if (j != kNoRidge) { last = x; lastj = j; }
+        if (last >= 0) {
+            const unsigned dx = unsigned(x - last);
+            dist[base + x] = dx * dx;
+            best[base + x] = lastj;
+        } else {
+            dist[base + x] = kNoRidge;
+            best[base + x] = kNoRidge;
+        }
+    }
+    last = -1;
+    for (int x = d2 - 1; x >= 0; --x) {
+        const unsigned j = ridx[base + x];
+        if (j != kNoRidge) { last = x; lastj = j; }
+        if (last >= 0) {
+            const unsigned dx = unsigned(last - x);
+            const unsigned dd = dx * dx;
+            const unsigned cur = dist[base + x];
+            if (dd < cur || (dd == cur && lastj < best[base + x])) {
+                dist[base + x] = dd;
+                best[base + x] = lastj;
+            }
+        }
+    }
+}
+
+// Pass along axis 1 or 0: scan outward from the voxel, stopping once the
+// squared offset alone reaches the best value found.
+template <int AXIS>
+__global__ void __launch_bounds__(mh::kBlock)
+edt_scan_kernel(const unsigned *__restrict__ din, const unsigned *__restrict__ bin,
+                unsigned *__restrict__ dout, unsigned *__restrict__ bout, int d0, int d1, int d2)
+{
+    const int64_t n = int64_t(d0) * d1 * d2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int len = AXIS == 1 ? d1 : d0;
+    const int64_t st = AXIS == 1 ? int64_t(d2) : int64_t(d1) * d2;
+    const int c = AXIS == 1 ? int((i / d2) % d1) : int(i / (int64_t(d2) * d1));
+    unsigned bd = din[i], bj = bin[i];
+    for (int off = 1; off < len; ++off) {
+        const unsigned o2 = unsigned(off) * unsigned(off);
+        if (o2 > bd) break;  // every further candidate is at least o2 away
+        if (c - off >= 0) {
+            const unsigned v = din[i - off * st];
+            if (v != kNoRidge) {
+                const unsigned dd = v + o2;
+                const unsigned j = bin[i - off * st];
+                if (dd < bd || (dd == bd && j < bj)) { bd = dd; bj = j; }
+            }
+        }
+        if (c + off < len) {
+            const unsigned v = din[i + off * st];
+            if (v != kNoRidge) {
+                const unsigned dd = v + o2;
+                const unsigned j = bin[i + off * st];
+                if (dd < bd || (dd == bd && j < bj)) { bd = dd; bj = j; }
+            }
+        }
+    }
+    dout[i] = bd;
+    bout[i] = bj;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+flip_by_nearest_kernel(float *__restrict__ dirs, const unsigned *__restrict__ best,
+                       const float *__restrict__ rdirs, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned j = best[i];
+    if (j == kNoRidge) return;
+    float *v = dirs + 3 * i;
+    const float *r = rdirs + 3 * int64_t(j);
+    const float x = v[0], y = v[1], z = v[2];
+    if (x * r[0] + y * r[1] + z * r[2] < 0.f) {
+        v[0] = -x;
+        v[1] = -y;
+        v[2] = -z;
+    }
+}
+
 __device__ __forceinline__ void normalize3(float v[3])
 {
     const float d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -228,19 +351,47 @@ MH_EXPORT int mh_main_dirs(mh_ctx *ctx, float *dirs, const float *qeig3, int64_t
     return MH_OK;
 }
 
+MH_EXPORT int64_t mh_nonridge_work_bytes(const int64_t dims[3])
+{
+    if (!mh::dims_ok(dims)) return -1;
+    return int64_t(5) * dims[0] * dims[1] * dims[2] * int64_t(sizeof(unsigned));
+}
+
 MH_EXPORT int mh_nonridge_signs(mh_ctx *ctx, float *dirs, const int64_t dims[3],
                                 const uint16_t *coords, const float *rdirs, int64_t nr,
-                                const float scaling[3], void *stream)
+                                const float scaling[3], void *work, int64_t work_bytes,
+                                void *stream)
 {
     MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
     MH_REQUIRE(ctx, dirs && scaling && nr >= 0 && nr < (int64_t(1) << 31), "bad arguments");
     MH_REQUIRE(ctx, nr == 0 || (coords && rdirs), "null ridge arrays");
     if (nr == 0) return MH_OK;  // no ridge point: nothing to align with
     const int64_t n = dims[0] * dims[1] * dims[2];
-    hipLaunchKernelGGL(nonridge_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0,
-                       (hipStream_t)stream, dirs, int(dims[0]), int(dims[1]), int(dims[2]), coords,
-                       rdirs, int(nr), scaling[0], scaling[1], scaling[2]);
-    MH_CHECK_LAUNCH(ctx, "nonridge_kernel");
+    hipStream_t s = (hipStream_t)stream;
+    const bool iso = scaling[0] == scaling[1] && scaling[1] == scaling[2] && scaling[0] > 0.f;
+    const bool small_dims = dims[0] <= 16384 && dims[1] <= 16384 && dims[2] <= 16384;
+    const bool have_ws = work && work_bytes >= mh_nonridge_work_bytes(dims);
+    // the brute-force scan costs N*R; the transform about N*(nearest distance)
+    if (!(iso && small_dims && have_ws) || double(n) * double(nr) < 2e9) {
+        hipLaunchKernelGGL(nonridge_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0, s, dirs,
+                           int(dims[0]), int(dims[1]), int(dims[2]), coords, rdirs, int(nr),
+                           scaling[0], scaling[1], scaling[2]);
+        MH_CHECK_LAUNCH(ctx, "nonridge_kernel");
+        return MH_OK;
+    }
+    unsigned *ridx = static_cast<unsigned *>(work);
+    unsigned *dA = ridx + n, *bA = dA + n, *dB = bA + n, *bB = dB + n;
+    const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
+    const unsigned nb = mh::blocks_for(n);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(nb), dim3(mh::kBlock), 0, s, ridx, n, kNoRidge);
+    hipLaunchKernelGGL(ridge_scatter_kernel, dim3(mh::blocks_for(nr)), dim3(mh::kBlock), 0, s, ridx,
+                       coords, int(nr), d0, d1, d2);
+    hipLaunchKernelGGL(edt_axis2_kernel, dim3(mh::blocks_for(int64_t(d0) * d1)), dim3(mh::kBlock), 0, s,
+                       ridx, dA, bA, d0, d1, d2);
+    hipLaunchKernelGGL(edt_scan_kernel<1>, dim3(nb), dim3(mh::kBlock), 0, s, dA, bA, dB, bB, d0, d1, d2);
+    hipLaunchKernelGGL(edt_scan_kernel<0>, dim3(nb), dim3(mh::kBlock), 0, s, dB, bB, dA, bA, d0, d1, d2);
+    hipLaunchKernelGGL(flip_by_nearest_kernel, dim3(nb), dim3(mh::kBlock), 0, s, dirs, bA, rdirs, n);
+    MH_CHECK_LAUNCH(ctx, "nearest-ridge transform");
     return MH_OK;
 }
 

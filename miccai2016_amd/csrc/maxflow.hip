@@ -12,6 +12,7 @@
 // an immediate return.
 #include <vector>
 
+#include "cmf_voxel.h"
 #include "mh_common.h"
 #include "mh_reduce.h"
 
@@ -30,9 +31,6 @@ struct Dims {
     int d0, d1, d2;
 };
 
-// min() of the source algorithm's kernel language: y if y < x, else x, so a
-// NaN residual propagates into the criterion while a NaN capacity is dropped.
-__device__ __forceinline__ float kmin(float x, float y) { return y < x ? y : x; }
 
 __device__ __forceinline__ void coords(int64_t i, const Dims D, int &i0, int &i1, int &i2)
 {
@@ -95,23 +93,11 @@ cmf_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp, const float4
         sq1 = eigs6[6 * i + 1];
         sq2 = eigs6[6 * i + 2];
     }
-    const float4 qq = q4[i];
-    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
-    const float h0 = hp[i], h1 = hp[n + i], h2 = hp[2 * n + i];
-    // p = diag(1/sq) R^T hp + diag(sq) R^T g
-    float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / sq0 + sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
-    float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / sq1 + sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
-    float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / sq2 + sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
-    // project onto the ball of radius alpha
-    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-    const float a = alpha[i];
-    const float sc = nrm > a ? a / nrm : 1.f;
-    p0 *= sc * sq0;
-    p1 *= sc * sq1;
-    p2 *= sc * sq2;
-    hp[i] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
-    hp[n + i] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
-    hp[2 * n + i] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+    float h0 = hp[i], h1 = hp[n + i], h2 = hp[2 * n + i];
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], sq0, sq1, sq2, alpha[i]);
+    hp[i] = h0;
+    hp[n + i] = h1;
+    hp[2 * n + i] = h2;
 }
 
 // ---------------------------------------------------------------- u-step ---
@@ -145,20 +131,17 @@ cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float 
         const float x1 = (i1 < D.d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - D.d2] : 0.f);
         const float x2 = (i2 < D.d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f);
         const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
-        float ui = u[i];
         float ub = 1.f, uf = 0.f;
         if (SUP) {
             const uint8_t l = labels[i];
             uf = float(l & 1);
             ub = (l & 2) ? 0.f : 1.f;
         }
-        const float ps = kmin(fmaf(-P.rcc, ui - ub, div + pt[i]), cs[i]);
-        const float pti = kmin(fmaf(P.rcc, ui - uf, ps - div), ct[i]);
-        const float err = div - ps + pti;
-        ui = ui - err / P.rcc;
+        float ui = u[i], pti = pt[i], ti;
+        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, ub, uf);
         pt[i] = pti;
         u[i] = ui;
-        tmp[i] = fmaf(-ui, P.rcc, err);
+        tmp[i] = ti;
         aerr = fabsf(err);
     }
     if (!CHECK) return;
@@ -366,7 +349,6 @@ cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
     const int64_t st0 = int64_t(D.d1) * D.d2;
     const bool e0 = i0 < D.d0 - 1, e1 = i1 < D.d1 - 1, e2 = i2 < D.d2 - 1;
     const float4 qq = q4[i];
-    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
     const float gm = gms[i];
 #pragma unroll
     for (int k = 0; k < kSI; ++k) {
@@ -378,19 +360,11 @@ cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
         const float g0 = e0 ? (ts[i + st0] - t) * P.ss0 : 0.f;
         const float g1 = e1 ? (ts[i + D.d2] - t) * P.ss1 : 0.f;
         const float g2 = e2 ? (ts[i + 1] - t) * P.ss2 : 0.f;
-        const float h0 = h[i], h1 = h[n + i], h2 = h[2 * n + i];
-        float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / P.sq0 + P.sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
-        float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / P.sq1 + P.sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
-        float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / P.sq2 + P.sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
-        const float a = SP.w[k] * expf(-gm * SP.k[k]);
-        const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-        const float sc = nrm > a ? a / nrm : 1.f;
-        p0 *= sc * P.sq0;
-        p1 *= sc * P.sq1;
-        p2 *= sc * P.sq2;
-        h[i] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
-        h[n + i] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
-        h[2 * n + i] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+        float h0 = h[i], h1 = h[n + i], h2 = h[2 * n + i];
+        mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, qq, P.sq0, P.sq1, P.sq2, SP.w[k] * expf(-gm * SP.k[k]));
+        h[i] = h0;
+        h[n + i] = h1;
+        h[2 * n + i] = h2;
     }
 }
 
@@ -422,14 +396,11 @@ cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
             const float x1 = (e1 ? h[n + i] : 0.f) - (f1 ? h[n + i - D.d2] : 0.f);
             const float x2 = (e2 ? h[2 * n + i] : 0.f) - (f2 ? h[2 * n + i - 1] : 0.f);
             const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
-            float ui = u[n * s + i];
-            const float ps = kmin(fmaf(-P.rcc, ui - 1.f, div + pt[n * s + i]), csi);
-            const float pti = kmin(fmaf(P.rcc, ui, ps - div), cti);
-            const float err = div - ps + pti;
-            ui = ui - err / P.rcc;
+            float ui = u[n * s + i], pti = pt[n * s + i], ti;
+            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csi, cti, P.rcc, 1.f, 0.f);
             pt[n * s + i] = pti;
             u[n * s + i] = ui;
-            tmp[n * s + i] = fmaf(-ui, P.rcc, err);
+            tmp[n * s + i] = ti;
             aerr[k] = fabsf(err);
         }
     }
@@ -656,22 +627,12 @@ ragged_p_kernel(const Vol *__restrict__ vols, int nvol, float *__restrict__ hp,
     const float g0 = i0 < V.d0 - 1 ? (tmp[i + st0] - t) * P.ss0 : 0.f;
     const float g1 = i1 < V.d1 - 1 ? (tmp[i + V.d2] - t) * P.ss1 : 0.f;
     const float g2 = i2 < V.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
-    const float4 qq = q4[i];
-    const mh::Mat3 R = mh::quat_to_mat(mh::Quat{qq.x, qq.y, qq.z, qq.w});
     float *h = hp + 3 * V.off;
-    const float h0 = h[li], h1 = h[n + li], h2 = h[2 * n + li];
-    float p0 = (R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2) / P.sq0 + P.sq0 * (R.m[0] * g0 + R.m[3] * g1 + R.m[6] * g2);
-    float p1 = (R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2) / P.sq1 + P.sq1 * (R.m[1] * g0 + R.m[4] * g1 + R.m[7] * g2);
-    float p2 = (R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2) / P.sq2 + P.sq2 * (R.m[2] * g0 + R.m[5] * g1 + R.m[8] * g2);
-    const float a = alpha[i];
-    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-    const float sc = nrm > a ? a / nrm : 1.f;
-    p0 *= sc * P.sq0;
-    p1 *= sc * P.sq1;
-    p2 *= sc * P.sq2;
-    h[li] = R.m[0] * p0 + R.m[1] * p1 + R.m[2] * p2;
-    h[n + li] = R.m[3] * p0 + R.m[4] * p1 + R.m[5] * p2;
-    h[2 * n + li] = R.m[6] * p0 + R.m[7] * p1 + R.m[8] * p2;
+    float h0 = h[li], h1 = h[n + li], h2 = h[2 * n + li];
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], P.sq0, P.sq1, P.sq2, alpha[i]);
+    h[li] = h0;
+    h[n + li] = h1;
+    h[2 * n + li] = h2;
 }
 
 template <bool CHECK>
@@ -698,14 +659,11 @@ ragged_u_kernel(const Vol *__restrict__ vols, int nvol, const float *__restrict_
         const float x1 = (i1 < V.d1 - 1 ? h[n + li] : 0.f) - (i1 > 0 ? h[n + li - V.d2] : 0.f);
         const float x2 = (i2 < V.d2 - 1 ? h[2 * n + li] : 0.f) - (i2 > 0 ? h[2 * n + li - 1] : 0.f);
         const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
-        float ui = u[i];
-        const float ps = kmin(fmaf(-P.rcc, ui - 1.f, div + pt[i]), cs[i]);
-        const float pti = kmin(fmaf(P.rcc, ui, ps - div), ct[i]);
-        const float err = div - ps + pti;
-        ui = ui - err / P.rcc;
+        float ui = u[i], pti = pt[i], ti;
+        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, 1.f, 0.f);
         pt[i] = pti;
         u[i] = ui;
-        tmp[i] = fmaf(-ui, P.rcc, err);
+        tmp[i] = ti;
         aerr = fabsf(err);
     }
     if (!CHECK) return;

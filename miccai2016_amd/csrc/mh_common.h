@@ -26,6 +26,9 @@ struct mh_ctx {
     // the context): [0, 4 KiB) scalars, then partials
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // job descriptor table of mh_cmf_jobs_step (grown on demand)
+    void *desc = nullptr;
+    size_t desc_bytes = 0;
 };
 
 namespace mh {
@@ -71,7 +74,7 @@ inline bool dims_ok(const int64_t *dims)
            dims[0] * dims[1] * dims[2] < (int64_t(1) << 31);
 }
 
-inline unsigned blocks_for(int64_t n, int block = kBlock)
+__host__ __device__ inline unsigned blocks_for(int64_t n, int block = kBlock)
 {
     return unsigned((n + block - 1) / block);
 }

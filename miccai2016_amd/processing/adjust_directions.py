@@ -185,9 +185,11 @@ class AdvancedAdjuster(TreePrintable):
             d_coords = dev.torch.from_numpy(raw).to(dev.device)
             d_rdirs = dev.upload(ridge_dirs)
             scaling = _lib.F32x3(*[float(v) for v in self.voxel_size])
+            wbytes = int(dev.lib.mh_nonridge_work_bytes(_lib.dims3(shape)))
+            work = dev.bytes(wbytes)
             rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(self._d_qeigs), _lib.dims3(shape),
                                            dev.ptr(d_coords), dev.ptr(d_rdirs), n_r, scaling,
-                                           dev.stream)
+                                           dev.ptr(work), wbytes, dev.stream)
             dev.check(rc, "mh_nonridge_signs")
         return dev.download(self._d_qeigs).reshape(self.qeigs.shape)
 

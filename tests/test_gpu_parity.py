@@ -210,7 +210,35 @@ def test_nonridge_signs_match_oracle(dev, rng):
     dc = torch.from_numpy(coords.view(np.int16)).to(dev.device)
     dr = dev.upload(rdirs)
     rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(d), _lib.dims3(shape), dev.ptr(dc),
-                                   dev.ptr(dr), len(coords), _lib.F32x3(1, 1, 1), dev.stream)
+                                   dev.ptr(dr), len(coords), _lib.F32x3(1, 1, 1), None, 0,
+                                   dev.stream)
+    assert rc == 0
+    got = dev.download(d).reshape(dirs.shape)
+    want = ref32.adjust_nonridge_signs(dirs, coords, rdirs)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("density,shape", [(0.3, (40, 50, 60)), (0.01, (96, 90, 100)),
+                                           (0.0005, (128, 128, 128))])
+def test_nearest_ridge_transform_equals_scan(dev, rng, density, shape):
+    """The distance-transform path (taken for many ridge points) picks the same
+    nearest ridge point, lowest index on ties, as the oracle's scan."""
+    import torch
+    from miccai2016_amd import _lib
+    n = int(np.prod(shape))
+    dirs = rng.normal(size=shape + (3,)).astype(np.float32)
+    coords = np.argwhere(rng.random(shape) < density)
+    coords = coords[rng.permutation(len(coords))].astype(np.uint16)   # arbitrary order
+    rdirs = rng.normal(size=(len(coords), 3)).astype(np.float32)
+    assert n * len(coords) >= 2e9   # the transform path is taken
+    d = dev.upload(dirs).reshape(-1)
+    dc = torch.from_numpy(coords.view(np.int16)).to(dev.device)
+    dr = dev.upload(rdirs)
+    wb = int(dev.lib.mh_nonridge_work_bytes(_lib.dims3(shape)))
+    work = dev.bytes(wb)
+    rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(d), _lib.dims3(shape), dev.ptr(dc),
+                                   dev.ptr(dr), len(coords), _lib.F32x3(1, 1, 1), dev.ptr(work), wb,
+                                   dev.stream)
     assert rc == 0
     got = dev.download(d).reshape(dirs.shape)
     want = ref32.adjust_nonridge_signs(dirs, coords, rdirs)
