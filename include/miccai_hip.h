@@ -238,6 +238,41 @@ int mh_cmf_solve_ragged(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const 
                         const mh_cmf_params *params, void *work_dev, int64_t work_bytes,
                         mh_cmf_result *results_host, void *stream);
 
+/* ---------------------------------------------- fields around the max-flow */
+
+/* (cs, ct) = rq * (max(r, 0), max(-r, 0)), r = log((N(x;1,s^2) + eps) /
+ * (N(x;0,s^2) + eps)), rq = 1 / log((p_hat + eps) / eps); FP32. Replaces the
+ * host weights of phantom_experiment.py:280-301. */
+int mh_terminal_capacities(mh_ctx *ctx, const float *img_dev, int64_t n, float noise_level,
+                           float eps, float *cs_dev, float *ct_dev, void *stream);
+/* |grad I|^2 with central differences inside, one-sided at the faces
+ * (numpy.gradient; phantom_experiment.py:345). */
+int mh_grad_mag_sq(mh_ctx *ctx, const float *img_dev, const int64_t dims[3], float *gms_dev,
+                   void *stream);
+/* alpha = w exp(-gms / decay^2) (phantom_experiment.py:359). */
+int mh_nonterminal_weight(mh_ctx *ctx, const float *gms_dev, int64_t n, float weighting,
+                          float decay, float *alpha_dev, void *stream);
+/* Min-max normalisation to [0, 1] (0 everywhere for a constant input);
+ * work_dev holds mh_normalize_work_floats(n) floats (helpers/misc.py:63-114,
+ * without percentiles). */
+int64_t mh_normalize_work_floats(int64_t n);
+int mh_minmax_normalize(mh_ctx *ctx, const float *x_dev, int64_t n, float *out_dev,
+                        float *work_dev, void *stream);
+/* v3 (SoA x3) = dirs3 (AoS x3) * resp: the GVF input of
+ * phantom_experiment.py:335. */
+int mh_weighted_dirs(mh_ctx *ctx, const float *dirs3_dev, const float *resp_dev, int64_t n,
+                     float *v3_dev, void *stream);
+/* SoA x3 -> AoS x3 (the GVF result layout, gradient_vector_flow.py:241-253). */
+int mh_soa_to_aos3(mh_ctx *ctx, const float *soa_dev, int64_t n, float *aos_dev, void *stream);
+/* Synthetic helix phantom for load generation: tube of radius r_tube around a
+ * helix of radius r_helix and pitch `pitch` along axis 0, plus N(0, sigma^2)
+ * noise from a counter-based hash of (seed, voxel). truth_dev may be NULL. No
+ * counterpart in the reference (its tvtk voxeliser, helpers/helix_phantom.py,
+ * draws a different, toric helix). */
+int mh_helix_phantom(mh_ctx *ctx, const int64_t dims[3], float r_tube, float r_helix,
+                     float pitch, float sigma, uint64_t seed, float *img_dev, uint8_t *truth_dev,
+                     void *stream);
+
 /* ------------------------------------------- resident jobs (continuous batching) */
 
 /* One unsupervised constant-eigenvalue solve whose state stays on the device

@@ -74,6 +74,14 @@ SIGNATURES = {
     "mh_cmf_batched_workspace_bytes": (i64, [i64, i32]),
     "mh_cmf_solve_batched": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, I64x3,
                                    C.POINTER(CmfParams), vp, i64, C.POINTER(CmfResult), vp]),
+    "mh_terminal_capacities": (i32, [vp, vp, i64, f32, f32, vp, vp, vp]),
+    "mh_grad_mag_sq": (i32, [vp, vp, I64x3, vp, vp]),
+    "mh_nonterminal_weight": (i32, [vp, vp, i64, f32, f32, vp, vp]),
+    "mh_normalize_work_floats": (i64, [i64]),
+    "mh_minmax_normalize": (i32, [vp, vp, i64, vp, vp, vp]),
+    "mh_weighted_dirs": (i32, [vp, vp, vp, i64, vp, vp]),
+    "mh_soa_to_aos3": (i32, [vp, vp, i64, vp, vp]),
+    "mh_helix_phantom": (i32, [vp, I64x3, f32, f32, f32, f32, C.c_uint64, vp, vp, vp]),
     "mh_cmf_job_state_bytes": (i64, [i64, C.POINTER(CmfParams)]),
     "mh_cmf_jobs_init": (i32, [vp, C.POINTER(CmfJob), i32, vp]),
     "mh_cmf_jobs_step": (i32, [vp, C.POINTER(CmfJob), i32, i32, vp]),

@@ -71,6 +71,60 @@ def prepare(img, noise_level, cfg=None, loader=None, verbose=False):
                     phantom.grad_mag_sq(img))
 
 
+@dataclass
+class PreparedDevice:
+    """Device tensors (float32, flat) shared by the cuts of one volume."""
+    quats: object    # completed frames, AoS x3 (3N)
+    cs: object
+    ct: object
+    gms: object      # |grad I|^2
+    ridge_voxels: int
+
+
+def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=False):
+    """prepare() on a device tensor (float32, N): every stage runs on the GPU
+    and only the ridge steps of the adjuster visit the host."""
+    from . import _lib
+    cfg = PipelineConfig() if cfg is None else cfg
+    dev = default_context() if loader is None else loader
+    shape = tuple(shape)
+    n = int(np.prod(shape))
+    img_t = img_t.reshape(-1)
+    vp = VesselnessCalculation.Parameters()
+    vp.alpha, vp.beta, vp.c = cfg.alpha, cfg.beta, cfg.c
+    vp.bright_objects, vp.scales = cfg.bright, cfg.scales
+    ff = FeatureFilter(vp, dev)
+    val, quat = ff.execute_tensor(img_t, shape, cfg.voxel_size)
+    ves = dev.empty(n)
+    work = dev.empty(int(dev.lib.mh_normalize_work_floats(n)))
+    dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(val), n, dev.ptr(ves), dev.ptr(work),
+                                          dev.stream), "mh_minmax_normalize")
+    adj = AdvancedAdjuster(threshold=cfg.ridge_thresholds, qeigs=None, response=None,
+                           voxel_size=cfg.voxel_size, clean_ridge=cfg.clean_ridge, loader=dev)
+    dirs = adj.execute_tensor(quat, ves, shape, verbose=verbose)
+    v3 = dev.empty(3 * n)
+    dev.check(dev.lib.mh_weighted_dirs(dev.handle, dev.ptr(dirs), dev.ptr(ves), n, dev.ptr(v3),
+                                       dev.stream), "mh_weighted_dirs")
+    del ff, val, quat
+    gp = GvfParameters()
+    gp.mu, gp.tol, gp.maxiter = cfg.gvf_mu, 0, cfg.gvf_iters
+    gp.callback, gp.update, gp.normalize = None, max(cfg.gvf_iters // 10, 1), False
+    gvf = GvfGpu(v3, gp, cfg.voxel_size, cl_program_loader=dev, shape=shape)
+    g = gvf.execute_device()
+    quats = dev.empty(3 * n)
+    dev.check(dev.lib.mh_soa_to_aos3(dev.handle, dev.ptr(g), n, dev.ptr(quats), dev.stream),
+              "mh_soa_to_aos3")
+    del gvf, g, v3
+    TransformationCompleter(None, dev).complete_device(quats)
+    cs, ct, gms = dev.empty(n), dev.empty(n), dev.empty(n)
+    dev.check(dev.lib.mh_terminal_capacities(dev.handle, dev.ptr(img_t), n, float(noise_level),
+                                             1e-9, dev.ptr(cs), dev.ptr(ct), dev.stream),
+              "mh_terminal_capacities")
+    dev.check(dev.lib.mh_grad_mag_sq(dev.handle, dev.ptr(img_t), _lib.dims3(shape), dev.ptr(gms),
+                                     dev.stream), "mh_grad_mag_sq")
+    return PreparedDevice(quats, cs, ct, gms, int(np.count_nonzero(adj.ridge)))
+
+
 def cut_parameters(cfg):
     p = CmfCutQAnisotropicGpu.CutParameters()
     p.num_iter, p.update, p.error_bound = cfg.cut_iters, cfg.cut_update, cfg.cut_error_bound

@@ -117,6 +117,56 @@ class AdvancedAdjuster(TreePrintable):
                 adjusted = self._adjust_all_directions(coords, rdirs, shape)
         return adjusted
 
+    def execute_tensor(self, qeigs_t, resp_t, shape, verbose=False):
+        """Device form of execute(): qeigs_t (squeezed frames, 3N) and resp_t
+        (N) device tensors; returns the adjusted directions (AoS x3 device
+        tensor). Only the ridge mask, the response and the ridge voxels'
+        directions visit the host (hysteresis, spanning tree, cleaning)."""
+        dev = self._loader
+        self._verbose = verbose
+        shape = tuple(shape)
+        n = int(np.prod(shape))
+        self._d_qeigs = qeigs_t.reshape(-1).clone()
+        self._d_resp = resp_t.reshape(-1)
+        d_ridge = dev.empty(n)
+        self._ridge_detector.execute(d_ridge, self._d_resp, self._d_qeigs, shape)
+        ridge = dev.download(d_ridge).reshape(shape)
+        if self._hysteresis:
+            resp = dev.download(self._d_resp).reshape(shape)
+            sure = np.logical_and(ridge, resp >= self.threshold[1])
+            ridge[:] = hysteresis_threshold(ridge, sure.astype(np.float32))
+        self.__ridge = ridge
+        rc = dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(self._d_qeigs), n,
+                                  dev.stream)
+        dev.check(rc, "mh_main_dirs")
+        md = self._d_qeigs.reshape(n, 3)
+
+        def ridge_dirs(mask):
+            """Coordinates of the mask's voxels and their main directions
+            (gathered on the device)."""
+            coords = np.argwhere(mask != 0)
+            if len(coords) == 0:
+                return coords, np.zeros((0, 3), np.float32)
+            lin = np.ravel_multi_index(tuple(coords.T), shape).astype(np.int64)
+            idx = dev.torch.from_numpy(lin).to(dev.device)
+            return coords, dev.download(md.index_select(0, idx))
+
+        coords, dirs = ridge_dirs(ridge)
+        coords, adjusted = self._tree_signs(coords, dirs)
+        if self.clean_ridge:
+            ridge = self._do_clean_ridge(ridge, coords, adjusted)
+            coords, dirs = ridge_dirs(ridge)
+            coords, adjusted = self._tree_signs(coords, dirs)
+        self._flip_nonridge(coords, adjusted, shape)
+        return self._d_qeigs
+
+    def _tree_signs(self, coords, dirs):
+        if len(coords) == 0:
+            return coords.astype(np.uint16), np.zeros((0, 3), np.float32)
+        pts = np.multiply(coords, self.voxel_size).astype(np.float64)
+        edges = ridge_graph.spanning_tree_edges(pts, float(np.linalg.norm(self.voxel_size)))
+        return np.require(coords, np.uint16, ["C"]), ridge_graph.propagate_signs(dirs, edges, 0)
+
     def _label_ridge(self):
         dev = self._loader
         shape = self.response.shape
@@ -146,14 +196,8 @@ class AdvancedAdjuster(TreePrintable):
         voxel 0 so that every tree edge has a non-negative dot product.
         Returns (uint16 coords R x 3 in C order, float32 adjusted dirs)."""
         coords = np.argwhere(labeled_ridge != 0)
-        if len(coords) == 0:
-            return coords.astype(np.uint16), np.zeros((0, 3), np.float32)
-        dirs = main_dirs[tuple(coords.T)]
-        pts = np.multiply(coords, self.voxel_size).astype(np.float64)
-        r0 = float(np.linalg.norm(self.voxel_size))   # the scaled 26-neighbourhood
-        edges = ridge_graph.spanning_tree_edges(pts, r0)
-        adjusted = ridge_graph.propagate_signs(dirs, edges, 0)
-        return np.require(coords, np.uint16, ["C"]), adjusted
+        # the tree's short-edge pass uses the scaled 26-neighbourhood
+        return self._tree_signs(coords, main_dirs[tuple(coords.T)] if len(coords) else None)
 
     def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):
         """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed
@@ -176,23 +220,31 @@ class AdvancedAdjuster(TreePrintable):
         self.__ridge = cleaned
         return cleaned
 
-    def _adjust_all_directions(self, ridge_coords, ridge_dirs, shape):
+    def _flip_nonridge(self, ridge_coords, ridge_dirs, shape):
+        """Every voxel's direction takes the sign that agrees with its nearest
+        (scaled Euclidean) ridge voxel, on the device."""
         dev = self._loader
         n_r = len(ridge_coords)
-        if n_r > 0:
-            # uint16 coordinates travel as their int16 bit pattern (torch has no uint16 copy)
-            raw = np.ascontiguousarray(ridge_coords, dtype=np.uint16).view(np.int16)
-            d_coords = dev.torch.from_numpy(raw).to(dev.device)
-            d_rdirs = dev.upload(ridge_dirs)
-            scaling = _lib.F32x3(*[float(v) for v in self.voxel_size])
-            wbytes = int(dev.lib.mh_nonridge_work_bytes(_lib.dims3(shape)))
-            work = dev.bytes(wbytes)
-            rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(self._d_qeigs), _lib.dims3(shape),
-                                           dev.ptr(d_coords), dev.ptr(d_rdirs), n_r, scaling,
-                                           dev.ptr(work), wbytes, dev.stream)
-            dev.check(rc, "mh_nonridge_signs")
-        return dev.download(self._d_qeigs).reshape(self.qeigs.shape)
+        if n_r == 0:
+            return
+        # uint16 coordinates travel as their int16 bit pattern (torch has no uint16 copy)
+        raw = np.ascontiguousarray(ridge_coords, dtype=np.uint16).view(np.int16)
+        d_coords = dev.torch.from_numpy(raw).to(dev.device)
+        d_rdirs = dev.upload(ridge_dirs)
+        scaling = _lib.F32x3(*[float(v) for v in self.voxel_size])
+        wbytes = int(dev.lib.mh_nonridge_work_bytes(_lib.dims3(shape)))
+        work = dev.bytes(wbytes)
+        rc = dev.lib.mh_nonridge_signs(dev.handle, dev.ptr(self._d_qeigs), _lib.dims3(shape),
+                                       dev.ptr(d_coords), dev.ptr(d_rdirs), n_r, scaling,
+                                       dev.ptr(work), wbytes, dev.stream)
+        dev.check(rc, "mh_nonridge_signs")
+        # the temporaries above are released when this returns; keep them alive
+        # until the kernels that read them are done
+        dev.synchronize()
 
+    def _adjust_all_directions(self, ridge_coords, ridge_dirs, shape):
+        self._flip_nonridge(ridge_coords, ridge_dirs, shape)
+        return self._loader.download(self._d_qeigs).reshape(self.qeigs.shape)
 
 
 class TransformationCompleter(TreePrintable):

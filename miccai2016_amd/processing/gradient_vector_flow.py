@@ -40,11 +40,17 @@ class GvfGpu:
     """3-D only. vfield: (D0, D1, D2, 3) vector field; response: optional
     fidelity weight (default |v|^2)."""
 
-    def __init__(self, vfield, params, voxel_size=None, response=None, cl_program_loader=None):
-        vfield = np.asarray(vfield)
-        assert vfield.ndim == 4 and vfield.shape[-1] == 3
+    def __init__(self, vfield, params, voxel_size=None, response=None, cl_program_loader=None,
+                 shape=None):
+        """vfield: numpy (D0, D1, D2, 3), or a device tensor holding the field
+        SoA x3 (3N floats) together with `shape`."""
+        self._device_input = shape is not None
+        if not self._device_input:
+            vfield = np.asarray(vfield)
+            assert vfield.ndim == 4 and vfield.shape[-1] == 3
+            shape = vfield.shape[:3]
         self.params = params
-        self.shape = vfield.shape[:3]
+        self.shape = tuple(shape)
         self.n_voxels = int(np.prod(self.shape))
         if min(self.shape) < 2:
             raise ValueError("GvfGpu needs at least 2 voxels along every axis")
@@ -53,8 +59,12 @@ class GvfGpu:
         self.loader = cl_program_loader if cl_program_loader is not None else default_context()
         dev = self.loader
         # SoA x3 on the device (gradient_vector_flow.py:77-79)
-        soa = np.stack([np.require(vfield[..., k], np.float32, ["C"]) for k in range(3)])
-        self.v = dev.upload(soa)
+        if self._device_input:
+            assert vfield.numel() == 3 * self.n_voxels
+            self.v = vfield.reshape(-1)
+        else:
+            soa = np.stack([np.require(vfield[..., k], np.float32, ["C"]) for k in range(3)])
+            self.v = dev.upload(soa)
         self.g = dev.zeros(3 * self.n_voxels)
         self.swap = dev.empty(3 * self.n_voxels)
         self.r = dev.empty(self.n_voxels)

@@ -276,6 +276,19 @@ class FeatureFilter(parameters.TreePrintable):
             b["quat"], self._gaussian_filter_3d, self._eigendecomposition, None)
         return val[:n], quat[:3 * n], slc, tuple(shape)
 
+    def execute_tensor(self, img_t, shape, voxel_size=None):
+        """Whole-volume response of a device tensor (float32, N): returns
+        device tensors (val (N), quat (3N)) owned by this filter."""
+        voxel_size = np.ones(3) if voxel_size is None else voxel_size
+        n = int(np.prod(shape))
+        self._init_buffers(n)
+        b = self._bufs
+        b["img"].copy_(img_t.reshape(-1))
+        val, quat = self._vesselness_calculation.execute(
+            self.vesselness_params, tuple(shape), voxel_size, b["img"], None, None, b["hess"],
+            b["val"], b["quat"], self._gaussian_filter_3d, self._eigendecomposition, None)
+        return val[:n], quat[:3 * n]
+
     def _determine_slicing_and_crop_voxels(self, voxels, roi, voxel_size):
         if roi is None:
             return voxels, None
