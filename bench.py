@@ -8,11 +8,13 @@ iterations each, run as one batched solve on device-resident inputs. The
 shared inputs (frames from Frangi -> adjustment -> GVF -> completion,
 capacities, |grad I|^2) are prepared once, outside the timed region.
 
-Reported: whole-job voxel-iterations/s of the batched grid; the same solves
-run one at a time (device-resident, and through the numpy drop-in API); a
-256^3 single solve (the bandwidth-scale plain workload); the HBM roofline of
-the max-flow iteration (84 algorithmic bytes per voxel-iteration); and the
-CPU oracle timed on a bounded sample on this host.
+Reported: whole-job voxel-iterations/s of the batched grid; every cell's
+fast Dice against the noise-free mask (counted on the device); the same
+solves run one at a time (device-resident, and through the numpy drop-in
+API); a 256^3 single solve (the bandwidth-scale plain workload); serving
+latency (p50/p99) and throughput under Poisson arrivals of mixed-size cuts;
+the HBM roofline of the max-flow iteration (84 algorithmic bytes per
+voxel-iteration); and the CPU oracle timed on a bounded sample on this host.
 """
 import argparse
 import json
@@ -57,6 +59,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-256", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=150)
+    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no "
@@ -113,6 +118,12 @@ def main():
     ms_step = secs / args.steps * 1e3
     log(f"batched grid: {value:.4e} voxel-iters/s ({ms_step:.1f} ms/step, wall {wall:.2f} s)")
 
+    # ---- quality of every grid cell against the noise-free mask, on the device
+    from miccai2016_amd.evaluation import DeviceDice
+    grid_dice = DeviceDice(truth, dev)(u, S)
+    best = int(np.nanargmax(grid_dice))
+    log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, decay={d_all[best]:.4g}")
+
     # ---- the same solves one at a time (device-resident) and through the API
     P = max(1, min(args.plain_solves, S))
     pick = np.linspace(0, S - 1, P).astype(int)
@@ -132,7 +143,7 @@ def main():
                                      shape=img.shape)
         t_dev += timed(torch, one)
         up = ud.detach().cpu().numpy()
-        dices.append(ph.fast_dice(ub[j], up >= 0.5))
+        dices.append(ph.dice_between(ub[j], up))
     per_dev = t_dev / P
     t_api = 0.0
     napi = min(P, 3)
@@ -145,25 +156,38 @@ def main():
     log(f"plain device-resident: {per_dev * 1e3:.2f} ms/solve; numpy API: {per_api * 1e3:.2f} "
         f"ms/solve; min Dice batched vs plain {min(dices):.5f}")
 
-    # ---- 256^3 single solve (bandwidth scale), synthetic inputs of that shape
+    # ---- 256^3 single solve (bandwidth scale): the 256^3 helix (tube 8, helix 64,
+    # pitch 96, noise 1.0) generated on the device with its centre-line frames
     big = None
     if not args.no_256:
-        shape = (256, 256, 256)
+        import ctypes as C
+        from miccai2016_amd import _lib
+        sp256 = ph.PLAIN_256
+        shape = sp256.shape
         nb = int(np.prod(shape))
-        g = torch.Generator(device=dev.device)
-        g.manual_seed(7)
-        qb = (torch.rand(nb * 3, generator=g, device=dev.device) * 3.0).contiguous()
-        img_b = torch.rand(nb, generator=g, device=dev.device)
-        cs_b = torch.rand(nb, generator=g, device=dev.device) * 0.5
-        ct_b = torch.rand(nb, generator=g, device=dev.device) * 0.5
-        al_b = torch.full((nb,), 0.02, device=dev.device)
+        L, h, st = dev.lib, dev.handle, dev.stream
+        img_b, qb = dev.empty(nb), dev.empty(3 * nb)
+        cs_b, ct_b, al_b, ub0 = dev.empty(nb), dev.empty(nb), dev.empty(nb), dev.empty(nb)
+        dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), sp256.r_tube, sp256.r_helix,
+                                     sp256.pitch, sp256.sigma, C.c_uint64(42), dev.ptr(img_b),
+                                     None, dev.ptr(qb), st), "mh_helix_phantom")
+        dev.check(L.mh_complete_frames(h, dev.ptr(qb), dev.ptr(qb), nb, st), "mh_complete_frames")
+        dev.check(L.mh_terminal_capacities(h, dev.ptr(img_b), nb, sp256.sigma, 1e-9,
+                                           dev.ptr(cs_b), dev.ptr(ct_b), st), "capacities")
+        dev.check(L.mh_grad_mag_sq(h, dev.ptr(img_b), _lib.dims3(shape), dev.ptr(al_b), st),
+                  "mh_grad_mag_sq")
+        dev.check(L.mh_nonterminal_weight(h, dev.ptr(al_b), nb, 10 ** -1.625, 10 ** 1.5,
+                                          dev.ptr(al_b), st), "mh_nonterminal_weight")
+        wk = dev.empty(int(L.mh_initial_u_work_floats(nb)))
+        dev.check(L.mh_initial_u(h, dev.ptr(img_b), nb, dev.ptr(ub0), dev.ptr(wk), st),
+                  "mh_initial_u")
         ub_ = torch.empty(nb, device=dev.device)
         p256 = cut_parameters(cfg)
         p256.num_iter = 1000
         c256 = CmfCutQAnisotropicGpu(p256, dev)
 
         def solve256():
-            ub_.copy_(img_b)
+            ub_.copy_(ub0)
             c256.execute_device(ub_, qb, al_b, cs_b, ct_b, const_sqevals=np.sqrt(eigs), shape=shape)
         solve256()
         t256 = timed(torch, solve256)
@@ -172,6 +196,35 @@ def main():
                "achieved_GBs": rate256 * BYTES_PER_VOXEL_ITER / 1e9}
         log(f"256^3 x 1000 iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s, "
             f"{big['achieved_GBs']:.0f} GB/s algorithmic")
+
+    # ---- serving: Poisson arrivals of synthetic helix cuts of mixed sizes
+    serve = None
+    if not args.no_serve:
+        from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
+        del u, ub
+        torch.cuda.empty_cache()
+        sp = cut_parameters(cfg)
+        srv = MaxFlowServer(sp, chunk=50).start()
+        log(f"serving {args.serve_requests} requests at {args.serve_qps} QPS "
+            f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
+        tickets = poisson_load(srv, synthetic_request, args.serve_qps, args.serve_requests,
+                               seed=11)
+        srv.stop(drain=True, timeout=600)
+        st = srv.stats()
+        errors = [t.error for t in tickets if t.error is not None]
+        dice = [t.dice for t in tickets if t.dice is not None]
+        serve = {"qps": args.serve_qps, "requests": args.serve_requests,
+                 "completed": st["completed"], "errors": len(errors),
+                 "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
+                 "voxel_iters_per_s": st.get("voxel_iters_per_s"),
+                 "peak_resident_jobs": srv.peak_resident,
+                 "peak_reserved_GiB": srv.peak_reserved / 2 ** 30,
+                 "budget_GiB": srv.budget / 2 ** 30, "chunk": srv.chunk,
+                 "median_dice_vs_truth": float(np.median(dice)) if dice else None,
+                 "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, "
+                             "helix/noise as the ragged set, (w, decay) from the grid"}
+        log(f"serving: p50 {serve['p50_s']:.3f} s, p99 {serve['p99_s']:.3f} s, "
+            f"{serve['voxel_iters_per_s']:.3e} voxel-iters/s, peak {srv.peak_resident} resident")
 
     # ---- CPU oracle on a bounded sample of the same workload
     cpu = None
@@ -214,7 +267,10 @@ def main():
                   "batched_speedup_vs_device": value / plain_rate,
                   "batched_speedup_vs_numpy_api": value / (n * args.iters / per_api),
                   "min_dice_batched_vs_plain": float(min(dices))},
+        "grid_quality": {"best_fast_dice": float(grid_dice[best]),
+                         "best_weighting": float(w_all[best]), "best_decay": float(d_all[best])},
         "plain_256": big,
+        "serving": serve,
         "prepare_s": t_prep,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

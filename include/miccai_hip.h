@@ -266,14 +266,31 @@ int mh_weighted_dirs(mh_ctx *ctx, const float *dirs3_dev, const float *resp_dev,
 int mh_soa_to_aos3(mh_ctx *ctx, const float *soa_dev, int64_t n, float *aos_dev, void *stream);
 /* Synthetic helix phantom for load generation: tube of radius r_tube around a
  * helix of radius r_helix and pitch `pitch` along axis 0, plus N(0, sigma^2)
- * noise from a counter-based hash of (seed, voxel). truth_dev may be NULL. No
- * counterpart in the reference (its tvtk voxeliser, helpers/helix_phantom.py,
- * draws a different, toric helix). */
+ * noise from a counter-based hash of (seed, voxel). truth_dev (u8) and
+ * tangent_dev (AoS x3: unit tangent of the closest centre-line point, in axis
+ * order) may be NULL. No counterpart in the reference (its tvtk voxeliser,
+ * helpers/helix_phantom.py, draws a different, toric helix). */
 int mh_helix_phantom(mh_ctx *ctx, const int64_t dims[3], float r_tube, float r_helix,
                      float pitch, float sigma, uint64_t seed, float *img_dev, uint8_t *truth_dev,
-                     void *stream);
+                     float *tangent_dev, void *stream);
 
 /* ------------------------------------------- resident jobs (continuous batching) */
+
+/* u0 = img / (max(img) + FLT_MIN) in FP32, a NaN in img giving a NaN maximum
+ * (replaces the host initialisation at continuous_cut_anisotropic_gpu.py:282-283
+ * for device-resident inputs). work: mh_initial_u_work_floats(n) floats. */
+int64_t mh_initial_u_work_floats(int64_t n);
+int mh_initial_u(mh_ctx *ctx, const float *img_dev, int64_t n, float *u_dev, float *work_dev,
+                 void *stream);
+
+/* Dice counts of nsolves labellings u (nsolves x n, u >= 0.5 is foreground)
+ * against one mask truth (n bytes, nonzero = foreground): counts_dev[3 s + k]
+ * = |S|, |S & T|, |T| for solve s, exact. The host forms
+ * 2 |S & T| / (|S| + |T|) as fast_dice_for does (phantom_experiment.py:36-47)
+ * without copying the labellings. work: mh_dice_work_bytes(n, nsolves). */
+int64_t mh_dice_work_bytes(int64_t n, int nsolves);
+int mh_dice_counts(mh_ctx *ctx, const float *u_dev, int nsolves, int64_t n, const uint8_t *truth_dev,
+                   void *work_dev, int64_t work_bytes, unsigned long long *counts_dev, void *stream);
 
 /* One unsupervised constant-eigenvalue solve whose state stays on the device
  * between calls, so jobs of different sizes can join and leave a running set

@@ -81,7 +81,11 @@ SIGNATURES = {
     "mh_minmax_normalize": (i32, [vp, vp, i64, vp, vp, vp]),
     "mh_weighted_dirs": (i32, [vp, vp, vp, i64, vp, vp]),
     "mh_soa_to_aos3": (i32, [vp, vp, i64, vp, vp]),
-    "mh_helix_phantom": (i32, [vp, I64x3, f32, f32, f32, f32, C.c_uint64, vp, vp, vp]),
+    "mh_initial_u_work_floats": (i64, [i64]),
+    "mh_initial_u": (i32, [vp, vp, i64, vp, vp, vp]),
+    "mh_dice_work_bytes": (i64, [i64, i32]),
+    "mh_dice_counts": (i32, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
+    "mh_helix_phantom": (i32, [vp, I64x3, f32, f32, f32, f32, C.c_uint64, vp, vp, vp, vp]),
     "mh_cmf_job_state_bytes": (i64, [i64, C.POINTER(CmfParams)]),
     "mh_cmf_jobs_init": (i32, [vp, C.POINTER(CmfJob), i32, vp]),
     "mh_cmf_jobs_step": (i32, [vp, C.POINTER(CmfJob), i32, i32, vp]),

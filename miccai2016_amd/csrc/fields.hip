@@ -55,6 +55,93 @@ nonterminal_kernel(const float *__restrict__ gms, int64_t n, float w, float k, f
     alpha[i] = w * expf(-gms[i] * k);
 }
 
+// Maximum that propagates NaN (numpy.max semantics).
+struct NanMaxOp {
+    __device__ static float identity() { return -INFINITY; }
+    __device__ static float combine(float a, float b) { return (a != a || b != b) ? NAN : fmaxf(a, b); }
+};
+
+template <class Op>
+__global__ void __launch_bounds__(mh::kBlock)
+reduce_partials_kernel(const float *__restrict__ x, int64_t n, float *__restrict__ part)
+{
+    float acc = Op::identity();
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        acc = Op::combine(acc, x[i]);
+    acc = mh::block_reduce<Op>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(mh::kBlock) reduce_final_kernel(float *__restrict__ part, int nparts)
+{
+    float acc = Op::identity();
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) acc = Op::combine(acc, part[k]);
+    acc = mh::block_reduce<Op>(acc);
+    if (threadIdx.x == 0) part[nparts] = acc;
+}
+
+// u0 = img / (max(img) + FLT_MIN), the division correctly rounded in FP32.
+__global__ void __launch_bounds__(mh::kBlock)
+initial_u_kernel(const float *__restrict__ img, int64_t n, const float *__restrict__ mx,
+                 float *__restrict__ u)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u[i] = img[i] / (mx[0] + FLT_MIN);
+}
+
+// Dice counts of S labellings (u >= 0.5) against one mask: per block
+// |S|, |S & T| and |T| as exact integers.
+__global__ void __launch_bounds__(mh::kBlock)
+dice_partials_kernel(const float *__restrict__ u, int64_t n, const uint8_t *__restrict__ truth,
+                     unsigned *__restrict__ part)
+{
+    const int s = blockIdx.y;
+    const float *us = u + int64_t(s) * n;
+    unsigned a = 0, both = 0, t = 0;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const bool in = us[i] >= 0.5f;
+        const bool tr = truth[i] != 0;
+        a += in;
+        both += in && tr;
+        t += tr;
+    }
+    __shared__ unsigned red[3][mh::kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        both += __shfl_xor(both, off);
+        t += __shfl_xor(t, off);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = a;
+        red[1][wid] = both;
+        red[2][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned acc = 0;
+        for (int w = 0; w < mh::kBlock / 64; ++w) acc += red[threadIdx.x][w];
+        part[(int64_t(s) * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+dice_final_kernel(const unsigned *__restrict__ part, int nparts, unsigned long long *__restrict__ out)
+{
+    const int s = blockIdx.x;
+    unsigned long long acc[3] = {0, 0, 0};
+    for (int k = threadIdx.x; k < nparts; k += 64)
+        for (int c = 0; c < 3; ++c) acc[c] += part[(int64_t(s) * nparts + k) * 3 + c];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 3; ++c) acc[c] += __shfl_xor(acc[c], off);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) out[3 * s + c] = acc[c];
+}
+
 struct MinOp {
     __device__ static float identity() { return INFINITY; }
     __device__ static float combine(float a, float b) { return fminf(a, b); }
@@ -146,7 +233,8 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
 
 __global__ void __launch_bounds__(mh::kBlock)
 helix_kernel(int d0, int d1, int d2, float r_tube, float r_helix, float pitch, float sigma,
-             uint64_t seed, float *__restrict__ img, uint8_t *__restrict__ truth)
+             uint64_t seed, float *__restrict__ img, uint8_t *__restrict__ truth,
+             float *__restrict__ tangent)
 {
     const int64_t n = int64_t(d0) * d1 * d2;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -186,6 +274,14 @@ helix_kernel(int d0, int d1, int d2, float r_tube, float r_helix, float pitch, f
     const float g = sqrtf(-2.f * logf(u1)) * cosf(6.28318530717958647692f * u2);
     img[i] = float(inside) + sigma * g;
     if (truth) truth[i] = inside;
+    if (tangent) {
+        // d/dt of (t, cy + Rh sin wt, cx + Rh cos wt) in axis order, unit length
+        const float t0 = 1.f, t1 = r_helix * w * c, t2 = -r_helix * w * s;
+        const float inv = 1.f / sqrtf(t0 * t0 + t1 * t1 + t2 * t2);
+        tangent[3 * i] = t0 * inv;
+        tangent[3 * i + 1] = t1 * inv;
+        tangent[3 * i + 2] = t2 * inv;
+    }
 }
 
 }  // namespace
@@ -272,13 +368,57 @@ MH_EXPORT int mh_soa_to_aos3(mh_ctx *ctx, const float *soa, int64_t n, float *ao
 
 MH_EXPORT int mh_helix_phantom(mh_ctx *ctx, const int64_t dims[3], float r_tube, float r_helix,
                                float pitch, float sigma, uint64_t seed, float *img, uint8_t *truth,
-                               void *stream)
+                               float *tangent, void *stream)
 {
     MH_REQUIRE(ctx, mh::dims_ok(dims) && img && pitch > 0.f && r_tube >= 0.f, "bad arguments");
     const int64_t n = dims[0] * dims[1] * dims[2];
     hipLaunchKernelGGL(helix_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0, (hipStream_t)stream,
                        int(dims[0]), int(dims[1]), int(dims[2]), r_tube, r_helix, pitch, sigma, seed,
-                       img, truth);
+                       img, truth, tangent);
     MH_CHECK_LAUNCH(ctx, "helix_kernel");
+    return MH_OK;
+}
+
+MH_EXPORT int64_t mh_initial_u_work_floats(int64_t n)
+{
+    if (n <= 0) return -1;
+    return int64_t(std::min<int64_t>(1024, mh::blocks_for(n))) + 1;
+}
+
+MH_EXPORT int mh_initial_u(mh_ctx *ctx, const float *img, int64_t n, float *u, float *work,
+                           void *stream)
+{
+    MH_REQUIRE(ctx, img && u && work && n > 0 && n < (int64_t(1) << 31), "bad arguments");
+    const unsigned nb = std::min<int64_t>(1024, mh::blocks_for(n));
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reduce_partials_kernel<NanMaxOp>, dim3(nb), dim3(mh::kBlock), 0, s, img, n, work);
+    hipLaunchKernelGGL(reduce_final_kernel<NanMaxOp>, dim3(1), dim3(mh::kBlock), 0, s, work, int(nb));
+    hipLaunchKernelGGL(initial_u_kernel, dim3(mh::blocks_for(n)), dim3(mh::kBlock), 0, s, img, n,
+                       work + nb, u);
+    MH_CHECK_LAUNCH(ctx, "initial_u");
+    return MH_OK;
+}
+
+MH_EXPORT int64_t mh_dice_work_bytes(int64_t n, int nsolves)
+{
+    if (n <= 0 || nsolves < 1) return -1;
+    const int64_t nb = std::min<int64_t>(256, mh::blocks_for(n));
+    return int64_t(nsolves) * nb * 3 * int64_t(sizeof(unsigned));
+}
+
+MH_EXPORT int mh_dice_counts(mh_ctx *ctx, const float *u, int nsolves, int64_t n,
+                             const uint8_t *truth, void *work, int64_t work_bytes,
+                             unsigned long long *counts_dev, void *stream)
+{
+    MH_REQUIRE(ctx, u && truth && work && counts_dev && nsolves >= 1 && nsolves <= 65535 && n > 0 &&
+               n < (int64_t(1) << 31), "bad arguments");
+    MH_REQUIRE(ctx, work_bytes >= mh_dice_work_bytes(n, nsolves), "workspace too small");
+    const unsigned nb = std::min<int64_t>(256, mh::blocks_for(n));
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(dice_partials_kernel, dim3(nb, unsigned(nsolves)), dim3(mh::kBlock), 0, s, u, n,
+                       truth, static_cast<unsigned *>(work));
+    hipLaunchKernelGGL(dice_final_kernel, dim3(unsigned(nsolves)), dim3(64), 0, s,
+                       static_cast<const unsigned *>(work), int(nb), counts_dev);
+    MH_CHECK_LAUNCH(ctx, "dice counts");
     return MH_OK;
 }

@@ -1,0 +1,80 @@
+"""Segmentation quality measures of the reference experiment
+(phantom_experiment.py:36-103), plus their device form for batched solves.
+
+``fast_dice_for`` thresholds at 0.5 and compares whole volumes;
+``select_largest`` keeps the largest face-connected component (scipy's
+default structure); ``dice_for`` is the Dice of that component.
+``DeviceDice`` counts |S|, |S & T| and |T| for many labellings on the GPU
+(exact integers) so a grid search's u fields never leave the device.
+"""
+import numpy as np
+import scipy.ndimage
+
+from . import _lib
+from .device import default_context
+
+
+def fast_dice_for(segmentation, ground_truth):
+    """2 |S & T| / (|S| + |T|), S = segmentation >= 0.5; NaN when both are empty
+    (phantom_experiment.py:36-47)."""
+    s = np.asarray(segmentation) >= 0.5
+    t = np.asarray(ground_truth)
+    den = float(np.sum(s)) + float(np.sum(t))
+    num = 2 * float(np.sum(np.multiply(s, t)))
+    return np.nan if den == 0 else num / den
+
+
+def select_largest(segmentation, return_size=False, threshold=0.5):
+    """The largest connected component of segmentation >= threshold (scipy
+    label, face connectivity); the first label wins ties. With no component:
+    (None, -1) (phantom_experiment.py:50-80)."""
+    labeled, n = scipy.ndimage.label(np.asarray(segmentation) >= threshold)
+    best, size = None, -1
+    if n:
+        sizes = np.bincount(labeled.ravel(), minlength=n + 1)[1:]
+        k = int(np.argmax(sizes))            # first of the largest
+        best, size = labeled == (k + 1), int(sizes[k])
+    return (best, size) if return_size else best
+
+
+def dice_for(segmentation, ground_truth):
+    """Dice of the largest component against ground_truth (phantom_experiment.py:83-103).
+    Raises ValueError when the segmentation has no foreground (the reference
+    fails there as well)."""
+    largest, size = select_largest(segmentation, return_size=True)
+    if largest is None:
+        raise ValueError("segmentation has no foreground component")
+    inter = float(np.sum(np.multiply(largest, ground_truth)))
+    return 2 * inter / (size + float(np.sum(ground_truth)))
+
+
+class DeviceDice:
+    """fast_dice_for of S labellings (device tensor S x N, float32) against one
+    mask (numpy or device uint8, N), reduced on the GPU."""
+
+    def __init__(self, truth, loader=None):
+        self.dev = loader if loader is not None else default_context()
+        dev = self.dev
+        if isinstance(truth, np.ndarray):
+            self.truth = dev.upload(np.asarray(truth) != 0, dtype=np.uint8).reshape(-1)
+        else:
+            self.truth = truth.reshape(-1)
+        self.n = self.truth.numel()
+
+    def counts(self, u, nsolves):
+        """(nsolves, 3) int64 array: |S|, |S & T|, |T| per labelling."""
+        dev = self.dev
+        assert u.numel() == nsolves * self.n
+        wb = int(dev.lib.mh_dice_work_bytes(self.n, int(nsolves)))
+        work = dev.bytes(wb)
+        out = dev.empty(3 * nsolves, dtype=dev.torch.int64)
+        dev.check(dev.lib.mh_dice_counts(dev.handle, dev.ptr(u), int(nsolves), self.n,
+                                         dev.ptr(self.truth), dev.ptr(work), wb, dev.ptr(out),
+                                         dev.stream), "mh_dice_counts")
+        return dev.download(out).reshape(nsolves, 3)
+
+    def __call__(self, u, nsolves=1):
+        c = self.counts(u, nsolves).astype(np.float64)
+        den = c[:, 0] + c[:, 2]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.where(den > 0, 2 * c[:, 1] / den, np.nan)

@@ -130,13 +130,15 @@ def grid_pairs():
 
 
 def fast_dice(seg, truth, threshold=0.5):
-    """2 |S & T| / (|S| + |T|) with S = seg >= threshold (phantom_experiment.py:38-49)."""
+    """2 |S & T| / (|S| + |T|) with S = seg >= threshold (phantom_experiment.py:36-47);
+    NaN when both are empty, as there."""
     b = np.asarray(seg) >= threshold
     t = np.asarray(truth) != 0
-    den = b.sum() + t.sum()
-    return 1.0 if den == 0 else 2.0 * np.logical_and(b, t).sum() / den
+    den = int(b.sum()) + int(t.sum())
+    return np.nan if den == 0 else 2.0 * int(np.logical_and(b, t).sum()) / den
 
 
 def dice_between(a, b, threshold=0.5):
-    """Dice overlap of two thresholded segmentations."""
-    return fast_dice(np.asarray(a), np.asarray(b) >= threshold, threshold)
+    """Dice overlap of two thresholded segmentations (1 when both are empty)."""
+    d = fast_dice(np.asarray(a), np.asarray(b) >= threshold, threshold)
+    return 1.0 if np.isnan(d) else d

@@ -1,0 +1,167 @@
+"""GPU tests of the resident-job solver and the continuous-batching server:
+a cut run in chunks beside other cuts of other shapes computes what it
+computes alone (max |d| <= 1e-5 against mh_cmf_solve, the same per-voxel
+arithmetic in the same order), admission never reserves more than the
+budget, and every request kind returns its result."""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from miccai2016_amd import _lib
+from miccai2016_amd import phantom as ph
+
+pytestmark = pytest.mark.gpu
+
+
+def _squeezed(rot):
+    from oracle import ref32
+    return ref32.qsqueeze(rot.as_quat()[..., [3, 0, 1, 2]].astype(np.float32))
+
+
+def _case(seed, shape, sigma=0.8):
+    spec = ph.HelixSpec(shape, 3, min(shape[1:]) / 4, 20, sigma, seed=seed)
+    img, truth = ph.make_helix(spec)
+    q = _squeezed(Rotation.random(int(np.prod(shape)), random_state=seed)).reshape(shape + (3,))
+    cs, ct = ph.terminal_capacities(img, sigma)
+    alpha = ph.nonterminal_weight(ph.grad_mag_sq(img), 10 ** -1.625, 10 ** 1.5)
+    return (img.astype(np.float32), q, alpha.astype(np.float32), cs.astype(np.float32),
+            ct.astype(np.float32), truth)
+
+
+def _params(num_iter=200, update=30, error_bound=0.0):
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    p = CmfCutQAnisotropicGpu.CutParameters()
+    p.num_iter, p.update, p.error_bound, p.steps = num_iter, update, error_bound, 0.16
+    return p
+
+
+def _alone(dev, case, eigs, num_iter, update=30, error_bound=0.0):
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    c = CmfCutQAnisotropicGpu(_params(num_iter, update, error_bound), dev)
+    u = c.execute(case[0], case[1], eigs, case[2], case[3], case[4], verbose=False)
+    return u, c.last_result
+
+
+SHAPES = [(16, 18, 20), (9, 31, 12), (24, 8, 17), (5, 5, 70)]
+EIGS = (1.0, 10 ** -1.5, 10 ** -1.5)
+
+
+def test_chunked_jobs_equal_lone_solves(dev):
+    from miccai2016_amd.serving import CutRequest, MaxFlowServer
+    cases = [_case(40 + k, s) for k, s in enumerate(SHAPES)]
+    iters = [200, 77, 131, 260]
+    srv = MaxFlowServer(_params(), chunk=37)
+    tickets = [srv.submit(CutRequest(c[0], c[1], c[2], c[3], c[4], EIGS, num_iter=it))
+               for c, it in zip(cases, iters)]
+    srv.run_until_idle()
+    assert srv.peak_resident == len(cases)
+    for c, it, t in zip(cases, iters, tickets):
+        u = t.wait(0)
+        want, res = _alone(dev, c, EIGS, it)
+        assert t.iterations == res.iterations == it and t.stopped == 0
+        assert np.abs(u - want).max() <= 1e-5
+
+
+def test_chunked_jobs_stop_like_lone_solves(dev):
+    from miccai2016_amd.serving import CutRequest, MaxFlowServer
+    cases = [_case(60 + k, s) for k, s in enumerate(SHAPES[:3])]
+    srv = MaxFlowServer(_params(num_iter=3000, update=7, error_bound=1e-3), chunk=50)
+    tickets = [srv.submit(CutRequest(c[0], c[1], c[2], c[3], c[4], (1, 1, 1))) for c in cases]
+    # a NaN frame stops its job at the first criterion, as it stops a lone solve
+    bad = list(cases[0])
+    bad[1] = bad[1].copy()
+    bad[1][3, 4, 5, 1] = np.nan
+    tickets.append(srv.submit(CutRequest(bad[0], bad[1], bad[2], bad[3], bad[4], (1, 1, 1))))
+    srv.run_until_idle()
+    for c, t in zip(cases + [bad], tickets):
+        u = t.wait(0)
+        want, res = _alone(dev, c, (1, 1, 1), 3000, 7, 1e-3)
+        assert (t.iterations, t.stopped) == (res.iterations, res.stopped)
+        if res.stopped == 2:
+            assert np.abs(u - want).max() <= 1e-5
+    assert tickets[-1].stopped == 1 and tickets[-1].iterations == 1
+
+
+def test_admission_respects_the_budget(dev):
+    from miccai2016_amd.serving import CutRequest, MaxFlowServer, request_footprint_bytes
+    cases = [_case(80 + k, s) for k, s in enumerate(SHAPES)]
+    reqs = [CutRequest(c[0], c[1], c[2], c[3], c[4], EIGS, num_iter=90) for c in cases]
+    need = [request_footprint_bytes(r, 90, 30) for r in reqs]
+    budget = max(need) + min(need) - 1         # never all four at once
+    srv = MaxFlowServer(_params(), budget_bytes=budget, chunk=20)
+    z = np.zeros((60, 60, 60), np.float32)
+    too_big = srv.submit(CutRequest(z, np.zeros(z.shape + (3,), np.float32), z, z, z))
+    with pytest.raises(MemoryError):
+        too_big.wait(0)
+    tickets = [srv.submit(r) for r in reqs]
+    srv.run_until_idle()
+    assert srv.peak_reserved <= budget and srv.peak_resident < len(reqs)
+    assert srv.reserved == 0
+    for c, t in zip(cases, tickets):
+        want, _ = _alone(dev, c, EIGS, 90)
+        assert np.abs(t.wait(0) - want).max() <= 1e-5
+
+
+def test_phantom_requests_equal_lone_solves(dev):
+    from miccai2016_amd.evaluation import fast_dice_for
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    from miccai2016_amd.serving import MaxFlowServer, PhantomCutRequest
+    reqs = [PhantomCutRequest((30, 26, 34), 4.0, 6.5, 30.0, 0.5, 0.1, 10.0, seed=7, num_iter=150),
+            PhantomCutRequest((20, 40, 22), 3.0, 5.0, 24.0, 0.9, 0.03, 3.0, seed=8, num_iter=90)]
+    srv = MaxFlowServer(_params(), chunk=40)
+    tickets = [srv.submit(r) for r in reqs]
+    srv.run_until_idle()
+    for r, t in zip(reqs, tickets):
+        u = t.wait(0)
+        n = int(np.prod(r.shape))
+        L, h, s = dev.lib, dev.handle, dev.stream
+        img, truth, tan = dev.empty(n), dev.empty(n, dtype=dev.torch.uint8), dev.empty(3 * n)
+        dev.check(L.mh_helix_phantom(h, _lib.dims3(r.shape), r.r_tube, r.r_helix, r.pitch,
+                                     r.sigma, C.c_uint64(r.seed), dev.ptr(img), dev.ptr(truth),
+                                     dev.ptr(tan), s))
+        dev.check(L.mh_complete_frames(h, dev.ptr(tan), dev.ptr(tan), n, s))
+        x = dev.download(img).reshape(r.shape)
+        cs, ct = ph.terminal_capacities(x.astype(np.float64), r.sigma)
+        alpha = ph.nonterminal_weight(ph.grad_mag_sq(x.astype(np.float64)), r.weighting, r.decay)
+        c = CmfCutQAnisotropicGpu(_params(r.num_iter), dev)
+        want = c.execute(x, dev.download(tan).reshape(r.shape + (3,)), r.eigs,
+                         alpha.astype(np.float32), cs.astype(np.float32), ct.astype(np.float32),
+                         verbose=False)
+        # the served fields are FP32 device formulas, the lone solve's float64 host ones
+        assert np.abs(u - want).max() <= 1e-3
+        assert ph.dice_between(u, want) >= 0.999
+        assert t.dice == pytest.approx(fast_dice_for(u, dev.download(truth).reshape(r.shape)),
+                                       abs=1e-12)
+
+
+def test_volume_requests_run_the_pipeline(dev):
+    from miccai2016_amd.pipeline import PipelineConfig, prepare, segment
+    from miccai2016_amd.serving import MaxFlowServer, VolumeRequest
+    spec = ph.HelixSpec((32, 36, 36), 3.0, 9.0, 20.0, 0.5, seed=21)
+    img, truth = ph.make_helix(spec)
+    cfg = PipelineConfig()
+    cfg.gvf_iters, cfg.cut_iters = 40, 120
+    srv = MaxFlowServer(_params(num_iter=120), chunk=30, prepare_cfg=cfg)
+    t = srv.submit(VolumeRequest(img, spec.sigma, 10 ** -1.625, 10 ** 1.5, truth=truth))
+    srv.run_until_idle()
+    u = t.wait(0)
+    host = segment(img, prepare(img, spec.sigma, cfg, dev), 10 ** -1.625, 10 ** 1.5, cfg, dev)
+    assert ph.dice_between(u, host) >= 0.99
+    assert t.dice == pytest.approx(ph.fast_dice(u, truth), abs=1e-12)
+    srv.stop()
+
+
+def test_poisson_load_on_a_background_thread(dev):
+    from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
+    srv = MaxFlowServer(_params(), chunk=25).start()
+    tickets = poisson_load(srv, lambda rng: synthetic_request(rng, side=(16, 40), iters=(30, 80)),
+                           qps=200.0, n_requests=12, seed=3)
+    srv.stop(drain=True, timeout=120)
+    assert all(t.done() and t.error is None for t in tickets)
+    assert all(t.iterations == t.request.num_iter for t in tickets)
+    st = srv.stats()
+    assert st["completed"] == 12 and st["p50_s"] <= st["p99_s"]
+    assert st["voxel_iterations"] == sum(int(np.prod(t.request.shape)) * t.iterations
+                                         for t in tickets)
