@@ -25,8 +25,8 @@ def test_terminal_capacities(dev, rng):
     img, _ = _img(rng)
     x = img.astype(np.float32)
     n = x.size
-    cs, ct = dev.empty(n), dev.empty(n)
-    dev.check(dev.lib.mh_terminal_capacities(dev.handle, dev.ptr(dev.upload(x)), n, 0.7, 1e-9,
+    cs, ct, x_t = dev.empty(n), dev.empty(n), dev.upload(x)
+    dev.check(dev.lib.mh_terminal_capacities(dev.handle, dev.ptr(x_t), n, 0.7, 1e-9,
                                              dev.ptr(cs), dev.ptr(ct), dev.stream))
     want_s, want_t = ph.terminal_capacities(x.astype(np.float64), 0.7)
     got_s, got_t = dev.download(cs).reshape(x.shape), dev.download(ct).reshape(x.shape)
@@ -39,8 +39,8 @@ def test_terminal_capacities(dev, rng):
 def test_grad_mag_sq_and_weight(dev, rng):
     for shape in [(17, 23, 30), (2, 9, 5), (1, 12, 7)]:
         x = rng.normal(size=shape).astype(np.float32)
-        g = dev.empty(x.size)
-        dev.check(dev.lib.mh_grad_mag_sq(dev.handle, dev.ptr(dev.upload(x)), _lib.dims3(shape),
+        g, x_t = dev.empty(x.size), dev.upload(x)
+        dev.check(dev.lib.mh_grad_mag_sq(dev.handle, dev.ptr(x_t), _lib.dims3(shape),
                                          dev.ptr(g), dev.stream))
         got = dev.download(g).reshape(shape)
         if shape[0] == 1:
@@ -63,7 +63,8 @@ def test_minmax_normalize(dev, rng):
     for n in [1, 1000, 3 * 2 ** 20 + 17]:
         x = (rng.normal(size=n) * 5 + 2).astype(np.float32)
         out, work = dev.empty(n), dev.empty(int(dev.lib.mh_normalize_work_floats(n)))
-        dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(dev.upload(x)), n, dev.ptr(out),
+        x_t = dev.upload(x)
+        dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(x_t), n, dev.ptr(out),
                                               dev.ptr(work), dev.stream))
         got = dev.download(out)
         want = normalize(x)
@@ -72,7 +73,8 @@ def test_minmax_normalize(dev, rng):
             assert got.min() == 0.0 and got[np.argmax(x)] == pytest.approx(1.0, abs=1e-6)
     x = np.full(77, 3.0, np.float32)      # empty range -> zeros, as normalize() gives
     out, work = dev.empty(77), dev.empty(int(dev.lib.mh_normalize_work_floats(77)))
-    dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(dev.upload(x)), 77, dev.ptr(out),
+    x_t = dev.upload(x)
+    dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(x_t), 77, dev.ptr(out),
                                           dev.ptr(work), dev.stream))
     assert (dev.download(out) == 0).all()
 
@@ -82,8 +84,9 @@ def test_layouts(dev, rng):
     d = rng.normal(size=(n, 3)).astype(np.float32)
     r = rng.uniform(size=n).astype(np.float32)
     v3 = dev.empty(3 * n)
-    dev.check(dev.lib.mh_weighted_dirs(dev.handle, dev.ptr(dev.upload(d)), dev.ptr(dev.upload(r)),
-                                       n, dev.ptr(v3), dev.stream))
+    d_t, r_t = dev.upload(d), dev.upload(r)     # keep the inputs alive across the launch
+    dev.check(dev.lib.mh_weighted_dirs(dev.handle, dev.ptr(d_t), dev.ptr(r_t), n, dev.ptr(v3),
+                                       dev.stream))
     want = np.ascontiguousarray((d * r[:, None]).T)
     assert np.array_equal(dev.download(v3).reshape(3, n), want)
     back = dev.empty(3 * n)
@@ -96,11 +99,13 @@ def test_initial_u_bit_exact(dev, rng):
     for n in [1, 4097, 2 ** 21 + 3]:
         x = rng.normal(size=n).astype(np.float32)
         u, work = dev.empty(n), dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
-        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(dev.upload(x)), n, dev.ptr(u),
+        x_t = dev.upload(x)
+        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), n, dev.ptr(u),
                                        dev.ptr(work), dev.stream))
         assert np.array_equal(dev.download(u), initial_u(x))
     x[123] = np.nan                       # numpy's max propagates NaN: u is NaN everywhere
-    dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(dev.upload(x)), x.size, dev.ptr(u),
+    x_t = dev.upload(x)
+    dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), x.size, dev.ptr(u),
                                    dev.ptr(work), dev.stream))
     assert np.isnan(dev.download(u)).all()
 
@@ -113,11 +118,12 @@ def test_dice_counts(dev, rng):
     u[1] = 0.0                            # empty labelling
     u[2] = np.where(truth, 0.5, 0.4999999)   # exactly the mask (0.5 counts as foreground)
     dd = DeviceDice(truth, dev)
-    c = dd.counts(dev.upload(u), S)
+    u_t = dev.upload(u)
+    c = dd.counts(u_t, S)
     for s in range(S):
         b = u[s] >= 0.5
         assert tuple(c[s]) == (b.sum(), (b & (truth != 0)).sum(), truth.sum())
-    got = dd(dev.upload(u), S)
+    got = dd(u_t, S)
     np.testing.assert_array_equal(got, [fast_dice_for(u[s], truth) for s in range(S)])
     assert got[2] == 1.0
     empty = DeviceDice(np.zeros(n, np.uint8), dev)(dev.upload(np.zeros(n, np.float32)), 1)
