@@ -227,6 +227,13 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
                          const float *cs_dev, const float *ct_dev, const int64_t dims[3],
                          const mh_cmf_params *params, void *work_dev, int64_t work_bytes,
                          mh_cmf_result *results_host, void *stream);
+/* Schedule of mh_cmf_solve_batched on this context (no counterpart in the
+ * reference): `group` solves (1..8) share one thread per voxel, and a group
+ * advances `visit` iterations before the next group takes its turn (visit >=
+ * num_iter runs the groups one after another). 0 selects the default: groups
+ * of 4 run one after another, which keeps a 128^3 group's state within the
+ * last-level cache. Results do not depend on the schedule, bit for bit. */
+int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
 
 /* Packed ragged batch: volume v has dims vdims_host[3v..3v+2] and its fields
  * start at element offset voff_host[v] in every packed array (u, eigs*3,

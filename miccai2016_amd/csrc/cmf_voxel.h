@@ -19,10 +19,23 @@ namespace mh {
 
 __device__ __forceinline__ float kmin(float x, float y) { return y < x ? y : x; }
 
+// Every helper below is evaluated without contraction into fused
+// multiply-adds (only the explicit fmaf calls fuse), so that each solver
+// variant, whatever the surrounding code, rounds identically.
+
+// divergence from the backward differences x_k of hp_k, scaled by rs
+__device__ __forceinline__ float cmf_div(float x0, float x1, float x2, float rs0, float rs1,
+                                         float rs2)
+{
+#pragma clang fp contract(off)
+    return x0 * rs0 + x1 * rs1 + x2 * rs2;
+}
+
 __device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, float g0, float g1,
                                             float g2, const float4 qq, float sq0, float sq1,
                                             float sq2, float alpha)
 {
+#pragma clang fp contract(off)
     const Mat3 R = quat_to_mat(Quat{qq.x, qq.y, qq.z, qq.w});
     const float *m = R.m;
     float p0 = (m[0] * h0 + m[3] * h1 + m[6] * h2) / sq0 + sq0 * (m[0] * g0 + m[3] * g1 + m[6] * g2);
@@ -42,6 +55,7 @@ __device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, flo
 __device__ __forceinline__ float cmf_u_voxel(float div, float &pt, float &u, float &tmp, float cs,
                                              float ct, float rcc, float ub, float uf)
 {
+#pragma clang fp contract(off)
     const float ps = kmin(fmaf(-rcc, u - ub, div + pt), cs);
     pt = kmin(fmaf(rcc, u - uf, ps - div), ct);
     const float err = div - ps + pt;

@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(mh::kBlock) jobs_u_kernel(const JobDesc *__res
         const float x0 = (i0 < D.d0 - 1 ? hp0[i] : 0.f) - (i0 > 0 ? hp0[i - s0] : 0.f);
         const float x1 = (i1 < D.d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - D.d2] : 0.f);
         const float x2 = (i2 < D.d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f);
-        const float div = x0 * D.rs0 + x1 * D.rs1 + x2 * D.rs2;
+        const float div = mh::cmf_div(x0, x1, x2, D.rs0, D.rs1, D.rs2);
         float ui = D.u[i], pti = D.pt[i], ti;
         const float err = mh::cmf_u_voxel(div, pti, ui, ti, D.cs[i], D.ct[i], D.rcc, 1.f, 0.f);
         D.pt[i] = pti;

@@ -130,7 +130,7 @@ cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float 
         const float x0 = (i0 < D.d0 - 1 ? hp0[i] : 0.f) - (i0 > 0 ? hp0[i - s0] : 0.f);
         const float x1 = (i1 < D.d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - D.d2] : 0.f);
         const float x2 = (i2 < D.d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f);
-        const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
+        const float div = mh::cmf_div(x0, x1, x2, P.rs0, P.rs1, P.rs2);
         float ub = 1.f, uf = 0.f;
         if (SUP) {
             const uint8_t l = labels[i];
@@ -335,10 +335,15 @@ quat_kernel(float4 *__restrict__ q4, const float *__restrict__ eigs, int64_t n)
     q4[i] = make_float4(q.w, q.x, q.y, q.z);
 }
 
+// NS solves per thread. Every load of all NS solves is issued before any
+// arithmetic (the addresses of absent neighbours fall back to the voxel
+// itself, and the selects below restore the boundary rules), so a thread
+// keeps 7 NS loads in flight; a stopped solve is computed but not stored.
+template <int NS>
 __global__ void __launch_bounds__(mh::kBlock)
 cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
                    const float4 *__restrict__ q4, const float *__restrict__ gms, const Dims D,
-                   const PParams P, const SolveParams SP, int s0, int ns, const int *__restrict__ stop)
+                   const PParams P, const SolveParams SP, int s0, const int *__restrict__ stop)
 {
     const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
     const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
@@ -348,67 +353,91 @@ cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
     coords(i, D, i0, i1, i2);
     const int64_t st0 = int64_t(D.d1) * D.d2;
     const bool e0 = i0 < D.d0 - 1, e1 = i1 < D.d1 - 1, e2 = i2 < D.d2 - 1;
+    const int64_t j0 = e0 ? i + st0 : i, j1 = e1 ? i + D.d2 : i, j2 = e2 ? i + 1 : i;
     const float4 qq = q4[i];
     const float gm = gms[i];
+    float t[NS], a0[NS], a1[NS], a2[NS], h0[NS], h1[NS], h2[NS];
 #pragma unroll
-    for (int k = 0; k < kSI; ++k) {
-        const int s = s0 + k;
-        if (k >= ns || stop[s]) continue;  // uniform over the block
-        const float *ts = tmp + n * s;
-        float *h = hp + 3 * n * s;
-        const float t = ts[i];
-        const float g0 = e0 ? (ts[i + st0] - t) * P.ss0 : 0.f;
-        const float g1 = e1 ? (ts[i + D.d2] - t) * P.ss1 : 0.f;
-        const float g2 = e2 ? (ts[i + 1] - t) * P.ss2 : 0.f;
-        float h0 = h[i], h1 = h[n + i], h2 = h[2 * n + i];
-        mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, qq, P.sq0, P.sq1, P.sq2, SP.w[k] * expf(-gm * SP.k[k]));
-        h[i] = h0;
-        h[n + i] = h1;
-        h[2 * n + i] = h2;
+    for (int k = 0; k < NS; ++k) {
+        const float *ts = tmp + n * (s0 + k);
+        const float *h = hp + 3 * n * (s0 + k);
+        t[k] = ts[i];
+        a0[k] = ts[j0];
+        a1[k] = ts[j1];
+        a2[k] = ts[j2];
+        h0[k] = h[i];
+        h1[k] = h[n + i];
+        h2[k] = h[2 * n + i];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        if (stop[s0 + k]) continue;  // uniform over the grid
+        const float g0 = e0 ? (a0[k] - t[k]) * P.ss0 : 0.f;
+        const float g1 = e1 ? (a1[k] - t[k]) * P.ss1 : 0.f;
+        const float g2 = e2 ? (a2[k] - t[k]) * P.ss2 : 0.f;
+        mh::cmf_p_voxel(h0[k], h1[k], h2[k], g0, g1, g2, qq, P.sq0, P.sq1, P.sq2,
+                        SP.w[k] * expf(-gm * SP.k[k]));
+        float *h = hp + 3 * n * (s0 + k);
+        h[i] = h0[k];
+        h[n + i] = h1[k];
+        h[2 * n + i] = h2[k];
     }
 }
 
-template <bool CHECK>
+template <int NS, bool CHECK>
 __global__ void __launch_bounds__(mh::kBlock)
 cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
                    const float *__restrict__ cs, const float *__restrict__ ct,
                    float *__restrict__ pt, float *__restrict__ u, const Dims D, const UParams P,
-                   int s0, int ns, BatchCtrl C, float *__restrict__ partials, int group)
+                   int s0, BatchCtrl C, float *__restrict__ partials)
 {
     const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
     const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
     const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
-    float aerr[kSI];
-    for (int k = 0; k < kSI; ++k) aerr[k] = 0.f;
+    float aerr[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) aerr[k] = 0.f;
     if (i < n) {
         int i0, i1, i2;
         coords(i, D, i0, i1, i2);
         const int64_t st0 = int64_t(D.d1) * D.d2;
         const bool e0 = i0 < D.d0 - 1, e1 = i1 < D.d1 - 1, e2 = i2 < D.d2 - 1;
         const bool f0 = i0 > 0, f1 = i1 > 0, f2 = i2 > 0;
+        const int64_t k0 = f0 ? i - st0 : i, k1 = f1 ? i - D.d2 : i, k2 = f2 ? i - 1 : i;
         const float csi = cs[i], cti = ct[i];
+        float ha0[NS], ha1[NS], ha2[NS], hb0[NS], hb1[NS], hb2[NS], uu[NS], pp[NS];
 #pragma unroll
-        for (int k = 0; k < kSI; ++k) {
-            const int s = s0 + k;
-            if (k >= ns || C.stop[s]) continue;  // uniform over the block
-            const float *h = hp + 3 * n * s;
-            const float x0 = (e0 ? h[i] : 0.f) - (f0 ? h[i - st0] : 0.f);
-            const float x1 = (e1 ? h[n + i] : 0.f) - (f1 ? h[n + i - D.d2] : 0.f);
-            const float x2 = (e2 ? h[2 * n + i] : 0.f) - (f2 ? h[2 * n + i - 1] : 0.f);
-            const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
-            float ui = u[n * s + i], pti = pt[n * s + i], ti;
-            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csi, cti, P.rcc, 1.f, 0.f);
-            pt[n * s + i] = pti;
-            u[n * s + i] = ui;
-            tmp[n * s + i] = ti;
+        for (int k = 0; k < NS; ++k) {
+            const float *h = hp + 3 * n * (s0 + k);
+            ha0[k] = h[i];
+            hb0[k] = h[k0];
+            ha1[k] = h[n + i];
+            hb1[k] = h[n + k1];
+            ha2[k] = h[2 * n + i];
+            hb2[k] = h[2 * n + k2];
+            uu[k] = u[n * (s0 + k) + i];
+            pp[k] = pt[n * (s0 + k) + i];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            if (C.stop[s0 + k]) continue;  // uniform over the grid
+            const float x0 = (e0 ? ha0[k] : 0.f) - (f0 ? hb0[k] : 0.f);
+            const float x1 = (e1 ? ha1[k] : 0.f) - (f1 ? hb1[k] : 0.f);
+            const float x2 = (e2 ? ha2[k] : 0.f) - (f2 ? hb2[k] : 0.f);
+            const float div = mh::cmf_div(x0, x1, x2, P.rs0, P.rs1, P.rs2);
+            float ti;
+            const float err = mh::cmf_u_voxel(div, pp[k], uu[k], ti, csi, cti, P.rcc, 1.f, 0.f);
+            const int64_t o = n * (s0 + k) + i;
+            pt[o] = pp[k];
+            u[o] = uu[k];
+            tmp[o] = ti;
             aerr[k] = fabsf(err);
         }
     }
     if (!CHECK) return;
     const unsigned nbx = gridDim.x;
 #pragma unroll
-    for (int k = 0; k < kSI; ++k) {
-        if (k >= ns) continue;  // uniform
+    for (int k = 0; k < NS; ++k) {
         const float v = mh::block_reduce<SumOp>(aerr[k]);
         if (threadIdx.x == 0) partials[int64_t(s0 + k) * nbx + blockIdx.x] = v;
     }
@@ -417,9 +446,9 @@ cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
 // One work-group per solve: criterion from its partials, log, stop flag.
 __global__ void __launch_bounds__(mh::kBlock)
 cmf_batch_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
-                      float error_bound, BatchCtrl C)
+                      float error_bound, BatchCtrl C, int s0)
 {
-    const int s = blockIdx.x;
+    const int s = s0 + blockIdx.x;
     if (C.stop[s]) return;
     float acc = 0.f;
     for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += partials[int64_t(s) * nparts + q];
@@ -464,6 +493,40 @@ int64_t batch_layout(int64_t n, int S, int maxcheck, char *base, BatchWorkspace 
     ws.maxcheck = maxcheck;
     if (w) *w = ws;
     return off;
+}
+
+template <int NS>
+void launch_batch_ns(bool chk, unsigned nb, hipStream_t s, const BatchWorkspace &W,
+                     const float *gms, const float *cs, const float *ct, float *u, const Dims &D,
+                     const PParams &PP, const UParams &UP, const SolveParams &sp, int s0,
+                     const BatchCtrl &C)
+{
+    hipLaunchKernelGGL(cmf_batch_p_kernel<NS>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
+                       gms, D, PP, sp, s0, W.stop);
+    if (chk)
+        hipLaunchKernelGGL((cmf_batch_u_kernel<NS, true>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
+                           W.tmp, cs, ct, W.pt, u, D, UP, s0, C, W.partials);
+    else
+        hipLaunchKernelGGL((cmf_batch_u_kernel<NS, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
+                           W.tmp, cs, ct, W.pt, u, D, UP, s0, C, W.partials);
+}
+
+// p- and u-step of one group of ns (1..kSI) solves starting at solve s0.
+void launch_batch_group(int ns, bool chk, unsigned nb, hipStream_t s, const BatchWorkspace &W,
+                        const float *gms, const float *cs, const float *ct, float *u,
+                        const Dims &D, const PParams &PP, const UParams &UP,
+                        const SolveParams &sp, int s0, const BatchCtrl &C)
+{
+    switch (ns) {
+    case 1: launch_batch_ns<1>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 2: launch_batch_ns<2>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 3: launch_batch_ns<3>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 4: launch_batch_ns<4>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 5: launch_batch_ns<5>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 6: launch_batch_ns<6>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    case 7: launch_batch_ns<7>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    default: launch_batch_ns<kSI>(chk, nb, s, W, gms, cs, ct, u, D, PP, UP, sp, s0, C); break;
+    }
 }
 
 int maxcheck_for(const mh_cmf_params *p)
@@ -513,33 +576,35 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
     const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
                      float(1.0 / (double(n) * double(prm->rcc)))};
-    std::vector<SolveParams> sp(groups);
-    for (int g = 0; g < groups; ++g)
-        for (int k = 0; k < kSI; ++k) {
-            const int j = std::min(g * kSI + k, S - 1);
-            sp[g].w[k] = w_host[j];
-            sp[g].k[k] = float(1.0 / (double(decay_host[j]) * double(decay_host[j])));
-        }
     const BatchCtrl C{W.stop, W.ncrit, W.count, W.crit_log, maxcheck};
-    for (int it = 0; it < prm->num_iter; ++it) {
-        const bool chk = (it % prm->update) == 0;
-        for (int g = 0; g < groups; ++g) {
-            const int s0 = g * kSI, ns = std::min(kSI, S - s0);
-            hipLaunchKernelGGL(cmf_batch_p_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
-                               gms, D, PP, sp[g], s0, ns, W.stop);
+    // Schedule: groups of G solves (one thread handles a voxel of all G); a
+    // group advances V iterations before the next group takes its turn. V =
+    // num_iter runs the groups one after another, so a group's state stays in
+    // the last-level cache between its iterations when it fits there.
+    int G = ctx->batch_group, V = ctx->batch_visit;
+    if (G <= 0) G = 4;
+    if (V <= 0) V = std::max(1, prm->num_iter);
+    G = std::min(G, kSI);
+    const int ngroups = (S + G - 1) / G;
+    std::vector<SolveParams> gp(ngroups);
+    for (int g = 0; g < ngroups; ++g)
+        for (int k = 0; k < kSI; ++k) {
+            const int j = std::min(g * G + k, S - 1);
+            gp[g].w[k] = w_host[j];
+            gp[g].k[k] = float(1.0 / (double(decay_host[j]) * double(decay_host[j])));
         }
-        for (int g = 0; g < groups; ++g) {
-            const int s0 = g * kSI, ns = std::min(kSI, S - s0);
-            if (chk)
-                hipLaunchKernelGGL(cmf_batch_u_kernel<true>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
-                                   cs, ct, W.pt, u, D, UP, s0, ns, C, W.partials, g);
-            else
-                hipLaunchKernelGGL(cmf_batch_u_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
-                                   W.tmp, cs, ct, W.pt, u, D, UP, s0, ns, C, W.partials, g);
+    for (int base = 0; base < prm->num_iter; base += V) {
+        const int last = std::min(prm->num_iter, base + V);
+        for (int g = 0; g < ngroups; ++g) {
+            const int s0 = g * G, ns = std::min(G, S - s0);
+            for (int it = base; it < last; ++it) {
+                const bool chk = (it % prm->update) == 0;
+                launch_batch_group(ns, chk, nb, s, W, gms, cs, ct, u, D, PP, UP, gp[g], s0, C);
+                if (chk)
+                    hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(ns), dim3(mh::kBlock), 0, s,
+                                       W.partials, int(nb), UP.inv_n_rcc, UP.error_bound, C, s0);
+            }
         }
-        if (chk)
-            hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(S), dim3(mh::kBlock), 0, s, W.partials, int(nb),
-                               UP.inv_n_rcc, UP.error_bound, C);
         MH_CHECK_LAUNCH(ctx, "batched cmf kernels");
     }
     std::vector<int> stop(2 * S);
@@ -658,7 +723,7 @@ ragged_u_kernel(const Vol *__restrict__ vols, int nvol, const float *__restrict_
         const float x0 = (i0 < V.d0 - 1 ? h[li] : 0.f) - (i0 > 0 ? h[li - st0] : 0.f);
         const float x1 = (i1 < V.d1 - 1 ? h[n + li] : 0.f) - (i1 > 0 ? h[n + li - V.d2] : 0.f);
         const float x2 = (i2 < V.d2 - 1 ? h[2 * n + li] : 0.f) - (i2 > 0 ? h[2 * n + li - 1] : 0.f);
-        const float div = x0 * P.rs0 + x1 * P.rs1 + x2 * P.rs2;
+        const float div = mh::cmf_div(x0, x1, x2, P.rs0, P.rs1, P.rs2);
         float ui = u[i], pti = pt[i], ti;
         const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, 1.f, 0.f);
         pt[i] = pti;
@@ -810,4 +875,12 @@ MH_EXPORT int mh_cmf_solve_ragged(mh_ctx *ctx, float *u, const float *eigs, cons
         }
     }
     return any_nan ? MH_STOPPED_NAN : MH_OK;
+}
+
+MH_EXPORT int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit)
+{
+    MH_REQUIRE(ctx, group >= 0 && group <= kSI && visit >= 0, "group in 0..8, visit >= 0");
+    ctx->batch_group = group;
+    ctx->batch_visit = visit;
+    return MH_OK;
 }

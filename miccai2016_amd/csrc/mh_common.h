@@ -29,6 +29,10 @@ struct mh_ctx {
     // job descriptor table of mh_cmf_jobs_step (grown on demand)
     void *desc = nullptr;
     size_t desc_bytes = 0;
+    // schedule of mh_cmf_solve_batched: solves per thread group and
+    // iterations per visit of a group (0: chosen automatically)
+    int batch_group = 0;
+    int batch_visit = 0;
 };
 
 namespace mh {
@@ -118,6 +122,7 @@ struct Mat3 {
 
 __device__ __forceinline__ Mat3 quat_to_mat(const Quat q)
 {
+#pragma clang fp contract(off)
     const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
     const float wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
     const float xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;

@@ -424,6 +424,28 @@ def test_batched_solves_equal_plain_solves(dev, rng):
         assert ph.dice_between(ub[s], up) >= 0.999
 
 
+def test_batched_schedules_give_identical_results(dev, rng):
+    """Every (group, visit) schedule computes the same iterations, bit for bit."""
+    img, q, _, cs, ct, gms = _cut_case(rng, (14, 19, 22))
+    w = np.geomspace(0.003, 3.0, 11).astype(np.float32)
+    d = np.geomspace(0.1, 1000.0, 11).astype(np.float32)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    outs = []
+    try:
+        for group, visit in [(0, 0), (1, 1), (3, 7), (4, 10 ** 6), (8, 13)]:
+            dev.check(dev.lib.mh_cmf_batched_set_schedule(dev.handle, group, visit))
+            c = _cutter(dev, 95, update=10, error_bound=2e-3)
+            outs.append(c.execute_batched(img, q, eigs, w, d, gms.astype(np.float32), cs, ct))
+            res = [(r.iterations, r.stopped) for r in c.last_result]
+            if group == 0:
+                res0 = res
+            assert res == res0
+    finally:
+        dev.check(dev.lib.mh_cmf_batched_set_schedule(dev.handle, 0, 0))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def test_ragged_solves_equal_plain_solves(dev, rng):
     shapes = [(9, 12, 15), (20, 7, 11), (16, 16, 16), (5, 33, 8)]
     cases = [_cut_case(rng, s) for s in shapes]
@@ -433,7 +455,7 @@ def test_ragged_solves_equal_plain_solves(dev, rng):
                           [k[2] for k in cases], [k[3] for k in cases], [k[4] for k in cases])
     for k, u in zip(cases, us):
         up = _cutter(dev, 150).execute(k[0], k[1], eigs, k[2], k[3], k[4], verbose=False)
-        assert np.abs(u - up).max() <= 1e-4
+        assert np.array_equal(u, up)
 
 
 def test_fill_and_reductions(dev, rng):

@@ -1,8 +1,8 @@
 """GPU tests of the resident-job solver and the continuous-batching server:
 a cut run in chunks beside other cuts of other shapes computes what it
-computes alone (max |d| <= 1e-5 against mh_cmf_solve, the same per-voxel
-arithmetic in the same order), admission never reserves more than the
-budget, and every request kind returns its result."""
+computes alone, bit for bit (the same per-voxel arithmetic in the same
+order, without contraction into fused multiply-adds), admission never
+reserves more than the budget, and every request kind returns its result."""
 import ctypes as C
 
 import numpy as np
@@ -61,7 +61,7 @@ def test_chunked_jobs_equal_lone_solves(dev):
         u = t.wait(0)
         want, res = _alone(dev, c, EIGS, it)
         assert t.iterations == res.iterations == it and t.stopped == 0
-        assert np.abs(u - want).max() <= 1e-5
+        assert np.array_equal(u, want)
 
 
 def test_chunked_jobs_stop_like_lone_solves(dev):
@@ -80,7 +80,7 @@ def test_chunked_jobs_stop_like_lone_solves(dev):
         want, res = _alone(dev, c, (1, 1, 1), 3000, 7, 1e-3)
         assert (t.iterations, t.stopped) == (res.iterations, res.stopped)
         if res.stopped == 2:
-            assert np.abs(u - want).max() <= 1e-5
+            assert np.array_equal(u, want)
     assert tickets[-1].stopped == 1 and tickets[-1].iterations == 1
 
 
@@ -101,7 +101,7 @@ def test_admission_respects_the_budget(dev):
     assert srv.reserved == 0
     for c, t in zip(cases, tickets):
         want, _ = _alone(dev, c, EIGS, 90)
-        assert np.abs(t.wait(0) - want).max() <= 1e-5
+        assert np.array_equal(t.wait(0), want)
 
 
 def test_phantom_requests_equal_lone_solves(dev):
