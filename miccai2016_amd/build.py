@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast", "-Wall", "-Wno-unused-result",
+    "-ffp-contract=fast-honor-pragmas", "-Wall", "-Wno-unused-result",
 ]
 
 

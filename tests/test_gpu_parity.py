@@ -417,11 +417,21 @@ def test_batched_solves_equal_plain_solves(dev, rng):
     ub = c.execute_batched(img, q, eigs, w, d, gms.astype(np.float32), cs, ct)
     assert ub.shape == (10,) + img.shape
     assert all(r.iterations == 200 for r in c.last_result)
+    g32 = gms.astype(np.float32)
+    g_t = dev.upload(g32)
+    a_t = dev.empty(g32.size)
     for s in range(10):
-        alpha = ph.nonterminal_weight(gms.astype(np.float32), w[s], d[s]).astype(np.float32)
+        alpha = ph.nonterminal_weight(g32, w[s], d[s]).astype(np.float32)
         up = _cutter(dev, 200).execute(img, q, eigs, alpha, cs, ct, verbose=False)
         assert np.abs(ub[s] - up).max() <= 1e-4
         assert ph.dice_between(ub[s], up) >= 0.999
+        # with the weight formed on the device as the batched solver forms it, the
+        # plain solve is the batched one bit for bit
+        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(g_t), g32.size, float(w[s]),
+                                                float(d[s]), dev.ptr(a_t), dev.stream))
+        a_dev = dev.download(a_t).reshape(img.shape)
+        up = _cutter(dev, 200).execute(img, q, eigs, a_dev, cs, ct, verbose=False)
+        assert np.array_equal(ub[s], up)
 
 
 def test_batched_schedules_give_identical_results(dev, rng):
