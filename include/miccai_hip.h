@@ -234,6 +234,16 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * of 4 run one after another, which keeps a 128^3 group's state within the
  * last-level cache. Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
+/* Two-level sweep of mh_cmf_solve on this context (no counterpart in the
+ * reference): unsupervised constant-eigenvalue solves may advance two
+ * iterations per pass over the state (temporal blocking along axis 0). mode 0
+ * (default) tunes: the first solve of a volume shape of >= 64 iterations times
+ * the plain kernels and a few sweep tile shapes on its own iterations and the
+ * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
+ * sweeps when the solve is eligible, with tile (1-based index of the built-in
+ * tile shapes, 0 the first) and zchunk (planes per work-group, 0: 16).
+ * Results do not depend on any of them, bit for bit. */
+int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk);
 
 /* Packed ragged batch: volume v has dims vdims_host[3v..3v+2] and its fields
  * start at element offset voff_host[v] in every packed array (u, eigs*3,

@@ -19,7 +19,7 @@
 namespace {
 
 // control block (int32 words) at the head of the solver workspace
-enum : int { kStop = 0, kNcrit = 1, kCounter = 2, kCtrlWords = 64 };
+enum : int { kStop = 0, kNcrit = 1, kCounter = 2, kParity = 3, kCtrlWords = 64 };
 constexpr int kCritLogMax = 1 << 16;
 
 struct SumOp {
@@ -168,6 +168,295 @@ cmf_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
     }
 }
 
+// ----------------------------------------------------- two-level sweep ---
+// Two iterations (levels A and B) in one pass over the state, temporally
+// blocked along axis 0. A work-group owns an output tile O of TY x TX voxels
+// in the (axis 1, axis 2) plane and a chunk of planes [z0, z1), and sweeps z
+// upward. At step z it computes
+//   1. the p-step of level A at plane z on O widened by (2, 1) voxels below
+//      and above in both in-plane axes (region E3),
+//   2. the u-step of level A at plane z on E3 minus its first row/column (E2),
+//   3. the p-step of level B at plane z-1 on O widened by one below (E1),
+//   4. the u-step of level B at plane z-1 on O, and stores the result,
+// so the in-plane neighbours each step needs (p: +1, u: -1) come from the
+// widened regions through LDS, and the axis-0 neighbours (plane z+1 for p,
+// plane z-1 for u) from registers carried between steps. A thread owns the
+// same in-plane points at every step. Level A reads the state of iteration t
+// from one buffer set and level B writes iteration t+2 to the other, so
+// work-groups never see each other's outputs within a pass. With levels = 1
+// only steps 1 and 2 run and step 2 stores. The per-voxel arithmetic is the
+// shared helpers', so the result equals two launches of the plain kernels bit
+// for bit.
+// Output tiles (axis 1 x axis 2) and work-group sizes: the widened tile
+// (TY + 3) x (TX + 3) has one point per thread.
+struct SweepShape {
+    int ty, tx, threads;
+};
+constexpr SweepShape kSweepShapes[] = {{12, 64, 1024}, {11, 32, 512}, {4, 64, 512}, {26, 32, 1024},
+                                       {13, 61, 1024}};
+constexpr int kNumSweepShapes = 5;
+
+// a way to run iterations: shape < 0 the plain kernels, else a two-level
+// sweep with that tile shape and plane chunk
+struct SweepMethod {
+    int shape, zchunk;
+};
+// candidates the tuner times on a solve of new dimensions
+constexpr SweepMethod kSweepCandidates[] = {{-1, 0}, {0, 16}, {0, 32}, {1, 8}, {1, 16}, {2, 16}, {4, 16}};
+
+struct SweepBufs {
+    const float *hp, *tmp, *pt, *u;  // iteration t
+    float *hpo, *tmpo, *pto, *uo;    // iteration t + levels
+};
+
+// Inputs of one point at one plane, loaded a step ahead of their use.
+struct SweepIn {
+    float t, t0, t1, t2;  // tmp at the voxel and its +axis-0/1/2 neighbours
+    float h0, h1, h2;     // hp
+    float4 q;
+    float a, pt, u, cs, ct;
+};
+
+// Plane-uniform base pointer plus a 32-bit byte offset: the loads and stores
+// then take the scalar-base + vector-offset form and keep one VGPR per
+// address instead of two.
+template <class T>
+__device__ __forceinline__ T ld(const T *base, unsigned byte_off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ void st(float *base, unsigned byte_off, float v)
+{
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
+__device__ __forceinline__ void sweep_load(SweepIn &L, const SweepBufs &B,
+                                           const float4 *__restrict__ q4,
+                                           const float *__restrict__ alpha,
+                                           const float *__restrict__ cs,
+                                           const float *__restrict__ ct, int64_t n, int64_t st0,
+                                           int d2, int64_t zoff, unsigned ob, bool p_on, bool u_on,
+                                           bool e0, unsigned f)
+{
+    if (p_on) {
+        const float *tz = B.tmp + zoff;
+        L.t = ld(tz, ob);
+        L.t0 = e0 ? ld(tz + st0, ob) : 0.f;
+        L.t1 = (f & 16u) ? ld(tz, ob + 4u * unsigned(d2)) : 0.f;
+        L.t2 = (f & 32u) ? ld(tz, ob + 4u) : 0.f;
+        L.h0 = ld(B.hp + zoff, ob);
+        L.h1 = ld(B.hp + n + zoff, ob);
+        L.h2 = ld(B.hp + 2 * n + zoff, ob);
+        L.q = ld(q4 + zoff, 4u * ob);
+        L.a = ld(alpha + zoff, ob);
+    }
+    if (u_on) {
+        L.pt = ld(B.pt + zoff, ob);
+        L.u = ld(B.u + zoff, ob);
+        L.cs = ld(cs + zoff, ob);
+        L.ct = ld(ct + zoff, ob);
+    }
+}
+
+template <int TY, int TX, int NT>
+__global__ void __launch_bounds__(NT)
+cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__restrict__ alpha,
+                 const float *__restrict__ cs, const float *__restrict__ ct, const Dims D,
+                 const PParams PP, const UParams UP, int zchunk, int levels, int check,
+                 int out_set, int *__restrict__ ctrl, float *__restrict__ partials)
+{
+    if (ctrl[kStop]) return;
+    constexpr int EY = TY + 3, EX = TX + 3, NE = EY * EX;
+    static_assert(NE <= NT, "one point per thread");
+    // axis-1 and axis-2 components of the hp planes (the axis-0 component is
+    // only needed at the thread's own voxel), and level-A tmp of plane z-1
+    __shared__ float sA1[NE], sA2[NE];  // level-A hp at plane z
+    __shared__ float sT[NE];            // level-A tmp at plane z-1
+    __shared__ float sB1[NE], sB2[NE];  // level-B hp at plane z-1
+
+    const int ntx = (D.d2 + TX - 1) / TX, nty = (D.d1 + TY - 1) / TY;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int tx = int(b % unsigned(ntx)), ty = int((b / unsigned(ntx)) % unsigned(nty));
+    const int tz = int(b / unsigned(ntx * nty));
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zchunk;
+    const int z1 = min(z0 + zchunk, D.d0);
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const int64_t st0 = int64_t(D.d1) * D.d2;
+
+    // this thread's point of the widened tile
+    const int p = int(threadIdx.x);
+    const int yy = p / EX, xx = p % EX;
+    const int y = y0 - 2 + yy, x = x0 - 2 + xx;
+    const bool inv = p < NE && y >= 0 && y < D.d1 && x >= 0 && x < D.d2;
+    const bool inE2 = inv && yy >= 1 && xx >= 1;
+    const bool inE1 = inv && yy >= 1 && yy < TY + 2 && xx >= 1 && xx < TX + 2;
+    const bool inO = inv && yy >= 2 && yy < TY + 2 && xx >= 2 && xx < TX + 2;
+    const unsigned f = (y < D.d1 - 1 ? 16u : 0u) | (x < D.d2 - 1 ? 32u : 0u) |
+                       (y > 0 ? 64u : 0u) | (x > 0 ? 128u : 0u);
+    const unsigned ob = inv ? 4u * unsigned(y * D.d2 + x) : 0u;  // in-plane byte offset
+
+    const bool two = levels == 2;
+    const int zs = max(z0 - (two ? 2 : 1), 0);
+    const int zend = two ? z1 : z1 - 1;
+    auto p_on = [&](int z) { return inv && z < D.d0; };
+    auto u_on = [&](int z) { return (two ? inE2 : inO) && z < D.d0 && z >= (two ? z0 - 1 : z0); };
+
+    // values carried from step z-1 to step z
+    float hA1_0 = 0.f, hA1_1 = 0.f, hA1_2 = 0.f;  // level-A hp at z-1
+    float hB2_0 = 0.f;                             // level-B hp (axis 0) at z-2
+    float ptAp = 0.f, uAp = 0.f, tAp = 0.f;       // level-A pt, u, tmp at z-1
+    float aP = 0.f, csP = 0.f, ctP = 0.f;
+    float4 qP = make_float4(1.f, 0.f, 0.f, 0.f);
+    float err_acc = 0.f;
+
+    SweepIn L;
+    L.t = L.t0 = L.t1 = L.t2 = L.h0 = L.h1 = L.h2 = L.a = L.pt = L.u = L.cs = L.ct = 0.f;
+    L.q = make_float4(1.f, 0.f, 0.f, 0.f);
+    if (zs <= zend)
+        sweep_load(L, B, q4, alpha, cs, ct, n, st0, D.d2, int64_t(zs) * st0, ob, p_on(zs),
+                   u_on(zs), zs < D.d0 - 1, f);
+    for (int z = zs; z <= zend; ++z) {
+        const bool ph1 = p_on(z), ph2 = u_on(z);
+        const bool ph3 = two && inE1 && z >= max(z0, 1);
+        const bool ph4 = two && inO && z - 1 >= z0 && z - 1 < z1;
+        const bool e0 = z < D.d0 - 1, f0 = z > 0;            // plane z
+        const bool e0b = z - 1 < D.d0 - 1, f0b = z - 1 > 0;  // plane z-1
+        const int64_t zoff = int64_t(z) * st0;
+        // 1. level-A p-step at plane z on the widened tile
+        float hA0_0 = 0.f, hA0_1 = 0.f, hA0_2 = 0.f;
+        if (ph1) {
+            const float g0 = e0 ? (L.t0 - L.t) * PP.ss0 : 0.f;
+            const float g1 = (f & 16u) ? (L.t1 - L.t) * PP.ss1 : 0.f;
+            const float g2 = (f & 32u) ? (L.t2 - L.t) * PP.ss2 : 0.f;
+            hA0_0 = L.h0;
+            hA0_1 = L.h1;
+            hA0_2 = L.h2;
+            mh::cmf_p_voxel(hA0_0, hA0_1, hA0_2, g0, g1, g2, L.q, PP.sq0, PP.sq1, PP.sq2, L.a);
+            if (!two && inO && z >= z0) {
+                st(B.hpo + zoff, ob, hA0_0);
+                st(B.hpo + n + zoff, ob, hA0_1);
+                st(B.hpo + 2 * n + zoff, ob, hA0_2);
+            }
+        }
+        if (p < NE) {
+            sA1[p] = hA0_1;
+            sA2[p] = hA0_2;
+        }
+        __syncthreads();
+        // 2. level-A u-step at plane z (stores when it is the only level)
+        float tA0 = 0.f, ptA0 = 0.f, uA0 = 0.f;
+        if (ph2) {
+            const float x0v = (e0 ? hA0_0 : 0.f) - (f0 ? hA1_0 : 0.f);
+            const float x1v = ((f & 16u) ? hA0_1 : 0.f) - ((f & 64u) ? sA1[p - EX] : 0.f);
+            const float x2v = ((f & 32u) ? hA0_2 : 0.f) - ((f & 128u) ? sA2[p - 1] : 0.f);
+            const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
+            ptA0 = L.pt;
+            uA0 = L.u;
+            const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, 1.f, 0.f);
+            if (!two) {
+                st(B.pto + zoff, ob, ptA0);
+                st(B.uo + zoff, ob, uA0);
+                st(B.tmpo + zoff, ob, tA0);
+                if (check) err_acc += fabsf(err);
+            }
+        }
+        // constants of plane z, needed at plane z-1's turn in the next step
+        const float4 qN = L.q;
+        const float aN = L.a, csN = L.cs, ctN = L.ct;
+        // inputs of the next step, in flight while the level-B work runs
+        if (z + 1 <= zend)
+            sweep_load(L, B, q4, alpha, cs, ct, n, st0, D.d2, zoff + st0, ob, p_on(z + 1),
+                       u_on(z + 1), z + 1 < D.d0 - 1, f);
+        // 3. level-B p-step at plane z-1 on E1
+        float hB1_0 = 0.f, hB1_1 = 0.f, hB1_2 = 0.f;
+        if (ph3) {
+            const float g0 = e0b ? (tA0 - tAp) * PP.ss0 : 0.f;
+            const float g1 = (f & 16u) ? (sT[p + EX] - tAp) * PP.ss1 : 0.f;
+            const float g2 = (f & 32u) ? (sT[p + 1] - tAp) * PP.ss2 : 0.f;
+            hB1_0 = hA1_0;
+            hB1_1 = hA1_1;
+            hB1_2 = hA1_2;
+            mh::cmf_p_voxel(hB1_0, hB1_1, hB1_2, g0, g1, g2, qP, PP.sq0, PP.sq1, PP.sq2, aP);
+        }
+        if (p < NE) {
+            sB1[p] = hB1_1;
+            sB2[p] = hB1_2;
+        }
+        __syncthreads();
+        // 4. level-B u-step at plane z-1 on O; store iteration t + 2
+        if (ph4) {
+            const int64_t zb = zoff - st0;
+            const float x0v = (e0b ? hB1_0 : 0.f) - (f0b ? hB2_0 : 0.f);
+            const float x1v = ((f & 16u) ? hB1_1 : 0.f) - ((f & 64u) ? sB1[p - EX] : 0.f);
+            const float x2v = ((f & 32u) ? hB1_2 : 0.f) - ((f & 128u) ? sB2[p - 1] : 0.f);
+            const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
+            float pti = ptAp, ui = uAp, ti;
+            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, 1.f, 0.f);
+            st(B.hpo + zb, ob, hB1_0);
+            st(B.hpo + n + zb, ob, hB1_1);
+            st(B.hpo + 2 * n + zb, ob, hB1_2);
+            st(B.pto + zb, ob, pti);
+            st(B.uo + zb, ob, ui);
+            st(B.tmpo + zb, ob, ti);
+            if (check) err_acc += fabsf(err);
+        }
+        // level-A tmp of plane z for the next step's level-B p-step (every read
+        // of plane z-1's values finished before the barrier above)
+        if (p < NE) sT[p] = tA0;
+        hB2_0 = hB1_0;
+        hA1_0 = hA0_0;
+        hA1_1 = hA0_1;
+        hA1_2 = hA0_2;
+        ptAp = ptA0;
+        uAp = uA0;
+        tAp = tA0;
+        qP = qN;
+        aP = aN;
+        csP = csN;
+        ctP = ctN;
+    }
+    if (check) {
+        const float v = mh::block_reduce<SumOp>(err_acc);
+        if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[kParity] = out_set;
+}
+
+template <int TY, int TX, int NT>
+void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SB, const float4 *q4,
+                    const float *alpha, const float *cs, const float *ct, const Dims &D,
+                    const PParams &PP, const UParams &UP, int zchunk, int lv, int chk, int out_set,
+                    int *ctrl, float *partials)
+{
+    hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT>), dim3(grid), dim3(NT), 0, s, SB, q4, alpha, cs,
+                       ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials);
+}
+
+void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &SB,
+                  const float4 *q4, const float *alpha, const float *cs, const float *ct,
+                  const Dims &D, const PParams &PP, const UParams &UP, int zchunk, int lv, int chk,
+                  int out_set, int *ctrl, float *partials)
+{
+    switch (shape_ix) {
+    case 1: launch_sweep_t<11, 32, 512>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    case 2: launch_sweep_t<4, 64, 512>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    case 3: launch_sweep_t<26, 32, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    case 4: launch_sweep_t<13, 61, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    default: launch_sweep_t<12, 64, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    }
+}
+
+// u of the buffer set that holds the final state, copied to the caller's u
+// when the last pass wrote the second set.
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_sweep_final_kernel(float *__restrict__ u, const float *__restrict__ u2, int64_t n,
+                       const int *__restrict__ ctrl)
+{
+    if (ctrl[kParity] != 1) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = u2[i];
+}
+
 struct Workspace {
     int *ctrl;
     float *crit_log;
@@ -176,6 +465,8 @@ struct Workspace {
     float *pt;
     float *tmp;
     float4 *q4;
+    // second state set of the two-level sweep (ping-pong)
+    float *hp2, *pt2, *tmp2, *u2;
 };
 
 int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
@@ -196,6 +487,10 @@ int64_t layout(int64_t n, char *base, Workspace *w)
     ws.pt = reinterpret_cast<float *>(take(sizeof(float) * n));
     ws.tmp = reinterpret_cast<float *>(take(sizeof(float) * n));
     ws.q4 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    ws.hp2 = reinterpret_cast<float *>(take(sizeof(float) * 3 * n));
+    ws.pt2 = reinterpret_cast<float *>(take(sizeof(float) * n));
+    ws.tmp2 = reinterpret_cast<float *>(take(sizeof(float) * n));
+    ws.u2 = reinterpret_cast<float *>(take(sizeof(float) * n));
     if (w) *w = ws;
     return off;
 }
@@ -243,30 +538,121 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
     const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
                      float(1.0 / (double(n) * double(prm->rcc)))};
     const bool sup = labels != nullptr;
-    for (int it = 0; it < prm->num_iter; ++it) {
-        if (prm->const_evals)
-            hipLaunchKernelGGL(cmf_p_kernel<true>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
-                               eigs, alpha, D, PP, W.ctrl);
-        else
-            hipLaunchKernelGGL(cmf_p_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp, W.q4,
-                               eigs, alpha, D, PP, W.ctrl);
-        const bool chk = (it % prm->update) == 0;
-        if (sup && chk)
-            hipLaunchKernelGGL((cmf_u_kernel<true, true>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
-                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
-        else if (sup)
-            hipLaunchKernelGGL((cmf_u_kernel<true, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
-                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
-        else if (chk)
-            hipLaunchKernelGGL((cmf_u_kernel<false, true>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp, W.tmp,
-                               cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
-        else
-            hipLaunchKernelGGL((cmf_u_kernel<false, false>), dim3(nb), dim3(mh::kBlock), 0, s, W.hp,
-                               W.tmp, cs, ct, W.pt, u, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+    // An iteration runs either as the plain kernels (in place on the current
+    // buffer set) or inside a two-level sweep (current set -> the other set).
+    // Both compute the same values, so a solve may mix them; the final u is in
+    // the set the last sweep wrote (ctrl[kParity]).
+    const bool sweep_elig = prm->const_evals && !sup && int64_t(D.d1) * D.d2 < (int64_t(1) << 27);
+    auto sweep_grid_of = [&](const SweepMethod &m) {
+        const SweepShape &S = kSweepShapes[m.shape];
+        return int64_t((D.d2 + S.tx - 1) / S.tx) * ((D.d1 + S.ty - 1) / S.ty) *
+               ((D.d0 + m.zchunk - 1) / m.zchunk);
+    };
+    auto usable = [&](const SweepMethod &m) {
+        return m.shape < 0 || (sweep_elig && sweep_grid_of(m) <= int64_t(nb) + 64);
+    };
+    int cur = 0, it = 0;
+    bool swept = false;
+    auto step = [&](const SweepMethod &m) -> int {
+        if (m.shape < 0) {
+            float *hp = cur ? W.hp2 : W.hp, *tmp = cur ? W.tmp2 : W.tmp;
+            float *pt = cur ? W.pt2 : W.pt, *uu = cur ? W.u2 : u;
+            if (prm->const_evals)
+                hipLaunchKernelGGL(cmf_p_kernel<true>, dim3(nb), dim3(mh::kBlock), 0, s, hp, tmp, W.q4,
+                                   eigs, alpha, D, PP, W.ctrl);
+            else
+                hipLaunchKernelGGL(cmf_p_kernel<false>, dim3(nb), dim3(mh::kBlock), 0, s, hp, tmp, W.q4,
+                                   eigs, alpha, D, PP, W.ctrl);
+            const bool chk = (it % prm->update) == 0;
+            if (sup && chk)
+                hipLaunchKernelGGL((cmf_u_kernel<true, true>), dim3(nb), dim3(mh::kBlock), 0, s, hp, tmp,
+                                   cs, ct, pt, uu, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+            else if (sup)
+                hipLaunchKernelGGL((cmf_u_kernel<true, false>), dim3(nb), dim3(mh::kBlock), 0, s, hp, tmp,
+                                   cs, ct, pt, uu, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+            else if (chk)
+                hipLaunchKernelGGL((cmf_u_kernel<false, true>), dim3(nb), dim3(mh::kBlock), 0, s, hp, tmp,
+                                   cs, ct, pt, uu, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+            else
+                hipLaunchKernelGGL((cmf_u_kernel<false, false>), dim3(nb), dim3(mh::kBlock), 0, s, hp,
+                                   tmp, cs, ct, pt, uu, labels, D, UP, W.ctrl, W.partials, W.crit_log);
+            if (chk)
+                hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(1024), 0, s, W.partials, int(nb),
+                                   UP.inv_n_rcc, UP.error_bound, W.ctrl, W.crit_log);
+            it += 1;
+            return 1;
+        }
+        // a criterion iteration is always the last level of its pass, so a
+        // stop never skips an iteration the pass already applied
+        const bool chk_a = (it % prm->update) == 0;
+        const int lv = (!chk_a && it + 1 < prm->num_iter) ? 2 : 1;
+        const bool chk = ((it + lv - 1) % prm->update) == 0;
+        SweepBufs SB;
+        if (cur == 0) SB = SweepBufs{W.hp, W.tmp, W.pt, u, W.hp2, W.tmp2, W.pt2, W.u2};
+        else SB = SweepBufs{W.hp2, W.tmp2, W.pt2, W.u2, W.hp, W.tmp, W.pt, u};
+        const int64_t grid = sweep_grid_of(m);
+        launch_sweep(m.shape, unsigned(grid), s, SB, W.q4, alpha, cs, ct, D, PP, UP, m.zchunk, lv,
+                     int(chk), 1 - cur, W.ctrl, W.partials);
         if (chk)
-            hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(1024), 0, s, W.partials, int(nb),
+            hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(1024), 0, s, W.partials, int(grid),
                                UP.inv_n_rcc, UP.error_bound, W.ctrl, W.crit_log);
+        cur ^= 1;
+        swept = true;
+        it += lv;
+        return lv;
+    };
+
+    // choose the method
+    const SweepMethod kPlain{-1, 0};
+    SweepMethod use = kPlain;
+    const std::array<int, 3> key{D.d0, D.d1, D.d2};
+    if (ctx->sweep_mode == 2) {
+        SweepMethod m{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
+                      ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+        if (usable(m)) use = m;
+    } else if (ctx->sweep_mode == 0 && sweep_elig) {
+        auto f = ctx->sweep_tuned.find(key);
+        if (f != ctx->sweep_tuned.end()) {
+            use = SweepMethod{f->second.first, f->second.second};
+        } else if (prm->num_iter >= 64) {
+            // time each candidate on this solve's own iterations (every method
+            // computes the same values): one warm-up pass, then >= 4 iterations
+            hipEvent_t e0, e1;
+            MH_CHECK(ctx, hipEventCreate(&e0));
+            MH_CHECK(ctx, hipEventCreate(&e1));
+            float best = 0.f;
+            for (const SweepMethod &m : kSweepCandidates) {
+                if (!usable(m) || it + 8 > prm->num_iter) continue;
+                step(m);
+                MH_CHECK(ctx, hipEventRecord(e0, s));
+                int done = 0;
+                while (done < 4) done += step(m);
+                MH_CHECK(ctx, hipEventRecord(e1, s));
+                MH_CHECK(ctx, hipEventSynchronize(e1));
+                float ms = 0.f;
+                MH_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+                const float per = ms / float(done);
+                if (best == 0.f || per < best) {
+                    best = per;
+                    use = m;
+                }
+            }
+            MH_CHECK_LAUNCH(ctx, "cmf tuning");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            int stopped = 0;
+            MH_CHECK(ctx, hipMemcpy(&stopped, W.ctrl + kStop, sizeof(int), hipMemcpyDeviceToHost));
+            // timings of a stopped solve (empty launches) choose nothing
+            if (!stopped) ctx->sweep_tuned[key] = {use.shape, use.zchunk};
+        }
+    }
+    while (it < prm->num_iter) {
+        step(use);
         MH_CHECK_LAUNCH(ctx, "cmf kernels");
+    }
+    if (swept) {
+        hipLaunchKernelGGL(cmf_sweep_final_kernel, dim3(nb), dim3(mh::kBlock), 0, s, u, W.u2, n, W.ctrl);
+        MH_CHECK_LAUNCH(ctx, "cmf sweep final");
     }
     int ctrl[kCtrlWords];
     MH_CHECK(ctx, hipMemcpyAsync(ctrl, W.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
@@ -882,5 +1268,15 @@ MH_EXPORT int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit)
     MH_REQUIRE(ctx, group >= 0 && group <= kSI && visit >= 0, "group in 0..8, visit >= 0");
     ctx->batch_group = group;
     ctx->batch_visit = visit;
+    return MH_OK;
+}
+
+MH_EXPORT int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk)
+{
+    MH_REQUIRE(ctx, mode >= 0 && mode <= 2, "mode 0 (automatic), 1 (off) or 2 (on)");
+    MH_REQUIRE(ctx, tile >= 0 && tile <= kNumSweepShapes && zchunk >= 0, "bad tile or zchunk");
+    ctx->sweep_mode = mode;
+    ctx->sweep_shape = tile;
+    ctx->sweep_zchunk = zchunk;
     return MH_OK;
 }

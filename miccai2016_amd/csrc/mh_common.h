@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <string>
 #include <algorithm>
+#include <array>
+#include <map>
+#include <utility>
 
 #include "../../include/miccai_hip.h"
 
@@ -33,6 +36,13 @@ struct mh_ctx {
     // iterations per visit of a group (0: chosen automatically)
     int batch_group = 0;
     int batch_visit = 0;
+    // two-level sweep of mh_cmf_solve: 0 automatic, 1 off, 2 whenever eligible
+    int sweep_mode = 0;
+    int sweep_shape = 0;   // tile shape (1-based; 0: default)
+    int sweep_zchunk = 0;  // planes per work-group (0: 16)
+    // method the tuner chose per volume dims: (tile shape or -1 for the plain
+    // kernels, plane chunk)
+    std::map<std::array<int, 3>, std::pair<int, int>> sweep_tuned;
 };
 
 namespace mh {

@@ -480,3 +480,28 @@ def test_fill_and_reductions(dev, rng):
     assert abs(s - np.abs(x.astype(np.float64)).sum()) <= 1e-5 * s
     dev.fill(t, 2.5)
     assert (dev.download(t) == 2.5).all()
+
+
+@pytest.mark.parametrize("shape,iters,update,bound", [
+    ((16, 16, 16), 120, 30, 0.0), ((20, 17, 23), 61, 7, 0.0), ((7, 64, 5), 90, 30, 0.0),
+    ((33, 9, 130), 75, 1, 0.0), ((2, 20, 20), 40, 3, 0.0), ((3, 3, 3), 17, 4, 0.0),
+    ((40, 70, 66), 3000, 7, 1e-3), ((24, 31, 140), 2, 30, 0.0)])
+def test_two_level_sweep_equals_plain_kernels(dev, rng, shape, iters, update, bound):
+    """The temporally blocked sweep computes the plain kernels' iterations bit
+    for bit, and stops at the same iteration."""
+    img, q, alpha, cs, ct, _ = _cut_case(rng, shape)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    out = {}
+    try:
+        for mode in (1, 2, 0):
+            dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, mode, 0, 0))
+            c = _cutter(dev, iters, update=update, error_bound=bound)
+            u = c.execute(img, q, eigs, alpha, cs, ct, verbose=False)
+            out[mode] = (u, c.last_result.iterations, c.last_result.stopped, c.last_criteria)
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    for mode in (2, 0):   # 0: the tuner times and mixes methods on the first solve
+        assert np.array_equal(out[1][0], out[mode][0])
+        assert out[1][1:3] == out[mode][1:3]
+        # the criterion sums the same |err| in another order
+        np.testing.assert_allclose(out[1][3], out[mode][3], rtol=1e-5)
