@@ -8,15 +8,20 @@ iterations each, run as one batched solve on device-resident inputs. The
 shared inputs (frames from Frangi -> adjustment -> GVF -> completion,
 capacities, |grad I|^2) are prepared once, outside the timed region.
 
-Reported: whole-job voxel-iterations/s of the batched grid; every cell's
-fast Dice against the noise-free mask (counted on the device); the same
-solves run one at a time (device-resident, and through the numpy drop-in
-API); a 256^3 single solve (the bandwidth-scale plain workload); serving
-latency (p50/p99) and throughput under Poisson arrivals of mixed-size cuts;
-the HBM roofline of the max-flow iteration (84 algorithmic bytes per
-voxel-iteration); and the CPU oracle timed on a bounded sample on this host.
+Reported beside it: every cell's fast Dice against the noise-free mask
+(counted on the device); the same solves run one at a time (device-resident,
+and through the numpy drop-in API); a 256^3 x 1000-iteration single solve
+(the bandwidth-scale plain workload); serving latency (p50/p99) and
+throughput under Poisson arrivals of mixed-size cuts; the HBM roofline of the
+max-flow iteration (84 algorithmic bytes per voxel-iteration) with the HBM
+traffic rocprofv3 measured for the same kernels; and the CPU oracle timed on
+a bounded sample on this host.
+
+``--only PART`` runs one part (grid, plain, plain256, serve, cpu, or copy:
+a 512 MiB device copy used to calibrate the rocprofv3 byte counters).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -29,6 +34,16 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_VOXEL_ITER = 84.0      # algorithmic bytes of one max-flow iteration
 HBM_PEAK_GBS = 8000.0            # MI355X
+
+# HBM bytes the same kernels moved per voxel-iteration, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE passes scaled by a 512 MiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0);
+# the passes and their commands are recorded under profiles/
+TRAFFIC_GRID_PER_VOXEL_ITER = 67.6     # batched p + u kernels, groups of 4 solves, 128^3
+TRAFFIC_SWEEP_PER_VOXEL_ITER = 45.0    # two-level sweep, 256^3
+TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
+GRID_GROUP = 4                         # solves per group of the default batched schedule
+
+PARTS = ("grid", "plain", "plain256", "serve", "cpu")
 
 
 def log(msg):
@@ -48,6 +63,30 @@ def timed(torch, fn, reps=1):
     return e0.elapsed_time(e1) / 1e3
 
 
+def device_helix(dev, spec, weighting, decay, seed=42):
+    """A helix phantom generated on the device with its centre-line frames and
+    the cut's fields: (u0, frames, alpha, cs, ct) device tensors."""
+    from miccai2016_amd import _lib
+    shape = spec.shape
+    n = int(np.prod(shape))
+    L, h, st = dev.lib, dev.handle, dev.stream
+    img, q = dev.empty(n), dev.empty(3 * n)
+    cs, ct, al, u0 = dev.empty(n), dev.empty(n), dev.empty(n), dev.empty(n)
+    dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), spec.r_tube, spec.r_helix, spec.pitch,
+                                 spec.sigma, C.c_uint64(seed), dev.ptr(img), None, dev.ptr(q), st),
+              "mh_helix_phantom")
+    dev.check(L.mh_complete_frames(h, dev.ptr(q), dev.ptr(q), n, st), "mh_complete_frames")
+    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, spec.sigma, 1e-9, dev.ptr(cs),
+                                       dev.ptr(ct), st), "mh_terminal_capacities")
+    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(shape), dev.ptr(al), st),
+              "mh_grad_mag_sq")
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(al), n, weighting, decay, dev.ptr(al), st),
+              "mh_nonterminal_weight")
+    wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
+    dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), st), "mh_initial_u")
+    return u0, q, al, cs, ct
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,17 +95,26 @@ def main():
     ap.add_argument("--iters", type=int, default=500, help="max-flow iterations per solve")
     ap.add_argument("--solves", type=int, default=425, help="grid cells per step (<= 425)")
     ap.add_argument("--plain-solves", type=int, default=6)
+    ap.add_argument("--only", choices=PARTS + ("copy",), default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-256", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--iters-256", type=int, default=1000)
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="two-level sweep of single solves: 0 tuned, 1 off, 2 on")
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no "
               "multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    run = {p: (args.only == p) if args.only else True for p in PARTS}
+    if not args.only:
+        run["cpu"] = not args.no_cpu
+        run["plain256"] = not args.no_256
+        run["serve"] = not args.no_serve
 
     import torch
     from miccai2016_amd import phantom as ph
@@ -76,207 +124,206 @@ def main():
         CmfCutQAnisotropicGpu, initial_u)
 
     dev = default_context(0)
+    dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, args.sweep, 0, 0), "mh_cmf_set_sweep")
+
+    if args.only == "copy":
+        a = torch.ones(128 << 20, device=dev.device)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
+        return
+
     cfg = PipelineConfig()
     cfg.cut_iters = args.iters
-    spec = ph.GRID_128
-    log(f"phantom {spec.shape} sigma={spec.sigma}")
-    img, truth = ph.make_helix(spec)
-    t0 = time.time()
-    prep = prepare(img, spec.sigma, cfg, dev)
-    torch.cuda.synchronize()
-    t_prep = time.time() - t0
-    log(f"prepare (Frangi, adjustment, GVF, frames, capacities): {t_prep:.2f} s")
-
+    eigs = (1, cfg.a_atv, cfg.a_atv)
     w_all, d_all = ph.grid_pairs()
     S = max(1, min(args.solves, len(w_all)))
-    w, d = w_all[:S].astype(np.float32), d_all[:S].astype(np.float32)
-    eigs = (1, cfg.a_atv, cfg.a_atv)
-    n = img.size
-    cutter = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
-    q_d = dev.upload(prep.quats).reshape(-1)
-    gms_d = dev.upload(prep.gms).reshape(-1)
-    cs_d, ct_d = dev.upload(prep.cs).reshape(-1), dev.upload(prep.ct).reshape(-1)
-    u0 = dev.upload(initial_u(img)).reshape(1, -1)
-    u = torch.empty((S, n), dtype=torch.float32, device=dev.device)
+    line = {
+        "metric": "ATV max-flow voxel-iters/s, batched vs plain at Dice>=0.999; p50/p99 latency",
+        "value": None, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic helix phantom (seeded)",
+        "config": {"workload": f"batched grid search, {S} ATV solves x {args.iters} iterations "
+                               f"on one 128^3 helix phantom (sigma 1.5)",
+                   "solves": S, "iterations": args.iters, "volume": [128, 128, 128]},
+    }
 
-    def step():
-        u.copy_(u0.expand(S, n))
-        cutter.execute_batched_device(u, S, w, d, q_d, gms_d, cs_d, ct_d, img.shape, eigs)
-
-    for k in range(args.warmup):
-        step()
+    if run["grid"] or run["plain"] or run["cpu"]:
+        spec = ph.GRID_128
+        log(f"phantom {spec.shape} sigma={spec.sigma}")
+        img, truth = ph.make_helix(spec)
+        t0 = time.time()
+        prep = prepare(img, spec.sigma, cfg, dev)
         torch.cuda.synchronize()
-        log(f"warmup step {k + 1}/{args.warmup} done")
-    wall0 = time.time()
-    secs = 0.0
-    for k in range(args.steps):
-        secs += timed(torch, step)
-        log(f"timed step {k + 1}/{args.steps}: {secs / (k + 1):.3f} s/step")
-    wall = time.time() - wall0
-    vox_iters = float(S) * n * args.iters
-    value = vox_iters * args.steps / secs
-    ms_step = secs / args.steps * 1e3
-    log(f"batched grid: {value:.4e} voxel-iters/s ({ms_step:.1f} ms/step, wall {wall:.2f} s)")
+        line["prepare_s"] = time.time() - t0
+        log(f"prepare (Frangi, adjustment, GVF, frames, capacities): {line['prepare_s']:.2f} s")
+        n = img.size
+        q_d = dev.upload(prep.quats).reshape(-1)
+        gms_d = dev.upload(prep.gms).reshape(-1)
+        cs_d, ct_d = dev.upload(prep.cs).reshape(-1), dev.upload(prep.ct).reshape(-1)
+        u0 = dev.upload(initial_u(img)).reshape(1, -1)
 
-    # ---- quality of every grid cell against the noise-free mask, on the device
-    from miccai2016_amd.evaluation import DeviceDice
-    grid_dice = DeviceDice(truth, dev)(u, S)
-    best = int(np.nanargmax(grid_dice))
-    log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, decay={d_all[best]:.4g}")
+    # ---- the batched grid (the headline)
+    ub = None
+    if run["grid"]:
+        w, d = w_all[:S].astype(np.float32), d_all[:S].astype(np.float32)
+        cutter = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
+        u = torch.empty((S, n), dtype=torch.float32, device=dev.device)
+
+        def step():
+            u.copy_(u0.expand(S, n))
+            cutter.execute_batched_device(u, S, w, d, q_d, gms_d, cs_d, ct_d, img.shape, eigs)
+
+        for k in range(args.warmup):
+            step()
+            torch.cuda.synchronize()
+            log(f"warmup step {k + 1}/{args.warmup} done")
+        secs = 0.0
+        for k in range(args.steps):
+            secs += timed(torch, step)
+            log(f"timed step {k + 1}/{args.steps}: {secs / (k + 1):.3f} s/step")
+        value = float(S) * n * args.iters * args.steps / secs
+        line["value"] = value
+        line["ms_per_step"] = secs / args.steps * 1e3
+        log(f"batched grid: {value:.4e} voxel-iters/s ({line['ms_per_step']:.1f} ms/step)")
+        from miccai2016_amd.evaluation import DeviceDice
+        grid_dice = DeviceDice(truth, dev)(u, S)
+        best = int(np.nanargmax(grid_dice))
+        line["grid_quality"] = {"best_fast_dice": float(grid_dice[best]),
+                                "best_weighting": float(w_all[best]),
+                                "best_decay": float(d_all[best])}
+        log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "
+            f"decay={d_all[best]:.4g}")
+        groups = -(-S // GRID_GROUP)
+        t_group_iter = secs / args.steps / (groups * args.iters)
+        achieved = value * BYTES_PER_VOXEL_ITER / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": TRAFFIC_GRID_PER_VOXEL_ITER * GRID_GROUP * n,
+            "kernel": "one max-flow iteration of a group of 4 solves: cmf_batch_p_kernel<4> + "
+                      "cmf_batch_u_kernel<4> (84 algorithmic B per solve-voxel-iteration; traffic "
+                      "= measured HBM bytes per such launch pair)",
+            "launch_pair_us": t_group_iter * 1e6,
+            "traffic_per_voxel_iteration": TRAFFIC_GRID_PER_VOXEL_ITER}
+        ub = u.detach().cpu().numpy()
+        del u
 
     # ---- the same solves one at a time (device-resident) and through the API
-    P = max(1, min(args.plain_solves, S))
-    pick = np.linspace(0, S - 1, P).astype(int)
-    ub = u.detach().cpu().numpy()
-    plain_cut = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
-    alphas = [ph.nonterminal_weight(prep.gms, w_all[j], d_all[j]).astype(np.float32) for j in pick]
-    eig_d = q_d
-    ud = dev.empty(n)
-    t_dev = 0.0
-    dices = []
-    for j, a in zip(pick, alphas):
-        a_d = dev.upload(a).reshape(-1)
+    if run["plain"]:
+        P = max(1, min(args.plain_solves, S))
+        pick = np.linspace(0, S - 1, P).astype(int)
+        plain_cut = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
+        alphas = [ph.nonterminal_weight(prep.gms, w_all[j], d_all[j]).astype(np.float32)
+                  for j in pick]
+        ud = dev.empty(n)
+        t_dev, dices = 0.0, []
+        for j, a in zip(pick, alphas):
+            a_d = dev.upload(a).reshape(-1)
 
-        def one():
-            ud.copy_(u0.reshape(-1))
-            plain_cut.execute_device(ud, eig_d, a_d, cs_d, ct_d, const_sqevals=np.sqrt(eigs),
-                                     shape=img.shape)
-        t_dev += timed(torch, one)
-        up = ud.detach().cpu().numpy()
-        dices.append(ph.dice_between(ub[j], up))
-    per_dev = t_dev / P
-    t_api = 0.0
-    napi = min(P, 3)
-    for j, a in zip(pick[:napi], alphas[:napi]):
-        t1 = time.time()
-        plain_cut.execute(img, prep.quats, eigs, a, prep.cs, prep.ct, verbose=False)
-        t_api += time.time() - t1
-    per_api = t_api / napi
-    plain_rate = n * args.iters / per_dev
-    log(f"plain device-resident: {per_dev * 1e3:.2f} ms/solve; numpy API: {per_api * 1e3:.2f} "
-        f"ms/solve; min Dice batched vs plain {min(dices):.5f}")
+            def one():
+                ud.copy_(u0.reshape(-1))
+                plain_cut.execute_device(ud, q_d, a_d, cs_d, ct_d, const_sqevals=np.sqrt(eigs),
+                                         shape=img.shape)
+            one()          # the first solve of a shape also tunes the method
+            t_dev += timed(torch, one)
+            if ub is not None:
+                dices.append(ph.dice_between(ub[j], ud.detach().cpu().numpy()))
+        per_dev = t_dev / P
+        t_api = 0.0
+        napi = min(P, 3)
+        for j, a in zip(pick[:napi], alphas[:napi]):
+            t1 = time.time()
+            plain_cut.execute(img, prep.quats, eigs, a, prep.cs, prep.ct, verbose=False)
+            t_api += time.time() - t1
+        per_api = t_api / napi
+        plain_rate = n * args.iters / per_dev
+        line["plain"] = {"voxel_iters_per_s": plain_rate, "ms_per_solve_device": per_dev * 1e3,
+                         "ms_per_solve_numpy_api": per_api * 1e3,
+                         "sequential_425_s_device": per_dev * 425,
+                         "sequential_425_s_numpy_api": per_api * 425}
+        if line["value"]:
+            line["plain"].update(
+                batched_speedup_vs_device=line["value"] / plain_rate,
+                batched_speedup_vs_numpy_api=line["value"] / (n * args.iters / per_api),
+                min_dice_batched_vs_plain=float(min(dices)))
+        log(f"plain device-resident: {per_dev * 1e3:.2f} ms/solve; numpy API: "
+            f"{per_api * 1e3:.2f} ms/solve")
 
-    # ---- 256^3 single solve (bandwidth scale): the 256^3 helix (tube 8, helix 64,
-    # pitch 96, noise 1.0) generated on the device with its centre-line frames
-    big = None
-    if not args.no_256:
-        import ctypes as C
-        from miccai2016_amd import _lib
-        sp256 = ph.PLAIN_256
-        shape = sp256.shape
-        nb = int(np.prod(shape))
-        L, h, st = dev.lib, dev.handle, dev.stream
-        img_b, qb = dev.empty(nb), dev.empty(3 * nb)
-        cs_b, ct_b, al_b, ub0 = dev.empty(nb), dev.empty(nb), dev.empty(nb), dev.empty(nb)
-        dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), sp256.r_tube, sp256.r_helix,
-                                     sp256.pitch, sp256.sigma, C.c_uint64(42), dev.ptr(img_b),
-                                     None, dev.ptr(qb), st), "mh_helix_phantom")
-        dev.check(L.mh_complete_frames(h, dev.ptr(qb), dev.ptr(qb), nb, st), "mh_complete_frames")
-        dev.check(L.mh_terminal_capacities(h, dev.ptr(img_b), nb, sp256.sigma, 1e-9,
-                                           dev.ptr(cs_b), dev.ptr(ct_b), st), "capacities")
-        dev.check(L.mh_grad_mag_sq(h, dev.ptr(img_b), _lib.dims3(shape), dev.ptr(al_b), st),
-                  "mh_grad_mag_sq")
-        dev.check(L.mh_nonterminal_weight(h, dev.ptr(al_b), nb, 10 ** -1.625, 10 ** 1.5,
-                                          dev.ptr(al_b), st), "mh_nonterminal_weight")
-        wk = dev.empty(int(L.mh_initial_u_work_floats(nb)))
-        dev.check(L.mh_initial_u(h, dev.ptr(img_b), nb, dev.ptr(ub0), dev.ptr(wk), st),
-                  "mh_initial_u")
+    # ---- 256^3 single solve (bandwidth scale)
+    if run["plain256"]:
+        spec = ph.PLAIN_256
+        it256 = args.iters_256
+        u0b, qb, al_b, cs_b, ct_b = device_helix(dev, spec, 10 ** -1.625, 10 ** 1.5)
+        nb = u0b.numel()
         ub_ = torch.empty(nb, device=dev.device)
         p256 = cut_parameters(cfg)
-        p256.num_iter = 1000
+        p256.num_iter = it256
         c256 = CmfCutQAnisotropicGpu(p256, dev)
 
         def solve256():
-            ub_.copy_(ub0)
-            c256.execute_device(ub_, qb, al_b, cs_b, ct_b, const_sqevals=np.sqrt(eigs), shape=shape)
-        solve256()
+            ub_.copy_(u0b)
+            c256.execute_device(ub_, qb, al_b, cs_b, ct_b, const_sqevals=np.sqrt(eigs),
+                                shape=spec.shape)
+        solve256()         # tunes the method for this shape
         t256 = timed(torch, solve256)
-        rate256 = nb * 1000 / t256
-        big = {"voxel_iters_per_s": rate256, "ms_per_solve": t256 * 1e3,
-               "achieved_GBs": rate256 * BYTES_PER_VOXEL_ITER / 1e9}
-        log(f"256^3 x 1000 iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s, "
-            f"{big['achieved_GBs']:.0f} GB/s algorithmic")
+        rate256 = nb * it256 / t256
+        line["plain_256"] = {"voxel_iters_per_s": rate256, "ms_per_solve": t256 * 1e3,
+                             "iterations": it256,
+                             "achieved_GBs_algorithmic": rate256 * BYTES_PER_VOXEL_ITER / 1e9,
+                             "frac_algorithmic": rate256 * BYTES_PER_VOXEL_ITER / 1e9 / HBM_PEAK_GBS,
+                             "traffic_per_voxel_iteration_sweep": TRAFFIC_SWEEP_PER_VOXEL_ITER,
+                             "traffic_per_voxel_iteration_plain_kernels":
+                                 TRAFFIC_PLAIN_PER_VOXEL_ITER}
+        log(f"256^3 x {it256} iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s")
+        del qb, al_b, cs_b, ct_b, u0b, ub_
 
     # ---- serving: Poisson arrivals of synthetic helix cuts of mixed sizes
-    serve = None
-    if not args.no_serve:
+    if run["serve"]:
         from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
-        del u, ub
         torch.cuda.empty_cache()
-        sp = cut_parameters(cfg)
-        srv = MaxFlowServer(sp, chunk=50).start()
+        srv = MaxFlowServer(cut_parameters(cfg), chunk=50).start()
         log(f"serving {args.serve_requests} requests at {args.serve_qps} QPS "
             f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
         tickets = poisson_load(srv, synthetic_request, args.serve_qps, args.serve_requests,
                                seed=11)
         srv.stop(drain=True, timeout=600)
         st = srv.stats()
-        errors = [t.error for t in tickets if t.error is not None]
-        dice = [t.dice for t in tickets if t.dice is not None]
-        serve = {"qps": args.serve_qps, "requests": args.serve_requests,
-                 "completed": st["completed"], "errors": len(errors),
-                 "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
-                 "voxel_iters_per_s": st.get("voxel_iters_per_s"),
-                 "peak_resident_jobs": srv.peak_resident,
-                 "peak_reserved_GiB": srv.peak_reserved / 2 ** 30,
-                 "budget_GiB": srv.budget / 2 ** 30, "chunk": srv.chunk,
-                 "median_dice_vs_truth": float(np.median(dice)) if dice else None,
-                 "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, "
-                             "helix/noise as the ragged set, (w, decay) from the grid"}
-        log(f"serving: p50 {serve['p50_s']:.3f} s, p99 {serve['p99_s']:.3f} s, "
-            f"{serve['voxel_iters_per_s']:.3e} voxel-iters/s, peak {srv.peak_resident} resident")
+        dice = [t.dice for t in tickets if t.dice is not None and not np.isnan(t.dice)]
+        line["serving"] = {
+            "qps": args.serve_qps, "requests": args.serve_requests,
+            "completed": st["completed"],
+            "errors": sum(t.error is not None for t in tickets),
+            "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
+            "voxel_iters_per_s": st.get("voxel_iters_per_s"),
+            "peak_resident_jobs": srv.peak_resident,
+            "peak_reserved_GiB": srv.peak_reserved / 2 ** 30, "budget_GiB": srv.budget / 2 ** 30,
+            "chunk": srv.chunk,
+            "median_dice_vs_truth": float(np.median(dice)) if dice else None,
+            "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, helix/noise "
+                        "as the ragged set, (w, decay) from the grid"}
+        log(f"serving: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s, "
+            f"peak {srv.peak_resident} resident")
 
     # ---- CPU oracle on a bounded sample of the same workload
-    cpu = None
-    if not args.no_cpu:
+    if run["cpu"]:
         from oracle import ref32
         threads = len(os.sched_getaffinity(0))
         ref32.set_num_threads(threads)
-        a0 = alphas[0]
+        a0 = ph.nonterminal_weight(prep.gms, w_all[0], d_all[0]).astype(np.float32)
         t1 = time.time()
         ref32.cmf_solve(img, prep.quats, eigs, a0, prep.cs, prep.ct, steps=cfg.steps, cc=cfg.cc,
                         num_iter=args.cpu_iters, update=cfg.cut_update, error_bound=0)
         tc = time.time() - t1
-        cpu = {"value": n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
-               "kind": "port",
-               "sample": f"1 of the 425 solves, {args.cpu_iters} of 500 iterations, 128^3, "
-                         f"C/OpenMP restatement of the reference kernels ({tc:.1f} s)"}
-        log(f"cpu oracle: {cpu['value']:.4e} voxel-iters/s on {threads} threads")
+        line["cpu_baseline"] = {
+            "value": n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 of the 425 solves, {args.cpu_iters} of 500 iterations, 128^3, "
+                      f"C/OpenMP restatement of the reference kernels ({tc:.1f} s)"}
+        log(f"cpu oracle: {line['cpu_baseline']['value']:.4e} voxel-iters/s on {threads} threads")
 
-    achieved = value * BYTES_PER_VOXEL_ITER / 1e9
-    line = {
-        "metric": "ATV max-flow voxel-iters/s, batched vs plain at Dice>=0.999; p50/p99 latency",
-        "value": value,
-        "unit": "voxel-iters/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic helix phantom (seeded)",
-        "config": {"workload": f"batched grid search, {S} ATV solves x {args.iters} iterations on "
-                               f"one 128^3 helix phantom (sigma 1.5)",
-                   "solves": S, "iterations": args.iters, "volume": list(img.shape)},
-        "plain": {"voxel_iters_per_s": plain_rate, "ms_per_solve_device": per_dev * 1e3,
-                  "ms_per_solve_numpy_api": per_api * 1e3,
-                  "sequential_425_s_device": per_dev * 425,
-                  "sequential_425_s_numpy_api": per_api * 425,
-                  "batched_speedup_vs_device": value / plain_rate,
-                  "batched_speedup_vs_numpy_api": value / (n * args.iters / per_api),
-                  "min_dice_batched_vs_plain": float(min(dices))},
-        "grid_quality": {"best_fast_dice": float(grid_dice[best]),
-                         "best_weighting": float(w_all[best]), "best_decay": float(d_all[best])},
-        "plain_256": big,
-        "serving": serve,
-        "prepare_s": t_prep,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "max-flow iteration (p-step + u-step), 84 B per voxel-iteration"},
-        "cpu_baseline": cpu,
-    }
     print(json.dumps(line), flush=True)
 
 
