@@ -237,7 +237,7 @@ int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
 /* Two-level sweep of mh_cmf_solve on this context (no counterpart in the
  * reference): unsupervised constant-eigenvalue solves may advance two
  * iterations per pass over the state (temporal blocking along axis 0). mode 0
- * (default) tunes: the first solve of a volume shape of >= 64 iterations times
+ * (default) tunes: the first solve of a volume shape of >= 160 iterations times
  * the plain kernels and a few sweep tile shapes on its own iterations and the
  * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
  * sweeps when the solve is eligible, with tile (1-based index of the built-in

@@ -614,19 +614,19 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
         auto f = ctx->sweep_tuned.find(key);
         if (f != ctx->sweep_tuned.end()) {
             use = SweepMethod{f->second.first, f->second.second};
-        } else if (prm->num_iter >= 64) {
+        } else if (prm->num_iter >= 160) {
             // time each candidate on this solve's own iterations (every method
-            // computes the same values): one warm-up pass, then >= 4 iterations
+            // computes the same values): one warm-up pass, then >= 8 iterations
             hipEvent_t e0, e1;
             MH_CHECK(ctx, hipEventCreate(&e0));
             MH_CHECK(ctx, hipEventCreate(&e1));
             float best = 0.f;
             for (const SweepMethod &m : kSweepCandidates) {
-                if (!usable(m) || it + 8 > prm->num_iter) continue;
+                if (!usable(m) || it + 12 > prm->num_iter) continue;
                 step(m);
                 MH_CHECK(ctx, hipEventRecord(e0, s));
                 int done = 0;
-                while (done < 4) done += step(m);
+                while (done < 8) done += step(m);
                 MH_CHECK(ctx, hipEventRecord(e1, s));
                 MH_CHECK(ctx, hipEventSynchronize(e1));
                 float ms = 0.f;
