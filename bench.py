@@ -11,13 +11,14 @@ capacities, |grad I|^2) are prepared once, outside the timed region.
 Reported beside it: every cell's fast Dice against the noise-free mask
 (counted on the device); the same solves run one at a time (device-resident,
 and through the numpy drop-in API); a 256^3 x 1000-iteration single solve
-(the bandwidth-scale plain workload); serving latency (p50/p99) and
+(the bandwidth-scale plain workload); a ragged set of helix phantoms through
+the whole pipeline with one packed max-flow solve; serving latency (p50/p99) and
 throughput under Poisson arrivals of mixed-size cuts; the HBM roofline of the
 max-flow iteration (84 algorithmic bytes per voxel-iteration) with the HBM
 traffic rocprofv3 measured for the same kernels; and the CPU oracle timed on
 a bounded sample on this host.
 
-``--only PART`` runs one part (grid, plain, plain256, serve, cpu, or copy:
+``--only PART`` runs one part (grid, plain, plain256, ragged, serve, cpu, or copy:
 a 512 MiB device copy used to calibrate the rocprofv3 byte counters).
 """
 import argparse
@@ -43,7 +44,7 @@ TRAFFIC_SWEEP_PER_VOXEL_ITER = 45.0    # two-level sweep, 256^3
 TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 GRID_GROUP = 4                         # solves per group of the default batched schedule
 
-PARTS = ("grid", "plain", "plain256", "serve", "cpu")
+PARTS = ("grid", "plain", "plain256", "ragged", "serve", "cpu")
 
 
 def log(msg):
@@ -105,6 +106,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--ragged-volumes", type=int, default=16)
+    ap.add_argument("--no-ragged", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no "
@@ -115,6 +118,7 @@ def main():
         run["cpu"] = not args.no_cpu
         run["plain256"] = not args.no_256
         run["serve"] = not args.no_serve
+        run["ragged"] = not args.no_ragged
 
     import torch
     from miccai2016_amd import phantom as ph
@@ -279,6 +283,40 @@ def main():
                                  TRAFFIC_PLAIN_PER_VOXEL_ITER}
         log(f"256^3 x {it256} iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s")
         del qb, al_b, cs_b, ct_b, u0b, ub_
+
+    # ---- ragged set: the full pipeline per volume, one packed max-flow solve
+    if run["ragged"]:
+        from miccai2016_amd import _lib
+        from miccai2016_amd.evaluation import DeviceDice
+        from miccai2016_amd.pipeline import segment_ragged
+        specs, rw, rd = ph.ragged_set(args.ragged_volumes, seed=2016)
+        vols, truths = [], []
+        for sp in specs:
+            nv = int(np.prod(sp.shape))
+            im, tr = dev.empty(nv), dev.empty(nv, dtype=torch.uint8)
+            dev.check(dev.lib.mh_helix_phantom(dev.handle, _lib.dims3(sp.shape), sp.r_tube,
+                                               sp.r_helix, sp.pitch, sp.sigma,
+                                               C.c_uint64(sp.seed), dev.ptr(im), dev.ptr(tr),
+                                               None, dev.stream), "mh_helix_phantom")
+            vols.append((im, sp.shape))
+            truths.append(tr)
+        rcfg = PipelineConfig()
+        rcfg.cut_iters = args.iters
+        rr = segment_ragged(vols, [sp.sigma for sp in specs], rw, rd, rcfg, dev)
+        total = int(sum(np.prod(sp.shape) for sp in specs))
+        dice = [float(DeviceDice(tr, dev)(rr.volume(v).reshape(-1), 1)[0])
+                for v, tr in enumerate(truths)]
+        line["ragged"] = {
+            "volumes": len(specs), "total_voxels": total, "iterations": args.iters,
+            "prepare_s": rr.prepare_s, "solve_s": rr.solve_s,
+            "voxel_iters_per_s": total * args.iters / rr.solve_s,
+            "median_fast_dice_vs_truth": float(np.nanmedian(dice)),
+            "workload": "dims U[48,192] per axis, tube U[3,8], helix radius min(dims)/4, pitch "
+                        "U[20,80], noise U[0.1,1.5], (w, decay) from the grid; Frangi, adjustment "
+                        "and GVF per volume, one packed ragged max-flow solve"}
+        log(f"ragged: {len(specs)} volumes, {total} voxels, prepare {rr.prepare_s:.2f} s, "
+            f"solve {rr.solve_s:.3f} s")
+        del vols, truths, rr
 
     # ---- serving: Poisson arrivals of synthetic helix cuts of mixed sizes
     if run["serve"]:

@@ -123,6 +123,23 @@ GRID_DECAYS = np.logspace(-1, 3, 16 + 1)
 A_ATV = 10 ** (-1.5)
 
 
+def ragged_set(count, seed=0, side=(48, 192)):
+    """Specs and (weighting, decay) of a ragged set: dims uniform per axis in
+    `side`, tube radius U[3, 8], helix radius a quarter of the smallest side,
+    pitch U[20, 80], noise U[0.1, 1.5], (w, decay) drawn from the grid; every
+    volume's noise seed is drawn from the master seed."""
+    rng = np.random.default_rng(seed)
+    specs, w, d = [], [], []
+    for _ in range(int(count)):
+        dims = tuple(int(v) for v in rng.integers(side[0], side[1] + 1, size=3))
+        specs.append(HelixSpec(dims, float(rng.uniform(3, 8)), 0.25 * min(dims),
+                               float(rng.uniform(20, 80)), float(rng.uniform(0.1, 1.5)),
+                               seed=int(rng.integers(1 << 31))))
+        w.append(float(rng.choice(GRID_WEIGHTINGS)))
+        d.append(float(rng.choice(GRID_DECAYS)))
+    return specs, np.array(w), np.array(d)
+
+
 def grid_pairs():
     """All (weighting, decay) pairs in the reference's itertools.product order."""
     w, d = np.meshgrid(GRID_WEIGHTINGS, GRID_DECAYS, indexing="ij")

@@ -140,3 +140,68 @@ def segment(img, prep, weighting, decay, cfg=None, loader=None, atv=True):
     ntw = phantom.nonterminal_weight(prep.gms, weighting, decay)
     return cutter.execute(img, prep.quats, (1, a, a), ntw, prep.cs, prep.ct, labels=None,
                           scaling=cfg.voxel_size, verbose=False)
+
+
+@dataclass
+class RaggedResult:
+    """Labellings of a ragged set: `u` packed (float32 device tensor), volume
+    v at element offset offsets[v] with shape shapes[v]."""
+    u: object
+    shapes: list
+    offsets: np.ndarray
+    results: list          # per-volume (iterations, stop flag)
+    prepare_s: float
+    solve_s: float
+
+    def volume(self, v):
+        o, s = int(self.offsets[v]), self.shapes[v]
+        return self.u[o:o + int(np.prod(s))].reshape(s)
+
+
+def segment_ragged(imgs, noise_levels, weightings, decays, cfg=None, loader=None, atv=True):
+    """The experiment pipeline over volumes of different shapes, each with its
+    own noise level and (weighting, decay): every volume is prepared on the
+    device (prepare_device), its fields are packed at element offsets, and
+    one ragged max-flow solve segments all of them together. imgs: numpy
+    volumes or (device tensor, shape) pairs. Returns a RaggedResult."""
+    import time
+    from . import _lib
+    cfg = PipelineConfig() if cfg is None else cfg
+    dev = default_context() if loader is None else loader
+    torch = dev.torch
+    t0 = time.perf_counter()
+    shapes, parts = [], {k: [] for k in ("u", "q", "a", "cs", "ct")}
+    for img, nl, w, d in zip(imgs, noise_levels, weightings, decays):
+        if isinstance(img, tuple):
+            img_t, shape = img[0].reshape(-1), tuple(img[1])
+        else:
+            shape = np.shape(img)
+            img_t = dev.upload(np.asarray(img, dtype=np.float32)).reshape(-1)
+        n = int(np.prod(shape))
+        prep = prepare_device(img_t, shape, nl, cfg, dev)
+        a, u0 = dev.empty(n), dev.empty(n)
+        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(prep.gms), n, float(w),
+                                                float(d), dev.ptr(a), dev.stream),
+                  "mh_nonterminal_weight")
+        work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(img_t), n, dev.ptr(u0), dev.ptr(work),
+                                       dev.stream), "mh_initial_u")
+        shapes.append(shape)
+        for k, t in zip(("u", "q", "a", "cs", "ct"), (u0, prep.quats, a, prep.cs, prep.ct)):
+            parts[k].append(t)
+        del prep
+    packed = {k: torch.cat(v) for k, v in parts.items()}
+    del parts
+    sizes = [int(np.prod(s)) for s in shapes]
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    dev.synchronize()
+    t1 = time.perf_counter()
+    a_atv = cfg.a_atv if atv else 1.0
+    cutter = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
+    cutter.execute_ragged_device(packed["u"], packed["q"], packed["a"], packed["cs"],
+                                 packed["ct"], shapes, offsets, (1, a_atv, a_atv),
+                                 cfg.voxel_size)
+    dev.synchronize()
+    t2 = time.perf_counter()
+    res = [(r.iterations, r.stopped) for r in cutter.last_result]
+    return RaggedResult(packed["u"], shapes, offsets, res, t1 - t0, t2 - t1)

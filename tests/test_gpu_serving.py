@@ -165,3 +165,32 @@ def test_poisson_load_on_a_background_thread(dev):
     assert st["completed"] == 12 and st["p50_s"] <= st["p99_s"]
     assert st["voxel_iterations"] == sum(int(np.prod(t.request.shape)) * t.iterations
                                          for t in tickets)
+
+
+def test_ragged_pipeline_equals_per_volume_pipeline(dev):
+    """segment_ragged packs every volume's device-prepared fields and solves
+    them in one ragged launch sequence: each labelling equals the volume's
+    own preparation plus a plain solve, bit for bit."""
+    from miccai2016_amd.pipeline import PipelineConfig, prepare_device, segment_ragged
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    specs, w, d = ph.ragged_set(3, seed=4, side=(20, 44))
+    imgs = [ph.make_helix(sp)[0] for sp in specs]
+    cfg = PipelineConfig()
+    cfg.gvf_iters, cfg.cut_iters = 40, 90
+    rr = segment_ragged(imgs, [sp.sigma for sp in specs], w, d, cfg, dev)
+    assert [r[0] for r in rr.results] == [90] * 3
+    for v, (img, sp) in enumerate(zip(imgs, specs)):
+        x = img.astype(np.float32)
+        n = x.size
+        x_t = dev.upload(x).reshape(-1)
+        prep = prepare_device(x_t, x.shape, sp.sigma, cfg, dev)
+        a, u = dev.empty(n), dev.empty(n)
+        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(prep.gms), n, float(w[v]),
+                                                float(d[v]), dev.ptr(a), dev.stream))
+        work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), n, dev.ptr(u), dev.ptr(work),
+                                       dev.stream))
+        c = CmfCutQAnisotropicGpu(_params(90), dev)
+        c.execute_device(u, prep.quats, a, prep.cs, prep.ct, const_sqevals=np.sqrt(EIGS),
+                         shape=x.shape)
+        assert np.array_equal(dev.download(rr.volume(v)), dev.download(u).reshape(x.shape))

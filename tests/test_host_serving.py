@@ -75,3 +75,15 @@ def test_footprints():
     assert serving.request_footprint_bytes(ph_req, 500, 30) == base + 5 * n
     with pytest.raises(ValueError):
         serving.job_footprint_bytes((2000, 2000, 1000), 500, 30)
+
+
+def test_ragged_set_follows_the_workload():
+    specs, w, d = ph.ragged_set(64, seed=9)
+    assert len(specs) == 64 and len(set(sp.seed for sp in specs)) == 64
+    for sp in specs:
+        assert all(48 <= v <= 192 for v in sp.shape)
+        assert 3 <= sp.r_tube <= 8 and 20 <= sp.pitch <= 80 and 0.1 <= sp.sigma <= 1.5
+        assert sp.r_helix == 0.25 * min(sp.shape)
+    assert set(w) <= set(ph.GRID_WEIGHTINGS) and set(d) <= set(ph.GRID_DECAYS)
+    again = ph.ragged_set(64, seed=9)[0]
+    assert [sp.shape for sp in again] == [sp.shape for sp in specs]
