@@ -36,7 +36,15 @@ def _shortest_foreign_edges(pts, labels, ncomp, tree):
     best_w = np.full(ncomp, np.inf)
     best_u = np.full(ncomp, -1, np.int64)
     best_v = np.full(ncomp, -1, np.int64)
-    pending = np.ones(n, bool)
+    # A component holding more than an eighth of the points (a long ridge) is
+    # searched from its own points against a tree of all other points: its
+    # points' nearest neighbours are mostly its own, so growing k would be
+    # slow. The rest start with the shared tree and a growing k.
+    sizes = np.bincount(labels, minlength=ncomp)
+    big = sizes > max(n // 8, 64)
+    pending = ~big[labels]
+    for c in np.nonzero(big)[0]:
+        _search_against_others(pts, labels, c, best_w, best_u, best_v)
     k = 8
     while pending.any() and k <= 64:
         idx = np.nonzero(pending)[0]
@@ -51,11 +59,14 @@ def _shortest_foreign_edges(pts, labels, ncomp, tree):
         d = dist[rows, first[rows]]
         u = idx[rows]
         v = nb[rows, first[rows]]
-        order = np.lexsort((v, u, d))
-        for r in order:
-            c = labels[u[r]]
-            if d[r] < best_w[c]:
-                best_w[c], best_u[c], best_v[c] = d[r], u[r], v[r]
+        # per component the lightest candidate (ties: smallest u, then v)
+        comp = labels[u]
+        order = np.lexsort((v, u, d, comp))
+        cs, at = np.unique(comp[order], return_index=True)
+        r = order[at]
+        better = d[r] < best_w[cs]
+        cs, r = cs[better], r[better]
+        best_w[cs], best_u[cs], best_v[cs] = d[r], u[r], v[r]
         pending[idx[has]] = False
         # a point whose k nearest are all in its own component may still hold
         # its component's shortest outgoing edge only if that edge is longer
@@ -69,16 +80,22 @@ def _shortest_foreign_edges(pts, labels, ncomp, tree):
     # points still pending: query against the points of other components only
     if pending.any():
         for c in np.unique(labels[pending]):
-            mine = np.nonzero(pending & (labels == c))[0]
-            others = np.nonzero(labels != c)[0]
-            if len(others) == 0:
-                continue
-            ot = cKDTree(pts[others])
-            d, j = ot.query(pts[mine], k=1)
-            r = np.argmin(d)
-            if d[r] < best_w[c]:
-                best_w[c], best_u[c], best_v[c] = d[r], mine[r], others[j[r]]
+            _search_against_others(pts, labels, c, best_w, best_u, best_v,
+                                   np.nonzero(pending & (labels == c))[0])
     return best_u, best_v, best_w
+
+
+def _search_against_others(pts, labels, c, best_w, best_u, best_v, mine=None):
+    """Shortest edge from component c (or its points `mine`) to any other
+    component, by a KD-tree of the other components' points."""
+    mine = np.nonzero(labels == c)[0] if mine is None else mine
+    others = np.nonzero(labels != c)[0]
+    if len(others) == 0 or len(mine) == 0:
+        return
+    d, j = cKDTree(pts[others]).query(pts[mine], k=1)
+    r = int(np.argmin(d))
+    if d[r] < best_w[c]:
+        best_w[c], best_u[c], best_v[c] = d[r], mine[r], others[j[r]]
 
 
 def spanning_tree_edges(pts, r0):
@@ -132,20 +149,37 @@ def spanning_tree_edges(pts, r0):
 def propagate_signs(dirs, edges, root=0):
     """Signs along the tree from `root`: a child keeps its direction if its
     dot product with its (already adjusted) parent is >= 0, else flips
-    (adjust_directions.py:384-417). Returns the adjusted float32 directions."""
+    (adjust_directions.py:384-417). Returns the adjusted float32 directions.
+
+    With s = dot(child, parent) of the unadjusted directions, the child's sign
+    is the parent's times sign(s) when s != 0, +1 when s == 0 and -1 when s is
+    NaN; the products along the root paths are formed by pointer jumping
+    (log2(depth) vectorised rounds) instead of a walk over the nodes."""
     n = len(dirs)
     out = np.full((n, 3), np.nan, np.float32)
     if n == 0:
         return out
-    out[root] = dirs[root]
+    d32 = np.asarray(dirs, np.float32)
+    out[root] = d32[root]
     if len(edges) == 0:
         return out
     g = coo_matrix((np.ones(len(edges)), (edges[:, 0], edges[:, 1])), shape=(n, n)).tocsr()
     order, pred = breadth_first_order(g, root, directed=False, return_predecessors=True)
-    d32 = np.asarray(dirs, np.float32)
-    sign = np.ones(n, np.float32)
-    for j in order[1:]:
-        p = pred[j]
-        sign[j] = 1.0 if float(np.dot(d32[j], d32[p] * sign[p])) >= 0 else -1.0
-    out[order] = d32[order] * sign[order][:, None]
+    nodes = order[1:]
+    par = pred[nodes]
+    dot = np.einsum("ij,ij->i", d32[nodes], d32[par], dtype=np.float32)
+    val = np.ones(n, np.int8)
+    nxt = np.arange(n)
+    link = (dot != 0) & ~np.isnan(dot)       # sign = parent's sign x sign(dot)
+    val[nodes[link]] = np.where(dot[link] > 0, 1, -1).astype(np.int8)
+    nxt[nodes[link]] = par[link]
+    val[nodes[np.isnan(dot)]] = -1           # the >= 0 test fails: flip
+    while True:
+        nn = nxt[nxt]
+        if np.array_equal(nn, nxt):
+            break
+        val = val * val[nxt]
+        nxt = nn
+    val = val * val[nxt]                     # the last hop onto a fixed node
+    out[order] = d32[order] * val[order].astype(np.float32)[:, None]
     return out
