@@ -234,6 +234,14 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * of 4 run one after another, which keeps a 128^3 group's state within the
  * last-level cache. Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
+/* Two-level sweep of mh_cmf_solve_batched's groups (no counterpart in the
+ * reference): 2 sweeps whenever the groups run one after another, with the
+ * tile shape forced by mh_cmf_set_sweep(mode 2) or else the one the tuner of
+ * mh_cmf_solve chose for these dims (default: the first shape, 16 planes);
+ * 0 (default) and 1 use the groups' per-iteration kernels, which measure as
+ * fast at 128^3 because they share the frame and capacities in registers.
+ * Results do not depend on it, bit for bit. */
+int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
 /* Two-level sweep of mh_cmf_solve on this context (no counterpart in the
  * reference): unsupervised constant-eigenvalue solves may advance two
  * iterations per pass over the state (temporal blocking along axis 0). mode 0
@@ -244,6 +252,10 @@ int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
  * tile shapes, 0 the first) and zchunk (planes per work-group, 0: 16).
  * Results do not depend on any of them, bit for bit. */
 int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk);
+/* The method the tuner chose on this context for volumes of these dims:
+ * returns 1 and sets tile (0: the per-iteration kernels, else the 1-based tile
+ * shape) and zchunk, or returns 0 when nothing was tuned for them. */
+int mh_cmf_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile, int *zchunk);
 
 /* Packed ragged batch: volume v has dims vdims_host[3v..3v+2] and its fields
  * start at element offset voff_host[v] in every packed array (u, eigs*3,

@@ -31,18 +31,22 @@ __device__ __forceinline__ float cmf_div(float x0, float x1, float x2, float rs0
     return x0 * rs0 + x1 * rs1 + x2 * rs2;
 }
 
-__device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, float g0, float g1,
-                                            float g2, const float4 qq, float sq0, float sq1,
-                                            float sq2, float alpha)
+// sq: sqrt of the eigenvalues of the frame; rq = 1 / sq (formed once on the
+// host, in double). The norm uses the hardware square root and the
+// projection the hardware reciprocal, as the reference's kernel uses
+// native_sqrt (update_qp.cl:35, 215-221).
+__device__ __forceinline__ void cmf_p_voxel_m(float &h0, float &h1, float &h2, float g0, float g1,
+                                              float g2, const Mat3 &R, float sq0, float sq1,
+                                              float sq2, float rq0, float rq1, float rq2,
+                                              float alpha)
 {
 #pragma clang fp contract(off)
-    const Mat3 R = quat_to_mat(Quat{qq.x, qq.y, qq.z, qq.w});
     const float *m = R.m;
-    float p0 = (m[0] * h0 + m[3] * h1 + m[6] * h2) / sq0 + sq0 * (m[0] * g0 + m[3] * g1 + m[6] * g2);
-    float p1 = (m[1] * h0 + m[4] * h1 + m[7] * h2) / sq1 + sq1 * (m[1] * g0 + m[4] * g1 + m[7] * g2);
-    float p2 = (m[2] * h0 + m[5] * h1 + m[8] * h2) / sq2 + sq2 * (m[2] * g0 + m[5] * g1 + m[8] * g2);
-    const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-    const float sc = nrm > alpha ? alpha / nrm : 1.f;
+    float p0 = (m[0] * h0 + m[3] * h1 + m[6] * h2) * rq0 + sq0 * (m[0] * g0 + m[3] * g1 + m[6] * g2);
+    float p1 = (m[1] * h0 + m[4] * h1 + m[7] * h2) * rq1 + sq1 * (m[1] * g0 + m[4] * g1 + m[7] * g2);
+    float p2 = (m[2] * h0 + m[5] * h1 + m[8] * h2) * rq2 + sq2 * (m[2] * g0 + m[5] * g1 + m[8] * g2);
+    const float nrm = __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    const float sc = nrm > alpha ? alpha * __builtin_amdgcn_rcpf(nrm) : 1.f;
     p0 *= sc * sq0;
     p1 *= sc * sq1;
     p2 *= sc * sq2;
@@ -51,15 +55,25 @@ __device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, flo
     h2 = m[6] * p0 + m[7] * p1 + m[8] * p2;
 }
 
-// Returns err; updates pt, u and writes the next residual to tmp.
+// the same with the rotation given as the unit quaternion (w, x, y, z) in qq
+__device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, float g0, float g1,
+                                            float g2, const float4 qq, float sq0, float sq1,
+                                            float sq2, float rq0, float rq1, float rq2, float alpha)
+{
+    cmf_p_voxel_m(h0, h1, h2, g0, g1, g2, quat_to_mat(Quat{qq.x, qq.y, qq.z, qq.w}), sq0, sq1, sq2,
+                  rq0, rq1, rq2, alpha);
+}
+
+// Returns err; updates pt, u and writes the next residual to tmp. inv_rcc =
+// 1 / rcc (formed once on the host, in double).
 __device__ __forceinline__ float cmf_u_voxel(float div, float &pt, float &u, float &tmp, float cs,
-                                             float ct, float rcc, float ub, float uf)
+                                             float ct, float rcc, float inv_rcc, float ub, float uf)
 {
 #pragma clang fp contract(off)
     const float ps = kmin(fmaf(-rcc, u - ub, div + pt), cs);
     pt = kmin(fmaf(rcc, u - uf, ps - div), ct);
     const float err = div - ps + pt;
-    u = u - err / rcc;
+    u = u - err * inv_rcc;
     tmp = fmaf(-u, rcc, err);
     return err;
 }

@@ -25,7 +25,8 @@ struct JobDesc {
     float4 *q4;
     float *hp, *pt, *tmp, *partials, *crit_log;
     int *ctrl;
-    float rcc, rs0, rs1, rs2, ss0, ss1, ss2, sq0, sq1, sq2, error_bound, inv_n_rcc;
+    float rcc, inv_rcc, rs0, rs1, rs2, ss0, ss1, ss2, sq0, sq1, sq2, rq0, rq1, rq2, error_bound,
+        inv_n_rcc;
     int iter0, num_iter, update, maxcheck;
 };
 
@@ -109,7 +110,8 @@ __global__ void __launch_bounds__(mh::kBlock) jobs_p_kernel(const JobDesc *__res
     const float g1 = i1 < D.d1 - 1 ? (tmp[i + D.d2] - t) * D.ss1 : 0.f;
     const float g2 = i2 < D.d2 - 1 ? (tmp[i + 1] - t) * D.ss2 : 0.f;
     float h0 = D.hp[i], h1 = D.hp[n + i], h2 = D.hp[2 * n + i];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, D.q4[i], D.sq0, D.sq1, D.sq2, D.alpha[i]);
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, D.q4[i], D.sq0, D.sq1, D.sq2, D.rq0, D.rq1, D.rq2,
+                    D.alpha[i]);
     D.hp[i] = h0;
     D.hp[n + i] = h1;
     D.hp[2 * n + i] = h2;
@@ -134,7 +136,7 @@ __global__ void __launch_bounds__(mh::kBlock) jobs_u_kernel(const JobDesc *__res
         const float x2 = (i2 < D.d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f);
         const float div = mh::cmf_div(x0, x1, x2, D.rs0, D.rs1, D.rs2);
         float ui = D.u[i], pti = D.pt[i], ti;
-        const float err = mh::cmf_u_voxel(div, pti, ui, ti, D.cs[i], D.ct[i], D.rcc, 1.f, 0.f);
+        const float err = mh::cmf_u_voxel(div, pti, ui, ti, D.cs[i], D.ct[i], D.rcc, D.inv_rcc, 1.f, 0.f);
         D.pt[i] = pti;
         D.u[i] = ui;
         D.tmp[i] = ti;
@@ -201,6 +203,10 @@ int build_descs(mh_ctx *ctx, mh_cmf_job *jobs, int nj, std::vector<JobDesc> &D, 
         d.rs0 = P.rs[0]; d.rs1 = P.rs[1]; d.rs2 = P.rs[2];
         d.ss0 = P.ss[0]; d.ss1 = P.ss[1]; d.ss2 = P.ss[2];
         d.sq0 = P.sqev[0]; d.sq1 = P.sqev[1]; d.sq2 = P.sqev[2];
+        d.rq0 = float(1.0 / double(P.sqev[0]));
+        d.rq1 = float(1.0 / double(P.sqev[1]));
+        d.rq2 = float(1.0 / double(P.sqev[2]));
+        d.inv_rcc = float(1.0 / double(P.rcc));
         d.error_bound = P.error_bound;
         d.inv_n_rcc = float(1.0 / (double(n) * double(P.rcc)));
         d.iter0 = J.iter;

@@ -67,7 +67,15 @@ cmf_init_kernel(float4 *__restrict__ q4, const float *__restrict__ eigs, int eig
 struct PParams {
     float ss0, ss1, ss2;
     float sq0, sq1, sq2;  // constant sqrt(lambda) (CONST_EV)
+    float rq0, rq1, rq2;  // 1 / sq (CONST_EV)
 };
+
+PParams p_params(const mh_cmf_params *prm)
+{
+    return PParams{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2],
+                   float(1.0 / double(prm->sqev[0])), float(1.0 / double(prm->sqev[1])),
+                   float(1.0 / double(prm->sqev[2]))};
+}
 
 template <bool CONST_EV>
 __global__ void __launch_bounds__(mh::kBlock)
@@ -87,14 +95,17 @@ cmf_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp, const float4
     const float g0 = i0 < D.d0 - 1 ? (tmp[i + s0] - t) * P.ss0 : 0.f;
     const float g1 = i1 < D.d1 - 1 ? (tmp[i + D.d2] - t) * P.ss1 : 0.f;
     const float g2 = i2 < D.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
-    float sq0 = P.sq0, sq1 = P.sq1, sq2 = P.sq2;
+    float sq0 = P.sq0, sq1 = P.sq1, sq2 = P.sq2, rq0 = P.rq0, rq1 = P.rq1, rq2 = P.rq2;
     if (!CONST_EV) {
         sq0 = eigs6[6 * i];
         sq1 = eigs6[6 * i + 1];
         sq2 = eigs6[6 * i + 2];
+        rq0 = 1.f / sq0;
+        rq1 = 1.f / sq1;
+        rq2 = 1.f / sq2;
     }
     float h0 = hp[i], h1 = hp[n + i], h2 = hp[2 * n + i];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], sq0, sq1, sq2, alpha[i]);
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], sq0, sq1, sq2, rq0, rq1, rq2, alpha[i]);
     hp[i] = h0;
     hp[n + i] = h1;
     hp[2 * n + i] = h2;
@@ -138,7 +149,7 @@ cmf_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp, const float 
             ub = (l & 2) ? 0.f : 1.f;
         }
         float ui = u[i], pti = pt[i], ti;
-        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, ub, uf);
+        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, P.inv_rcc, ub, uf);
         pt[i] = pti;
         u[i] = ui;
         tmp[i] = ti;
@@ -167,6 +178,14 @@ cmf_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
         else if (crit < error_bound) ctrl[kStop] = 2;
     }
 }
+
+constexpr int kSI = 8;  // most solves per group of the batched solver
+
+// per-solve constants of a group: alpha = w exp(-|grad I|^2 k), k = 1 / decay^2
+struct SolveParams {
+    float w[kSI];
+    float k[kSI];  // 1 / decay^2
+};
 
 // ----------------------------------------------------- two-level sweep ---
 // Two iterations (levels A and B) in one pass over the state, temporally
@@ -207,6 +226,16 @@ constexpr SweepMethod kSweepCandidates[] = {{-1, 0}, {0, 16}, {0, 32}, {1, 8}, {
 struct SweepBufs {
     const float *hp, *tmp, *pt, *u;  // iteration t
     float *hpo, *tmpo, *pto, *uo;    // iteration t + levels
+};
+
+// A batched pass: `ns` solves of one volume in one launch, block b serving
+// solve b % ns; solve k's state and alpha are at offset k n (hp: 3 k n) of
+// their buffers, and it has its own stop flag, parity and criterion partials
+// (nper per solve).
+struct SweepBatch {
+    int s0, ns, nper;
+    const int *stop;
+    int *parity;
 };
 
 // Inputs of one point at one plane, loaded a step ahead of their use.
@@ -258,14 +287,35 @@ __device__ __forceinline__ void sweep_load(SweepIn &L, const SweepBufs &B,
     }
 }
 
-template <int TY, int TX, int NT>
+template <int TY, int TX, int NT, bool BATCH>
 __global__ void __launch_bounds__(NT)
 cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__restrict__ alpha,
                  const float *__restrict__ cs, const float *__restrict__ ct, const Dims D,
                  const PParams PP, const UParams UP, int zchunk, int levels, int check,
-                 int out_set, int *__restrict__ ctrl, float *__restrict__ partials)
+                 int out_set, int *__restrict__ ctrl, float *__restrict__ partials,
+                 const SweepBatch SB)
 {
-    if (ctrl[kStop]) return;
+    // block -> (solve of the batch, tile)
+    unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    int ks = 0;
+    if (BATCH) {
+        ks = int(b % unsigned(SB.ns));
+        b /= unsigned(SB.ns);
+    }
+    if (BATCH ? SB.stop[SB.s0 + ks] : ctrl[kStop]) return;
+    const int64_t nvox = int64_t(D.d0) * D.d1 * D.d2;
+    if (BATCH) {
+        const int64_t o = nvox * ks;
+        B.hp += 3 * o;
+        B.tmp += o;
+        B.pt += o;
+        B.u += o;
+        B.hpo += 3 * o;
+        B.tmpo += o;
+        B.pto += o;
+        B.uo += o;
+        alpha += o;
+    }
     constexpr int EY = TY + 3, EX = TX + 3, NE = EY * EX;
     static_assert(NE <= NT, "one point per thread");
     // axis-1 and axis-2 components of the hp planes (the axis-0 component is
@@ -275,7 +325,6 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     __shared__ float sB1[NE], sB2[NE];  // level-B hp at plane z-1
 
     const int ntx = (D.d2 + TX - 1) / TX, nty = (D.d1 + TY - 1) / TY;
-    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
     const int tx = int(b % unsigned(ntx)), ty = int((b / unsigned(ntx)) % unsigned(nty));
     const int tz = int(b / unsigned(ntx * nty));
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zchunk;
@@ -306,7 +355,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     float hB2_0 = 0.f;                             // level-B hp (axis 0) at z-2
     float ptAp = 0.f, uAp = 0.f, tAp = 0.f;       // level-A pt, u, tmp at z-1
     float aP = 0.f, csP = 0.f, ctP = 0.f;
-    float4 qP = make_float4(1.f, 0.f, 0.f, 0.f);
+    mh::Mat3 RP = mh::quat_to_mat(mh::Quat{1.f, 0.f, 0.f, 0.f});  // frame at z-1
     float err_acc = 0.f;
 
     SweepIn L;
@@ -324,6 +373,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         const int64_t zoff = int64_t(z) * st0;
         // 1. level-A p-step at plane z on the widened tile
         float hA0_0 = 0.f, hA0_1 = 0.f, hA0_2 = 0.f;
+        // the frame of plane z, built once for both levels
+        const mh::Mat3 RN = mh::quat_to_mat(mh::Quat{L.q.x, L.q.y, L.q.z, L.q.w});
         if (ph1) {
             const float g0 = e0 ? (L.t0 - L.t) * PP.ss0 : 0.f;
             const float g1 = (f & 16u) ? (L.t1 - L.t) * PP.ss1 : 0.f;
@@ -331,7 +382,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             hA0_0 = L.h0;
             hA0_1 = L.h1;
             hA0_2 = L.h2;
-            mh::cmf_p_voxel(hA0_0, hA0_1, hA0_2, g0, g1, g2, L.q, PP.sq0, PP.sq1, PP.sq2, L.a);
+            mh::cmf_p_voxel_m(hA0_0, hA0_1, hA0_2, g0, g1, g2, RN, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
+                              PP.rq1, PP.rq2, L.a);
             if (!two && inO && z >= z0) {
                 st(B.hpo + zoff, ob, hA0_0);
                 st(B.hpo + n + zoff, ob, hA0_1);
@@ -352,7 +404,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
             ptA0 = L.pt;
             uA0 = L.u;
-            const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, 1.f, 0.f);
+            const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, UP.inv_rcc, 1.f, 0.f);
             if (!two) {
                 st(B.pto + zoff, ob, ptA0);
                 st(B.uo + zoff, ob, uA0);
@@ -361,7 +413,6 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             }
         }
         // constants of plane z, needed at plane z-1's turn in the next step
-        const float4 qN = L.q;
         const float aN = L.a, csN = L.cs, ctN = L.ct;
         // inputs of the next step, in flight while the level-B work runs
         if (z + 1 <= zend)
@@ -376,7 +427,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             hB1_0 = hA1_0;
             hB1_1 = hA1_1;
             hB1_2 = hA1_2;
-            mh::cmf_p_voxel(hB1_0, hB1_1, hB1_2, g0, g1, g2, qP, PP.sq0, PP.sq1, PP.sq2, aP);
+            mh::cmf_p_voxel_m(hB1_0, hB1_1, hB1_2, g0, g1, g2, RP, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
+                              PP.rq1, PP.rq2, aP);
         }
         if (p < NE) {
             sB1[p] = hB1_1;
@@ -391,7 +443,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             const float x2v = ((f & 32u) ? hB1_2 : 0.f) - ((f & 128u) ? sB2[p - 1] : 0.f);
             const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
             float pti = ptAp, ui = uAp, ti;
-            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, 1.f, 0.f);
+            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, UP.inv_rcc, 1.f, 0.f);
             st(B.hpo + zb, ob, hB1_0);
             st(B.hpo + n + zb, ob, hB1_1);
             st(B.hpo + 2 * n + zb, ob, hB1_2);
@@ -410,40 +462,74 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         ptAp = ptA0;
         uAp = uA0;
         tAp = tA0;
-        qP = qN;
+        RP = RN;
         aP = aN;
         csP = csN;
         ctP = ctN;
     }
     if (check) {
         const float v = mh::block_reduce<SumOp>(err_acc);
-        if (threadIdx.x == 0) partials[blockIdx.x] = v;
+        if (threadIdx.x == 0) {
+            if (BATCH) partials[int64_t(SB.s0 + ks) * SB.nper + b] = v;
+            else partials[blockIdx.x] = v;
+        }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl[kParity] = out_set;
+    if (threadIdx.x == 0) {
+        if (BATCH && b == 0) SB.parity[SB.s0 + ks] = out_set;
+        else if (!BATCH && blockIdx.x == 0) ctrl[kParity] = out_set;
+    }
 }
 
 template <int TY, int TX, int NT>
-void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SB, const float4 *q4,
+void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const float4 *q4,
                     const float *alpha, const float *cs, const float *ct, const Dims &D,
                     const PParams &PP, const UParams &UP, int zchunk, int lv, int chk, int out_set,
-                    int *ctrl, float *partials)
+                    int *ctrl, float *partials, const SweepBatch *batch)
 {
-    hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT>), dim3(grid), dim3(NT), 0, s, SB, q4, alpha, cs,
-                       ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials);
+    if (batch)
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, true>), dim3(grid), dim3(NT), 0, s, SBf, q4,
+                           alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, *batch);
+    else
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, false>), dim3(grid), dim3(NT), 0, s, SBf, q4,
+                           alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials,
+                           SweepBatch{});
 }
 
-void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &SB,
+// grid: work-groups per solve times the number of solves (batch->ns)
+void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &SBf,
                   const float4 *q4, const float *alpha, const float *cs, const float *ct,
                   const Dims &D, const PParams &PP, const UParams &UP, int zchunk, int lv, int chk,
-                  int out_set, int *ctrl, float *partials)
+                  int out_set, int *ctrl, float *partials, const SweepBatch *batch = nullptr)
 {
     switch (shape_ix) {
-    case 1: launch_sweep_t<11, 32, 512>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
-    case 2: launch_sweep_t<4, 64, 512>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
-    case 3: launch_sweep_t<26, 32, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
-    case 4: launch_sweep_t<13, 61, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
-    default: launch_sweep_t<12, 64, 1024>(grid, s, SB, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials); break;
+    case 1: launch_sweep_t<11, 32, 512>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 2: launch_sweep_t<4, 64, 512>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 3: launch_sweep_t<26, 32, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 4: launch_sweep_t<13, 61, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    default: launch_sweep_t<12, 64, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
     }
+}
+
+// alpha of the solves of a group: w exp(-|grad I|^2 k), the expression the
+// batched p-step forms in registers
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_group_alpha_kernel(const float *__restrict__ gms, int64_t n, const SolveParams SP,
+                       float *__restrict__ alpha)
+{
+    const int k = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) alpha[n * k + i] = SP.w[k] * expf(-gms[i] * SP.k[k]);
+}
+
+// u of every solve of a batched group whose last pass wrote the group buffers
+__global__ void __launch_bounds__(mh::kBlock)
+cmf_batch_sweep_final_kernel(float *__restrict__ u, const float *__restrict__ u2, int64_t n, int s0,
+                             const int *__restrict__ parity)
+{
+    const int k = blockIdx.y;
+    if (parity[s0 + k] != 1) return;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) u[n * (s0 + k) + i] = u2[n * k + i];
 }
 
 // u of the buffer set that holds the final state, copied to the caller's u
@@ -534,8 +620,8 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
                        W.pt, W.tmp, u, cs, ct, n, prm->rcc);
     MH_CHECK_LAUNCH(ctx, "cmf_init_kernel");
 
-    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
-    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
+    const PParams PP = p_params(prm);
+    const UParams UP{prm->rcc, float(1.0 / double(prm->rcc)), prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
                      float(1.0 / (double(n) * double(prm->rcc)))};
     const bool sup = labels != nullptr;
     // An iteration runs either as the plain kernels (in place on the current
@@ -681,7 +767,6 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
 // =========================================================================
 namespace {
 
-constexpr int kSI = 8;  // solves per thread
 
 struct BatchCtrl {
     int *stop;        // [S]
@@ -691,10 +776,7 @@ struct BatchCtrl {
     int maxcheck;
 };
 
-struct SolveParams {
-    float w[kSI];
-    float k[kSI];  // 1 / decay^2
-};
+
 
 __global__ void __launch_bounds__(mh::kBlock)
 cmf_batch_init_kernel(float *__restrict__ hp, float *__restrict__ pt, float *__restrict__ tmp,
@@ -761,7 +843,7 @@ cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
         const float g0 = e0 ? (a0[k] - t[k]) * P.ss0 : 0.f;
         const float g1 = e1 ? (a1[k] - t[k]) * P.ss1 : 0.f;
         const float g2 = e2 ? (a2[k] - t[k]) * P.ss2 : 0.f;
-        mh::cmf_p_voxel(h0[k], h1[k], h2[k], g0, g1, g2, qq, P.sq0, P.sq1, P.sq2,
+        mh::cmf_p_voxel(h0[k], h1[k], h2[k], g0, g1, g2, qq, P.sq0, P.sq1, P.sq2, P.rq0, P.rq1, P.rq2,
                         SP.w[k] * expf(-gm * SP.k[k]));
         float *h = hp + 3 * n * (s0 + k);
         h[i] = h0[k];
@@ -812,7 +894,7 @@ cmf_batch_u_kernel(const float *__restrict__ hp, float *__restrict__ tmp,
             const float x2 = (e2 ? ha2[k] : 0.f) - (f2 ? hb2[k] : 0.f);
             const float div = mh::cmf_div(x0, x1, x2, P.rs0, P.rs1, P.rs2);
             float ti;
-            const float err = mh::cmf_u_voxel(div, pp[k], uu[k], ti, csi, cti, P.rcc, 1.f, 0.f);
+            const float err = mh::cmf_u_voxel(div, pp[k], uu[k], ti, csi, cti, P.rcc, P.inv_rcc, 1.f, 0.f);
             const int64_t o = n * (s0 + k) + i;
             pt[o] = pp[k];
             u[o] = uu[k];
@@ -850,10 +932,12 @@ cmf_batch_crit_kernel(const float *__restrict__ partials, int nparts, float inv_
 }
 
 struct BatchWorkspace {
-    int *stop, *ncrit;
+    int *stop, *ncrit, *parity;
     unsigned *count;
     float *crit_log, *partials, *hp, *pt, *tmp;
     float4 *q4;
+    // second state set of one group for the two-level sweep
+    float *hp2, *pt2, *tmp2, *u2, *alpha2;  // and the group's alpha fields
     int maxcheck;
 };
 
@@ -876,6 +960,13 @@ int64_t batch_layout(int64_t n, int S, int maxcheck, char *base, BatchWorkspace 
     ws.hp = reinterpret_cast<float *>(take(sizeof(float) * 3 * n * S));
     ws.pt = reinterpret_cast<float *>(take(sizeof(float) * n * S));
     ws.tmp = reinterpret_cast<float *>(take(sizeof(float) * n * S));
+    ws.parity = reinterpret_cast<int *>(take(sizeof(int) * int64_t(S)));
+    const int64_t g = std::min(S, kSI);
+    ws.hp2 = reinterpret_cast<float *>(take(sizeof(float) * 3 * n * g));
+    ws.pt2 = reinterpret_cast<float *>(take(sizeof(float) * n * g));
+    ws.tmp2 = reinterpret_cast<float *>(take(sizeof(float) * n * g));
+    ws.u2 = reinterpret_cast<float *>(take(sizeof(float) * n * g));
+    ws.alpha2 = reinterpret_cast<float *>(take(sizeof(float) * n * g));
     ws.maxcheck = maxcheck;
     if (w) *w = ws;
     return off;
@@ -959,8 +1050,8 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                        u, cs, ct, n, prm->rcc);
     MH_CHECK_LAUNCH(ctx, "batched init");
 
-    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
-    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
+    const PParams PP = p_params(prm);
+    const UParams UP{prm->rcc, float(1.0 / double(prm->rcc)), prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
                      float(1.0 / (double(n) * double(prm->rcc)))};
     const BatchCtrl C{W.stop, W.ncrit, W.count, W.crit_log, maxcheck};
     // Schedule: groups of G solves (one thread handles a voxel of all G); a
@@ -979,7 +1070,54 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
             gp[g].w[k] = w_host[j];
             gp[g].k[k] = float(1.0 / (double(decay_host[j]) * double(decay_host[j])));
         }
-    for (int base = 0; base < prm->num_iter; base += V) {
+    hipLaunchKernelGGL(zero_ints, dim3((S + 255) / 256), dim3(256), 0, s, W.parity, S);
+    // the two-level sweep runs a group's iterations in passes over two buffer
+    // sets; it needs the groups one after another
+    const bool sweep_elig = V >= prm->num_iter && int64_t(D.d1) * D.d2 < (int64_t(1) << 27) &&
+                            ctx->batch_sweep != 1;
+    SweepMethod sm{0, 16};
+    if (ctx->sweep_mode == 2) {
+        sm = SweepMethod{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
+                         ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+    } else {
+        auto f = ctx->sweep_tuned.find({D.d0, D.d1, D.d2});
+        if (f != ctx->sweep_tuned.end() && f->second.first >= 0)
+            sm = SweepMethod{f->second.first, f->second.second};
+    }
+    const SweepShape &SS = kSweepShapes[sm.shape];
+    const int64_t nper = int64_t((D.d2 + SS.tx - 1) / SS.tx) * ((D.d1 + SS.ty - 1) / SS.ty) *
+                         ((D.d0 + sm.zchunk - 1) / sm.zchunk);
+    // measured at 128^3: the groups' per-iteration kernels (which share the
+    // frame and capacities in registers) are as fast as the sweep, so the
+    // sweep of groups is used only when asked for
+    const bool use_sweep = sweep_elig && nper <= int64_t(nb) && ctx->batch_sweep == 2;
+    for (int g = 0; use_sweep && g < ngroups; ++g) {
+        const int s0 = g * G, ns = std::min(G, S - s0);
+        const SweepBatch SBt{s0, ns, int(nper), W.stop, W.parity};
+        hipLaunchKernelGGL(cmf_group_alpha_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, gms, n, gp[g],
+                           W.alpha2);
+        int cur = 0;
+        for (int it = 0; it < prm->num_iter;) {
+            const bool chk_a = (it % prm->update) == 0;
+            const int lv = (!chk_a && it + 1 < prm->num_iter) ? 2 : 1;
+            const bool chk = ((it + lv - 1) % prm->update) == 0;
+            const int64_t o = n * s0;
+            SweepBufs B0{W.hp + 3 * o, W.tmp + o, W.pt + o, u + o, W.hp2, W.tmp2, W.pt2, W.u2};
+            SweepBufs B1{W.hp2, W.tmp2, W.pt2, W.u2, W.hp + 3 * o, W.tmp + o, W.pt + o, u + o};
+            launch_sweep(sm.shape, unsigned(nper * ns), s, cur == 0 ? B0 : B1, W.q4, W.alpha2, cs, ct, D,
+                         PP, UP, sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBt);
+            if (chk)
+                hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(ns), dim3(mh::kBlock), 0, s, W.partials,
+                                   int(nper), UP.inv_n_rcc, UP.error_bound, C, s0);
+            MH_CHECK_LAUNCH(ctx, "batched sweep");
+            cur ^= 1;
+            it += lv;
+        }
+        hipLaunchKernelGGL(cmf_batch_sweep_final_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, u, W.u2,
+                           n, s0, W.parity);
+        MH_CHECK_LAUNCH(ctx, "batched sweep final");
+    }
+    for (int base = 0; !use_sweep && base < prm->num_iter; base += V) {
         const int last = std::min(prm->num_iter, base + V);
         for (int g = 0; g < ngroups; ++g) {
             const int s0 = g * G, ns = std::min(G, S - s0);
@@ -1080,7 +1218,8 @@ ragged_p_kernel(const Vol *__restrict__ vols, int nvol, float *__restrict__ hp,
     const float g2 = i2 < V.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
     float *h = hp + 3 * V.off;
     float h0 = h[li], h1 = h[n + li], h2 = h[2 * n + li];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], P.sq0, P.sq1, P.sq2, alpha[i]);
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], P.sq0, P.sq1, P.sq2, P.rq0, P.rq1, P.rq2,
+                    alpha[i]);
     h[li] = h0;
     h[n + li] = h1;
     h[2 * n + li] = h2;
@@ -1111,7 +1250,7 @@ ragged_u_kernel(const Vol *__restrict__ vols, int nvol, const float *__restrict_
         const float x2 = (i2 < V.d2 - 1 ? h[2 * n + li] : 0.f) - (i2 > 0 ? h[2 * n + li - 1] : 0.f);
         const float div = mh::cmf_div(x0, x1, x2, P.rs0, P.rs1, P.rs2);
         float ui = u[i], pti = pt[i], ti;
-        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, 1.f, 0.f);
+        const float err = mh::cmf_u_voxel(div, pti, ui, ti, cs[i], ct[i], P.rcc, P.inv_rcc, 1.f, 0.f);
         pt[i] = pti;
         u[i] = ui;
         tmp[i] = ti;
@@ -1228,8 +1367,8 @@ MH_EXPORT int mh_cmf_solve_ragged(mh_ctx *ctx, float *u, const float *eigs, cons
     hipLaunchKernelGGL(ragged_init_kernel, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols, nvol,
                        W.q4, eigs, W.hp, W.pt, W.tmp, u, cs, ct, prm->rcc);
     MH_CHECK_LAUNCH(ctx, "ragged init");
-    const PParams PP{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2]};
-    const UParams UP{prm->rcc, 1.f / prm->rcc, prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound, 0.f};
+    const PParams PP = p_params(prm);
+    const UParams UP{prm->rcc, float(1.0 / double(prm->rcc)), prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound, 0.f};
     const BatchCtrl C{W.stop, W.ncrit, W.count, W.crit_log, maxcheck};
     for (int it = 0; it < prm->num_iter; ++it) {
         hipLaunchKernelGGL(ragged_p_kernel, dim3(unsigned(nblocks)), dim3(mh::kBlock), 0, s, W.vols, nvol,
@@ -1279,4 +1418,21 @@ MH_EXPORT int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk)
     ctx->sweep_shape = tile;
     ctx->sweep_zchunk = zchunk;
     return MH_OK;
+}
+
+MH_EXPORT int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode)
+{
+    MH_REQUIRE(ctx, mode >= 0 && mode <= 2, "mode 0 (automatic), 1 (off) or 2 (on)");
+    ctx->batch_sweep = mode;
+    return MH_OK;
+}
+
+MH_EXPORT int mh_cmf_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile, int *zchunk)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims) && tile && zchunk, "bad arguments");
+    auto f = ctx->sweep_tuned.find({int(dims[0]), int(dims[1]), int(dims[2])});
+    if (f == ctx->sweep_tuned.end()) return 0;
+    *tile = f->second.first + 1;  // 0: the per-iteration kernels
+    *zchunk = f->second.second;
+    return 1;
 }

@@ -435,15 +435,18 @@ def test_batched_solves_equal_plain_solves(dev, rng):
 
 
 def test_batched_schedules_give_identical_results(dev, rng):
-    """Every (group, visit) schedule computes the same iterations, bit for bit."""
+    """Every (group, visit) schedule, with and without the two-level sweep of
+    the groups, computes the same iterations, bit for bit."""
     img, q, _, cs, ct, gms = _cut_case(rng, (14, 19, 22))
     w = np.geomspace(0.003, 3.0, 11).astype(np.float32)
     d = np.geomspace(0.1, 1000.0, 11).astype(np.float32)
     eigs = (1, 10 ** -1.5, 10 ** -1.5)
     outs = []
     try:
-        for group, visit in [(0, 0), (1, 1), (3, 7), (4, 10 ** 6), (8, 13)]:
+        for group, visit, sweep in [(0, 0, 1), (1, 1, 0), (3, 7, 0), (4, 10 ** 6, 1), (8, 13, 0),
+                                    (4, 10 ** 6, 2), (3, 10 ** 6, 2), (8, 0, 2)]:
             dev.check(dev.lib.mh_cmf_batched_set_schedule(dev.handle, group, visit))
+            dev.check(dev.lib.mh_cmf_batched_set_sweep(dev.handle, sweep))
             c = _cutter(dev, 95, update=10, error_bound=2e-3)
             outs.append(c.execute_batched(img, q, eigs, w, d, gms.astype(np.float32), cs, ct))
             res = [(r.iterations, r.stopped) for r in c.last_result]
@@ -452,6 +455,7 @@ def test_batched_schedules_give_identical_results(dev, rng):
             assert res == res0
     finally:
         dev.check(dev.lib.mh_cmf_batched_set_schedule(dev.handle, 0, 0))
+        dev.check(dev.lib.mh_cmf_batched_set_sweep(dev.handle, 0))
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
 
