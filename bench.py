@@ -40,9 +40,12 @@ HBM_PEAK_GBS = 8000.0            # MI355X
 # and WRITE_SIZE passes scaled by a 512 MiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0);
 # the passes and their commands are recorded under profiles/
 TRAFFIC_GRID_PER_VOXEL_ITER = 67.6     # batched p + u kernels, groups of 4 solves, 128^3
-TRAFFIC_SWEEP_PER_VOXEL_ITER = 45.0    # two-level sweep, 256^3
 TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
-GRID_GROUP = 4                         # solves per group of the default batched schedule
+# two-level sweep by tile shape (1-based, as mh_cmf_sweep_choice reports it), at 128^3
+# (the grid) and at 256^3
+TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 41.7, 2: 43.8, 3: 42.1, 5: 43.3}
+TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.5, 2: 49.0, 3: 46.1, 5: 48.6}
+GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
 PARTS = ("grid", "plain", "plain256", "ragged", "serve", "cpu")
 
@@ -62,6 +65,26 @@ def timed(torch, fn, reps=1):
     e1.record(s)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / 1e3
+
+
+def sweep_choice(dev, shape):
+    """The tile shape (1-based) of the two-level sweep the library tuned for
+    volumes of this shape, or 0 for the per-iteration kernels / not tuned."""
+    from miccai2016_amd import _lib
+    tile, zc = C.c_int(0), C.c_int(0)
+    ok = dev.lib.mh_cmf_sweep_choice(dev.handle, _lib.dims3(shape), C.byref(tile), C.byref(zc))
+    return tile.value if ok == 1 else 0
+
+
+def sweep_passes(iters, update):
+    """Passes of the two-level sweep for a solve of `iters` iterations: a
+    criterion iteration always ends its pass."""
+    it = passes = 0
+    while it < iters:
+        lv = 2 if (it % update != 0 and it + 1 < iters) else 1
+        it += lv
+        passes += 1
+    return passes
 
 
 def device_helix(dev, spec, weighting, decay, seed=42):
@@ -200,18 +223,31 @@ def main():
                                 "best_decay": float(d_all[best])}
         log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "
             f"decay={d_all[best]:.4g}")
-        groups = -(-S // GRID_GROUP)
-        t_group_iter = secs / args.steps / (groups * args.iters)
         achieved = value * BYTES_PER_VOXEL_ITER / 1e9
-        line["roofline"] = {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": TRAFFIC_GRID_PER_VOXEL_ITER * GRID_GROUP * n,
-            "kernel": "one max-flow iteration of a group of 4 solves: cmf_batch_p_kernel<4> + "
-                      "cmf_batch_u_kernel<4> (84 algorithmic B per solve-voxel-iteration; traffic "
-                      "= measured HBM bytes per such launch pair)",
-            "launch_pair_us": t_group_iter * 1e6,
-            "traffic_per_voxel_iteration": TRAFFIC_GRID_PER_VOXEL_ITER}
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
+        tile = sweep_choice(dev, img.shape)
+        if tile:
+            # solves ran one at a time through the two-level sweep: one launch
+            # is one pass of up to two iterations over one solve
+            passes = sweep_passes(args.iters, cfg.cut_update)
+            per_vi = TRAFFIC_SWEEP_PER_VOXEL_ITER.get(tile)
+            line["roofline"].update(
+                kernel=f"cmf_sweep_kernel (tile shape {tile}): one pass = two max-flow iterations of "
+                       f"one solve (84 algorithmic B per voxel-iteration; traffic = measured HBM "
+                       f"bytes per pass)",
+                launch_us=secs / args.steps / (S * passes) * 1e6,
+                traffic=None if per_vi is None else per_vi * 2 * n,
+                traffic_per_voxel_iteration=per_vi)
+        else:
+            groups = -(-S // GRID_GROUP)
+            line["roofline"].update(
+                kernel="one max-flow iteration of a group of 4 solves: cmf_batch_p_kernel<4> + "
+                       "cmf_batch_u_kernel<4> (84 algorithmic B per solve-voxel-iteration; traffic "
+                       "= measured HBM bytes per such launch pair)",
+                launch_us=secs / args.steps / (groups * args.iters) * 1e6,
+                traffic=TRAFFIC_GRID_PER_VOXEL_ITER * GRID_GROUP * n,
+                traffic_per_voxel_iteration=TRAFFIC_GRID_PER_VOXEL_ITER)
         ub = u.detach().cpu().numpy()
         del u
 
@@ -274,11 +310,13 @@ def main():
         solve256()         # tunes the method for this shape
         t256 = timed(torch, solve256)
         rate256 = nb * it256 / t256
+        tile256 = sweep_choice(dev, spec.shape)
         line["plain_256"] = {"voxel_iters_per_s": rate256, "ms_per_solve": t256 * 1e3,
-                             "iterations": it256,
+                             "iterations": it256, "sweep_tile": tile256,
                              "achieved_GBs_algorithmic": rate256 * BYTES_PER_VOXEL_ITER / 1e9,
                              "frac_algorithmic": rate256 * BYTES_PER_VOXEL_ITER / 1e9 / HBM_PEAK_GBS,
-                             "traffic_per_voxel_iteration_sweep": TRAFFIC_SWEEP_PER_VOXEL_ITER,
+                             "traffic_per_voxel_iteration_sweep":
+                                 TRAFFIC_SWEEP_256_PER_VOXEL_ITER.get(tile256),
                              "traffic_per_voxel_iteration_plain_kernels":
                                  TRAFFIC_PLAIN_PER_VOXEL_ITER}
         log(f"256^3 x {it256} iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s")

@@ -235,17 +235,19 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * last-level cache. Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
 /* Two-level sweep of mh_cmf_solve_batched's groups (no counterpart in the
- * reference): 2 sweeps whenever the groups run one after another, with the
- * tile shape forced by mh_cmf_set_sweep(mode 2) or else the one the tuner of
- * mh_cmf_solve chose for these dims (default: the first shape, 16 planes);
- * 0 (default) and 1 use the groups' per-iteration kernels, which measure as
- * fast at 128^3 because they share the frame and capacities in registers.
+ * reference). 0 (default): when the groups run one after another and the
+ * single-solve tuner picks a sweep for these dims (it is run on a probe copy
+ * of solve 0 if it has not been for these dims), solves run one at a time
+ * through that sweep; otherwise groups of 4 use the per-iteration kernels,
+ * which share the frame and capacities in registers. 1: never sweep. 2: sweep
+ * the groups whenever eligible, with the tile shape forced by
+ * mh_cmf_set_sweep(mode 2) or the tuned one.
  * Results do not depend on it, bit for bit. */
 int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
 /* Two-level sweep of mh_cmf_solve on this context (no counterpart in the
  * reference): unsupervised constant-eigenvalue solves may advance two
  * iterations per pass over the state (temporal blocking along axis 0). mode 0
- * (default) tunes: the first solve of a volume shape of >= 160 iterations times
+ * (default) tunes: the first solve of a volume shape of >= 96 iterations times
  * the plain kernels and a few sweep tile shapes on its own iterations and the
  * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
  * sweeps when the solve is eligible, with tile (1-based index of the built-in

@@ -589,30 +589,17 @@ __global__ void zero_ints(int *p, int n)
 
 }  // namespace
 
-MH_EXPORT int64_t mh_cmf_workspace_bytes(int64_t n)
-{
-    if (n <= 0 || n >= (int64_t(1) << 31)) return -1;
-    return layout(n, nullptr, nullptr);
-}
+namespace {
 
-MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float *alpha,
-                           const float *cs, const float *ct, const uint8_t *labels,
-                           const int64_t dims[3], const mh_cmf_params *prm, void *work,
-                           int64_t work_bytes, mh_cmf_result *res, float *crit_host, void *stream)
+// The iterations of one solve on workspace W: initialisation, then the
+// method chosen for these dims (tuned on the first solve of a shape), the
+// final copy of u when a sweep left it in the second buffer set. Enqueues
+// only, except while tuning.
+int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, const float *cs,
+               const float *ct, const uint8_t *labels, const Dims D, int64_t n,
+               const mh_cmf_params *prm, const Workspace &W, hipStream_t s)
 {
-    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
-    MH_REQUIRE(ctx, u && eigs && alpha && cs && ct && prm && work, "null pointer");
-    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1, "num_iter >= 0 and update >= 1 required");
-    MH_REQUIRE(ctx, prm->rcc > 0.f, "rcc must be positive");
-    const int64_t n = dims[0] * dims[1] * dims[2];
-    MH_REQUIRE(ctx, work_bytes >= layout(n, nullptr, nullptr), "workspace too small");
-    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(work) & 255) == 0, "workspace must be 256-byte aligned");
-    Workspace W;
-    layout(n, static_cast<char *>(work), &W);
-    hipStream_t s = (hipStream_t)stream;
     const unsigned nb = mh::blocks_for(n);
-    const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
-
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, W.ctrl, int(kCtrlWords));
     MH_CHECK_LAUNCH(ctx, "zero_ints");
     const int es = prm->const_evals ? 3 : 6, eo = prm->const_evals ? 0 : 3;
@@ -700,7 +687,7 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
         auto f = ctx->sweep_tuned.find(key);
         if (f != ctx->sweep_tuned.end()) {
             use = SweepMethod{f->second.first, f->second.second};
-        } else if (prm->num_iter >= 160) {
+        } else if (prm->num_iter >= 96) {
             // time each candidate on this solve's own iterations (every method
             // computes the same values): one warm-up pass, then >= 8 iterations
             hipEvent_t e0, e1;
@@ -740,6 +727,37 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
         hipLaunchKernelGGL(cmf_sweep_final_kernel, dim3(nb), dim3(mh::kBlock), 0, s, u, W.u2, n, W.ctrl);
         MH_CHECK_LAUNCH(ctx, "cmf sweep final");
     }
+    return MH_OK;
+}
+
+}  // namespace
+
+MH_EXPORT int64_t mh_cmf_workspace_bytes(int64_t n)
+{
+    if (n <= 0 || n >= (int64_t(1) << 31)) return -1;
+    return layout(n, nullptr, nullptr);
+}
+
+MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float *alpha,
+                           const float *cs, const float *ct, const uint8_t *labels,
+                           const int64_t dims[3], const mh_cmf_params *prm, void *work,
+                           int64_t work_bytes, mh_cmf_result *res, float *crit_host, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, u && eigs && alpha && cs && ct && prm && work, "null pointer");
+    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1, "num_iter >= 0 and update >= 1 required");
+    MH_REQUIRE(ctx, prm->rcc > 0.f, "rcc must be positive");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    MH_REQUIRE(ctx, work_bytes >= layout(n, nullptr, nullptr), "workspace too small");
+    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(work) & 255) == 0, "workspace must be 256-byte aligned");
+    Workspace W;
+    layout(n, static_cast<char *>(work), &W);
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned nb = mh::blocks_for(n);
+    const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
+
+    const int rc_core = solve_core(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
+    if (rc_core) return rc_core;
     int ctrl[kCtrlWords];
     MH_CHECK(ctx, hipMemcpyAsync(ctrl, W.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
     MH_CHECK(ctx, hipStreamSynchronize(s));
@@ -1013,6 +1031,43 @@ int maxcheck_for(const mh_cmf_params *p)
 
 }  // namespace
 
+namespace {
+
+// Tune the single-solve method for these dims on a probe: solve 0's fields
+// (u0, alpha from |grad I|^2) copied into scratch memory and 96 iterations of
+// a solve that cannot stop early; the context keeps the choice.
+int tune_on_probe(mh_ctx *ctx, const float *u, float w, float decay, const float *eigs,
+                  const float *gms, const float *cs, const float *ct, const Dims D, int64_t n,
+                  const mh_cmf_params *prm, hipStream_t s)
+{
+    const int64_t wbytes = layout(n, nullptr, nullptr);
+    char *mem = nullptr;
+    MH_CHECK(ctx, hipMalloc(&mem, wbytes + 2 * align256(sizeof(float) * n)));
+    float *us = reinterpret_cast<float *>(mem + wbytes);
+    float *as = reinterpret_cast<float *>(mem + wbytes + align256(sizeof(float) * n));
+    SolveParams sp{};
+    sp.w[0] = w;
+    sp.k[0] = float(1.0 / (double(decay) * double(decay)));
+    int rc = mh::check_hip(ctx, hipMemcpyAsync(us, u, sizeof(float) * n, hipMemcpyDeviceToDevice, s),
+                           "probe copy");
+    if (!rc) {
+        hipLaunchKernelGGL(cmf_group_alpha_kernel, dim3(mh::blocks_for(n), 1), dim3(mh::kBlock), 0, s,
+                           gms, n, sp, as);
+        Workspace W;
+        layout(n, mem, &W);
+        mh_cmf_params probe = *prm;
+        probe.num_iter = 96;
+        probe.error_bound = 0.f;
+        rc = solve_core(ctx, us, eigs, as, cs, ct, nullptr, D, n, &probe, W, s);
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(mem);
+    if (!rc && e != hipSuccess) rc = mh::check_hip(ctx, e, "probe");
+    return rc;
+}
+
+}  // namespace
+
 MH_EXPORT int64_t mh_cmf_batched_workspace_bytes(int64_t n, int nsolves)
 {
     if (n <= 0 || n >= (int64_t(1) << 31) || nsolves <= 0) return -1;
@@ -1059,8 +1114,37 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     // num_iter runs the groups one after another, so a group's state stays in
     // the last-level cache between its iterations when it fits there.
     int G = ctx->batch_group, V = ctx->batch_visit;
-    if (G <= 0) G = 4;
     if (V <= 0) V = std::max(1, prm->num_iter);
+    // Method. The two-level sweep runs a group's iterations in passes over two
+    // buffer sets, so it needs the groups one after another. Automatic: solve
+    // one at a time through the sweep when the tuner of single solves chose a
+    // sweep for these dims (tuning now on solve 0's fields in scratch memory
+    // if it has not), else groups of 4 through the per-iteration kernels,
+    // which share the frame and capacities in registers.
+    const std::array<int, 3> key{D.d0, D.d1, D.d2};
+    const bool sweep_elig = V >= prm->num_iter && int64_t(D.d1) * D.d2 < (int64_t(1) << 27) &&
+                            ctx->batch_sweep != 1;
+    bool want_sweep = sweep_elig && ctx->batch_sweep == 2;
+    SweepMethod sm{0, 16};
+    if (ctx->sweep_mode == 2) {
+        sm = SweepMethod{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
+                         ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+    } else {
+        if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0 && prm->num_iter >= 96 &&
+            ctx->sweep_tuned.find(key) == ctx->sweep_tuned.end()) {
+            const int rc = tune_on_probe(ctx, u, w_host[0], decay_host[0], eigs, gms, cs, ct, D, n, prm, s);
+            if (rc) return rc;
+        }
+        auto f = ctx->sweep_tuned.find(key);
+        if (f != ctx->sweep_tuned.end() && f->second.first >= 0) {
+            sm = SweepMethod{f->second.first, f->second.second};
+            if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0) {
+                want_sweep = true;
+                if (G <= 0) G = 1;
+            }
+        }
+    }
+    if (G <= 0) G = 4;
     G = std::min(G, kSI);
     const int ngroups = (S + G - 1) / G;
     std::vector<SolveParams> gp(ngroups);
@@ -1071,26 +1155,10 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
             gp[g].k[k] = float(1.0 / (double(decay_host[j]) * double(decay_host[j])));
         }
     hipLaunchKernelGGL(zero_ints, dim3((S + 255) / 256), dim3(256), 0, s, W.parity, S);
-    // the two-level sweep runs a group's iterations in passes over two buffer
-    // sets; it needs the groups one after another
-    const bool sweep_elig = V >= prm->num_iter && int64_t(D.d1) * D.d2 < (int64_t(1) << 27) &&
-                            ctx->batch_sweep != 1;
-    SweepMethod sm{0, 16};
-    if (ctx->sweep_mode == 2) {
-        sm = SweepMethod{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
-                         ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
-    } else {
-        auto f = ctx->sweep_tuned.find({D.d0, D.d1, D.d2});
-        if (f != ctx->sweep_tuned.end() && f->second.first >= 0)
-            sm = SweepMethod{f->second.first, f->second.second};
-    }
     const SweepShape &SS = kSweepShapes[sm.shape];
     const int64_t nper = int64_t((D.d2 + SS.tx - 1) / SS.tx) * ((D.d1 + SS.ty - 1) / SS.ty) *
                          ((D.d0 + sm.zchunk - 1) / sm.zchunk);
-    // measured at 128^3: the groups' per-iteration kernels (which share the
-    // frame and capacities in registers) are as fast as the sweep, so the
-    // sweep of groups is used only when asked for
-    const bool use_sweep = sweep_elig && nper <= int64_t(nb) && ctx->batch_sweep == 2;
+    const bool use_sweep = want_sweep && nper <= int64_t(nb);
     for (int g = 0; use_sweep && g < ngroups; ++g) {
         const int s0 = g * G, ns = std::min(G, S - s0);
         const SweepBatch SBt{s0, ns, int(nper), W.stop, W.parity};
