@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--iters-256", type=int, default=1000)
     ap.add_argument("--sweep", type=int, default=0,
                     help="two-level sweep of single solves: 0 tuned, 1 off, 2 on")
+    ap.add_argument("--batched-sweep", type=int, default=0,
+                    help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -152,6 +154,8 @@ def main():
 
     dev = default_context(0)
     dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, args.sweep, 0, 0), "mh_cmf_set_sweep")
+    dev.check(dev.lib.mh_cmf_batched_set_sweep(dev.handle, args.batched_sweep),
+              "mh_cmf_batched_set_sweep")
 
     if args.only == "copy":
         a = torch.ones(128 << 20, device=dev.device)
