@@ -222,7 +222,10 @@ def main():
         from miccai2016_amd.evaluation import DeviceDice
         grid_dice = DeviceDice(truth, dev)(u, S)
         best = int(np.nanargmax(grid_dice))
+        from miccai2016_amd.evaluation import device_dice_for
+        true_dice, _ = device_dice_for(u[best], truth, img.shape, dev)
         line["grid_quality"] = {"best_fast_dice": float(grid_dice[best]),
+                                "dice_of_largest_component": float(true_dice),
                                 "best_weighting": float(w_all[best]),
                                 "best_decay": float(d_all[best])}
         log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "

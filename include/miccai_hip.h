@@ -323,6 +323,19 @@ int64_t mh_dice_work_bytes(int64_t n, int nsolves);
 int mh_dice_counts(mh_ctx *ctx, const float *u_dev, int nsolves, int64_t n, const uint8_t *truth_dev,
                    void *work_dev, int64_t work_bytes, unsigned long long *counts_dev, void *stream);
 
+/* Dice of the largest face-connected component of u >= threshold against the
+ * mask truth (the experiment's dice_for / select_largest, phantom_experiment.py:
+ * 50-103, with scipy's default structure): out_dev[0] = |L| (-1 if there is no
+ * foreground), [1] = |L & T|, [2] = |T|, [3] = the smallest voxel index of L.
+ * Ties of the largest size go to the component whose first voxel comes first
+ * in raster order, as there. labels_dev (int32 per voxel, optional) receives
+ * every foreground voxel's component as its smallest voxel index, -1 for
+ * background. work: mh_largest_component_work_bytes(dims). */
+int64_t mh_largest_component_work_bytes(const int64_t dims[3]);
+int mh_largest_component_dice(mh_ctx *ctx, const float *u_dev, float threshold,
+                              const uint8_t *truth_dev, const int64_t dims[3], int *labels_dev,
+                              void *work_dev, int64_t work_bytes, long long *out_dev, void *stream);
+
 /* One unsupervised constant-eigenvalue solve whose state stays on the device
  * between calls, so jobs of different sizes can join and leave a running set
  * at iteration-chunk boundaries. All pointers are device pointers owned by

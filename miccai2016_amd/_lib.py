@@ -88,6 +88,8 @@ SIGNATURES = {
     "mh_initial_u_work_floats": (i64, [i64]),
     "mh_initial_u": (i32, [vp, vp, i64, vp, vp, vp]),
     "mh_dice_work_bytes": (i64, [i64, i32]),
+    "mh_largest_component_work_bytes": (i64, [I64x3]),
+    "mh_largest_component_dice": (i32, [vp, vp, f32, vp, I64x3, vp, vp, i64, vp, vp]),
     "mh_dice_counts": (i32, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
     "mh_helix_phantom": (i32, [vp, I64x3, f32, f32, f32, f32, C.c_uint64, vp, vp, vp, vp]),
     "mh_cmf_job_state_bytes": (i64, [i64, C.POINTER(CmfParams)]),

@@ -5,7 +5,8 @@
 ``select_largest`` keeps the largest face-connected component (scipy's
 default structure); ``dice_for`` is the Dice of that component.
 ``DeviceDice`` counts |S|, |S & T| and |T| for many labellings on the GPU
-(exact integers) so a grid search's u fields never leave the device.
+(exact integers) so a grid search's u fields never leave the device, and
+``device_dice_for`` is dice_for with the components labelled on the GPU.
 """
 import numpy as np
 import scipy.ndimage
@@ -78,3 +79,28 @@ class DeviceDice:
         den = c[:, 0] + c[:, 2]
         with np.errstate(invalid="ignore", divide="ignore"):
             return np.where(den > 0, 2 * c[:, 1] / den, np.nan)
+
+
+def device_dice_for(u, ground_truth, shape, loader=None, labels=None):
+    """dice_for on the GPU: u (device tensor, N floats) thresholded at 0.5, its
+    largest face-connected component (union-find labelling) against
+    ground_truth (numpy or device uint8). Returns (dice, size of the largest
+    component). Raises ValueError when u has no foreground, as dice_for does.
+    labels: optional int32 device tensor (N) that receives the component of
+    every voxel (its smallest voxel index; -1 for background)."""
+    dev = loader if loader is not None else default_context()
+    if isinstance(ground_truth, np.ndarray):
+        t = dev.upload(np.asarray(ground_truth) != 0, dtype=np.uint8).reshape(-1)
+    else:
+        t = ground_truth.reshape(-1)
+    wb = int(dev.lib.mh_largest_component_work_bytes(_lib.dims3(shape)))
+    work = dev.bytes(wb)
+    out = dev.empty(4, dtype=dev.torch.int64)
+    dev.check(dev.lib.mh_largest_component_dice(dev.handle, dev.ptr(u), 0.5, dev.ptr(t),
+                                                _lib.dims3(shape), dev.ptr(labels), dev.ptr(work),
+                                                wb, dev.ptr(out), dev.stream),
+              "mh_largest_component_dice")
+    size, inter, tsum, _ = (int(v) for v in dev.download(out))
+    if size < 0:
+        raise ValueError("segmentation has no foreground component")
+    return 2.0 * inter / (size + tsum), size

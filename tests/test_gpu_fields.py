@@ -178,3 +178,43 @@ def test_prepare_device_matches_host(dev):
     vh = ref32.quat2mat(ref32.qexpand(qh))[:, :, 0]
     cosang = np.abs(np.sum(vd * vh, axis=1))
     assert np.mean(cosang > np.cos(1e-2)) > 0.99
+
+
+def test_largest_component_dice(dev, rng):
+    """dice_for with the components labelled on the GPU: the same labels as
+    scipy (face connectivity, smallest voxel index per component), the same
+    largest component (first one on ties) and the same Dice."""
+    import scipy.ndimage as ndi
+    from miccai2016_amd.evaluation import device_dice_for, dice_for, select_largest
+    cases = []
+    for shape, p in [((30, 41, 37), 0.45), ((64, 64, 64), 0.3), ((5, 200, 3), 0.6), ((1, 1, 9), 0.5)]:
+        u = (rng.uniform(size=shape) < p).astype(np.float32) * 0.8 + 0.1
+        u.flat[u.size // 2] = 0.9          # at least one foreground voxel
+        cases.append((u, shape))
+    spec = ph.HelixSpec((48, 52, 50), 4.0, 12.0, 24.0, 0.6, seed=2)
+    img, truth_h = ph.make_helix(spec)
+    cases.append((img.astype(np.float32), spec.shape))
+    tie = np.zeros((4, 5, 6), np.float32)     # two components of 3 voxels: the first wins
+    tie[0, 0, :3] = 1
+    tie[3, 4, 3:] = 1
+    cases.append((tie, tie.shape))
+    for u, shape in cases:
+        truth = truth_h if shape == spec.shape else (rng.uniform(size=shape) < 0.3).astype(np.uint8)
+        lab = dev.empty(int(np.prod(shape)), dtype=dev.torch.int32)
+        u_t = dev.upload(u).reshape(-1)
+        d, size = device_dice_for(u_t, truth, shape, dev, labels=lab)
+        want_comp, want_size = select_largest(u, return_size=True)
+        assert size == want_size
+        assert d == pytest.approx(dice_for(u, truth), abs=1e-12)
+        # labels: scipy's components, each named by its smallest voxel index
+        sl, nl = ndi.label(u >= 0.5)
+        got = dev.download(lab).reshape(shape)
+        assert np.array_equal(got < 0, sl == 0)
+        flat = sl.ravel()
+        first = np.full(nl + 1, -1, np.int64)
+        order = np.argsort(flat, kind="stable")
+        starts = np.searchsorted(flat[order], np.arange(1, nl + 1))
+        first[1:] = order[starts]
+        assert np.array_equal(got.ravel()[flat > 0], first[flat[flat > 0]])
+    with pytest.raises(ValueError):
+        device_dice_for(dev.zeros(27), np.ones((3, 3, 3), np.uint8), (3, 3, 3), dev)
