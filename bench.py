@@ -129,7 +129,8 @@ def main():
     ap.add_argument("--batched-sweep", type=int, default=0,
                     help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
     ap.add_argument("--cpu-iters", type=int, default=150)
-    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--serve-qps", type=str, default="8,16",
+                    help="comma-separated arrival rates; one serving run each")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--ragged-volumes", type=int, default=16)
     ap.add_argument("--no-ragged", action="store_true")
@@ -367,28 +368,32 @@ def main():
     if run["serve"]:
         from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
         torch.cuda.empty_cache()
-        srv = MaxFlowServer(cut_parameters(cfg), chunk=50).start()
-        log(f"serving {args.serve_requests} requests at {args.serve_qps} QPS "
-            f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
-        tickets = poisson_load(srv, synthetic_request, args.serve_qps, args.serve_requests,
-                               seed=11)
-        srv.stop(drain=True, timeout=600)
-        st = srv.stats()
-        dice = [t.dice for t in tickets if t.dice is not None and not np.isnan(t.dice)]
-        line["serving"] = {
-            "qps": args.serve_qps, "requests": args.serve_requests,
-            "completed": st["completed"],
-            "errors": sum(t.error is not None for t in tickets),
-            "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
-            "voxel_iters_per_s": st.get("voxel_iters_per_s"),
-            "peak_resident_jobs": srv.peak_resident,
-            "peak_reserved_GiB": srv.peak_reserved / 2 ** 30, "budget_GiB": srv.budget / 2 ** 30,
-            "chunk": srv.chunk,
-            "median_dice_vs_truth": float(np.median(dice)) if dice else None,
-            "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, helix/noise "
-                        "as the ragged set, (w, decay) from the grid"}
-        log(f"serving: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s, "
-            f"peak {srv.peak_resident} resident")
+        line["serving"] = []
+        for qps in (float(q) for q in args.serve_qps.split(",")):
+            srv = MaxFlowServer(cut_parameters(cfg), chunk=50).start()
+            log(f"serving {args.serve_requests} requests at {qps} QPS "
+                f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
+            tickets = poisson_load(srv, synthetic_request, qps, args.serve_requests, seed=11)
+            srv.stop(drain=True, timeout=600)
+            st = srv.stats()
+            dice = [t.dice for t in tickets if t.dice is not None and not np.isnan(t.dice)]
+            line["serving"].append({
+                "qps": qps, "requests": args.serve_requests, "completed": st["completed"],
+                "errors": sum(t.error is not None for t in tickets),
+                "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
+                "voxel_iters_per_s": st.get("voxel_iters_per_s"),
+                "offered_voxel_iters_per_s": qps * float(np.mean(
+                    [np.prod(t.request.shape) * t.request.num_iter for t in tickets])),
+                "peak_resident_jobs": srv.peak_resident,
+                "peak_reserved_GiB": srv.peak_reserved / 2 ** 30,
+                "budget_GiB": srv.budget / 2 ** 30, "chunk": srv.chunk,
+                "median_dice_vs_truth": float(np.median(dice)) if dice else None,
+                "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, helix/noise "
+                            "as the ragged set, (w, decay) from the grid; cuts of >= 2^20 voxels "
+                            "as resident solves, smaller ones as packed jobs"})
+            log(f"serving at {qps} QPS: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s,"
+                f" peak {srv.peak_resident} resident")
+            del srv
 
     # ---- CPU oracle on a bounded sample of the same workload
     if run["cpu"]:

@@ -323,6 +323,42 @@ int64_t mh_dice_work_bytes(int64_t n, int nsolves);
 int mh_dice_counts(mh_ctx *ctx, const float *u_dev, int nsolves, int64_t n, const uint8_t *truth_dev,
                    void *work_dev, int64_t work_bytes, unsigned long long *counts_dev, void *stream);
 
+/* A plain solve advanced in steps (no counterpart in the reference, whose
+ * solves run start to finish, continuous_cut_anisotropic_gpu.py:276-343): the
+ * serving loop keeps large cuts resident this way and advances each by a
+ * chunk of iterations per turn, with the same methods as mh_cmf_solve
+ * (per-iteration kernels or two-level sweep), so the result is mh_cmf_solve's
+ * bit for bit. Unsupervised. work_dev: mh_cmf_workspace_bytes(n) bytes,
+ * 256-byte aligned; every pointer is a device pointer the caller owns. */
+typedef struct {
+    int64_t dims[3];
+    float *u_dev;            /* u0 before init, the labelling after finish */
+    const float *eigs_dev;   /* squeezed frames AoS x3 (const_evals) or AoS x6 */
+    const float *alpha_dev;
+    const float *cs_dev;
+    const float *ct_dev;
+    void *work_dev;
+    mh_cmf_params params;
+    int iter;                /* iterations enqueued so far (set by the calls) */
+    int cur;                 /* buffer set holding the state (set by the calls) */
+    int swept;               /* a two-level sweep has run (set by the calls) */
+    int method;              /* -2 not chosen yet, -1 per-iteration kernels, else tile shape */
+    int zchunk;
+} mh_cmf_resident;
+
+/* Enqueue the initialisation; sets iter = 0 and the method to "not chosen". */
+int mh_cmf_resident_init(mh_ctx *ctx, mh_cmf_resident *r, void *stream);
+/* Enqueue up to `iters` more iterations (never past num_iter). The method is
+ * chosen at the first step: tuned for these dims on this context, tuned now
+ * on this solve's iterations when allow_tune (blocks while timing), or the
+ * default (the sweep when the volume gives enough work-groups). */
+int mh_cmf_resident_step(mh_ctx *ctx, mh_cmf_resident *r, int iters, int allow_tune, void *stream);
+/* Blocks; *stopped = 0 running, 1 NaN criterion, 2 converged. */
+int mh_cmf_resident_status(mh_ctx *ctx, const mh_cmf_resident *r, int *stopped, void *stream);
+/* Blocks; puts the final u in u_dev and reports like mh_cmf_solve. */
+int mh_cmf_resident_finish(mh_ctx *ctx, mh_cmf_resident *r, mh_cmf_result *res, float *crit_host,
+                           void *stream);
+
 /* Dice of the largest face-connected component of u >= threshold against the
  * mask truth (the experiment's dice_for / select_largest, phantom_experiment.py:
  * 50-103, with scipy's default structure): out_dev[0] = |L| (-1 if there is no

@@ -38,6 +38,12 @@ class CmfResult(C.Structure):
     _fields_ = [("iterations", i32), ("stopped", i32), ("ncrit", i32), ("last_crit", f32)]
 
 
+class CmfResident(C.Structure):
+    _fields_ = [("dims", i64 * 3), ("u", vp), ("eigs", vp), ("alpha", vp), ("cs", vp), ("ct", vp),
+                ("work", vp), ("params", CmfParams), ("iter", i32), ("cur", i32), ("swept", i32),
+                ("method", i32), ("zchunk", i32)]
+
+
 class CmfJob(C.Structure):
     _fields_ = [("dims", i64 * 3), ("u", vp), ("eigs", vp), ("alpha", vp), ("cs", vp), ("ct", vp),
                 ("state", vp), ("params", CmfParams), ("iter", i32), ("stopped", i32)]
@@ -89,6 +95,10 @@ SIGNATURES = {
     "mh_initial_u": (i32, [vp, vp, i64, vp, vp, vp]),
     "mh_dice_work_bytes": (i64, [i64, i32]),
     "mh_largest_component_work_bytes": (i64, [I64x3]),
+    "mh_cmf_resident_init": (i32, [vp, vp, vp]),
+    "mh_cmf_resident_step": (i32, [vp, vp, i32, i32, vp]),
+    "mh_cmf_resident_status": (i32, [vp, vp, vp, vp]),
+    "mh_cmf_resident_finish": (i32, [vp, vp, vp, vp, vp]),
     "mh_largest_component_dice": (i32, [vp, vp, f32, vp, I64x3, vp, vp, i64, vp, vp]),
     "mh_dice_counts": (i32, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
     "mh_helix_phantom": (i32, [vp, I64x3, f32, f32, f32, f32, C.c_uint64, vp, vp, vp, vp]),

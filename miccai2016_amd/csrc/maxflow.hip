@@ -591,42 +591,66 @@ __global__ void zero_ints(int *p, int n)
 
 namespace {
 
-// The iterations of one solve on workspace W: initialisation, then the
-// method chosen for these dims (tuned on the first solve of a shape), the
-// final copy of u when a sweep left it in the second buffer set. Enqueues
-// only, except while tuning.
-int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, const float *cs,
-               const float *ct, const uint8_t *labels, const Dims D, int64_t n,
-               const mh_cmf_params *prm, const Workspace &W, hipStream_t s)
-{
-    const unsigned nb = mh::blocks_for(n);
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, W.ctrl, int(kCtrlWords));
-    MH_CHECK_LAUNCH(ctx, "zero_ints");
-    const int es = prm->const_evals ? 3 : 6, eo = prm->const_evals ? 0 : 3;
-    hipLaunchKernelGGL(cmf_init_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.q4, eigs, es, eo, W.hp,
-                       W.pt, W.tmp, u, cs, ct, n, prm->rcc);
-    MH_CHECK_LAUNCH(ctx, "cmf_init_kernel");
+// One solve on workspace W, advanced in steps. An iteration runs either as
+// the plain kernels (in place on the current buffer set) or inside a
+// two-level sweep (current set -> the other set); both compute the same
+// values, so a solve may mix them, and the final u is in the set the last
+// sweep wrote (ctrl[kParity]).
+struct Runner {
+    mh_ctx *ctx;
+    float *u;
+    const float *eigs, *alpha, *cs, *ct;
+    const uint8_t *labels;
+    Dims D;
+    int64_t n;
+    unsigned nb;
+    const mh_cmf_params *prm;
+    Workspace W;
+    hipStream_t s;
+    PParams PP;
+    UParams UP;
+    bool sup, sweep_elig;
+    int cur = 0, it = 0, limit = 0;
+    bool swept = false;
 
-    const PParams PP = p_params(prm);
-    const UParams UP{prm->rcc, float(1.0 / double(prm->rcc)), prm->rs[0], prm->rs[1], prm->rs[2], prm->error_bound,
-                     float(1.0 / (double(n) * double(prm->rcc)))};
-    const bool sup = labels != nullptr;
-    // An iteration runs either as the plain kernels (in place on the current
-    // buffer set) or inside a two-level sweep (current set -> the other set).
-    // Both compute the same values, so a solve may mix them; the final u is in
-    // the set the last sweep wrote (ctrl[kParity]).
-    const bool sweep_elig = prm->const_evals && !sup && int64_t(D.d1) * D.d2 < (int64_t(1) << 27);
-    auto sweep_grid_of = [&](const SweepMethod &m) {
+    Runner(mh_ctx *c, float *u_, const float *e, const float *a, const float *cs_, const float *ct_,
+           const uint8_t *l, Dims d, int64_t n_, const mh_cmf_params *p, const Workspace &w,
+           hipStream_t st)
+        : ctx(c), u(u_), eigs(e), alpha(a), cs(cs_), ct(ct_), labels(l), D(d), n(n_),
+          nb(mh::blocks_for(n_)), prm(p), W(w), s(st), PP(p_params(p)),
+          UP{p->rcc, float(1.0 / double(p->rcc)), p->rs[0], p->rs[1], p->rs[2], p->error_bound,
+             float(1.0 / (double(n_) * double(p->rcc)))},
+          sup(l != nullptr),
+          sweep_elig(p->const_evals && l == nullptr && int64_t(d.d1) * d.d2 < (int64_t(1) << 27)),
+          limit(p->num_iter)
+    {
+    }
+
+    int init()
+    {
+        hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, W.ctrl, int(kCtrlWords));
+        const int es = prm->const_evals ? 3 : 6, eo = prm->const_evals ? 0 : 3;
+        hipLaunchKernelGGL(cmf_init_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.q4, eigs, es, eo,
+                           W.hp, W.pt, W.tmp, u, cs, ct, n, prm->rcc);
+        MH_CHECK_LAUNCH(ctx, "cmf init");
+        return MH_OK;
+    }
+
+    int64_t sweep_grid_of(const SweepMethod &m) const
+    {
         const SweepShape &S = kSweepShapes[m.shape];
         return int64_t((D.d2 + S.tx - 1) / S.tx) * ((D.d1 + S.ty - 1) / S.ty) *
                ((D.d0 + m.zchunk - 1) / m.zchunk);
-    };
-    auto usable = [&](const SweepMethod &m) {
+    }
+    bool usable(const SweepMethod &m) const
+    {
         return m.shape < 0 || (sweep_elig && sweep_grid_of(m) <= int64_t(nb) + 64);
-    };
-    int cur = 0, it = 0;
-    bool swept = false;
-    auto step = [&](const SweepMethod &m) -> int {
+    }
+
+    // one iteration (plain kernels) or one pass of up to two (sweep) that
+    // does not go past `limit`; returns the iterations advanced
+    int step(const SweepMethod &m)
+    {
         if (m.shape < 0) {
             float *hp = cur ? W.hp2 : W.hp, *tmp = cur ? W.tmp2 : W.tmp;
             float *pt = cur ? W.pt2 : W.pt, *uu = cur ? W.u2 : u;
@@ -657,8 +681,9 @@ int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, con
         }
         // a criterion iteration is always the last level of its pass, so a
         // stop never skips an iteration the pass already applied
+        const int lim = std::min(prm->num_iter, limit);
         const bool chk_a = (it % prm->update) == 0;
-        const int lv = (!chk_a && it + 1 < prm->num_iter) ? 2 : 1;
+        const int lv = (!chk_a && it + 1 < lim) ? 2 : 1;
         const bool chk = ((it + lv - 1) % prm->update) == 0;
         SweepBufs SB;
         if (cur == 0) SB = SweepBufs{W.hp, W.tmp, W.pt, u, W.hp2, W.tmp2, W.pt2, W.u2};
@@ -673,63 +698,108 @@ int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, con
         swept = true;
         it += lv;
         return lv;
-    };
+    }
 
-    // choose the method
-    const SweepMethod kPlain{-1, 0};
-    SweepMethod use = kPlain;
-    const std::array<int, 3> key{D.d0, D.d1, D.d2};
-    if (ctx->sweep_mode == 2) {
-        SweepMethod m{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
-                      ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
-        if (usable(m)) use = m;
-    } else if (ctx->sweep_mode == 0 && sweep_elig) {
+    // The method for this solve: forced by the context, tuned earlier for
+    // these dims, tuned now on this solve's own iterations (allow_tune, at
+    // least 96 iterations), or the default (the sweep when eligible and the
+    // volume gives >= 512 work-groups, else the plain kernels).
+    int choose(bool allow_tune, SweepMethod &use)
+    {
+        use = SweepMethod{-1, 0};
+        const std::array<int, 3> key{D.d0, D.d1, D.d2};
+        if (ctx->sweep_mode == 1 || !sweep_elig) return MH_OK;
+        if (ctx->sweep_mode == 2) {
+            SweepMethod m{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
+                          ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+            if (usable(m)) use = m;
+            return MH_OK;
+        }
         auto f = ctx->sweep_tuned.find(key);
         if (f != ctx->sweep_tuned.end()) {
             use = SweepMethod{f->second.first, f->second.second};
-        } else if (prm->num_iter >= 96) {
-            // time each candidate on this solve's own iterations (every method
-            // computes the same values): one warm-up pass, then >= 8 iterations
-            hipEvent_t e0, e1;
-            MH_CHECK(ctx, hipEventCreate(&e0));
-            MH_CHECK(ctx, hipEventCreate(&e1));
-            float best = 0.f;
-            for (const SweepMethod &m : kSweepCandidates) {
-                if (!usable(m) || it + 12 > prm->num_iter) continue;
-                step(m);
-                MH_CHECK(ctx, hipEventRecord(e0, s));
-                int done = 0;
-                while (done < 8) done += step(m);
-                MH_CHECK(ctx, hipEventRecord(e1, s));
-                MH_CHECK(ctx, hipEventSynchronize(e1));
-                float ms = 0.f;
-                MH_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
-                const float per = ms / float(done);
-                if (best == 0.f || per < best) {
-                    best = per;
-                    use = m;
-                }
-            }
-            MH_CHECK_LAUNCH(ctx, "cmf tuning");
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            int stopped = 0;
-            MH_CHECK(ctx, hipMemcpy(&stopped, W.ctrl + kStop, sizeof(int), hipMemcpyDeviceToHost));
-            // timings of a stopped solve (empty launches) choose nothing
-            if (!stopped) ctx->sweep_tuned[key] = {use.shape, use.zchunk};
+            return MH_OK;
         }
+        if (!allow_tune || prm->num_iter < 96) {
+            const SweepMethod m{0, 16};
+            if (usable(m) && sweep_grid_of(m) >= 512) use = m;
+            return MH_OK;
+        }
+        // time each candidate on this solve's own iterations: one warm-up
+        // pass, then >= 8 iterations
+        hipEvent_t e0, e1;
+        MH_CHECK(ctx, hipEventCreate(&e0));
+        MH_CHECK(ctx, hipEventCreate(&e1));
+        float best = 0.f;
+        for (const SweepMethod &m : kSweepCandidates) {
+            if (!usable(m) || it + 12 > prm->num_iter) continue;
+            step(m);
+            MH_CHECK(ctx, hipEventRecord(e0, s));
+            int done = 0;
+            while (done < 8) done += step(m);
+            MH_CHECK(ctx, hipEventRecord(e1, s));
+            MH_CHECK(ctx, hipEventSynchronize(e1));
+            float ms = 0.f;
+            MH_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+            const float per = ms / float(done);
+            if (best == 0.f || per < best) {
+                best = per;
+                use = m;
+            }
+        }
+        MH_CHECK_LAUNCH(ctx, "cmf tuning");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        int stopped = 0;
+        MH_CHECK(ctx, hipMemcpy(&stopped, W.ctrl + kStop, sizeof(int), hipMemcpyDeviceToHost));
+        // timings of a stopped solve (empty launches) choose nothing
+        if (!stopped) ctx->sweep_tuned[key] = {use.shape, use.zchunk};
+        return MH_OK;
     }
-    while (it < prm->num_iter) {
-        step(use);
+
+    int run_until(const SweepMethod &m, int target)
+    {
+        limit = std::min(target, prm->num_iter);
+        while (it < limit) step(m);
         MH_CHECK_LAUNCH(ctx, "cmf kernels");
+        return MH_OK;
     }
-    if (swept) {
-        hipLaunchKernelGGL(cmf_sweep_final_kernel, dim3(nb), dim3(mh::kBlock), 0, s, u, W.u2, n, W.ctrl);
-        MH_CHECK_LAUNCH(ctx, "cmf sweep final");
+
+    int finish()
+    {
+        if (swept) {
+            hipLaunchKernelGGL(cmf_sweep_final_kernel, dim3(nb), dim3(mh::kBlock), 0, s, u, W.u2, n,
+                               W.ctrl);
+            MH_CHECK_LAUNCH(ctx, "cmf sweep final");
+        }
+        return MH_OK;
     }
-    return MH_OK;
+};
+
+// The iterations of one solve on workspace W: initialisation, the method
+// chosen for these dims (tuned on the first solve of a shape), the final copy
+// of u when a sweep left it in the second buffer set. Enqueues only, except
+// while tuning.
+int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, const float *cs,
+               const float *ct, const uint8_t *labels, const Dims D, int64_t n,
+               const mh_cmf_params *prm, const Workspace &W, hipStream_t s)
+{
+    Runner R(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
+    int rc = R.init();
+    if (rc) return rc;
+    SweepMethod use;
+    rc = R.choose(true, use);
+    if (rc) return rc;
+    rc = R.run_until(use, prm->num_iter);
+    if (rc) return rc;
+    return R.finish();
 }
 
+}  // namespace
+
+namespace {
+int read_result(mh_ctx *ctx, const Workspace &W, const mh_cmf_params *prm, mh_cmf_result *res,
+                float *crit_host, hipStream_t s);
 }  // namespace
 
 MH_EXPORT int64_t mh_cmf_workspace_bytes(int64_t n)
@@ -758,6 +828,15 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
 
     const int rc_core = solve_core(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
     if (rc_core) return rc_core;
+    return read_result(ctx, W, prm, res, crit_host, s);
+}
+
+namespace {
+
+// Blocks; the criterion log, stop flag and iteration count of a solve.
+int read_result(mh_ctx *ctx, const Workspace &W, const mh_cmf_params *prm, mh_cmf_result *res,
+                float *crit_host, hipStream_t s)
+{
     int ctrl[kCtrlWords];
     MH_CHECK(ctx, hipMemcpyAsync(ctrl, W.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
     MH_CHECK(ctx, hipStreamSynchronize(s));
@@ -773,6 +852,95 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
         res->last_crit = ncrit > 0 ? log[ncrit - 1] : 0.f;
     }
     return ctrl[kStop] == 1 ? MH_STOPPED_NAN : MH_OK;
+}
+
+Runner runner_of(mh_ctx *ctx, mh_cmf_resident *r, Workspace &W, hipStream_t s)
+{
+    const int64_t n = r->dims[0] * r->dims[1] * r->dims[2];
+    layout(n, static_cast<char *>(r->work_dev), &W);
+    Runner R(ctx, r->u_dev, r->eigs_dev, r->alpha_dev, r->cs_dev, r->ct_dev, nullptr,
+             Dims{int(r->dims[0]), int(r->dims[1]), int(r->dims[2])}, n, &r->params, W, s);
+    R.cur = r->cur;
+    R.it = r->iter;
+    R.swept = r->swept != 0;
+    return R;
+}
+
+int resident_check(mh_ctx *ctx, const mh_cmf_resident *r)
+{
+    MH_REQUIRE(ctx, r && mh::dims_ok(r->dims), "bad resident solve");
+    MH_REQUIRE(ctx, r->u_dev && r->eigs_dev && r->alpha_dev && r->cs_dev && r->ct_dev && r->work_dev,
+               "null pointer");
+    MH_REQUIRE(ctx, (reinterpret_cast<uintptr_t>(r->work_dev) & 255) == 0,
+               "workspace must be 256-byte aligned");
+    const mh_cmf_params &P = r->params;
+    MH_REQUIRE(ctx, P.num_iter >= 0 && P.update >= 1 && P.rcc > 0.f, "bad params");
+    return MH_OK;
+}
+
+}  // namespace
+
+MH_EXPORT int mh_cmf_resident_init(mh_ctx *ctx, mh_cmf_resident *r, void *stream)
+{
+    int rc = resident_check(ctx, r);
+    if (rc) return rc;
+    r->iter = 0;
+    r->cur = 0;
+    r->swept = 0;
+    r->method = -2;
+    r->zchunk = 0;
+    Workspace W;
+    Runner R = runner_of(ctx, r, W, (hipStream_t)stream);
+    return R.init();
+}
+
+MH_EXPORT int mh_cmf_resident_step(mh_ctx *ctx, mh_cmf_resident *r, int iters, int allow_tune,
+                                   void *stream)
+{
+    int rc = resident_check(ctx, r);
+    if (rc) return rc;
+    MH_REQUIRE(ctx, iters >= 0, "iters >= 0");
+    Workspace W;
+    Runner R = runner_of(ctx, r, W, (hipStream_t)stream);
+    SweepMethod m{r->method, r->zchunk};
+    if (r->method == -2) {
+        rc = R.choose(allow_tune != 0, m);
+        if (rc) return rc;
+        r->method = m.shape;
+        r->zchunk = m.zchunk;
+    }
+    rc = R.run_until(m, r->iter + iters > R.it ? std::max(R.it, r->iter + iters) : R.it);
+    r->iter = R.it;
+    r->cur = R.cur;
+    r->swept = R.swept ? 1 : 0;
+    return rc;
+}
+
+MH_EXPORT int mh_cmf_resident_status(mh_ctx *ctx, const mh_cmf_resident *r, int *stopped,
+                                     void *stream)
+{
+    int rc = resident_check(ctx, r);
+    if (rc) return rc;
+    MH_REQUIRE(ctx, stopped, "null pointer");
+    Workspace W;
+    const int64_t n = r->dims[0] * r->dims[1] * r->dims[2];
+    layout(n, static_cast<char *>(r->work_dev), &W);
+    hipStream_t s = (hipStream_t)stream;
+    MH_CHECK(ctx, hipMemcpyAsync(stopped, W.ctrl + kStop, sizeof(int), hipMemcpyDeviceToHost, s));
+    MH_CHECK(ctx, hipStreamSynchronize(s));
+    return MH_OK;
+}
+
+MH_EXPORT int mh_cmf_resident_finish(mh_ctx *ctx, mh_cmf_resident *r, mh_cmf_result *res,
+                                     float *crit_host, void *stream)
+{
+    int rc = resident_check(ctx, r);
+    if (rc) return rc;
+    Workspace W;
+    Runner R = runner_of(ctx, r, W, (hipStream_t)stream);
+    rc = R.finish();
+    if (rc) return rc;
+    return read_result(ctx, W, &r->params, res, crit_host, (hipStream_t)stream);
 }
 
 // =========================================================================

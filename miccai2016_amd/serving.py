@@ -2,10 +2,13 @@
 
 Requests wait in a FIFO queue. A request is admitted when its device
 footprint fits the unreserved part of an HBM budget. An admitted cut joins
-the resident set at the next chunk boundary (``mh_cmf_jobs_init``); the whole
-set, whatever the jobs' shapes, advances ``chunk`` iterations per
-``mh_cmf_jobs_step``; a finished cut leaves the set and releases its memory.
-Every job computes exactly the iterations of a lone
+the resident set at the next chunk boundary; every chunk, each cut advances
+``chunk`` iterations: cuts of at least ``resident_min_voxels`` voxels as
+resident solves (``mh_cmf_resident_step``: per-iteration kernels or the
+two-level sweep, whichever the library uses for that shape), the smaller ones
+together in one packed launch sequence (``mh_cmf_jobs_step``, any mix of
+shapes). A finished cut leaves the set and releases its memory. Every cut
+computes exactly the iterations of a lone
 ``CmfCutQAnisotropicGpu.execute_device`` call.
 
 Three request kinds:
@@ -41,6 +44,9 @@ FIELD_BYTES_PER_VOXEL = 28
 # Frangi working set (image, Hessian AoS x6, response, frames), the GVF
 # buffers (field, result and swap SoA x3, response) and the cut's fields
 PREPARE_BYTES_PER_VOXEL = 160
+# cuts of at least this many voxels run as resident solves, which may use the
+# two-level sweep; smaller ones share the packed job launches
+RESIDENT_MIN_VOXELS = 1 << 20
 
 
 @dataclass
@@ -122,10 +128,26 @@ def job_footprint_bytes(shape, num_iter, update):
     return FIELD_BYTES_PER_VOXEL * n + st + 5 * 256
 
 
-def request_footprint_bytes(req, num_iter, update):
+def resident_footprint_bytes(shape):
+    """Device bytes a cut advanced as a resident solve holds: its fields plus
+    the solve's workspace (both buffer sets of the two-level sweep)."""
+    n = int(np.prod(shape))
+    w = int(_lib.lib().mh_cmf_workspace_bytes(n))
+    if w < 0:
+        raise ValueError(f"volume of shape {tuple(shape)} is not supported")
+    return FIELD_BYTES_PER_VOXEL * n + w + 5 * 256
+
+
+def request_footprint_bytes(req, num_iter, update, resident_min=None):
+    """Reservation of a request: a cut of at least `resident_min` voxels runs
+    as a resident solve (mh_cmf_resident_*), a smaller one as a packed job
+    (mh_cmf_jobs_*); a volume to prepare needs the preparation's working set
+    first."""
+    resident_min = RESIDENT_MIN_VOXELS if resident_min is None else resident_min
     shape = req.shape if isinstance(req, PhantomCutRequest) else np.shape(req.img)
     n = int(np.prod(shape))
-    b = job_footprint_bytes(shape, num_iter, update)
+    b = (resident_footprint_bytes(shape) if n >= resident_min
+         else job_footprint_bytes(shape, num_iter, update))
     if isinstance(req, VolumeRequest):
         b = max(b, PREPARE_BYTES_PER_VOXEL * n)
     elif isinstance(req, PhantomCutRequest):
@@ -143,6 +165,7 @@ class _Job:
         self.t = {}               # device tensors kept alive while resident
         self.truth = None
         self.event = None         # producer stream event to wait on before use
+        self.resident = False     # advanced as a resident solve (else a packed job)
 
 
 class MaxFlowServer:
@@ -155,8 +178,9 @@ class MaxFlowServer:
     """
 
     def __init__(self, cut_params, budget_bytes=None, chunk=50, device=0, return_device=False,
-                 max_jobs=1024, prepare_cfg=None):
+                 max_jobs=1024, prepare_cfg=None, resident_min_voxels=RESIDENT_MIN_VOXELS):
         import torch
+        self.resident_min = int(resident_min_voxels)
         self.torch = torch
         self.params = cut_params
         self.chunk = int(chunk)
@@ -196,7 +220,8 @@ class MaxFlowServer:
         num_iter = int(request.num_iter or self.params.num_iter)
         t = Ticket(request)
         try:
-            need = request_footprint_bytes(request, num_iter, int(self.params.update))
+            need = request_footprint_bytes(request, num_iter, int(self.params.update),
+                                           self.resident_min)
         except Exception as e:        # unsupported shape
             t.error = e
             t._done.set()
@@ -451,8 +476,12 @@ class MaxFlowServer:
     # ------------------------------------------------------ resident set --
     def _init(self, new):
         dev = self.dev
-        arr = (_lib.CmfJob * len(new))()
-        for k, job in enumerate(new):
+        jobs = [j for j in new if j.n < self.resident_min]
+        for job in new:
+            if job.n >= self.resident_min:
+                self._init_resident(job)
+        arr = (_lib.CmfJob * len(jobs))()
+        for k, job in enumerate(jobs):
             job.state = dev.bytes(int(dev.lib.mh_cmf_job_state_bytes(job.n, C.byref(job.P))))
             a = arr[k]
             for i in range(3):
@@ -461,38 +490,79 @@ class MaxFlowServer:
                                     job.t["alpha"].data_ptr())
             a.cs, a.ct, a.state = job.t["cs"].data_ptr(), job.t["ct"].data_ptr(), job.state.data_ptr()
             a.params = job.P
-        for i in range(0, len(new), 1024):
-            part = (_lib.CmfJob * min(1024, len(new) - i)).from_buffer(arr, i * C.sizeof(_lib.CmfJob))
+        for i in range(0, len(jobs), 1024):
+            part = (_lib.CmfJob * min(1024, len(jobs) - i)).from_buffer(arr, i * C.sizeof(_lib.CmfJob))
             dev.check(dev.lib.mh_cmf_jobs_init(dev.handle, part, len(part), dev.stream),
                       "mh_cmf_jobs_init")
-        for k, job in enumerate(new):
+        for k, job in enumerate(jobs):
             job.handle = _lib.CmfJob.from_buffer_copy(arr[k])
         self._active.extend(new)
         self.peak_resident = max(self.peak_resident, len(self._active))
         self._jobs_arr = None
 
+    def _init_resident(self, job):
+        dev = self.dev
+        job.resident = True
+        job.state = dev.bytes(int(dev.lib.mh_cmf_workspace_bytes(job.n)))
+        r = _lib.CmfResident()
+        for i in range(3):
+            r.dims[i] = job.shape[i]
+        r.u, r.eigs, r.alpha = (job.t["u"].data_ptr(), job.t["eigs"].data_ptr(),
+                                job.t["alpha"].data_ptr())
+        r.cs, r.ct, r.work = job.t["cs"].data_ptr(), job.t["ct"].data_ptr(), job.state.data_ptr()
+        r.params = job.P
+        dev.check(dev.lib.mh_cmf_resident_init(dev.handle, C.byref(r), dev.stream),
+                  "mh_cmf_resident_init")
+        job.handle = r
+
     def _step(self):
         dev = self.dev
-        if self._jobs_arr is None:
-            self._jobs_arr = (_lib.CmfJob * len(self._active))(*[j.handle for j in self._active])
-        arr = self._jobs_arr
-        dev.check(dev.lib.mh_cmf_jobs_step(dev.handle, arr, len(arr), self.chunk, dev.stream),
-                  "mh_cmf_jobs_step")
-        keep, done = [], []
-        for k, job in enumerate(self._active):
-            job.handle = arr[k]
-            (done if arr[k].stopped or arr[k].iter >= job.P.num_iter else keep).append(job)
+        residents = [j for j in self._active if j.resident]
+        jobs = [j for j in self._active if not j.resident]
+        # large cuts: each advances by a chunk through its own launches
+        for job in residents:
+            dev.check(dev.lib.mh_cmf_resident_step(dev.handle, C.byref(job.handle), self.chunk, 0,
+                                                   dev.stream), "mh_cmf_resident_step")
+        done = []
+        if jobs:
+            if self._jobs_arr is None:
+                self._jobs_arr = (_lib.CmfJob * len(jobs))(*[j.handle for j in jobs])
+            arr = self._jobs_arr
+            dev.check(dev.lib.mh_cmf_jobs_step(dev.handle, arr, len(arr), self.chunk, dev.stream),
+                      "mh_cmf_jobs_step")   # blocks: every launch above has run
+            for k, job in enumerate(jobs):
+                job.handle = arr[k]
+                if arr[k].stopped or arr[k].iter >= job.P.num_iter:
+                    done.append(job)
+        for job in residents:
+            if job.handle.iter >= job.P.num_iter:
+                done.append(job)
+                continue
+            st = C.c_int(0)
+            dev.check(dev.lib.mh_cmf_resident_status(dev.handle, C.byref(job.handle), C.byref(st),
+                                                     dev.stream), "mh_cmf_resident_status")
+            if st.value:
+                done.append(job)
         if done:
             for job in done:
                 self._finish(job)
-            self._active = keep
-            self._jobs_arr = None
+            gone = set(id(j) for j in done)
+            self._active = [j for j in self._active if id(j) not in gone]
+            if any(not j.resident for j in done):
+                self._jobs_arr = None
 
     def _finish(self, job):
         dev = self.dev
         t = job.ticket
         h = job.handle
-        t.iterations, t.stopped = int(h.iter), int(h.stopped)
+        if job.resident:
+            res = _lib.CmfResult()
+            rc = dev.lib.mh_cmf_resident_finish(dev.handle, C.byref(h), C.byref(res), None,
+                                                dev.stream)
+            dev.check(rc, "mh_cmf_resident_finish")
+            t.iterations, t.stopped = int(res.iterations), int(res.stopped)
+        else:
+            t.iterations, t.stopped = int(h.iter), int(h.stopped)
         if job.truth is not None:
             from .evaluation import DeviceDice
             t.dice = float(DeviceDice(job.truth, dev)(job.t["u"], 1)[0])

@@ -48,11 +48,12 @@ SHAPES = [(16, 18, 20), (9, 31, 12), (24, 8, 17), (5, 5, 70)]
 EIGS = (1.0, 10 ** -1.5, 10 ** -1.5)
 
 
-def test_chunked_jobs_equal_lone_solves(dev):
+@pytest.mark.parametrize("resident_min", [1 << 40, 0, 5000])   # all packed, all resident, mixed
+def test_chunked_jobs_equal_lone_solves(dev, resident_min):
     from miccai2016_amd.serving import CutRequest, MaxFlowServer
     cases = [_case(40 + k, s) for k, s in enumerate(SHAPES)]
     iters = [200, 77, 131, 260]
-    srv = MaxFlowServer(_params(), chunk=37)
+    srv = MaxFlowServer(_params(), chunk=37, resident_min_voxels=resident_min)
     tickets = [srv.submit(CutRequest(c[0], c[1], c[2], c[3], c[4], EIGS, num_iter=it))
                for c, it in zip(cases, iters)]
     srv.run_until_idle()
@@ -64,10 +65,12 @@ def test_chunked_jobs_equal_lone_solves(dev):
         assert np.array_equal(u, want)
 
 
-def test_chunked_jobs_stop_like_lone_solves(dev):
+@pytest.mark.parametrize("resident_min", [1 << 40, 0])
+def test_chunked_jobs_stop_like_lone_solves(dev, resident_min):
     from miccai2016_amd.serving import CutRequest, MaxFlowServer
     cases = [_case(60 + k, s) for k, s in enumerate(SHAPES[:3])]
-    srv = MaxFlowServer(_params(num_iter=3000, update=7, error_bound=1e-3), chunk=50)
+    srv = MaxFlowServer(_params(num_iter=3000, update=7, error_bound=1e-3), chunk=50,
+                        resident_min_voxels=resident_min)
     tickets = [srv.submit(CutRequest(c[0], c[1], c[2], c[3], c[4], (1, 1, 1))) for c in cases]
     # a NaN frame stops its job at the first criterion, as it stops a lone solve
     bad = list(cases[0])
@@ -194,3 +197,41 @@ def test_ragged_pipeline_equals_per_volume_pipeline(dev):
         c.execute_device(u, prep.quats, a, prep.cs, prep.ct, const_sqevals=np.sqrt(EIGS),
                          shape=x.shape)
         assert np.array_equal(dev.download(rr.volume(v)), dev.download(u).reshape(x.shape))
+
+
+def test_resident_sweep_in_serving_equals_lone_solve(dev):
+    """A cut large enough for the two-level sweep, served as a resident solve
+    in chunks of an odd size, equals the lone solve bit for bit."""
+    from miccai2016_amd.serving import MaxFlowServer, PhantomCutRequest
+    req = PhantomCutRequest((64, 96, 128), 5.0, 16.0, 40.0, 0.8, 0.05, 30.0, seed=5, num_iter=123)
+    try:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, 0, 0))
+        srv = MaxFlowServer(_params(), chunk=17, resident_min_voxels=0)
+        dev2 = srv.dev
+        dev2.check(dev2.lib.mh_cmf_set_sweep(dev2.handle, 2, 0, 0))
+        t = srv.submit(req)
+        srv.run_until_idle()
+        u = t.wait(0)
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    n = int(np.prod(req.shape))
+    L, h, s = dev.lib, dev.handle, dev.stream
+    img, tan = dev.empty(n), dev.empty(3 * n)
+    dev.check(L.mh_helix_phantom(h, _lib.dims3(req.shape), req.r_tube, req.r_helix, req.pitch,
+                                 req.sigma, C.c_uint64(req.seed), dev.ptr(img), None, dev.ptr(tan), s))
+    dev.check(L.mh_complete_frames(h, dev.ptr(tan), dev.ptr(tan), n, s))
+    cs, ct, a, u0 = dev.empty(n), dev.empty(n), dev.empty(n), dev.empty(n)
+    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, req.sigma, 1e-9, dev.ptr(cs), dev.ptr(ct), s))
+    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(req.shape), dev.ptr(a), s))
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(a), n, req.weighting, req.decay, dev.ptr(a), s))
+    wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
+    dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), s))
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    c = CmfCutQAnisotropicGpu(_params(123), dev)
+    dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 1, 0, 0))
+    try:
+        c.execute_device(u0, tan, a, cs, ct, const_sqevals=np.sqrt(req.eigs), shape=req.shape)
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    assert t.iterations == 123
+    assert np.array_equal(u, dev.download(u0).reshape(req.shape))
