@@ -254,6 +254,14 @@ int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
  * tile shapes, 0 the first) and zchunk (planes per work-group, 0: 16).
  * Results do not depend on any of them, bit for bit. */
 int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk);
+/* Graph replay in mh_cmf_solve and the resident solves (no counterpart in the
+ * reference): when the per-iteration kernels run, one criterion cycle (update
+ * iterations starting at a criterion iteration) is captured as a HIP graph
+ * and replayed for the following cycles: on 1, off 0 (the default: on gfx950
+ * launched kernels were measured as fast or faster). The graph holds kernel
+ * launches only and runs on a stream of the context's own, ordered after and
+ * before the caller's stream. Results do not depend on it, bit for bit. */
+int mh_cmf_set_graphs(mh_ctx *ctx, int on);
 /* The method the tuner chose on this context for volumes of these dims:
  * returns 1 and sets tile (0: the per-iteration kernels, else the 1-based tile
  * shape) and zchunk, or returns 0 when nothing was tuned for them. */

@@ -98,6 +98,7 @@ MH_EXPORT int mh_destroy(mh_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->desc) (void)hipFree(ctx->desc);
+    if (ctx->graph_stream) (void)hipStreamDestroy(ctx->graph_stream);
     delete ctx;
     return MH_OK;
 }

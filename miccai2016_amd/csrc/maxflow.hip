@@ -760,8 +760,67 @@ struct Runner {
     int run_until(const SweepMethod &m, int target)
     {
         limit = std::min(target, prm->num_iter);
+        // With graphs on, the per-iteration kernels' criterion cycle (update
+        // iterations from a criterion iteration: the same launches every
+        // time) is captured once as a graph and replayed. The kernels read
+        // the stop flag themselves, as launched ones do, so a stop inside a
+        // replay ends the solve the same way. Off by default: on gfx950
+        // back-to-back launches on one stream already run at the dispatch
+        // rate, and replayed cycles measured 1-20% slower from 32^3 to 128^3.
+        if (m.shape < 0 && ctx->graph_mode == 1) {
+            while (it < limit && it % prm->update != 0) step(m);
+            const int cycles = (limit - it) / prm->update;
+            if (cycles >= 2) {
+                const int rc = replay_cycles(m, cycles);
+                if (rc) return rc;
+            }
+        }
         while (it < limit) step(m);
         MH_CHECK_LAUNCH(ctx, "cmf kernels");
+        return MH_OK;
+    }
+
+    // Capture one criterion cycle and launch it `cycles` times. Capture needs
+    // a stream other than the legacy default one, so the work goes to the
+    // context's own non-blocking stream, ordered after and before the
+    // caller's stream by events. If capture is refused, nothing is lost: the
+    // captured launches never ran and the loop launches them normally.
+    int replay_cycles(const SweepMethod &m, int cycles)
+    {
+        if (!ctx->graph_stream)
+            MH_CHECK(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
+        hipStream_t gs = ctx->graph_stream;
+        hipEvent_t before, after;
+        MH_CHECK(ctx, hipEventCreateWithFlags(&before, hipEventDisableTiming));
+        MH_CHECK(ctx, hipEventCreateWithFlags(&after, hipEventDisableTiming));
+        MH_CHECK(ctx, hipEventRecord(before, s));
+        MH_CHECK(ctx, hipStreamWaitEvent(gs, before, 0));
+        const hipStream_t caller = s;
+        s = gs;
+        const int it0 = it;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool ok = hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            for (int k = 0; k < prm->update; ++k) step(m);
+            ok = hipStreamEndCapture(gs, &graph) == hipSuccess && graph != nullptr;
+            it = it0;
+        }
+        if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        hipError_t e = hipSuccess;
+        for (int c = 0; ok && c < cycles && e == hipSuccess; ++c) {
+            e = hipGraphLaunch(exec, gs);
+            it += prm->update;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();  // a refused capture leaves nothing behind
+        s = caller;
+        MH_CHECK(ctx, hipEventRecord(after, gs));
+        MH_CHECK(ctx, hipStreamWaitEvent(s, after, 0));
+        (void)hipEventDestroy(before);
+        (void)hipEventDestroy(after);
+        MH_CHECK(ctx, e);
         return MH_OK;
     }
 
@@ -1671,4 +1730,11 @@ MH_EXPORT int mh_cmf_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile,
     *tile = f->second.first + 1;  // 0: the per-iteration kernels
     *zchunk = f->second.second;
     return 1;
+}
+
+MH_EXPORT int mh_cmf_set_graphs(mh_ctx *ctx, int on)
+{
+    MH_REQUIRE(ctx, on == 0 || on == 1, "0 (off) or 1 (on)");
+    ctx->graph_mode = on;
+    return MH_OK;
 }

@@ -41,6 +41,9 @@ struct mh_ctx {
     int sweep_mode = 0;
     int sweep_shape = 0;   // tile shape (1-based; 0: default)
     int sweep_zchunk = 0;  // planes per work-group (0: 16)
+    // graph replay of the per-iteration kernels' criterion cycles: 1 on, 0 off
+    int graph_mode = 0;
+    hipStream_t graph_stream = nullptr;  // capture stream, created on first use
     // method the tuner chose per volume dims: (tile shape or -1 for the plain
     // kernels, plane chunk)
     std::map<std::array<int, 3>, std::pair<int, int>> sweep_tuned;

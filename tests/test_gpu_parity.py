@@ -509,3 +509,41 @@ def test_two_level_sweep_equals_plain_kernels(dev, rng, shape, iters, update, bo
         assert out[1][1:3] == out[mode][1:3]
         # the criterion sums the same |err| in another order
         np.testing.assert_allclose(out[1][3], out[mode][3], rtol=1e-5)
+
+
+@pytest.mark.parametrize("shape,iters,update,bound,labelled", [
+    ((20, 17, 23), 200, 7, 0.0, False), ((16, 16, 16), 3000, 7, 1e-3, False),
+    ((12, 14, 16), 95, 30, 0.0, True), ((5, 6, 7), 64, 1, 0.0, False)])
+def test_graph_replay_equals_launched_kernels(dev, rng, shape, iters, update, bound, labelled):
+    """Criterion cycles replayed from a captured graph compute the launched
+    kernels' iterations bit for bit and stop at the same iteration; every
+    replayed solve is checked, with other device work between them."""
+    img, q, alpha, cs, ct, _ = _cut_case(rng, shape)
+    labels = None
+    if labelled:
+        labels = np.zeros(shape, np.uint8)
+        labels[6, 7, 2:5] = 1
+        labels[0, :3, :3] = 2
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+
+    def solve():
+        c = _cutter(dev, iters, update=update, error_bound=bound)
+        u = c.execute(img, q, eigs, alpha, cs, ct, labels=labels, verbose=False)
+        return u, c.last_result.iterations, c.last_result.stopped, c.last_criteria
+
+    try:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 1, 0, 0))
+        want = solve()
+        dev.check(dev.lib.mh_cmf_set_graphs(dev.handle, 1))
+        for k in range(3):
+            got = solve()
+            assert np.array_equal(got[0], want[0]), k
+            assert got[1:3] == want[1:3], k
+            np.testing.assert_array_equal(got[3], want[3])
+            # eager work of the program's own between the replayed solves
+            t = dev.upload(rng.normal(size=4096).astype(np.float32))
+            dev.fill(t, float(k))
+            assert (dev.download(t) == float(k)).all()
+    finally:
+        dev.check(dev.lib.mh_cmf_set_graphs(dev.handle, 0))
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
