@@ -148,6 +148,22 @@ int mh_complete_frames(mh_ctx *ctx, float *out3_dev, const float *dirs3_dev, int
 int mh_hysteresis_host(const uint8_t *cand_host, const uint8_t *sure_host, uint8_t *out_host,
                        const int64_t dims[3]);
 
+/* Host: a minimum spanning tree of the complete Euclidean graph over n points
+ * (pts_host: n x 3 doubles, e.g. voxel-size scaled ridge coordinates), as
+ * n - 1 index pairs in edges_host (2(n - 1) int64). r0 is the longest
+ * neighbour distance (the scaled 26-neighbourhood diagonal): Kruskal on the
+ * pairs within r0, then Boruvka rounds by k-d tree search. Ties in length go
+ * to the lexicographically smallest index pair. Replaces the pdist +
+ * squareform + minimum_spanning_tree of _adjust_ridge_directions
+ * (processing/adjust_directions.py:365-379) without its O(n^2) matrix. */
+int mh_spanning_tree_host(const double *pts_host, int64_t n, double r0, int64_t *edges_host);
+/* Host: directions (n x 3 floats) with signs propagated along a tree from
+ * root: a child keeps its direction when its dot product with its adjusted
+ * parent is >= 0, else flips (NaN flips); unreached points are NaN. Replaces
+ * the traversal of _adjust_ridge_directions (processing/adjust_directions.py:384-417). */
+int mh_tree_signs_host(const float *dirs_host, int64_t n, const int64_t *edges_host, int64_t nedges,
+                       int64_t root, float *out_host);
+
 /* --------------------------------------------------------------------- GVF */
 
 typedef struct {

@@ -1,6 +1,7 @@
 """The sparse Euclidean MST used for ridge sign propagation against the
 reference's dense routine (scipy minimum_spanning_tree on pdist). CPU only."""
 import numpy as np
+import pytest
 from scipy.sparse.csgraph import minimum_spanning_tree
 from scipy.spatial.distance import pdist, squareform
 
@@ -53,4 +54,47 @@ def test_large_ridge_is_fast(rng):
     t = time.time()
     edges = ridge_graph.spanning_tree_edges(grid, np.sqrt(3))
     assert _is_spanning_tree(len(grid), edges)
-    assert time.time() - t < 60
+    assert time.time() - t < 10
+
+
+@pytest.mark.parametrize("density,seed", [(0.02, 1), (0.08, 2), (0.3, 3), (0.6, 4)])
+def test_sparse_mst_weight_on_lattices(monkeypatch, density, seed):
+    """Lattice points tie heavily; the sparse tree has the dense routine's
+    total length, also with anisotropic voxel sizes and far-apart clusters."""
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
+    rng = np.random.default_rng(seed)
+    grid = np.argwhere(rng.random((16, 16, 16)) < density).astype(np.float64)
+    far = np.argwhere(rng.random((6, 6, 6)) < 0.3).astype(np.float64) + 40
+    for vs in ((1, 1, 1), (2.0, 0.5, 1.25)):
+        pts = np.concatenate([grid, far]) * np.asarray(vs)
+        edges = ridge_graph.spanning_tree_edges(pts, float(np.linalg.norm(vs)))
+        assert _is_spanning_tree(len(pts), edges)
+        dense = minimum_spanning_tree(squareform(pdist(pts))).tocoo()
+        assert abs(_tree_weight(pts, edges) - dense.data.sum()) <= 1e-9 * dense.data.sum()
+
+
+def test_sparse_mst_is_the_dense_tree_for_distinct_lengths(rng, monkeypatch):
+    """Continuous coordinates in clusters of very different sizes (one
+    component holding most points): the unique tree, edge for edge."""
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
+    t = np.linspace(0, 6 * np.pi, 1500)
+    helix = np.stack([t * 2, 10 * np.cos(t), 10 * np.sin(t)], 1) + rng.normal(0, 0.05, (1500, 3))
+    pts = np.concatenate([helix, rng.uniform(-30, 60, size=(300, 3))])
+    edges = ridge_graph.spanning_tree_edges(pts, 1.0)
+    dense = minimum_spanning_tree(squareform(pdist(pts))).tocoo()
+    assert {tuple(sorted(e)) for e in edges.tolist()} == \
+        {tuple(sorted(e)) for e in zip(dense.row.tolist(), dense.col.tolist())}
+
+
+def test_propagate_signs_zero_nan_and_unreached():
+    """dot >= 0 keeps (a zero product keeps), dot < 0 flips, a NaN product
+    flips; the sign is taken against the adjusted parent; points off the
+    tree stay NaN (adjust_directions.py:384-417)."""
+    dirs = np.array([[1, 0, 0], [0, 1, 0], [0, -1, 0], [np.nan, 0, 0], [-1, 0, 0],
+                     [-0.5, 0.5, 0], [3, 3, 3]], np.float32)
+    edges = np.array([[0, 1], [2, 1], [3, 0], [4, 0], [1, 5]], np.int64)
+    out = ridge_graph.propagate_signs(dirs, edges, 0)
+    want = np.array([[1, 0, 0], [0, 1, 0], [0, 1, 0], [np.nan, -0.0, -0.0], [1, 0, 0],
+                     [-0.5, 0.5, 0]], np.float32)
+    np.testing.assert_array_equal(out[:6], want)
+    assert np.signbit(out[3, 1]) and np.isnan(out[6]).all()
