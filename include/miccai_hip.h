@@ -157,6 +157,11 @@ int mh_hysteresis_host(const uint8_t *cand_host, const uint8_t *sure_host, uint8
  * squareform + minimum_spanning_tree of _adjust_ridge_directions
  * (processing/adjust_directions.py:365-379) without its O(n^2) matrix. */
 int mh_spanning_tree_host(const double *pts_host, int64_t n, double r0, int64_t *edges_host);
+/* Host: the same tree for n distinct voxels (coords_host: n x 3 int64 voxel
+ * indices inside dims) at scaled positions coords * voxel_size; the short
+ * edges come from a dense neighbour lookup instead of a k-d tree query. */
+int mh_ridge_tree_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
+                       const double voxel_size[3], int64_t *edges_host);
 /* Host: directions (n x 3 floats) with signs propagated along a tree from
  * root: a child keeps its direction when its dot product with its adjusted
  * parent is >= 0, else flips (NaN flips); unreached points are NaN. Replaces

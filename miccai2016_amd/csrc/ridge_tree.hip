@@ -219,41 +219,27 @@ private:
     }
 };
 
-// float dot product, summed left to right without fused multiply-adds
-inline float dot3(const float *a, const float *b)
+// Kruskal over `shorts` (sorted here), extending the forest in `sets`.
+void kruskal(std::vector<Edge> &shorts, DisjointSets &sets, int64_t *edges, int64_t &ne)
 {
-#pragma clang fp contract(off)
-    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+    std::sort(shorts.begin(), shorts.end(), lighter);
+    for (const Edge &e : shorts)
+        if (sets.unite(e.a, e.b)) {
+            edges[2 * ne] = e.a;
+            edges[2 * ne + 1] = e.b;
+            ++ne;
+        }
 }
 
-}  // namespace
-
-MH_EXPORT int mh_spanning_tree_host(const double *pts, int64_t n, double r0, int64_t *edges)
+// Boruvka rounds that join the forest's components into one tree.
+//
+// Per point, a lower bound on its distance^2 to the nearest point of
+// another component. Components only merge, so the set of foreign points
+// only shrinks and a bound stays valid in later rounds. A component's
+// points are searched in increasing bound order, and the search stops
+// once the bound exceeds the component's best edge so far.
+int complete_tree(KdTree &tree, int64_t n, DisjointSets &sets, int64_t *edges, int64_t ne)
 {
-    if (n < 0 || (n > 0 && (!pts || (n > 1 && !edges))) || !(r0 >= 0.0)) return MH_ERR_ARG;
-    if (n < 2) return MH_OK;
-    KdTree tree(pts, n);
-    DisjointSets sets(n);
-    int64_t ne = 0;
-    // 1. the forest of short edges
-    {
-        std::vector<Edge> shorts;
-        const double r = r0 * (1.0 + 1e-9);
-        tree.pairs_within(r * r, shorts);
-        std::sort(shorts.begin(), shorts.end(), lighter);
-        for (const Edge &e : shorts)
-            if (sets.unite(e.a, e.b)) {
-                edges[2 * ne] = e.a;
-                edges[2 * ne + 1] = e.b;
-                ++ne;
-            }
-    }
-    // 2. Boruvka rounds over the components of the forest
-    // Per point, a lower bound on its distance^2 to the nearest point of
-    // another component. Components only merge, so the set of foreign points
-    // only shrinks and a bound stays valid in later rounds. A component's
-    // points are searched in increasing bound order, and the search stops
-    // once the bound exceeds the component's best edge so far.
     std::vector<int64_t> comp(static_cast<size_t>(n)), root_ix(static_cast<size_t>(n), -1);
     std::vector<int64_t> members(static_cast<size_t>(n)), first;
     std::vector<double> lb(static_cast<size_t>(n), 0.0);
@@ -322,6 +308,87 @@ MH_EXPORT int mh_spanning_tree_host(const double *pts, int64_t n, double r0, int
         if (ne == before) return MH_ERR_ARG;  // non-finite coordinates
     }
     return MH_OK;
+}
+
+// float dot product, summed left to right without fused multiply-adds
+inline float dot3(const float *a, const float *b)
+{
+#pragma clang fp contract(off)
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+}  // namespace
+
+MH_EXPORT int mh_spanning_tree_host(const double *pts, int64_t n, double r0, int64_t *edges)
+{
+    if (n < 0 || (n > 0 && (!pts || (n > 1 && !edges))) || !(r0 >= 0.0)) return MH_ERR_ARG;
+    if (n < 2) return MH_OK;
+    KdTree tree(pts, n);
+    DisjointSets sets(n);
+    int64_t ne = 0;
+    std::vector<Edge> shorts;
+    const double r = r0 * (1.0 + 1e-9);
+    tree.pairs_within(r * r, shorts);
+    kruskal(shorts, sets, edges, ne);
+    return complete_tree(tree, n, sets, edges, ne);
+}
+
+// Lattice points (voxel coordinates, scaled by the voxel size): the short
+// edges come from the 13 forward neighbour offsets looked up in a dense index
+// volume. Every pair shorter than twice the smallest voxel size is a
+// 26-neighbour pair, so Kruskal on the neighbour pairs below that length is
+// the full Kruskal's prefix, and the tree is mh_spanning_tree_host's on the
+// scaled points.
+MH_EXPORT int mh_ridge_tree_host(const int64_t *coords, int64_t n, const int64_t dims[3],
+                                 const double voxel_size[3], int64_t *edges)
+{
+    if (n < 0 || !mh::dims_ok(dims) || !voxel_size || (n > 0 && !coords) || (n > 1 && !edges))
+        return MH_ERR_ARG;
+    for (int k = 0; k < 3; ++k)
+        if (!(voxel_size[k] > 0.0) || !std::isfinite(voxel_size[k])) return MH_ERR_ARG;
+    if (n < 2) return MH_OK;
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    std::vector<int32_t> at(size_t(d0 * d1 * d2), -1);
+    std::vector<double> pts(size_t(3 * n));
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c0 = coords[3 * i], c1 = coords[3 * i + 1], c2 = coords[3 * i + 2];
+        if (c0 < 0 || c0 >= d0 || c1 < 0 || c1 >= d1 || c2 < 0 || c2 >= d2) return MH_ERR_ARG;
+        int32_t &slot = at[size_t(c2 + d2 * (c1 + d1 * c0))];
+        if (slot >= 0) return MH_ERR_ARG;  // repeated voxel
+        slot = int32_t(i);
+        pts[size_t(3 * i)] = double(c0) * voxel_size[0];
+        pts[size_t(3 * i + 1)] = double(c1) * voxel_size[1];
+        pts[size_t(3 * i + 2)] = double(c2) * voxel_size[2];
+    }
+    const double vmin = std::min(voxel_size[0], std::min(voxel_size[1], voxel_size[2]));
+    const double limit = (2.0 * vmin) * (2.0 * vmin) * (1.0 - 1e-9);
+    std::vector<Edge> shorts;
+    shorts.reserve(size_t(4 * n));
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c0 = coords[3 * i], c1 = coords[3 * i + 1], c2 = coords[3 * i + 2];
+        for (int o0 = 0; o0 <= 1; ++o0)
+            for (int o1 = -1; o1 <= 1; ++o1)
+                for (int o2 = -1; o2 <= 1; ++o2) {
+                    // forward offsets only: (0,0,1), (0,1,*), (1,*,*)
+                    if (o0 == 0 && (o1 < 0 || (o1 == 0 && o2 <= 0))) continue;
+                    const int64_t e0 = c0 + o0, e1 = c1 + o1, e2 = c2 + o2;
+                    if (e0 >= d0 || e1 < 0 || e1 >= d1 || e2 < 0 || e2 >= d2) continue;
+                    const int32_t j = at[size_t(e2 + d2 * (e1 + d1 * e0))];
+                    if (j < 0) continue;
+                    const double *a = &pts[size_t(3 * i)], *b = &pts[size_t(3 * j)];
+                    const double x0 = a[0] - b[0], x1 = a[1] - b[1], x2 = a[2] - b[2];
+                    const double w2 = x0 * x0 + x1 * x1 + x2 * x2;
+                    if (w2 < limit) shorts.push_back(Edge{w2, std::min(i, int64_t(j)), std::max(i, int64_t(j))});
+                }
+    }
+    std::vector<int32_t>().swap(at);
+    DisjointSets sets(n);
+    int64_t ne = 0;
+    kruskal(shorts, sets, edges, ne);
+    std::vector<Edge>().swap(shorts);
+    KdTree tree(pts.data(), n);
+    return complete_tree(tree, n, sets, edges, ne);
 }
 
 // Signs along the tree from `root`: a child keeps its direction when its dot

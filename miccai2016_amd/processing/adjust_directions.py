@@ -152,19 +152,18 @@ class AdvancedAdjuster(TreePrintable):
             return coords, dev.download(md.index_select(0, idx))
 
         coords, dirs = ridge_dirs(ridge)
-        coords, adjusted = self._tree_signs(coords, dirs)
+        coords, adjusted = self._tree_signs(coords, dirs, shape)
         if self.clean_ridge:
             ridge = self._do_clean_ridge(ridge, coords, adjusted)
             coords, dirs = ridge_dirs(ridge)
-            coords, adjusted = self._tree_signs(coords, dirs)
+            coords, adjusted = self._tree_signs(coords, dirs, shape)
         self._flip_nonridge(coords, adjusted, shape)
         return self._d_qeigs
 
-    def _tree_signs(self, coords, dirs):
+    def _tree_signs(self, coords, dirs, shape):
         if len(coords) == 0:
             return coords.astype(np.uint16), np.zeros((0, 3), np.float32)
-        pts = np.multiply(coords, self.voxel_size).astype(np.float64)
-        edges = ridge_graph.spanning_tree_edges(pts, float(np.linalg.norm(self.voxel_size)))
+        edges = ridge_graph.ridge_tree_edges(coords, shape, self.voxel_size)
         return np.require(coords, np.uint16, ["C"]), ridge_graph.propagate_signs(dirs, edges, 0)
 
     def _label_ridge(self):
@@ -197,7 +196,8 @@ class AdvancedAdjuster(TreePrintable):
         Returns (uint16 coords R x 3 in C order, float32 adjusted dirs)."""
         coords = np.argwhere(labeled_ridge != 0)
         # the tree's short-edge pass uses the scaled 26-neighbourhood
-        return self._tree_signs(coords, main_dirs[tuple(coords.T)] if len(coords) else None)
+        return self._tree_signs(coords, main_dirs[tuple(coords.T)] if len(coords) else None,
+                                labeled_ridge.shape)
 
     def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):
         """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed

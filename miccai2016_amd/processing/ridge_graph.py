@@ -17,7 +17,7 @@ from scipy.spatial.distance import pdist, squareform
 
 from .. import _lib
 
-DENSE_LIMIT = 1000   # up to this many points the reference's dense routine is used
+DENSE_LIMIT = 256    # up to this many points the reference's dense routine is used
 
 
 def _dense_mst(pts):
@@ -39,6 +39,25 @@ def spanning_tree_edges(pts, r0):
     rc = _lib.lib().mh_spanning_tree_host(p.ctypes.data, n, float(r0), edges.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"mh_spanning_tree_host failed (status {rc}): non-finite coordinates?")
+    return edges
+
+
+def ridge_tree_edges(coords, shape, voxel_size):
+    """spanning_tree_edges of the voxels `coords` (R x 3 indices into
+    `shape`) at positions coords * voxel_size, with the scaled voxel diagonal
+    as the neighbourhood radius."""
+    coords = np.asarray(coords)
+    n = len(coords)
+    if n <= DENSE_LIMIT:
+        pts = np.multiply(coords, voxel_size).astype(np.float64)
+        return spanning_tree_edges(pts, float(np.linalg.norm(voxel_size)))
+    c = np.require(coords, np.int64, ["C"])
+    vs = np.require(np.multiply(np.ones(3), voxel_size), np.float64, ["C"])
+    edges = np.empty((n - 1, 2), np.int64)
+    rc = _lib.lib().mh_ridge_tree_host(c.ctypes.data, n, _lib.dims3(shape), vs.ctypes.data,
+                                       edges.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"mh_ridge_tree_host failed (status {rc})")
     return edges
 
 

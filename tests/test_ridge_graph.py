@@ -98,3 +98,30 @@ def test_propagate_signs_zero_nan_and_unreached():
                      [-0.5, 0.5, 0]], np.float32)
     np.testing.assert_array_equal(out[:6], want)
     assert np.signbit(out[3, 1]) and np.isnan(out[6]).all()
+
+
+@pytest.mark.parametrize("vs", [(1, 1, 1), (2.0, 0.5, 1.25), (1, 1, 3)])
+def test_lattice_tree_is_the_point_tree(monkeypatch, vs):
+    """The dense-lookup lattice construction gives the k-d tree
+    construction's tree on the scaled points, edge for edge (both break ties
+    on the index pair), with the dense routine's total length."""
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
+    rng = np.random.default_rng(7)
+    shape = (20, 18, 22)
+    mask = rng.random(shape) < 0.12
+    mask[2:18, 9, 11] = True
+    coords = np.argwhere(mask)
+    pts = coords * np.asarray(vs, np.float64)
+    a = ridge_graph.ridge_tree_edges(coords, shape, vs)
+    b = ridge_graph.spanning_tree_edges(pts, float(np.linalg.norm(vs)))
+    assert {tuple(sorted(e)) for e in a.tolist()} == {tuple(sorted(e)) for e in b.tolist()}
+    dense = minimum_spanning_tree(squareform(pdist(pts))).tocoo()
+    assert abs(_tree_weight(pts, a) - dense.data.sum()) <= 1e-9 * dense.data.sum()
+
+
+def test_lattice_tree_rejects_bad_coordinates(monkeypatch):
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 1)
+    with pytest.raises(RuntimeError):
+        ridge_graph.ridge_tree_edges(np.array([[0, 0, 0], [0, 0, 0]]), (2, 2, 2), 1)
+    with pytest.raises(RuntimeError):
+        ridge_graph.ridge_tree_edges(np.array([[0, 0, 0], [0, 0, 2]]), (2, 2, 2), 1)
