@@ -133,6 +133,8 @@ def main():
                     help="comma-separated arrival rates; one serving run each")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--ragged-volumes", type=int, default=16)
+    ap.add_argument("--ragged-workers", type=int, default=8,
+                    help="threads preparing the ragged volumes (own context and stream each)")
     ap.add_argument("--no-ragged", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -348,13 +350,15 @@ def main():
             truths.append(tr)
         rcfg = PipelineConfig()
         rcfg.cut_iters = args.iters
-        rr = segment_ragged(vols, [sp.sigma for sp in specs], rw, rd, rcfg, dev)
+        rr = segment_ragged(vols, [sp.sigma for sp in specs], rw, rd, rcfg, dev,
+                            workers=args.ragged_workers)
         total = int(sum(np.prod(sp.shape) for sp in specs))
         dice = [float(DeviceDice(tr, dev)(rr.volume(v).reshape(-1), 1)[0])
                 for v, tr in enumerate(truths)]
         line["ragged"] = {
             "volumes": len(specs), "total_voxels": total, "iterations": args.iters,
-            "prepare_s": rr.prepare_s, "solve_s": rr.solve_s,
+            "prepare_s": rr.prepare_s, "prepare_workers": args.ragged_workers,
+            "solve_s": rr.solve_s,
             "voxel_iters_per_s": total * args.iters / rr.solve_s,
             "median_fast_dice_vs_truth": float(np.nanmedian(dice)),
             "workload": "dims U[48,192] per axis, tube U[3,8], helix radius min(dims)/4, pitch "

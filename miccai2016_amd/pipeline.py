@@ -158,43 +158,83 @@ class RaggedResult:
         return self.u[o:o + int(np.prod(s))].reshape(s)
 
 
-def segment_ragged(imgs, noise_levels, weightings, decays, cfg=None, loader=None, atv=True):
+_worker_contexts = {}
+
+
+def _worker_context(device, k):
+    """The k-th preparation worker's own library context and stream on
+    `device` (kept for the process: a context per thread, as the library
+    requires of concurrent callers)."""
+    key = (device, k)
+    if key not in _worker_contexts:
+        from .device import DeviceContext
+        ctx = DeviceContext(device)
+        _worker_contexts[key] = (ctx, ctx.torch.cuda.Stream(ctx.device))
+    return _worker_contexts[key]
+
+
+def _prepare_cut_fields(img, nl, w, d, cfg, dev):
+    """One volume's packed-solve inputs: (shape, u0, frames, alpha, cs, ct)."""
+    if isinstance(img, tuple):
+        img_t, shape = img[0].reshape(-1), tuple(img[1])
+    else:
+        shape = np.shape(img)
+        img_t = dev.upload(np.asarray(img, dtype=np.float32)).reshape(-1)
+    n = int(np.prod(shape))
+    prep = prepare_device(img_t, shape, nl, cfg, dev)
+    a, u0 = dev.empty(n), dev.empty(n)
+    dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(prep.gms), n, float(w), float(d),
+                                            dev.ptr(a), dev.stream), "mh_nonterminal_weight")
+    work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+    dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(img_t), n, dev.ptr(u0), dev.ptr(work),
+                                   dev.stream), "mh_initial_u")
+    return shape, u0, prep.quats, a, prep.cs, prep.ct
+
+
+def segment_ragged(imgs, noise_levels, weightings, decays, cfg=None, loader=None, atv=True,
+                   workers=4):
     """The experiment pipeline over volumes of different shapes, each with its
     own noise level and (weighting, decay): every volume is prepared on the
     device (prepare_device), its fields are packed at element offsets, and
     one ragged max-flow solve segments all of them together. imgs: numpy
-    volumes or (device tensor, shape) pairs. Returns a RaggedResult."""
+    volumes or (device tensor, shape) pairs. Returns a RaggedResult.
+
+    With workers > 1, volumes are prepared by that many threads, each with its
+    own library context and stream, so one volume's host ridge steps overlap
+    other volumes' kernels; results do not depend on it."""
     import time
-    from . import _lib
+    from concurrent.futures import ThreadPoolExecutor
     cfg = PipelineConfig() if cfg is None else cfg
     dev = default_context() if loader is None else loader
     torch = dev.torch
     t0 = time.perf_counter()
-    shapes, parts = [], {k: [] for k in ("u", "q", "a", "cs", "ct")}
-    for img, nl, w, d in zip(imgs, noise_levels, weightings, decays):
-        if isinstance(img, tuple):
-            img_t, shape = img[0].reshape(-1), tuple(img[1])
-        else:
-            shape = np.shape(img)
-            img_t = dev.upload(np.asarray(img, dtype=np.float32)).reshape(-1)
-        n = int(np.prod(shape))
-        prep = prepare_device(img_t, shape, nl, cfg, dev)
-        a, u0 = dev.empty(n), dev.empty(n)
-        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(prep.gms), n, float(w),
-                                                float(d), dev.ptr(a), dev.stream),
-                  "mh_nonterminal_weight")
-        work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
-        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(img_t), n, dev.ptr(u0), dev.ptr(work),
-                                       dev.stream), "mh_initial_u")
-        shapes.append(shape)
-        for k, t in zip(("u", "q", "a", "cs", "ct"), (u0, prep.quats, a, prep.cs, prep.ct)):
-            parts[k].append(t)
-        del prep
-    packed = {k: torch.cat(v) for k, v in parts.items()}
-    del parts
+    jobs = list(zip(imgs, noise_levels, weightings, decays))
+    workers = max(1, min(int(workers), len(jobs)))
+    if workers == 1:
+        prepared = [_prepare_cut_fields(*j, cfg, dev) for j in jobs]
+    else:
+        dev.synchronize()   # device inputs are complete before other streams read them
+        slots = list(range(workers))
+
+        def run(job):
+            k = slots.pop()
+            try:
+                ctx, stream = _worker_context(dev.index, k)
+                with torch.cuda.stream(stream):
+                    out = _prepare_cut_fields(*job, cfg, ctx)
+                    stream.synchronize()
+                return out
+            finally:
+                slots.append(k)
+        with ThreadPoolExecutor(workers) as pool:
+            prepared = list(pool.map(run, jobs))
+    shapes = [p[0] for p in prepared]
+    packed = {k: torch.cat([p[i + 1] for p in prepared])
+              for i, k in enumerate(("u", "q", "a", "cs", "ct"))}
     sizes = [int(np.prod(s)) for s in shapes]
     offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     dev.synchronize()
+    del prepared
     t1 = time.perf_counter()
     a_atv = cfg.a_atv if atv else 1.0
     cutter = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)

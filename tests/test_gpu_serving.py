@@ -170,17 +170,19 @@ def test_poisson_load_on_a_background_thread(dev):
                                          for t in tickets)
 
 
-def test_ragged_pipeline_equals_per_volume_pipeline(dev):
+@pytest.mark.parametrize("workers", [1, 3])
+def test_ragged_pipeline_equals_per_volume_pipeline(dev, workers):
     """segment_ragged packs every volume's device-prepared fields and solves
     them in one ragged launch sequence: each labelling equals the volume's
-    own preparation plus a plain solve, bit for bit."""
+    own preparation plus a plain solve, bit for bit, whether the volumes are
+    prepared in turn or by concurrent workers."""
     from miccai2016_amd.pipeline import PipelineConfig, prepare_device, segment_ragged
     from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
     specs, w, d = ph.ragged_set(3, seed=4, side=(20, 44))
     imgs = [ph.make_helix(sp)[0] for sp in specs]
     cfg = PipelineConfig()
     cfg.gvf_iters, cfg.cut_iters = 40, 90
-    rr = segment_ragged(imgs, [sp.sigma for sp in specs], w, d, cfg, dev)
+    rr = segment_ragged(imgs, [sp.sigma for sp in specs], w, d, cfg, dev, workers=workers)
     assert [r[0] for r in rr.results] == [90] * 3
     for v, (img, sp) in enumerate(zip(imgs, specs)):
         x = img.astype(np.float32)
