@@ -132,6 +132,10 @@ def main():
     ap.add_argument("--serve-qps", type=str, default="8,16",
                     help="comma-separated arrival rates; one serving run each")
     ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-pipeline-qps", type=str, default="2",
+                    help="QPS of requests served through the full preparation ('' skips)")
+    ap.add_argument("--serve-pipeline-requests", type=int, default=20)
+    ap.add_argument("--serve-prepare-workers", type=int, default=4)
     ap.add_argument("--ragged-volumes", type=int, default=16)
     ap.add_argument("--ragged-workers", type=int, default=8,
                     help="threads preparing the ragged volumes (own context and stream each)")
@@ -373,16 +377,21 @@ def main():
         from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
         torch.cuda.empty_cache()
         line["serving"] = []
-        for qps in (float(q) for q in args.serve_qps.split(",")):
-            srv = MaxFlowServer(cut_parameters(cfg), chunk=50).start()
-            log(f"serving {args.serve_requests} requests at {qps} QPS "
+        runs = [(float(q), False, args.serve_requests) for q in args.serve_qps.split(",") if q]
+        runs += [(float(q), True, args.serve_pipeline_requests)
+                 for q in args.serve_pipeline_qps.split(",") if q]
+        for qps, full, nreq in runs:
+            srv = MaxFlowServer(cut_parameters(cfg), chunk=50,
+                                prepare_workers=args.serve_prepare_workers).start()
+            log(f"serving {nreq} {'prepared ' if full else ''}requests at {qps} QPS "
                 f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
-            tickets = poisson_load(srv, synthetic_request, qps, args.serve_requests, seed=11)
+            tickets = poisson_load(srv, lambda rng: synthetic_request(rng, prepare=full), qps,
+                                   nreq, seed=11)
             srv.stop(drain=True, timeout=600)
             st = srv.stats()
             dice = [t.dice for t in tickets if t.dice is not None and not np.isnan(t.dice)]
             line["serving"].append({
-                "qps": qps, "requests": args.serve_requests, "completed": st["completed"],
+                "qps": qps, "requests": nreq, "completed": st["completed"],
                 "errors": sum(t.error is not None for t in tickets),
                 "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
                 "voxel_iters_per_s": st.get("voxel_iters_per_s"),
@@ -392,6 +401,9 @@ def main():
                 "peak_reserved_GiB": srv.peak_reserved / 2 ** 30,
                 "budget_GiB": srv.budget / 2 ** 30, "chunk": srv.chunk,
                 "median_dice_vs_truth": float(np.median(dice)) if dice else None,
+                "frames": ("full preparation (Frangi, adjustment, GVF, completion) on "
+                           f"{args.serve_prepare_workers} workers" if full
+                           else "the phantom's centre-line frames"),
                 "workload": "side log-U[64,320], aspect <= 2:1, 300-1000 iterations, helix/noise "
                             "as the ragged set, (w, decay) from the grid; cuts of >= 2^20 voxels "
                             "as resident solves, smaller ones as packed jobs"})

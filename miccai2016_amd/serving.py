@@ -84,6 +84,14 @@ class PhantomCutRequest:
     seed: int = 0
     eigs: tuple = (1.0, 10 ** -1.5, 10 ** -1.5)
     num_iter: int = None
+    # False: the phantom's own centre-line frames; True: the full preparation
+    # (Frangi, direction adjustment, GVF, frame completion) on a worker
+    prepare: bool = False
+    cfg: object = None           # pipeline.PipelineConfig when prepared
+
+
+def _prepares(req):
+    return isinstance(req, VolumeRequest) or (isinstance(req, PhantomCutRequest) and req.prepare)
 
 
 class Ticket:
@@ -148,9 +156,9 @@ def request_footprint_bytes(req, num_iter, update, resident_min=None):
     n = int(np.prod(shape))
     b = (resident_footprint_bytes(shape) if n >= resident_min
          else job_footprint_bytes(shape, num_iter, update))
-    if isinstance(req, VolumeRequest):
+    if _prepares(req):
         b = max(b, PREPARE_BYTES_PER_VOXEL * n)
-    elif isinstance(req, PhantomCutRequest):
+    if isinstance(req, PhantomCutRequest):
         b += 5 * n            # the generated image and mask while the job is set up
     return b
 
@@ -174,11 +182,13 @@ class MaxFlowServer:
     cut_params: CmfCutQAnisotropicGpu.CutParameters (cc, steps, update,
     error_bound, default num_iter). budget_bytes: HBM the server may reserve
     for requests (default: 90 % of the device's free memory at start).
-    chunk: iterations per step of the resident set.
+    chunk: iterations per step of the resident set. prepare_workers: threads
+    preparing VolumeRequests (each with its own library context and stream).
     """
 
     def __init__(self, cut_params, budget_bytes=None, chunk=50, device=0, return_device=False,
-                 max_jobs=1024, prepare_cfg=None, resident_min_voxels=RESIDENT_MIN_VOXELS):
+                 max_jobs=1024, prepare_cfg=None, resident_min_voxels=RESIDENT_MIN_VOXELS,
+                 prepare_workers=2):
         import torch
         self.resident_min = int(resident_min_voxels)
         self.torch = torch
@@ -205,8 +215,8 @@ class MaxFlowServer:
         self._jobs_arr = None
         self._stop = False
         self._thread = None
-        self._prep_thread = None
-        self._prep_dev = None
+        self.prepare_workers = max(1, int(prepare_workers))
+        self._prep_threads = []
         self._preparing = 0
         self._inflight = 0        # admitted, not yet finished or failed
         if int(cut_params.update) < 1:
@@ -251,9 +261,10 @@ class MaxFlowServer:
             self._wake.notify()
         if self._thread is not None:
             self._thread.join(timeout)
-        if self._prep_thread is not None:
+        for _ in self._prep_threads:
             self._prep_q.put(None)
-            self._prep_thread.join(timeout)
+        for th in self._prep_threads:
+            th.join(timeout)
 
     def wait_idle(self, timeout=None):
         t0 = time.perf_counter()
@@ -337,7 +348,7 @@ class MaxFlowServer:
                 self.peak_reserved = max(self.peak_reserved, self.reserved)
             t.t_admit = time.perf_counter()
             try:
-                if isinstance(t.request, VolumeRequest):
+                if _prepares(t.request):
                     with self._lock:
                         self._preparing += 1
                     self._ensure_prep_thread()
@@ -419,16 +430,18 @@ class MaxFlowServer:
 
     # --------------------------------------------------------- preparation --
     def _ensure_prep_thread(self):
-        if self._prep_thread is None:
-            self._prep_thread = threading.Thread(target=self._prep_loop, name="maxflow-prepare",
-                                                 daemon=True)
-            self._prep_thread.start()
+        if not self._prep_threads:
+            for k in range(self.prepare_workers):
+                th = threading.Thread(target=self._prep_loop, name=f"maxflow-prepare-{k}",
+                                      daemon=True)
+                th.start()
+                self._prep_threads.append(th)
 
     def _prep_loop(self):
         from .pipeline import prepare_device
         torch = self.torch
         with torch.cuda.device(self.dev.device):
-            self._prep_dev = DeviceContext(self.dev.index)
+            pdev = DeviceContext(self.dev.index)
             pstream = torch.cuda.Stream(device=self.dev.device)
             with torch.cuda.stream(pstream):
                 while True:
@@ -437,7 +450,7 @@ class MaxFlowServer:
                         return
                     t, need = item
                     try:
-                        job = self._volume_job(t, need, prepare_device, pstream)
+                        job = self._volume_job(t, need, prepare_device, pdev, pstream)
                         self._ready.put(job)
                     except Exception as e:
                         self._fail(t, need, e)
@@ -446,14 +459,26 @@ class MaxFlowServer:
                             self._preparing -= 1
                             self._wake.notify()
 
-    def _volume_job(self, t, need, prepare_device, pstream):
+    def _volume_job(self, t, need, prepare_device, dev, pstream):
         r = t.request
-        dev = self._prep_dev
-        img = np.require(r.img, np.float32, ["C", "A"])
-        shape = img.shape
-        n = img.size
-        img_t = dev.upload(img).reshape(-1)
-        prep = prepare_device(img_t, shape, r.noise_level, r.cfg or self.prepare_cfg, dev)
+        truth_t = None
+        if isinstance(r, PhantomCutRequest):
+            shape = tuple(int(v) for v in r.shape)
+            n = int(np.prod(shape))
+            img_t, truth_t = dev.empty(n), dev.empty(n, dtype=self.torch.uint8)
+            dev.check(dev.lib.mh_helix_phantom(dev.handle, _lib.dims3(shape), float(r.r_tube),
+                                               float(r.r_helix), float(r.pitch), float(r.sigma),
+                                               C.c_uint64(int(r.seed)), dev.ptr(img_t),
+                                               dev.ptr(truth_t), None, dev.stream),
+                      "mh_helix_phantom")
+            noise = r.sigma
+        else:
+            img = np.require(r.img, np.float32, ["C", "A"])
+            shape = img.shape
+            n = img.size
+            img_t = dev.upload(img).reshape(-1)
+            noise = r.noise_level
+        prep = prepare_device(img_t, shape, noise, r.cfg or self.prepare_cfg, dev)
         L, h, s = dev.lib, dev.handle, dev.stream
         alpha, u = dev.empty(n), dev.empty(n)
         dev.check(L.mh_nonterminal_weight(h, dev.ptr(prep.gms), n, float(r.weighting),
@@ -464,7 +489,10 @@ class MaxFlowServer:
                   "mh_initial_u")
         job = _Job(t, shape, self._params_for(r, shape), need)
         job.t = dict(u=u, eigs=prep.quats, alpha=alpha, cs=prep.cs, ct=prep.ct)
-        if r.truth is not None:
+        if truth_t is not None:
+            job.truth = truth_t
+            job.t["truth"] = job.truth
+        elif r.truth is not None:
             job.truth = dev.upload(np.asarray(r.truth) != 0, dtype=np.uint8).reshape(-1)
             job.t["truth"] = job.truth
         job.event = self.torch.cuda.Event()
@@ -583,12 +611,13 @@ class MaxFlowServer:
 
 # ------------------------------------------------------------- load tests --
 
-def synthetic_request(rng, side=(64, 320), aspect=2.0, iters=(300, 1000)):
+def synthetic_request(rng, side=(64, 320), aspect=2.0, iters=(300, 1000), prepare=False):
     """A PhantomCutRequest drawn like the serving workload: base side
     log-uniform in `side`, each axis scaled by a factor within sqrt(aspect)
     of 1 (so no two axes differ by more than `aspect`), tube radius U[3, 8],
     helix radius a quarter of the smallest side, pitch U[20, 80], noise
-    U[0.1, 1.5], (w, decay) from the 25 x 17 grid, iterations U[lo, hi]."""
+    U[0.1, 1.5], (w, decay) from the 25 x 17 grid, iterations U[lo, hi].
+    prepare: frames from the full preparation instead of the centre line."""
     from . import phantom as ph
     base = float(np.exp(rng.uniform(np.log(side[0]), np.log(side[1]))))
     f = np.exp(rng.uniform(-0.5, 0.5, size=3) * np.log(aspect))
@@ -599,7 +628,7 @@ def synthetic_request(rng, side=(64, 320), aspect=2.0, iters=(300, 1000)):
     return PhantomCutRequest(dims, float(rng.uniform(3, 8)), 0.25 * min(dims),
                              float(rng.uniform(20, 80)), float(rng.uniform(0.1, 1.5)), w, d,
                              seed=int(rng.integers(1 << 62)),
-                             num_iter=int(rng.integers(iters[0], iters[1] + 1)))
+                             num_iter=int(rng.integers(iters[0], iters[1] + 1)), prepare=prepare)
 
 
 def poisson_load(server, make_request, qps, n_requests, seed=0):

@@ -156,6 +156,49 @@ def test_volume_requests_run_the_pipeline(dev):
     srv.stop()
 
 
+def test_prepared_phantom_requests_equal_the_device_pipeline(dev):
+    """Phantom requests with the full preparation, prepared by two workers
+    while other cuts run: each labelling equals the device pipeline run alone
+    (phantom, prepare_device, weight, initial u, plain solve), bit for bit."""
+    from miccai2016_amd.pipeline import PipelineConfig, prepare_device
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    from miccai2016_amd.serving import MaxFlowServer, PhantomCutRequest
+    cfg = PipelineConfig()
+    cfg.gvf_iters = 40
+    reqs = [PhantomCutRequest((30, 26, 34), 4.0, 6.5, 30.0, 0.5, 0.1, 10.0, seed=7, num_iter=150,
+                              prepare=True),
+            PhantomCutRequest((20, 40, 22), 3.0, 5.0, 24.0, 0.9, 0.03, 3.0, seed=8, num_iter=90),
+            PhantomCutRequest((36, 20, 28), 3.5, 5.0, 22.0, 1.2, 0.3, 30.0, seed=9, num_iter=120,
+                              prepare=True),
+            PhantomCutRequest((24, 24, 24), 3.0, 6.0, 25.0, 0.3, 0.1, 3.0, seed=10, num_iter=60,
+                              prepare=True)]
+    srv = MaxFlowServer(_params(), chunk=40, prepare_cfg=cfg, prepare_workers=2)
+    tickets = [srv.submit(r) for r in reqs]
+    srv.run_until_idle()
+    srv.stop()
+    for r, t in zip(reqs, tickets):
+        u = t.wait(0)
+        if not r.prepare:
+            continue
+        n = int(np.prod(r.shape))
+        L, h, s = dev.lib, dev.handle, dev.stream
+        img, truth = dev.empty(n), dev.empty(n, dtype=dev.torch.uint8)
+        dev.check(L.mh_helix_phantom(h, _lib.dims3(r.shape), r.r_tube, r.r_helix, r.pitch,
+                                     r.sigma, C.c_uint64(r.seed), dev.ptr(img), dev.ptr(truth),
+                                     None, s))
+        prep = prepare_device(img, r.shape, r.sigma, cfg, dev)
+        a, u0 = dev.empty(n), dev.empty(n)
+        dev.check(L.mh_nonterminal_weight(h, dev.ptr(prep.gms), n, r.weighting, r.decay,
+                                          dev.ptr(a), s))
+        work = dev.empty(int(L.mh_initial_u_work_floats(n)))
+        dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(work), s))
+        c = CmfCutQAnisotropicGpu(_params(r.num_iter), dev)
+        c.execute_device(u0, prep.quats, a, prep.cs, prep.ct, const_sqevals=np.sqrt(r.eigs),
+                         shape=r.shape)
+        assert np.array_equal(u, dev.download(u0).reshape(r.shape))
+        assert t.iterations == r.num_iter and t.dice is not None
+
+
 def test_poisson_load_on_a_background_thread(dev):
     from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
     srv = MaxFlowServer(_params(), chunk=25).start()
