@@ -47,7 +47,7 @@ TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 41.7, 2: 43.8, 3: 42.1, 5: 43.3}
 TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.5, 2: 49.0, 3: 46.1, 5: 48.6}
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
-PARTS = ("grid", "plain", "plain256", "ragged", "serve", "cpu")
+PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu")
 
 
 def log(msg):
@@ -87,28 +87,28 @@ def sweep_passes(iters, update):
     return passes
 
 
-def device_helix(dev, spec, weighting, decay, seed=42):
-    """A helix phantom generated on the device with its centre-line frames and
-    the cut's fields: (u0, frames, alpha, cs, ct) device tensors."""
+def device_pipeline(dev, spec, weighting, decay, cfg, timings, seed=42):
+    """A helix phantom generated on the device and the full preparation
+    (Frangi, direction adjustment, GVF, frame completion, capacities) of its
+    cut: (u0, frames, alpha, cs, ct, truth) device tensors; stage seconds go
+    into `timings`."""
     from miccai2016_amd import _lib
+    from miccai2016_amd.pipeline import prepare_device
     shape = spec.shape
     n = int(np.prod(shape))
     L, h, st = dev.lib, dev.handle, dev.stream
-    img, q = dev.empty(n), dev.empty(3 * n)
-    cs, ct, al, u0 = dev.empty(n), dev.empty(n), dev.empty(n), dev.empty(n)
+    img, truth = dev.empty(n), dev.empty(n, dtype=dev.torch.uint8)
     dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), spec.r_tube, spec.r_helix, spec.pitch,
-                                 spec.sigma, C.c_uint64(seed), dev.ptr(img), None, dev.ptr(q), st),
-              "mh_helix_phantom")
-    dev.check(L.mh_complete_frames(h, dev.ptr(q), dev.ptr(q), n, st), "mh_complete_frames")
-    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, spec.sigma, 1e-9, dev.ptr(cs),
-                                       dev.ptr(ct), st), "mh_terminal_capacities")
-    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(shape), dev.ptr(al), st),
-              "mh_grad_mag_sq")
-    dev.check(L.mh_nonterminal_weight(h, dev.ptr(al), n, weighting, decay, dev.ptr(al), st),
+                                 spec.sigma, C.c_uint64(seed), dev.ptr(img), dev.ptr(truth), None,
+                                 st), "mh_helix_phantom")
+    prep = prepare_device(img, shape, spec.sigma, cfg, dev, timings=timings)
+    al, u0 = dev.empty(n), dev.empty(n)
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(prep.gms), n, weighting, decay, dev.ptr(al), st),
               "mh_nonterminal_weight")
     wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
     dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), st), "mh_initial_u")
-    return u0, q, al, cs, ct
+    timings["ridge_voxels"] = prep.ridge_voxels
+    return u0, prep.quats, al, prep.cs, prep.ct, truth
 
 
 def main():
@@ -306,11 +306,49 @@ def main():
         log(f"plain device-resident: {per_dev * 1e3:.2f} ms/solve; numpy API: "
             f"{per_api * 1e3:.2f} ms/solve")
 
+    # ---- 64^3 single solve through the whole pipeline (the reference's own case)
+    if run["plain64"]:
+        from miccai2016_amd.evaluation import DeviceDice, device_dice_for
+        spec = ph.PLAIN_64
+        prep64 = {}
+        t0 = time.perf_counter()
+        u0s, qs, als, css, cts, trs = device_pipeline(dev, spec, 10 ** -1.625, 10 ** 1.5, cfg,
+                                                      prep64)
+        t_prep = time.perf_counter() - t0
+        us = torch.empty_like(u0s)
+        c64 = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
+
+        def solve64():
+            us.copy_(u0s)
+            c64.execute_device(us, qs, als, css, cts, const_sqevals=np.sqrt(eigs),
+                               shape=spec.shape)
+        solve64()
+        t64 = timed(torch, solve64)
+        n64 = u0s.numel()
+        try:
+            dice64, _ = device_dice_for(us, trs, spec.shape, dev)
+        except ValueError:          # an empty cut
+            dice64 = float("nan")
+        line["plain_64"] = {
+            "pipeline_s": t_prep + t64, "prepare_s": {k: v for k, v in prep64.items()
+                                                      if k != "ridge_voxels"},
+            "ridge_voxels": prep64["ridge_voxels"], "solve_ms": t64 * 1e3,
+            "iterations": cfg.cut_iters, "voxel_iters_per_s": n64 * cfg.cut_iters / t64,
+            "fast_dice_vs_truth": float(DeviceDice(trs, dev)(us, 1)[0]),
+            "dice_of_largest_component": float(dice64),
+            "workload": "64^3 helix (tube 4, radius 16, pitch 24, noise 1.5): Frangi (3 scales), "
+                        "adjustment, GVF (200 iterations), 500 ATV max-flow iterations"}
+        log(f"64^3 pipeline: prepare {prep64}, solve {t64 * 1e3:.1f} ms, Dice "
+            f"{line['plain_64']['fast_dice_vs_truth']:.4f}")
+        del u0s, qs, als, css, cts, trs, us
+
     # ---- 256^3 single solve (bandwidth scale)
     if run["plain256"]:
         spec = ph.PLAIN_256
         it256 = args.iters_256
-        u0b, qb, al_b, cs_b, ct_b = device_helix(dev, spec, 10 ** -1.625, 10 ** 1.5)
+        prep256 = {}
+        u0b, qb, al_b, cs_b, ct_b, tr_b = device_pipeline(dev, spec, 10 ** -1.625, 10 ** 1.5, cfg,
+                                                          prep256)
         nb = u0b.numel()
         ub_ = torch.empty(nb, device=dev.device)
         p256 = cut_parameters(cfg)
@@ -333,8 +371,13 @@ def main():
                                  TRAFFIC_SWEEP_256_PER_VOXEL_ITER.get(tile256),
                              "traffic_per_voxel_iteration_plain_kernels":
                                  TRAFFIC_PLAIN_PER_VOXEL_ITER}
-        log(f"256^3 x {it256} iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s")
-        del qb, al_b, cs_b, ct_b, u0b, ub_
+        from miccai2016_amd.evaluation import DeviceDice
+        line["plain_256"]["prepare_s"] = {k: v for k, v in prep256.items() if k != "ridge_voxels"}
+        line["plain_256"]["ridge_voxels"] = prep256["ridge_voxels"]
+        line["plain_256"]["fast_dice_vs_truth"] = float(DeviceDice(tr_b, dev)(ub_, 1)[0])
+        log(f"256^3 x {it256} iterations: {t256:.3f} s -> {rate256:.4e} voxel-iters/s; "
+            f"preparation {prep256}")
+        del qb, al_b, cs_b, ct_b, u0b, ub_, tr_b
 
     # ---- ragged set: the full pipeline per volume, one packed max-flow solve
     if run["ragged"]:

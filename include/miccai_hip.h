@@ -162,9 +162,16 @@ int mh_spanning_tree_host(const double *pts_host, int64_t n, double r0, int64_t 
  * edges come from a dense neighbour lookup instead of a k-d tree query. */
 int mh_ridge_tree_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
                        const double voxel_size[3], int64_t *edges_host);
+/* Host: the first phase of mh_ridge_tree_host alone, the spanning forest of
+ * the neighbour pairs shorter than twice the smallest voxel size; returns the
+ * number of edges written (< 0 on error). The direction adjustment's cleaning
+ * scores need no more than this (processing/adjust_directions.py:290-339). */
+int64_t mh_ridge_forest_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
+                             const double voxel_size[3], int64_t *edges_host);
 /* Host: directions (n x 3 floats) with signs propagated along a tree from
  * root: a child keeps its direction when its dot product with its adjusted
- * parent is >= 0, else flips (NaN flips); unreached points are NaN. Replaces
+ * parent is >= 0, else flips (NaN flips). Trees of a forest not reached from
+ * root start at their smallest index, which keeps its direction. Replaces
  * the traversal of _adjust_ridge_directions (processing/adjust_directions.py:384-417). */
 int mh_tree_signs_host(const float *dirs_host, int64_t n, const int64_t *edges_host, int64_t nedges,
                        int64_t root, float *out_host);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "mh_hysteresis_host": (i32, [vp, vp, vp, I64x3]),
     "mh_spanning_tree_host": (i32, [vp, i64, C.c_double, vp]),
     "mh_ridge_tree_host": (i32, [vp, i64, I64x3, vp, vp]),
+    "mh_ridge_forest_host": (i64, [vp, i64, I64x3, vp, vp]),
     "mh_tree_signs_host": (i32, [vp, i64, vp, i64, i64, vp]),
     "mh_gvf_response": (i32, [vp, vp, vp, i64, vp]),
     "mh_gvf_work_floats": (i64, [I64x3]),

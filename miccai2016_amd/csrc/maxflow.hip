@@ -882,7 +882,6 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
     Workspace W;
     layout(n, static_cast<char *>(work), &W);
     hipStream_t s = (hipStream_t)stream;
-    const unsigned nb = mh::blocks_for(n);
     const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
 
     const int rc_core = solve_core(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);

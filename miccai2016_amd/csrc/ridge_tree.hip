@@ -40,7 +40,10 @@ inline bool lighter(const Edge &x, const Edge &y)
 
 struct DisjointSets {
     std::vector<int64_t> parent;
-    explicit DisjointSets(int64_t n) : parent(size_t(n)) { std::iota(parent.begin(), parent.end(), 0); }
+    explicit DisjointSets(int64_t n) : parent(static_cast<size_t>(n))
+    {
+        std::iota(parent.begin(), parent.end(), 0);
+    }
     int64_t find(int64_t x)
     {
         while (parent[size_t(x)] != x) {
@@ -60,87 +63,144 @@ struct DisjointSets {
     }
 };
 
-// k-d tree over the points: nodes hold a contiguous range of `order`, their
-// bounding box and, during a Boruvka round, the component shared by all their
-// points (-1 when mixed).
+unsigned worker_count()
+{
+    return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+// f(k) for every k in [0, count), dealt to up to `threads` threads
+template <class F>
+void parallel_for(int64_t count, unsigned threads, F f)
+{
+    if (threads <= 1 || count <= 1) {
+        for (int64_t k = 0; k < count; ++k) f(k);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (int64_t k = next++; k < count; k = next++) f(k);
+    };
+    std::vector<std::thread> pool;
+    const int64_t nt = std::min<int64_t>(threads, count);
+    for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
+}
+
+// k-d tree over the points, stored contiguously in tree order. Node ids
+// follow a fixed pre-order layout (left child id + 1, right child after the
+// left subtree), so subtrees can be built by different threads. During a
+// Boruvka round every node knows the component shared by all its points
+// (-1 when mixed).
 class KdTree {
 public:
-    static constexpr int kLeaf = 8;
+    static constexpr int64_t kLeaf = 8;
+    struct Point {
+        double x[3];
+        int64_t id;
+    };
     struct Node {
         double lo[3], hi[3];
         int64_t begin, end;
         int64_t left, right;  // -1 for leaves
-        int64_t comp;
     };
 
-    KdTree(const double *pts, int64_t n) : p_(pts), order_(size_t(n))
+    KdTree(const double *pts, int64_t n) : p_(static_cast<size_t>(n))
     {
-        std::iota(order_.begin(), order_.end(), 0);
-        nodes_.reserve(size_t(2 * (n / kLeaf + 1)));
-        build(0, n);
+        for (int64_t i = 0; i < n; ++i)
+            p_[size_t(i)] = Point{{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, i};
+        nodes_.resize(size_t(count_nodes(n)));
+        build(0, 0, n, 0);
+        comp_.assign(nodes_.size(), -1);
+        pcomp_.assign(p_.size(), -1);
     }
+
+    int64_t size() const { return int64_t(p_.size()); }
+    int64_t id_at(int64_t t) const { return p_[size_t(t)].id; }
 
     // every pair (i, j), i < j, with |pi - pj|^2 <= r2
     void pairs_within(double r2, std::vector<Edge> &out) const
     {
-        for (int64_t i = 0; i < int64_t(order_.size()); ++i) pairs_of(0, i, r2, out);
+        for (int64_t t = 0; t < size(); ++t) pairs_of(0, t, r2, out);
     }
 
-    // component of every node from the point components
-    void label(const std::vector<int64_t> &comp) { label_node(0, comp); }
-
-    // shortest edge from point i (component c) to a point of another
-    // component, if lighter than `best`
-    void nearest_foreign(int64_t i, int64_t c, const std::vector<int64_t> &comp, Edge &best) const
+    // components of the points (by point id) and of every node
+    void label(const std::vector<int64_t> &comp)
     {
-        foreign(0, i, c, comp, best);
+        for (size_t t = 0; t < p_.size(); ++t) pcomp_[t] = comp[size_t(p_[t].id)];
+        // children have larger ids than their parent
+        for (int64_t id = int64_t(nodes_.size()) - 1; id >= 0; --id) {
+            const Node &nd = nodes_[size_t(id)];
+            int64_t c;
+            if (nd.left < 0) {
+                c = pcomp_[size_t(nd.begin)];
+                for (int64_t t = nd.begin + 1; t < nd.end && c >= 0; ++t)
+                    if (pcomp_[size_t(t)] != c) c = -1;
+            } else {
+                const int64_t a = comp_[size_t(nd.left)], b = comp_[size_t(nd.right)];
+                c = (a == b) ? a : -1;
+            }
+            comp_[size_t(id)] = c;
+        }
     }
+    int64_t comp_at(int64_t t) const { return pcomp_[size_t(t)]; }
 
-    int64_t size() const { return int64_t(order_.size()); }
+    // shortest edge from the point at tree position t to a point of another
+    // component, if lighter than `best`
+    void nearest_foreign(int64_t t, Edge &best) const { foreign(0, t, pcomp_[size_t(t)], best); }
 
 private:
-    const double *p_;
-    std::vector<int64_t> order_;
+    std::vector<Point> p_;
     std::vector<Node> nodes_;
+    std::vector<int64_t> comp_, pcomp_;
 
-    const double *pt(int64_t i) const { return p_ + 3 * i; }
-
-    int64_t build(int64_t b, int64_t e)
+    static int64_t count_nodes(int64_t m)
     {
-        const int64_t id = int64_t(nodes_.size());
-        nodes_.push_back(Node{});
-        Node nd{};
+        if (m <= kLeaf) return 1;
+        return 1 + count_nodes(m / 2) + count_nodes(m - m / 2);
+    }
+
+    void build(int64_t id, int64_t b, int64_t e, int depth)
+    {
+        Node nd;
         nd.begin = b;
         nd.end = e;
         nd.left = nd.right = -1;
-        nd.comp = -1;
         for (int k = 0; k < 3; ++k) {
             nd.lo[k] = HUGE_VAL;
             nd.hi[k] = -HUGE_VAL;
         }
         for (int64_t t = b; t < e; ++t)
             for (int k = 0; k < 3; ++k) {
-                nd.lo[k] = std::min(nd.lo[k], pt(order_[size_t(t)])[k]);
-                nd.hi[k] = std::max(nd.hi[k], pt(order_[size_t(t)])[k]);
+                nd.lo[k] = std::min(nd.lo[k], p_[size_t(t)].x[k]);
+                nd.hi[k] = std::max(nd.hi[k], p_[size_t(t)].x[k]);
             }
         if (e - b > kLeaf) {
             int axis = 0;
             for (int k = 1; k < 3; ++k)
                 if (nd.hi[k] - nd.lo[k] > nd.hi[axis] - nd.lo[axis]) axis = k;
             const int64_t m = b + (e - b) / 2;
-            std::nth_element(order_.begin() + b, order_.begin() + m, order_.begin() + e,
-                             [&](int64_t x, int64_t y) {
-                                 const double px = pt(x)[axis], py = pt(y)[axis];
-                                 return px < py || (px == py && x < y);
+            std::nth_element(p_.begin() + b, p_.begin() + m, p_.begin() + e,
+                             [axis](const Point &x, const Point &y) {
+                                 return x.x[axis] < y.x[axis] ||
+                                        (x.x[axis] == y.x[axis] && x.id < y.id);
                              });
-            nd.left = build(b, m);
-            nd.right = build(m, e);
+            nd.left = id + 1;
+            nd.right = id + 1 + count_nodes(m - b);
+            if (depth < 4 && e - b > (int64_t(1) << 16)) {
+                const int64_t left = nd.left;
+                std::thread th([this, left, b, m, depth]() { build(left, b, m, depth + 1); });
+                build(nd.right, m, e, depth + 1);
+                th.join();
+            } else {
+                build(nd.left, b, m, depth + 1);
+                build(nd.right, m, e, depth + 1);
+            }
         }
         nodes_[size_t(id)] = nd;
-        return id;
     }
 
-    double box_dist2(const Node &nd, const double *q) const
+    static double box_dist2(const Node &nd, const double *q)
     {
         double s = 0.0;
         for (int k = 0; k < 3; ++k) {
@@ -150,101 +210,98 @@ private:
         return s;
     }
 
-    double dist2(int64_t i, int64_t j) const
+    static double dist2(const double *a, const double *b)
     {
-        const double *a = pt(i), *b = pt(j);
         const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
         return d0 * d0 + d1 * d1 + d2 * d2;
     }
 
-    void pairs_of(int64_t id, int64_t i, double r2, std::vector<Edge> &out) const
+    void pairs_of(int64_t id, int64_t t, double r2, std::vector<Edge> &out) const
     {
         const Node &nd = nodes_[size_t(id)];
-        if (box_dist2(nd, pt(i)) > r2) return;
+        const Point &p = p_[size_t(t)];
+        if (box_dist2(nd, p.x) > r2) return;
         if (nd.left < 0) {
-            for (int64_t t = nd.begin; t < nd.end; ++t) {
-                const int64_t j = order_[size_t(t)];
-                if (j > i) {
-                    const double d = dist2(i, j);
-                    if (d <= r2) out.push_back(Edge{d, i, j});
+            for (int64_t u = nd.begin; u < nd.end; ++u) {
+                const Point &q = p_[size_t(u)];
+                if (q.id > p.id) {
+                    const double d = dist2(p.x, q.x);
+                    if (d <= r2) out.push_back(Edge{d, p.id, q.id});
                 }
             }
             return;
         }
-        pairs_of(nd.left, i, r2, out);
-        pairs_of(nd.right, i, r2, out);
+        pairs_of(nd.left, t, r2, out);
+        pairs_of(nd.right, t, r2, out);
     }
 
-    int64_t label_node(int64_t id, const std::vector<int64_t> &comp)
+    void foreign(int64_t id, int64_t t, int64_t c, Edge &best) const
     {
-        Node &nd = nodes_[size_t(id)];
-        if (nd.left < 0) {
-            int64_t c = comp[size_t(order_[size_t(nd.begin)])];
-            for (int64_t t = nd.begin + 1; t < nd.end && c >= 0; ++t)
-                if (comp[size_t(order_[size_t(t)])] != c) c = -1;
-            nd.comp = c;
-        } else {
-            const int64_t a = label_node(nd.left, comp);
-            const int64_t b = label_node(nd.right, comp);
-            nodes_[size_t(id)].comp = (a == b) ? a : -1;
-        }
-        return nodes_[size_t(id)].comp;
-    }
-
-    void foreign(int64_t id, int64_t i, int64_t c, const std::vector<int64_t> &comp, Edge &best) const
-    {
+        if (comp_[size_t(id)] == c) return;
         const Node &nd = nodes_[size_t(id)];
-        if (nd.comp == c) return;
+        const Point &p = p_[size_t(t)];
         // strict: a box at exactly the best distance may hold a tie that
         // wins on the indices
-        if (box_dist2(nd, pt(i)) > best.w2) return;
+        if (box_dist2(nd, p.x) > best.w2) return;
         if (nd.left < 0) {
-            for (int64_t t = nd.begin; t < nd.end; ++t) {
-                const int64_t j = order_[size_t(t)];
-                if (comp[size_t(j)] == c) continue;
-                const Edge e{dist2(i, j), std::min(i, j), std::max(i, j)};
+            for (int64_t u = nd.begin; u < nd.end; ++u) {
+                if (pcomp_[size_t(u)] == c) continue;
+                const Point &q = p_[size_t(u)];
+                const Edge e{dist2(p.x, q.x), std::min(p.id, q.id), std::max(p.id, q.id)};
                 if (lighter(e, best)) best = e;
             }
             return;
         }
-        const double dl = box_dist2(nodes_[size_t(nd.left)], pt(i));
-        const double dr = box_dist2(nodes_[size_t(nd.right)], pt(i));
+        const double dl = box_dist2(nodes_[size_t(nd.left)], p.x);
+        const double dr = box_dist2(nodes_[size_t(nd.right)], p.x);
         if (dl <= dr) {
-            foreign(nd.left, i, c, comp, best);
-            foreign(nd.right, i, c, comp, best);
+            foreign(nd.left, t, c, best);
+            foreign(nd.right, t, c, best);
         } else {
-            foreign(nd.right, i, c, comp, best);
-            foreign(nd.left, i, c, comp, best);
+            foreign(nd.right, t, c, best);
+            foreign(nd.left, t, c, best);
         }
     }
 };
+
+void add_edge(const Edge &e, DisjointSets &sets, int64_t *edges, int64_t &ne)
+{
+    if (sets.unite(e.a, e.b)) {
+        edges[2 * ne] = e.a;
+        edges[2 * ne + 1] = e.b;
+        ++ne;
+    }
+}
 
 // Kruskal over `shorts` (sorted here), extending the forest in `sets`.
 void kruskal(std::vector<Edge> &shorts, DisjointSets &sets, int64_t *edges, int64_t &ne)
 {
     std::sort(shorts.begin(), shorts.end(), lighter);
-    for (const Edge &e : shorts)
-        if (sets.unite(e.a, e.b)) {
-            edges[2 * ne] = e.a;
-            edges[2 * ne + 1] = e.b;
-            ++ne;
-        }
+    for (const Edge &e : shorts) add_edge(e, sets, edges, ne);
 }
 
 // Boruvka rounds that join the forest's components into one tree.
 //
-// Per point, a lower bound on its distance^2 to the nearest point of
-// another component. Components only merge, so the set of foreign points
-// only shrinks and a bound stays valid in later rounds. A component's
-// points are searched in increasing bound order, and the search stops
-// once the bound exceeds the component's best edge so far.
+// Per point, a lower bound on its distance^2 to the nearest point of another
+// component. Components only merge, so the set of foreign points only
+// shrinks and a bound stays valid in later rounds. A component's points are
+// searched in increasing bound order, and the search stops once the bound
+// exceeds the component's best edge so far. Components are dealt to worker
+// threads; each component's search is sequential, so the result does not
+// depend on the thread count.
 int complete_tree(KdTree &tree, int64_t n, DisjointSets &sets, int64_t *edges, int64_t ne)
 {
     std::vector<int64_t> comp(static_cast<size_t>(n)), root_ix(static_cast<size_t>(n), -1);
     std::vector<int64_t> members(static_cast<size_t>(n)), first;
-    std::vector<double> lb(static_cast<size_t>(n), 0.0);
+    std::vector<double> lb(static_cast<size_t>(n), 0.0);  // by tree position
     std::vector<Edge> best, picks;
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    struct Task {
+        int64_t c, begin, end;
+        Edge best;
+    };
+    constexpr int64_t kSlice = 32768;
+    std::vector<Task> tasks;
+    const unsigned hw = worker_count();
     while (ne < n - 1) {
         int64_t ncomp = 0;
         for (int64_t i = 0; i < n; ++i) {
@@ -254,44 +311,49 @@ int complete_tree(KdTree &tree, int64_t n, DisjointSets &sets, int64_t *edges, i
         }
         for (int64_t i = 0; i < n; ++i) root_ix[size_t(sets.find(i))] = -1;
         tree.label(comp);
-        // members of each component, by increasing bound (then index)
+        // tree positions of each component's points, by increasing bound
+        // (then position: the order only decides how soon the search stops)
         first.assign(size_t(ncomp) + 1, 0);
-        for (int64_t i = 0; i < n; ++i) ++first[size_t(comp[size_t(i)]) + 1];
+        for (int64_t t = 0; t < n; ++t) ++first[size_t(tree.comp_at(t)) + 1];
         for (int64_t c = 0; c < ncomp; ++c) first[size_t(c) + 1] += first[size_t(c)];
         {
             std::vector<int64_t> fill(first.begin(), first.end() - 1);
-            for (int64_t i = 0; i < n; ++i) members[size_t(fill[size_t(comp[size_t(i)])]++)] = i;
+            for (int64_t t = 0; t < n; ++t) members[size_t(fill[size_t(tree.comp_at(t))]++)] = t;
         }
-        best.assign(size_t(ncomp), Edge{HUGE_VAL, -1, -1});
-        std::atomic<int64_t> next{0};
-        auto work = [&]() {
-            for (int64_t c = next++; c < ncomp; c = next++) {
-                auto b0 = members.begin() + first[size_t(c)], b1 = members.begin() + first[size_t(c) + 1];
-                std::sort(b0, b1, [&](int64_t x, int64_t y) {
-                    return lb[size_t(x)] < lb[size_t(y)] || (lb[size_t(x)] == lb[size_t(y)] && x < y);
-                });
-                Edge &bc = best[size_t(c)];
-                for (auto it = b0; it != b1; ++it) {
-                    const int64_t i = *it;
-                    // strict: a point at exactly the best distance may hold a
-                    // tie that wins on the indices
-                    if (lb[size_t(i)] > bc.w2) break;
-                    const double bound = bc.w2;
-                    const Edge was = bc;
-                    tree.nearest_foreign(i, c, comp, bc);
-                    const bool found = bc.w2 != was.w2 || bc.a != was.a || bc.b != was.b;
-                    lb[size_t(i)] = std::max(lb[size_t(i)], found ? bc.w2 : bound);
-                }
+        // a large component is searched in slices by several threads, each
+        // slice with its own best; a component's edge is the lightest of its
+        // slices' (each slice finds its exact minimum below its bound)
+        tasks.clear();
+        for (int64_t c = 0; c < ncomp; ++c)
+            for (int64_t b = first[size_t(c)]; b < first[size_t(c) + 1]; b += kSlice)
+                tasks.push_back(Task{c, b, std::min(b + kSlice, first[size_t(c) + 1]),
+                                     Edge{HUGE_VAL, -1, -1}});
+        auto search = [&](int64_t k) {
+            Task &task = tasks[size_t(k)];
+            auto b0 = members.begin() + task.begin, b1 = members.begin() + task.end;
+            auto by_bound = [&](int64_t x, int64_t y) {
+                return lb[size_t(x)] < lb[size_t(y)] || (lb[size_t(x)] == lb[size_t(y)] && x < y);
+            };
+            if (!std::is_sorted(b0, b1, by_bound)) std::sort(b0, b1, by_bound);
+            // a private copy: neighbouring entries belong to other threads
+            Edge bc = task.best;
+            for (auto it = b0; it != b1; ++it) {
+                const int64_t t = *it;
+                // strict: a point at exactly the best distance may hold a tie
+                // that wins on the indices
+                if (lb[size_t(t)] > bc.w2) break;
+                const double bound = bc.w2;
+                const Edge was = bc;
+                tree.nearest_foreign(t, bc);
+                const bool found = bc.w2 != was.w2 || bc.a != was.a || bc.b != was.b;
+                lb[size_t(t)] = std::max(lb[size_t(t)], found ? bc.w2 : bound);
             }
+            task.best = bc;
         };
-        const unsigned nt = (n >= 20000 && ncomp >= 64) ? hw : 1u;
-        if (nt > 1) {
-            std::vector<std::thread> pool;
-            for (unsigned t = 0; t < nt; ++t) pool.emplace_back(work);
-            for (std::thread &t : pool) t.join();
-        } else {
-            work();
-        }
+        parallel_for(int64_t(tasks.size()), (n >= 20000 && tasks.size() >= 2) ? hw : 1u, search);
+        best.assign(size_t(ncomp), Edge{HUGE_VAL, -1, -1});
+        for (const Task &task : tasks)
+            if (task.best.a >= 0 && lighter(task.best, best[size_t(task.c)])) best[size_t(task.c)] = task.best;
         picks.clear();
         for (const Edge &e : best)
             if (e.a >= 0) picks.push_back(e);
@@ -299,13 +361,106 @@ int complete_tree(KdTree &tree, int64_t n, DisjointSets &sets, int64_t *edges, i
         // add in order, only edges that join two different components
         std::sort(picks.begin(), picks.end(), lighter);
         const int64_t before = ne;
-        for (const Edge &e : picks)
-            if (sets.unite(e.a, e.b)) {
-                edges[2 * ne] = e.a;
-                edges[2 * ne + 1] = e.b;
-                ++ne;
-            }
+        for (const Edge &e : picks) add_edge(e, sets, edges, ne);
         if (ne == before) return MH_ERR_ARG;  // non-finite coordinates
+    }
+    return MH_OK;
+}
+
+// The forest of the lattice's short edges: pairs of 26-neighbours shorter
+// than twice the smallest voxel size (every pair that short is a 26-neighbour
+// pair), added in the full Kruskal order. pts receives the scaled positions.
+// With point indices in C order of the voxels, each forward offset's edges
+// come out in (a, b) order, so the offsets of one exact length are merged in
+// order without a sort.
+int lattice_forest(const int64_t *coords, int64_t n, const int64_t dims[3], const double vs[3],
+                   std::vector<double> &pts, DisjointSets &sets, int64_t *edges, int64_t &ne)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    std::vector<int32_t> at(static_cast<size_t>(d0 * d1 * d2), -1);
+    pts.resize(size_t(3 * n));
+    bool c_order = true;
+    int64_t prev = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c0 = coords[3 * i], c1 = coords[3 * i + 1], c2 = coords[3 * i + 2];
+        if (c0 < 0 || c0 >= d0 || c1 < 0 || c1 >= d1 || c2 < 0 || c2 >= d2) return MH_ERR_ARG;
+        const int64_t lin = c2 + d2 * (c1 + d1 * c0);
+        int32_t &slot = at[size_t(lin)];
+        if (slot >= 0) return MH_ERR_ARG;  // repeated voxel
+        slot = int32_t(i);
+        c_order = c_order && lin > prev;
+        prev = lin;
+        pts[size_t(3 * i)] = double(c0) * vs[0];
+        pts[size_t(3 * i + 1)] = double(c1) * vs[1];
+        pts[size_t(3 * i + 2)] = double(c2) * vs[2];
+    }
+    const double vmin = std::min(vs[0], std::min(vs[1], vs[2]));
+    const double limit = (2.0 * vmin) * (2.0 * vmin) * (1.0 - 1e-9);
+    // the 13 forward offsets: (0,0,1), (0,1,*), (1,*,*)
+    struct Off {
+        int o0, o1, o2;
+    };
+    std::vector<Off> offs;
+    for (int o0 = 0; o0 <= 1; ++o0)
+        for (int o1 = -1; o1 <= 1; ++o1)
+            for (int o2 = -1; o2 <= 1; ++o2)
+                if (!(o0 == 0 && (o1 < 0 || (o1 == 0 && o2 <= 0)))) offs.push_back(Off{o0, o1, o2});
+    std::vector<std::vector<Edge>> lists(offs.size());
+    parallel_for(int64_t(offs.size()), worker_count(), [&](int64_t k) {
+        const Off &o = offs[size_t(k)];
+        std::vector<Edge> &L = lists[size_t(k)];
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e0 = coords[3 * i] + o.o0, e1 = coords[3 * i + 1] + o.o1,
+                          e2 = coords[3 * i + 2] + o.o2;
+            if (e0 >= d0 || e1 < 0 || e1 >= d1 || e2 < 0 || e2 >= d2) continue;
+            const int32_t j = at[size_t(e2 + d2 * (e1 + d1 * e0))];
+            if (j < 0) continue;
+            const double *a = &pts[size_t(3 * i)], *b = &pts[size_t(3 * j)];
+            const double x0 = a[0] - b[0], x1 = a[1] - b[1], x2 = a[2] - b[2];
+            const double w2 = x0 * x0 + x1 * x1 + x2 * x2;
+            if (w2 < limit) L.push_back(Edge{w2, std::min(i, int64_t(j)), std::max(i, int64_t(j))});
+        }
+    });
+    std::vector<int32_t>().swap(at);
+    // without a sort only when every offset's edges share one exact length
+    // (and C order keeps each list in (a, b) order)
+    bool uniform = c_order;
+    std::vector<double> len(offs.size(), -1.0);
+    for (size_t k = 0; k < lists.size() && uniform; ++k)
+        for (const Edge &e : lists[k]) {
+            if (len[k] < 0) len[k] = e.w2;
+            if (e.w2 != len[k]) {
+                uniform = false;
+                break;
+            }
+        }
+    if (!uniform) {
+        std::vector<Edge> all;
+        for (auto &L : lists) all.insert(all.end(), L.begin(), L.end());
+        kruskal(all, sets, edges, ne);
+        return MH_OK;
+    }
+    std::vector<double> lens;
+    for (double l : len)
+        if (l >= 0) lens.push_back(l);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    for (double l : lens) {
+        std::vector<size_t> cls, pos;
+        for (size_t k = 0; k < offs.size(); ++k)
+            if (len[k] == l) cls.push_back(k);
+        pos.assign(cls.size(), 0);
+        while (true) {
+            size_t pick = cls.size();
+            for (size_t q = 0; q < cls.size(); ++q) {
+                const std::vector<Edge> &L = lists[cls[q]];
+                if (pos[q] >= L.size()) continue;
+                if (pick == cls.size() || lighter(L[pos[q]], lists[cls[pick]][pos[pick]])) pick = q;
+            }
+            if (pick == cls.size()) break;
+            add_edge(lists[cls[pick]][pos[pick]++], sets, edges, ne);
+        }
     }
     return MH_OK;
 }
@@ -315,6 +470,14 @@ inline float dot3(const float *a, const float *b)
 {
 #pragma clang fp contract(off)
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+bool voxel_size_ok(const double *vs)
+{
+    if (!vs) return false;
+    for (int k = 0; k < 3; ++k)
+        if (!(vs[k] > 0.0) || !std::isfinite(vs[k])) return false;
+    return true;
 }
 
 }  // namespace
@@ -335,73 +498,51 @@ MH_EXPORT int mh_spanning_tree_host(const double *pts, int64_t n, double r0, int
 
 // Lattice points (voxel coordinates, scaled by the voxel size): the short
 // edges come from the 13 forward neighbour offsets looked up in a dense index
-// volume. Every pair shorter than twice the smallest voxel size is a
-// 26-neighbour pair, so Kruskal on the neighbour pairs below that length is
-// the full Kruskal's prefix, and the tree is mh_spanning_tree_host's on the
+// volume. Kruskal on the neighbour pairs below twice the smallest voxel size
+// is the full Kruskal's prefix, so the tree is mh_spanning_tree_host's on the
 // scaled points.
 MH_EXPORT int mh_ridge_tree_host(const int64_t *coords, int64_t n, const int64_t dims[3],
                                  const double voxel_size[3], int64_t *edges)
 {
-    if (n < 0 || !mh::dims_ok(dims) || !voxel_size || (n > 0 && !coords) || (n > 1 && !edges))
+    if (n < 0 || !mh::dims_ok(dims) || !voxel_size_ok(voxel_size) || (n > 0 && !coords) ||
+        (n > 1 && !edges))
         return MH_ERR_ARG;
-    for (int k = 0; k < 3; ++k)
-        if (!(voxel_size[k] > 0.0) || !std::isfinite(voxel_size[k])) return MH_ERR_ARG;
     if (n < 2) return MH_OK;
-    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
-    std::vector<int32_t> at(size_t(d0 * d1 * d2), -1);
-    std::vector<double> pts(size_t(3 * n));
-    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
-    for (int64_t i = 0; i < n; ++i) {
-        const int64_t c0 = coords[3 * i], c1 = coords[3 * i + 1], c2 = coords[3 * i + 2];
-        if (c0 < 0 || c0 >= d0 || c1 < 0 || c1 >= d1 || c2 < 0 || c2 >= d2) return MH_ERR_ARG;
-        int32_t &slot = at[size_t(c2 + d2 * (c1 + d1 * c0))];
-        if (slot >= 0) return MH_ERR_ARG;  // repeated voxel
-        slot = int32_t(i);
-        pts[size_t(3 * i)] = double(c0) * voxel_size[0];
-        pts[size_t(3 * i + 1)] = double(c1) * voxel_size[1];
-        pts[size_t(3 * i + 2)] = double(c2) * voxel_size[2];
-    }
-    const double vmin = std::min(voxel_size[0], std::min(voxel_size[1], voxel_size[2]));
-    const double limit = (2.0 * vmin) * (2.0 * vmin) * (1.0 - 1e-9);
-    std::vector<Edge> shorts;
-    shorts.reserve(size_t(4 * n));
-    for (int64_t i = 0; i < n; ++i) {
-        const int64_t c0 = coords[3 * i], c1 = coords[3 * i + 1], c2 = coords[3 * i + 2];
-        for (int o0 = 0; o0 <= 1; ++o0)
-            for (int o1 = -1; o1 <= 1; ++o1)
-                for (int o2 = -1; o2 <= 1; ++o2) {
-                    // forward offsets only: (0,0,1), (0,1,*), (1,*,*)
-                    if (o0 == 0 && (o1 < 0 || (o1 == 0 && o2 <= 0))) continue;
-                    const int64_t e0 = c0 + o0, e1 = c1 + o1, e2 = c2 + o2;
-                    if (e0 >= d0 || e1 < 0 || e1 >= d1 || e2 < 0 || e2 >= d2) continue;
-                    const int32_t j = at[size_t(e2 + d2 * (e1 + d1 * e0))];
-                    if (j < 0) continue;
-                    const double *a = &pts[size_t(3 * i)], *b = &pts[size_t(3 * j)];
-                    const double x0 = a[0] - b[0], x1 = a[1] - b[1], x2 = a[2] - b[2];
-                    const double w2 = x0 * x0 + x1 * x1 + x2 * x2;
-                    if (w2 < limit) shorts.push_back(Edge{w2, std::min(i, int64_t(j)), std::max(i, int64_t(j))});
-                }
-    }
-    std::vector<int32_t>().swap(at);
+    std::vector<double> pts;
     DisjointSets sets(n);
     int64_t ne = 0;
-    kruskal(shorts, sets, edges, ne);
-    std::vector<Edge>().swap(shorts);
+    const int rc = lattice_forest(coords, n, dims, voxel_size, pts, sets, edges, ne);
+    if (rc) return rc;
+    if (ne == n - 1) return MH_OK;
     KdTree tree(pts.data(), n);
     return complete_tree(tree, n, sets, edges, ne);
 }
 
-// Signs along the tree from `root`: a child keeps its direction when its dot
-// product with its adjusted parent is >= 0 and flips otherwise (a NaN product
-// flips), processed in breadth-first order from the root. Points not reached
-// from the root are NaN.
+MH_EXPORT int64_t mh_ridge_forest_host(const int64_t *coords, int64_t n, const int64_t dims[3],
+                                       const double voxel_size[3], int64_t *edges)
+{
+    if (n < 0 || !mh::dims_ok(dims) || !voxel_size_ok(voxel_size) || (n > 0 && !coords) ||
+        (n > 1 && !edges))
+        return MH_ERR_ARG;
+    if (n < 2) return 0;
+    std::vector<double> pts;
+    DisjointSets sets(n);
+    int64_t ne = 0;
+    const int rc = lattice_forest(coords, n, dims, voxel_size, pts, sets, edges, ne);
+    return rc ? rc : ne;
+}
+
+// Signs along the trees of a forest: breadth first from `root`, then from
+// every point not reached yet, in index order (such a point keeps its own
+// direction). A child keeps its direction when its dot product with its
+// adjusted parent is >= 0 and flips otherwise (a NaN product flips).
 MH_EXPORT int mh_tree_signs_host(const float *dirs, int64_t n, const int64_t *edges, int64_t ne,
                                  int64_t root, float *out)
 {
     if (n < 0 || ne < 0 || (n > 0 && (!dirs || !out || root < 0 || root >= n)) || (ne > 0 && !edges))
         return MH_ERR_ARG;
     if (n == 0) return MH_OK;
-    std::vector<int64_t> start(size_t(n) + 1, 0), adj(size_t(2 * ne));
+    std::vector<int64_t> start(static_cast<size_t>(n) + 1, 0), adj(static_cast<size_t>(2 * ne));
     for (int64_t k = 0; k < 2 * ne; ++k) {
         if (edges[k] < 0 || edges[k] >= n) return MH_ERR_ARG;
         ++start[size_t(edges[k]) + 1];
@@ -413,29 +554,32 @@ MH_EXPORT int mh_tree_signs_host(const float *dirs, int64_t n, const int64_t *ed
         adj[size_t(fill[size_t(a)]++)] = b;
         adj[size_t(fill[size_t(b)]++)] = a;
     }
-    const float nan = std::numeric_limits<float>::quiet_NaN();
-    for (int64_t i = 0; i < 3 * n; ++i) out[i] = nan;
-    std::vector<char> seen(size_t(n), 0);
+    std::vector<char> seen(static_cast<size_t>(n), 0);
     std::vector<int64_t> queue;
     queue.reserve(size_t(n));
-    std::memcpy(out + 3 * root, dirs + 3 * root, 3 * sizeof(float));
-    seen[size_t(root)] = 1;
-    queue.push_back(root);
-    for (size_t h = 0; h < queue.size(); ++h) {
-        const int64_t p = queue[h];
-        const float *pp = out + 3 * p;
-        for (int64_t t = start[size_t(p)]; t < start[size_t(p) + 1]; ++t) {
-            const int64_t c = adj[size_t(t)];
-            if (seen[size_t(c)]) continue;
-            seen[size_t(c)] = 1;
-            const float *cd = dirs + 3 * c;
-            const float dot = dot3(cd, pp);
-            const float s = (dot >= 0.f) ? 1.f : -1.f;
-            out[3 * c] = s * cd[0];
-            out[3 * c + 1] = s * cd[1];
-            out[3 * c + 2] = s * cd[2];
-            queue.push_back(c);
+    auto grow = [&](int64_t r) {
+        std::memcpy(out + 3 * r, dirs + 3 * r, 3 * sizeof(float));
+        seen[size_t(r)] = 1;
+        queue.clear();
+        queue.push_back(r);
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int64_t p = queue[h];
+            const float *pp = out + 3 * p;
+            for (int64_t t = start[size_t(p)]; t < start[size_t(p) + 1]; ++t) {
+                const int64_t c = adj[size_t(t)];
+                if (seen[size_t(c)]) continue;
+                seen[size_t(c)] = 1;
+                const float *cd = dirs + 3 * c;
+                const float s = (dot3(cd, pp) >= 0.f) ? 1.f : -1.f;
+                out[3 * c] = s * cd[0];
+                out[3 * c + 1] = s * cd[1];
+                out[3 * c + 2] = s * cd[2];
+                queue.push_back(c);
+            }
         }
-    }
+    };
+    grow(root);
+    for (int64_t i = 0; i < n; ++i)
+        if (!seen[size_t(i)]) grow(i);
     return MH_OK;
 }

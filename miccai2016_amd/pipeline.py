@@ -81,12 +81,24 @@ class PreparedDevice:
     ridge_voxels: int
 
 
-def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=False):
+def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=False,
+                   timings=None):
     """prepare() on a device tensor (float32, N): every stage runs on the GPU
-    and only the ridge steps of the adjuster visit the host."""
+    and only the ridge steps of the adjuster visit the host. With a dict as
+    `timings`, the stream is synchronised after each stage and the seconds
+    of 'frangi', 'adjust', 'gvf' and 'fields' are stored in it."""
+    import time
     from . import _lib
     cfg = PipelineConfig() if cfg is None else cfg
     dev = default_context() if loader is None else loader
+    clock = [time.perf_counter()]
+
+    def lap(name):
+        if timings is not None:
+            dev.synchronize()
+            now = time.perf_counter()
+            timings[name] = now - clock[0]
+            clock[0] = now
     shape = tuple(shape)
     n = int(np.prod(shape))
     img_t = img_t.reshape(-1)
@@ -99,18 +111,21 @@ def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=Fal
     work = dev.empty(int(dev.lib.mh_normalize_work_floats(n)))
     dev.check(dev.lib.mh_minmax_normalize(dev.handle, dev.ptr(val), n, dev.ptr(ves), dev.ptr(work),
                                           dev.stream), "mh_minmax_normalize")
+    lap("frangi")
     adj = AdvancedAdjuster(threshold=cfg.ridge_thresholds, qeigs=None, response=None,
                            voxel_size=cfg.voxel_size, clean_ridge=cfg.clean_ridge, loader=dev)
     dirs = adj.execute_tensor(quat, ves, shape, verbose=verbose)
     v3 = dev.empty(3 * n)
     dev.check(dev.lib.mh_weighted_dirs(dev.handle, dev.ptr(dirs), dev.ptr(ves), n, dev.ptr(v3),
                                        dev.stream), "mh_weighted_dirs")
+    lap("adjust")
     del ff, val, quat
     gp = GvfParameters()
     gp.mu, gp.tol, gp.maxiter = cfg.gvf_mu, 0, cfg.gvf_iters
     gp.callback, gp.update, gp.normalize = None, max(cfg.gvf_iters // 10, 1), False
     gvf = GvfGpu(v3, gp, cfg.voxel_size, cl_program_loader=dev, shape=shape)
     g = gvf.execute_device()
+    lap("gvf")
     quats = dev.empty(3 * n)
     dev.check(dev.lib.mh_soa_to_aos3(dev.handle, dev.ptr(g), n, dev.ptr(quats), dev.stream),
               "mh_soa_to_aos3")
@@ -122,6 +137,7 @@ def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=Fal
               "mh_terminal_capacities")
     dev.check(dev.lib.mh_grad_mag_sq(dev.handle, dev.ptr(img_t), _lib.dims3(shape), dev.ptr(gms),
                                      dev.stream), "mh_grad_mag_sq")
+    lap("fields")
     return PreparedDevice(quats, cs, ct, gms, int(np.count_nonzero(adj.ridge)))
 
 

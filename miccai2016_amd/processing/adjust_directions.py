@@ -108,7 +108,8 @@ class AdvancedAdjuster(TreePrintable):
         adjusted = None
         if (not self.ridge_detection_only) or self.clean_ridge:
             main_dirs = self._qeigs_to_main_dirs()
-            coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
+            coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs,
+                                                          for_cleaning=self.clean_ridge)
             if self.clean_ridge:
                 labeled = self._do_clean_ridge(labeled, coords, rdirs)
                 if not self.ridge_detection_only:
@@ -152,13 +153,35 @@ class AdvancedAdjuster(TreePrintable):
             return coords, dev.download(md.index_select(0, idx))
 
         coords, dirs = ridge_dirs(ridge)
-        coords, adjusted = self._tree_signs(coords, dirs, shape)
+        if self.clean_ridge:
+            coords, adjusted = self._cleaning_signs(coords, dirs, shape)
+        else:
+            coords, adjusted = self._tree_signs(coords, dirs, shape)
         if self.clean_ridge:
             ridge = self._do_clean_ridge(ridge, coords, adjusted)
             coords, dirs = ridge_dirs(ridge)
             coords, adjusted = self._tree_signs(coords, dirs, shape)
         self._flip_nonridge(coords, adjusted, shape)
         return self._d_qeigs
+
+    def _cleaning_signs(self, coords, dirs, shape):
+        """Ridge directions good for the cleaning scores, which only need
+        signs that agree with the spanning tree within each 26-connected
+        segment, up to one flip per segment (the score is the norm of the
+        segment's summed directions). With an isotropic voxel size the tree
+        restricted to a segment is the forest of its neighbour edges, so the
+        forest's signs suffice, unless a forest edge's dot product is zero or
+        NaN (such a decision does not flip with its parent); then, and for
+        anisotropic voxels, the full tree is used."""
+        vs = self.voxel_size
+        if len(coords) > ridge_graph.DENSE_LIMIT and vs[0] == vs[1] == vs[2]:
+            edges = ridge_graph.ridge_forest_edges(coords, shape, vs)
+            d = np.asarray(dirs, np.float64)
+            dots = np.einsum("ij,ij->i", d[edges[:, 0]], d[edges[:, 1]])
+            if not (np.abs(dots) < 1e-6).any() and not np.isnan(dots).any():
+                return (np.require(coords, np.uint16, ["C"]),
+                        ridge_graph.propagate_signs(dirs, edges, 0))
+        return self._tree_signs(coords, dirs, shape)
 
     def _tree_signs(self, coords, dirs, shape):
         if len(coords) == 0:
@@ -189,15 +212,17 @@ class AdvancedAdjuster(TreePrintable):
         self.__main_dirs_unadjusted = main_dirs.copy()
         return main_dirs
 
-    def _adjust_ridge_directions(self, labeled_ridge, main_dirs):
+    def _adjust_ridge_directions(self, labeled_ridge, main_dirs, for_cleaning=False):
         """Minimum spanning tree over the ridge voxels (scaled Euclidean
         distances, adjust_directions.py:365-417); signs propagated from ridge
         voxel 0 so that every tree edge has a non-negative dot product.
-        Returns (uint16 coords R x 3 in C order, float32 adjusted dirs)."""
+        Returns (uint16 coords R x 3 in C order, float32 adjusted dirs).
+        for_cleaning: the directions only feed the cleaning scores."""
         coords = np.argwhere(labeled_ridge != 0)
-        # the tree's short-edge pass uses the scaled 26-neighbourhood
-        return self._tree_signs(coords, main_dirs[tuple(coords.T)] if len(coords) else None,
-                                labeled_ridge.shape)
+        dirs = main_dirs[tuple(coords.T)] if len(coords) else None
+        if for_cleaning and len(coords):
+            return self._cleaning_signs(coords, dirs, labeled_ridge.shape)
+        return self._tree_signs(coords, dirs, labeled_ridge.shape)
 
     def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):
         """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed

@@ -61,11 +61,30 @@ def ridge_tree_edges(coords, shape, voxel_size):
     return edges
 
 
+def ridge_forest_edges(coords, shape, voxel_size):
+    """The first phase of ridge_tree_edges alone: the spanning forest of the
+    neighbour pairs shorter than twice the smallest voxel size (for an
+    isotropic voxel size, a spanning tree of every 26-connected segment)."""
+    coords = np.asarray(coords)
+    n = len(coords)
+    if n < 2:
+        return np.zeros((0, 2), np.int64)
+    c = np.require(coords, np.int64, ["C"])
+    vs = np.require(np.multiply(np.ones(3), voxel_size), np.float64, ["C"])
+    edges = np.empty((n - 1, 2), np.int64)
+    ne = _lib.lib().mh_ridge_forest_host(c.ctypes.data, n, _lib.dims3(shape), vs.ctypes.data,
+                                         edges.ctypes.data)
+    if ne < 0:
+        raise RuntimeError(f"mh_ridge_forest_host failed (status {ne})")
+    return edges[:ne]
+
+
 def propagate_signs(dirs, edges, root=0):
     """Signs along the tree from `root`: a child keeps its direction if its
     dot product with its (already adjusted) parent is >= 0, else flips
-    (adjust_directions.py:384-417). Returns the adjusted float32 directions;
-    points the tree does not reach are NaN."""
+    (adjust_directions.py:384-417). Returns the adjusted float32 directions.
+    For a forest, every tree not reached from `root` starts at its smallest
+    index, which keeps its direction."""
     d32 = np.require(dirs, np.float32, ["C"]).reshape(-1, 3)
     n = len(d32)
     out = np.empty((n, 3), np.float32)

@@ -86,10 +86,11 @@ def test_sparse_mst_is_the_dense_tree_for_distinct_lengths(rng, monkeypatch):
         {tuple(sorted(e)) for e in zip(dense.row.tolist(), dense.col.tolist())}
 
 
-def test_propagate_signs_zero_nan_and_unreached():
+def test_propagate_signs_zero_nan_and_forests():
     """dot >= 0 keeps (a zero product keeps), dot < 0 flips, a NaN product
-    flips; the sign is taken against the adjusted parent; points off the
-    tree stay NaN (adjust_directions.py:384-417)."""
+    flips; the sign is taken against the adjusted parent
+    (adjust_directions.py:384-417); a tree the root does not reach starts at
+    its smallest index, which keeps its direction."""
     dirs = np.array([[1, 0, 0], [0, 1, 0], [0, -1, 0], [np.nan, 0, 0], [-1, 0, 0],
                      [-0.5, 0.5, 0], [3, 3, 3]], np.float32)
     edges = np.array([[0, 1], [2, 1], [3, 0], [4, 0], [1, 5]], np.int64)
@@ -97,7 +98,58 @@ def test_propagate_signs_zero_nan_and_unreached():
     want = np.array([[1, 0, 0], [0, 1, 0], [0, 1, 0], [np.nan, -0.0, -0.0], [1, 0, 0],
                      [-0.5, 0.5, 0]], np.float32)
     np.testing.assert_array_equal(out[:6], want)
-    assert np.signbit(out[3, 1]) and np.isnan(out[6]).all()
+    assert np.signbit(out[3, 1]) and np.array_equal(out[6], dirs[6])
+    # a second tree: 6 - 7 - 8, started at 6
+    dirs2 = np.concatenate([dirs, [[-3, -3, 0], [1, 1, 0]]]).astype(np.float32)
+    edges2 = np.concatenate([edges, [[7, 6], [7, 8]]])
+    out2 = ridge_graph.propagate_signs(dirs2, edges2, 0)
+    np.testing.assert_array_equal(out2[6:], [[3, 3, 3], [3, 3, 0], [1, 1, 0]])
+
+
+class _NoDevice:
+    """Stand-in loader for the adjuster's host steps."""
+
+
+def _segments(rng, shape, count, zero_dot=False):
+    """Random 26-connected ridge segments (random walks) and random
+    directions; with zero_dot, one segment is two neighbours with orthogonal
+    axis directions."""
+    ridge = np.zeros(shape, np.float32)
+    for _ in range(count):
+        p = rng.integers(2, np.array(shape) - 2)
+        for _ in range(int(rng.integers(5, 60))):
+            ridge[tuple(p)] = 1
+            p = np.clip(p + rng.integers(-1, 2, size=3), 0, np.array(shape) - 1)
+    dirs = rng.normal(size=shape + (3,)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=-1, keepdims=True)
+    if zero_dot:                      # a segment of two voxels, one forest edge
+        ridge[:4, :4, :4] = 0
+        ridge[1, 1, 1] = ridge[1, 1, 2] = 1
+        dirs[1, 1, 1] = (1, 0, 0)
+        dirs[1, 1, 2] = (0, 1, 0)
+    return ridge, dirs
+
+
+@pytest.mark.parametrize("seed,zero_dot", [(1, False), (2, False), (3, True)])
+def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, zero_dot):
+    """The cleaning keeps the same segments whether its scores come from the
+    neighbour forest's signs or from the full spanning tree's."""
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
+    rng = np.random.default_rng(seed)
+    shape = (64, 64, 64)
+    ridge, dirs = _segments(rng, shape, 80, zero_dot)
+    adj = AdvancedAdjuster((0.01, 0.3), None, None, voxel_size=1, loader=_NoDevice())
+    c_full, d_full = adj._adjust_ridge_directions(ridge, dirs)
+    c_forest, d_forest = adj._adjust_ridge_directions(ridge, dirs, for_cleaning=True)
+    np.testing.assert_array_equal(c_full, c_forest)
+    np.testing.assert_array_equal(np.abs(d_full), np.abs(d_forest))
+    if zero_dot:
+        np.testing.assert_array_equal(d_full, d_forest)       # fell back to the tree
+    keep_full = adj._do_clean_ridge(ridge, c_full, d_full)
+    keep_forest = adj._do_clean_ridge(ridge, c_forest, d_forest)
+    assert 0 < keep_full.sum() < ridge.sum()
+    np.testing.assert_array_equal(keep_full, keep_forest)
 
 
 @pytest.mark.parametrize("vs", [(1, 1, 1), (2.0, 0.5, 1.25), (1, 1, 3)])
