@@ -27,9 +27,9 @@ struct ConvGroup {
     ConvOut out[3];
 };
 
-// One pass along one axis; each group reads one input field once per tap and
-// feeds up to three outputs (derivative orders), so intermediates shared by
-// several Hessian entries are computed once.
+// One pass along one axis; each group reads one input field and feeds up to
+// three outputs (derivative orders), so intermediates shared by several
+// Hessian entries are computed once.
 struct ConvPass {
     ConvGroup grp[6];
     int ngroups;
@@ -37,62 +37,187 @@ struct ConvPass {
     float taps[3][kMaxTaps];
 };
 
-template <int AXIS>
-__global__ void __launch_bounds__(mh::kBlock)
-conv_pass_kernel(const ConvPass P, int d0, int d1, int d2)
+// Index of position p of an axis of length len, mirror-completed and tiled
+// with period 2 (len - 1) (scipy's 'mirror' mode; a length-1 axis repeats its
+// one element).
+__device__ __forceinline__ int mirror_index(int p, int len)
 {
-    const int64_t n = int64_t(d0) * d1 * d2;
-    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
-    const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int i2 = int(i % d2);
-    const int i1 = int((i / d2) % d1);
-    const int i0 = int(i / (int64_t(d2) * d1));
-    const int len = AXIS == 0 ? d0 : (AXIS == 1 ? d1 : d2);
-    const int ia = AXIS == 0 ? i0 : (AXIS == 1 ? i1 : i2);
-    const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : (AXIS == 1 ? d2 : 1);
-    const int64_t base = i - int64_t(ia) * sa;
-    const int R = P.radius;
-    const int K = 2 * R + 1;
+    if (p >= 0 && p < len) return p;
     const int period = 2 * (len - 1);
+    if (period == 0) return 0;
+    if (p < 0 && -p < len) return -p;                 // one reflection at 0
+    if (p >= len && p < period) return period - p;    // one reflection at len - 1
+    int t = p % period;
+    t += t < 0 ? period : 0;
+    return t < len ? t : period - t;
+}
 
+// Both pass kernels stage a tile of the input line segments plus a halo of
+// R taps on each side in LDS (each element loaded once, coalesced along axis
+// 2), together with the taps in convolution order; every output then sums
+// its K taps from LDS in the order j = 0 .. K-1 of the true convolution (tap
+// K-1-j against element p - R + j).
+
+constexpr int kTileX = 64;    // consecutive voxels along axis 2 per tile
+constexpr int kRunA = 16;     // outputs per thread along the pass axis (axes 0, 1)
+constexpr int kTileA = 4 * kRunA;
+constexpr int kRows2 = 16;    // rows (axis 1) per tile of the axis-2 pass
+
+// Pass along axis 0 or 1. Tile: 64 voxels along axis 2 x 64 along the pass
+// axis; thread (x, y) computes the 16 consecutive outputs y*16 .. y*16+15 of
+// column x. LDS: (64 + 2R) x 64 floats and the group's taps (up to three
+// orders).
+template <int AXIS>
+__global__ void __launch_bounds__(256)
+conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
+{
+    extern __shared__ float lds[];
+    const int len = AXIS == 0 ? d0 : d1;
+    const int other = AXIS == 0 ? d1 : d0;
+    const int n2 = (d2 + kTileX - 1) / kTileX;
+    const int na = (len + kTileA - 1) / kTileA;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int t2 = int(b % unsigned(n2));
+    const int ta = int((b / unsigned(n2)) % unsigned(na));
+    const int o = int(b / (unsigned(n2) * unsigned(na)));
+    if (o >= other) return;
+    const int x = threadIdx.x & (kTileX - 1), y = threadIdx.x / kTileX;
+    const int i2 = t2 * kTileX + x;
+    const int a0 = ta * kTileA;
+    const int R = P.radius, K = 2 * R + 1, rows = kTileA + 2 * R;
+    const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : int64_t(d2);
+    const int64_t base = AXIS == 0 ? int64_t(o) * d2 + i2 : int64_t(o) * d1 * d2 + i2;
+    const bool live = i2 < d2;
+    float *wts = lds + rows * kTileX;
     for (int g = 0; g < P.ngroups; ++g) {
         const ConvGroup &G = P.grp[g];
-        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-        // position in the mirror-completed, periodically tiled axis
-        int t = 0;
-        if (period > 0) {
-            t = (ia - R - 1) % period;
-            t += t < 0 ? period : 0;
-        }
+        for (int r = y; r < rows; r += 4)
+            lds[r * kTileX + x] = live ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa] : 0.f;
+        for (int t = threadIdx.x; t < 3 * K; t += 256)
+            wts[t] = P.taps[G.out[t / K].order][K - 1 - t % K];
+        __syncthreads();
+        float acc[3][kRunA];
+#pragma unroll
+        for (int q = 0; q < kRunA; ++q) acc[0][q] = acc[1][q] = acc[2][q] = 0.f;
+        const float *col = lds + (y * kRunA) * kTileX + x;
         for (int j = 0; j < K; ++j) {
-            int m = 0;
-            if (period > 0) {
-                ++t;
-                t = t == period ? 0 : t;
-                m = t < len ? t : period - t;
+            const float w0 = wts[j], w1 = wts[K + j], w2 = wts[2 * K + j];
+#pragma unroll
+            for (int q = 0; q < kRunA; ++q) {
+                const float v = col[(q + j) * kTileX];
+                acc[0][q] += w0 * v;
+                if (G.nout > 1) acc[1][q] += w1 * v;
+                if (G.nout > 2) acc[2][q] += w2 * v;
             }
-            const float x = G.in[base + int64_t(m) * sa];
-            const int k = K - 1 - j;
-            acc0 += P.taps[G.out[0].order][k] * x;
-            if (G.nout > 1) acc1 += P.taps[G.out[1].order][k] * x;
-            if (G.nout > 2) acc2 += P.taps[G.out[2].order][k] * x;
         }
-        G.out[0].ptr[i * G.out[0].stride + G.out[0].offset] = acc0;
-        if (G.nout > 1) G.out[1].ptr[i * G.out[1].stride + G.out[1].offset] = acc1;
-        if (G.nout > 2) G.out[2].ptr[i * G.out[2].stride + G.out[2].offset] = acc2;
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < kRunA; ++q) {
+                const int a = a0 + y * kRunA + q;
+                if (a < len) {
+                    const int64_t i = base + int64_t(a) * sa;
+                    G.out[0].ptr[i * G.out[0].stride + G.out[0].offset] = acc[0][q];
+                    if (G.nout > 1) G.out[1].ptr[i * G.out[1].stride + G.out[1].offset] = acc[1][q];
+                    if (G.nout > 2) G.out[2].ptr[i * G.out[2].stride + G.out[2].offset] = acc[2][q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Pass along axis 2 (contiguous), one output per group. Tile: 64 outputs
+// along axis 2 x 16 rows along axis 1; thread (x, y) computes voxel x of rows
+// y, y+4, y+8, y+12; the tile rows are loaded by 16 threads each. LDS:
+// 16 x (64 + 2R) floats and the group's K taps. The results of all groups
+// stay in registers and are stored together, so a voxel's AoS components
+// leave in one burst.
+__global__ void __launch_bounds__(256)
+conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
+{
+    extern __shared__ float lds[];
+    const int n2 = (d2 + kTileX - 1) / kTileX;
+    const int n1 = (d1 + kRows2 - 1) / kRows2;
+    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int t2 = int(b % unsigned(n2));
+    const int t1 = int((b / unsigned(n2)) % unsigned(n1));
+    const int i0 = int(b / (unsigned(n2) * unsigned(n1)));
+    if (i0 >= d0) return;
+    const int x = threadIdx.x & (kTileX - 1), y = threadIdx.x / kTileX;
+    const int R = P.radius, K = 2 * R + 1, width = kTileX + 2 * R;
+    const int s0 = t2 * kTileX;
+    // loading: row lr of the tile, columns lc, lc + 16, ...
+    const int lr = threadIdx.x / 16, lc = threadIdx.x % 16;
+    const int li1 = t1 * kRows2 + lr;
+    const int64_t lrow = (int64_t(i0) * d1 + li1) * d2;
+    float *wts = lds + kRows2 * width;
+    float res[4][6];
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+        if (g >= P.ngroups) break;
+        const ConvGroup &G = P.grp[g];
+        for (int c = lc; c < width; c += 16)
+            lds[lr * width + c] = li1 < d1 ? G.in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
+        if (int(threadIdx.x) < K) wts[threadIdx.x] = P.taps[G.out[0].order][K - 1 - int(threadIdx.x)];
+        __syncthreads();
+        float a[4] = {0.f, 0.f, 0.f, 0.f};
+        const float *line = lds + y * width + x;
+#pragma unroll 4
+        for (int j = 0; j < K; ++j) {
+            const float w = wts[j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] += w * line[4 * q * width + j];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) res[q][g] = a[q];
+        __syncthreads();
+    }
+    const int i2 = s0 + x;
+    if (i2 >= d2) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i1 = t1 * kRows2 + y + 4 * q;
+        if (i1 >= d1) continue;
+        const int64_t i = (int64_t(i0) * d1 + i1) * d2 + i2;
+#pragma unroll
+        for (int g = 0; g < 6; ++g) {
+            if (g >= P.ngroups) break;
+            const ConvOut &O = P.grp[g].out[0];
+            O.ptr[i * O.stride + O.offset] = res[q][g];
+        }
     }
 }
 
 int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, hipStream_t s)
 {
-    const int64_t n = dims[0] * dims[1] * dims[2];
-    const dim3 grid(mh::blocks_for(n)), block(mh::kBlock);
+    // no output may alias an input (tiles read halos that other tiles would
+    // overwrite); the axis-2 pass keeps one output per group in registers
+    MH_REQUIRE(ctx, P.ngroups >= 1 && P.ngroups <= 6, "1 to 6 groups per pass");
+    for (int g = 0; g < P.ngroups; ++g) {
+        MH_REQUIRE(ctx, P.grp[g].nout >= 1 && P.grp[g].nout <= (axis == 2 ? 1 : 3),
+                   "outputs per group: 1 to 3 (1 along axis 2)");
+        for (int h = 0; h < P.ngroups; ++h)
+            for (int u = 0; u < P.grp[h].nout; ++u)
+                MH_REQUIRE(ctx, P.grp[h].out[u].ptr != P.grp[g].in, "a pass cannot work in place");
+    }
     const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
-    if (axis == 0) hipLaunchKernelGGL(conv_pass_kernel<0>, grid, block, 0, s, P, d0, d1, d2);
-    else if (axis == 1) hipLaunchKernelGGL(conv_pass_kernel<1>, grid, block, 0, s, P, d0, d1, d2);
-    else hipLaunchKernelGGL(conv_pass_kernel<2>, grid, block, 0, s, P, d0, d1, d2);
-    MH_CHECK_LAUNCH(ctx, "conv_pass_kernel");
+    const int64_t n2 = (d2 + kTileX - 1) / kTileX;
+    if (axis == 2) {
+        const int64_t blocks = n2 * ((d1 + kRows2 - 1) / kRows2) * d0;
+        MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
+        const size_t shm = sizeof(float) * (kRows2 * (kTileX + 2 * P.radius) + kMaxTaps);
+        hipLaunchKernelGGL(conv_row_kernel, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+    } else {
+        const int len = axis == 0 ? d0 : d1, other = axis == 0 ? d1 : d0;
+        const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other;
+        MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
+        const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius) + 3 * kMaxTaps);
+        if (axis == 0)
+            hipLaunchKernelGGL(conv_tile_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+        else
+            hipLaunchKernelGGL(conv_tile_kernel<1>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+    }
+    MH_CHECK_LAUNCH(ctx, "conv pass");
     return MH_OK;
 }
 

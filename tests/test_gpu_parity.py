@@ -45,7 +45,24 @@ def test_gauss_filter_matches_oracle(dev, rng):
     assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (9, 20, 31), (40, 3, 7)])
+def test_gauss_filter_kernel_longer_than_axis(dev, rng):
+    """Radius 16 taps against axes of 3 and 50 voxels: the mirror completion
+    repeats with period 2 (len - 1), as scipy's 'mirror' mode does."""
+    from miccai2016_amd.processing.qfrangi_gpu import GaussianFilter3D
+    img = rng.normal(size=(3, 50, 70)).astype(np.float32)
+    gf = GaussianFilter3D(dev, round_up=False)
+    a, b = dev.upload(img).reshape(-1), dev.empty(img.size)
+    res, _ = gf.execute(a, b, img.shape, (4.0, 4.0, 1.0), (0, 2, 1))
+    got = dev.download(res).reshape(img.shape)
+    taps = gf.taps_for((4.0, 4.0, 1.0), (0, 2, 1))
+    want = img
+    for ax in range(3):
+        want = ref32.convolve_axis(want, ax, taps[ax])
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (9, 20, 31), (40, 3, 7), (70, 130, 67),
+                                   (1, 5, 80), (33, 1, 1)])
 def test_hessian_matches_oracle(dev, rng, shape):
     from miccai2016_amd.processing.qfrangi_gpu import GaussianFilter3D, Hessian
     img = _smooth_img(rng, shape, 1.0)
@@ -54,7 +71,13 @@ def test_hessian_matches_oracle(dev, rng, shape):
     out = dev.empty(6 * img.size)
     h.execute(dev.upload(img).reshape(-1), None, None, shape, out, sig)
     got = dev.download(out).reshape(shape + (6,))
-    want = ref32.hessian(img, sig)
+    if min(shape) > 1:
+        want = ref32.hessian(img, sig)
+    else:
+        # the reference's mirror indexing divides by 2 (len - 1) = 0 on a
+        # length-1 axis; the kernels repeat the one element there, as scipy's
+        # 'mirror' mode does, so scipy is the oracle for these shapes
+        want = fp64.hessian_scipy(img, sig)
     assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
 
 
