@@ -10,16 +10,19 @@ capacities, |grad I|^2) are prepared once, outside the timed region.
 
 Reported beside it: every cell's fast Dice against the noise-free mask
 (counted on the device); the same solves run one at a time (device-resident,
-and through the numpy drop-in API); a 256^3 x 1000-iteration single solve
-(the bandwidth-scale plain workload); a ragged set of helix phantoms through
-the whole pipeline with one packed max-flow solve; serving latency (p50/p99) and
-throughput under Poisson arrivals of mixed-size cuts; the HBM roofline of the
-max-flow iteration (84 algorithmic bytes per voxel-iteration) with the HBM
-traffic rocprofv3 measured for the same kernels; and the CPU oracle timed on
-a bounded sample on this host.
+and through the numpy drop-in API); the 64^3 and 256^3 single solves through
+the whole pipeline (the plain workloads, 500 and 1000 iterations); a ragged
+set of helix phantoms through the whole pipeline with one packed max-flow
+solve; serving latency (p50/p99) and throughput under Poisson arrivals of
+mixed-size cuts, and of requests that need the whole preparation; the HBM
+roofline of the max-flow iteration (84 algorithmic bytes per voxel-iteration)
+with the HBM traffic rocprofv3 measured for the same kernels; and the CPU
+oracle timed on a bounded sample on this host.
 
-``--only PART`` runs one part (grid, plain, plain256, ragged, serve, cpu, or copy:
-a 512 MiB device copy used to calibrate the rocprofv3 byte counters).
+``--only PART`` runs one part (grid, plain, plain64, plain256, ragged, serve,
+cpu; sigmas: the grid at all 15 noise levels of the experiment, not part of
+the default run; or copy: a 512 MiB device copy used to calibrate the
+rocprofv3 byte counters).
 """
 import argparse
 import ctypes as C
@@ -47,7 +50,7 @@ TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 41.7, 2: 43.8, 3: 42.1, 5: 43.3}
 TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.5, 2: 49.0, 3: 46.1, 5: 48.6}
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
-PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu")
+PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu", "sigmas")
 
 
 def log(msg):
@@ -137,6 +140,8 @@ def main():
     ap.add_argument("--serve-pipeline-requests", type=int, default=20)
     ap.add_argument("--serve-prepare-workers", type=int, default=4)
     ap.add_argument("--ragged-volumes", type=int, default=16)
+    ap.add_argument("--sigmas", type=int, default=15,
+                    help="noise levels of --only sigmas (the first of linspace(0.1, 1.5, 15))")
     ap.add_argument("--ragged-workers", type=int, default=8,
                     help="threads preparing the ragged volumes (own context and stream each)")
     ap.add_argument("--no-ragged", action="store_true")
@@ -148,6 +153,7 @@ def main():
     run = {p: (args.only == p) if args.only else True for p in PARTS}
     if not args.only:
         run["cpu"] = not args.no_cpu
+        run["sigmas"] = False      # the whole experiment: --only sigmas
         run["plain256"] = not args.no_256
         run["serve"] = not args.no_serve
         run["ragged"] = not args.no_ragged
@@ -453,6 +459,60 @@ def main():
             log(f"serving at {qps} QPS: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s,"
                 f" peak {srv.peak_resident} resident")
             del srv
+
+    # ---- the experiment's grid at every noise level (config 3 in full)
+    if run["sigmas"]:
+        from miccai2016_amd import _lib
+        from miccai2016_amd.evaluation import DeviceDice, device_dice_for
+        from miccai2016_amd.pipeline import prepare_device
+        w, d = w_all[:S].astype(np.float32), d_all[:S].astype(np.float32)
+        base = ph.GRID_128
+        shape = base.shape
+        n = int(np.prod(shape))
+        cutter = CmfCutQAnisotropicGpu(cut_parameters(cfg), dev)
+        u = torch.empty((S, n), dtype=torch.float32, device=dev.device)
+        rows, t_all = [], time.perf_counter()
+        for sigma in np.linspace(0.1, 1.5, 15)[:max(1, args.sigmas)]:
+            t0 = time.perf_counter()
+            img, truth = dev.empty(n), dev.empty(n, dtype=torch.uint8)
+            dev.check(dev.lib.mh_helix_phantom(dev.handle, _lib.dims3(shape), base.r_tube,
+                                               base.r_helix, base.pitch, float(sigma),
+                                               C.c_uint64(42), dev.ptr(img), dev.ptr(truth), None,
+                                               dev.stream), "mh_helix_phantom")
+            prep = prepare_device(img, shape, float(sigma), cfg, dev)
+            u0 = dev.empty(n)
+            wk = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+            dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk),
+                                           dev.stream), "mh_initial_u")
+            dev.synchronize()
+            t_prep = time.perf_counter() - t0
+
+            def grid_step():
+                u.copy_(u0.reshape(1, -1).expand(S, n))
+                cutter.execute_batched_device(u, S, w, d, prep.quats, prep.gms, prep.cs, prep.ct,
+                                              shape, eigs)
+            t_grid = timed(torch, grid_step)
+            fd = DeviceDice(truth, dev)(u, S)
+            best = int(np.nanargmax(fd))
+            try:
+                true_dice = float(device_dice_for(u[best], truth, shape, dev)[0])
+            except ValueError:
+                true_dice = float("nan")
+            rows.append({"sigma": float(sigma), "prepare_s": t_prep, "grid_s": t_grid,
+                         "voxel_iters_per_s": S * n * args.iters / t_grid,
+                         "ridge_voxels": prep.ridge_voxels, "best_fast_dice": float(fd[best]),
+                         "dice_of_largest_component": true_dice,
+                         "best_weighting": float(w_all[best]), "best_decay": float(d_all[best])})
+            log(f"sigma {sigma:.2f}: prepare {t_prep:.2f} s, grid {t_grid:.2f} s, best fast Dice "
+                f"{fd[best]:.4f} (largest component {true_dice:.4f}) at w={w_all[best]:.4g}, "
+                f"decay={d_all[best]:.4g}")
+            del prep, img, truth, u0
+        line["grid_sigmas"] = {
+            "levels": rows, "total_s": time.perf_counter() - t_all,
+            "workload": f"{S} ATV solves x {args.iters} iterations per noise level on the 128^3 helix "
+                        "(tube 6, radius 32, pitch 48), noise linspace(0.1, 1.5, 15); Frangi, "
+                        "adjustment and GVF once per level on the device"}
+        del u
 
     # ---- CPU oracle on a bounded sample of the same workload
     if run["cpu"]:
