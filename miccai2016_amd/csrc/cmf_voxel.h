@@ -19,16 +19,16 @@ namespace mh {
 
 __device__ __forceinline__ float kmin(float x, float y) { return y < x ? y : x; }
 
-// Every helper below is evaluated without contraction into fused
-// multiply-adds (only the explicit fmaf calls fuse), so that each solver
-// variant, whatever the surrounding code, rounds identically.
+// Every helper below is evaluated without implicit contraction into fused
+// multiply-adds: the fused operations are the explicit fmaf calls, so that
+// each solver variant, whatever the surrounding code, rounds identically.
 
 // divergence from the backward differences x_k of hp_k, scaled by rs
 __device__ __forceinline__ float cmf_div(float x0, float x1, float x2, float rs0, float rs1,
                                          float rs2)
 {
 #pragma clang fp contract(off)
-    return x0 * rs0 + x1 * rs1 + x2 * rs2;
+    return fmaf(x2, rs2, fmaf(x1, rs1, x0 * rs0));
 }
 
 // sq: sqrt of the eigenvalues of the frame; rq = 1 / sq (formed once on the
@@ -42,17 +42,24 @@ __device__ __forceinline__ void cmf_p_voxel_m(float &h0, float &h1, float &h2, f
 {
 #pragma clang fp contract(off)
     const float *m = R.m;
-    float p0 = (m[0] * h0 + m[3] * h1 + m[6] * h2) * rq0 + sq0 * (m[0] * g0 + m[3] * g1 + m[6] * g2);
-    float p1 = (m[1] * h0 + m[4] * h1 + m[7] * h2) * rq1 + sq1 * (m[1] * g0 + m[4] * g1 + m[7] * g2);
-    float p2 = (m[2] * h0 + m[5] * h1 + m[8] * h2) * rq2 + sq2 * (m[2] * g0 + m[5] * g1 + m[8] * g2);
-    const float nrm = __builtin_amdgcn_sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+    // R^T hp and R^T g
+    const float a0 = fmaf(m[6], h2, fmaf(m[3], h1, m[0] * h0));
+    const float a1 = fmaf(m[7], h2, fmaf(m[4], h1, m[1] * h0));
+    const float a2 = fmaf(m[8], h2, fmaf(m[5], h1, m[2] * h0));
+    const float b0 = fmaf(m[6], g2, fmaf(m[3], g1, m[0] * g0));
+    const float b1 = fmaf(m[7], g2, fmaf(m[4], g1, m[1] * g0));
+    const float b2 = fmaf(m[8], g2, fmaf(m[5], g1, m[2] * g0));
+    float p0 = fmaf(a0, rq0, sq0 * b0);
+    float p1 = fmaf(a1, rq1, sq1 * b1);
+    float p2 = fmaf(a2, rq2, sq2 * b2);
+    const float nrm = __builtin_amdgcn_sqrtf(fmaf(p2, p2, fmaf(p1, p1, p0 * p0)));
     const float sc = nrm > alpha ? alpha * __builtin_amdgcn_rcpf(nrm) : 1.f;
     p0 *= sc * sq0;
     p1 *= sc * sq1;
     p2 *= sc * sq2;
-    h0 = m[0] * p0 + m[1] * p1 + m[2] * p2;
-    h1 = m[3] * p0 + m[4] * p1 + m[5] * p2;
-    h2 = m[6] * p0 + m[7] * p1 + m[8] * p2;
+    h0 = fmaf(m[2], p2, fmaf(m[1], p1, m[0] * p0));
+    h1 = fmaf(m[5], p2, fmaf(m[4], p1, m[3] * p0));
+    h2 = fmaf(m[8], p2, fmaf(m[7], p1, m[6] * p0));
 }
 
 // the same with the rotation given as the unit quaternion (w, x, y, z) in qq

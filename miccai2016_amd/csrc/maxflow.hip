@@ -246,44 +246,57 @@ struct SweepIn {
     float a, pt, u, cs, ct;
 };
 
-// Plane-uniform base pointer plus a 32-bit byte offset: the loads and stores
-// then take the scalar-base + vector-offset form and keep one VGPR per
-// address instead of two.
-template <class T>
-__device__ __forceinline__ T ld(const T *base, unsigned byte_off)
+// The sweep addresses its fields through buffer resources: a uniform byte
+// offset (the plane, and the component of hp) plus the thread's 32-bit byte
+// offset within the plane, so no 64-bit address is formed per lane. Every
+// byte offset stays below 2^32 (the sweep takes volumes of fewer than 2^28
+// voxels), and each resource spans exactly its field.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int64_t kSweepMaxVoxels = int64_t(1) << 28;
+constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit data, gfx9 family
+
+__device__ __forceinline__ Rsrc rsrc_of(const void *p, int64_t bytes)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(unsigned(bytes)), kRsrcWord3);
 }
-__device__ __forceinline__ void st(float *base, unsigned byte_off, float v)
+__device__ __forceinline__ float bld(Rsrc r, unsigned voff, unsigned soff)
 {
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off) = v;
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+__device__ __forceinline__ float4 bld4(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, int(voff), int(soff), 0);
 }
 
-__device__ __forceinline__ void sweep_load(SweepIn &L, const SweepBufs &B,
-                                           const float4 *__restrict__ q4,
-                                           const float *__restrict__ alpha,
-                                           const float *__restrict__ cs,
-                                           const float *__restrict__ ct, int64_t n, int64_t st0,
-                                           int d2, int64_t zoff, unsigned ob, bool p_on, bool u_on,
-                                           bool e0, unsigned f)
+struct SweepRsrc {
+    Rsrc tmp, hp, q, a, pt, u, cs, ct;  // inputs
+    Rsrc hpo, tmpo, pto, uo;            // outputs
+};
+
+__device__ __forceinline__ void sweep_load(SweepIn &L, const SweepRsrc &R, unsigned nb,
+                                           unsigned pb, unsigned d2b, unsigned zb, unsigned ob,
+                                           bool p_on, bool u_on, bool e0, unsigned f)
 {
     if (p_on) {
-        const float *tz = B.tmp + zoff;
-        L.t = ld(tz, ob);
-        L.t0 = e0 ? ld(tz + st0, ob) : 0.f;
-        L.t1 = (f & 16u) ? ld(tz, ob + 4u * unsigned(d2)) : 0.f;
-        L.t2 = (f & 32u) ? ld(tz, ob + 4u) : 0.f;
-        L.h0 = ld(B.hp + zoff, ob);
-        L.h1 = ld(B.hp + n + zoff, ob);
-        L.h2 = ld(B.hp + 2 * n + zoff, ob);
-        L.q = ld(q4 + zoff, 4u * ob);
-        L.a = ld(alpha + zoff, ob);
+        L.t = bld(R.tmp, ob, zb);
+        L.t0 = e0 ? bld(R.tmp, ob, zb + pb) : 0.f;
+        L.t1 = (f & 16u) ? bld(R.tmp, ob + d2b, zb) : 0.f;
+        L.t2 = (f & 32u) ? bld(R.tmp, ob + 4u, zb) : 0.f;
+        L.h0 = bld(R.hp, ob, zb);
+        L.h1 = bld(R.hp, ob, zb + nb);
+        L.h2 = bld(R.hp, ob, zb + 2u * nb);
+        L.q = bld4(R.q, 4u * ob, 4u * zb);
+        L.a = bld(R.a, ob, zb);
     }
     if (u_on) {
-        L.pt = ld(B.pt + zoff, ob);
-        L.u = ld(B.u + zoff, ob);
-        L.cs = ld(cs + zoff, ob);
-        L.ct = ld(ct + zoff, ob);
+        L.pt = bld(R.pt, ob, zb);
+        L.u = bld(R.u, ob, zb);
+        L.cs = bld(R.cs, ob, zb);
+        L.ct = bld(R.ct, ob, zb);
     }
 }
 
@@ -331,6 +344,11 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     const int z1 = min(z0 + zchunk, D.d0);
     const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
     const int64_t st0 = int64_t(D.d1) * D.d2;
+    const SweepRsrc R{rsrc_of(B.tmp, 4 * n), rsrc_of(B.hp, 12 * n), rsrc_of(q4, 16 * n),
+                      rsrc_of(alpha, 4 * n), rsrc_of(B.pt, 4 * n), rsrc_of(B.u, 4 * n),
+                      rsrc_of(cs, 4 * n), rsrc_of(ct, 4 * n), rsrc_of(B.hpo, 12 * n),
+                      rsrc_of(B.tmpo, 4 * n), rsrc_of(B.pto, 4 * n), rsrc_of(B.uo, 4 * n)};
+    const unsigned nb = unsigned(4 * n), pb = unsigned(4 * st0), d2b = 4u * unsigned(D.d2);
 
     // this thread's point of the widened tile
     const int p = int(threadIdx.x);
@@ -362,15 +380,14 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     L.t = L.t0 = L.t1 = L.t2 = L.h0 = L.h1 = L.h2 = L.a = L.pt = L.u = L.cs = L.ct = 0.f;
     L.q = make_float4(1.f, 0.f, 0.f, 0.f);
     if (zs <= zend)
-        sweep_load(L, B, q4, alpha, cs, ct, n, st0, D.d2, int64_t(zs) * st0, ob, p_on(zs),
-                   u_on(zs), zs < D.d0 - 1, f);
+        sweep_load(L, R, nb, pb, d2b, unsigned(zs) * pb, ob, p_on(zs), u_on(zs), zs < D.d0 - 1, f);
     for (int z = zs; z <= zend; ++z) {
         const bool ph1 = p_on(z), ph2 = u_on(z);
         const bool ph3 = two && inE1 && z >= max(z0, 1);
         const bool ph4 = two && inO && z - 1 >= z0 && z - 1 < z1;
         const bool e0 = z < D.d0 - 1, f0 = z > 0;            // plane z
         const bool e0b = z - 1 < D.d0 - 1, f0b = z - 1 > 0;  // plane z-1
-        const int64_t zoff = int64_t(z) * st0;
+        const unsigned zb = unsigned(z) * pb;  // plane z, bytes
         // 1. level-A p-step at plane z on the widened tile
         float hA0_0 = 0.f, hA0_1 = 0.f, hA0_2 = 0.f;
         // the frame of plane z, built once for both levels
@@ -385,9 +402,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             mh::cmf_p_voxel_m(hA0_0, hA0_1, hA0_2, g0, g1, g2, RN, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
                               PP.rq1, PP.rq2, L.a);
             if (!two && inO && z >= z0) {
-                st(B.hpo + zoff, ob, hA0_0);
-                st(B.hpo + n + zoff, ob, hA0_1);
-                st(B.hpo + 2 * n + zoff, ob, hA0_2);
+                bst(R.hpo, ob, zb, hA0_0);
+                bst(R.hpo, ob, zb + nb, hA0_1);
+                bst(R.hpo, ob, zb + 2u * nb, hA0_2);
             }
         }
         if (p < NE) {
@@ -406,9 +423,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             uA0 = L.u;
             const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, UP.inv_rcc, 1.f, 0.f);
             if (!two) {
-                st(B.pto + zoff, ob, ptA0);
-                st(B.uo + zoff, ob, uA0);
-                st(B.tmpo + zoff, ob, tA0);
+                bst(R.pto, ob, zb, ptA0);
+                bst(R.uo, ob, zb, uA0);
+                bst(R.tmpo, ob, zb, tA0);
                 if (check) err_acc += fabsf(err);
             }
         }
@@ -416,8 +433,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         const float aN = L.a, csN = L.cs, ctN = L.ct;
         // inputs of the next step, in flight while the level-B work runs
         if (z + 1 <= zend)
-            sweep_load(L, B, q4, alpha, cs, ct, n, st0, D.d2, zoff + st0, ob, p_on(z + 1),
-                       u_on(z + 1), z + 1 < D.d0 - 1, f);
+            sweep_load(L, R, nb, pb, d2b, zb + pb, ob, p_on(z + 1), u_on(z + 1), z + 1 < D.d0 - 1, f);
         // 3. level-B p-step at plane z-1 on E1
         float hB1_0 = 0.f, hB1_1 = 0.f, hB1_2 = 0.f;
         if (ph3) {
@@ -437,19 +453,19 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         __syncthreads();
         // 4. level-B u-step at plane z-1 on O; store iteration t + 2
         if (ph4) {
-            const int64_t zb = zoff - st0;
+            const unsigned zp = zb - pb;  // plane z-1
             const float x0v = (e0b ? hB1_0 : 0.f) - (f0b ? hB2_0 : 0.f);
             const float x1v = ((f & 16u) ? hB1_1 : 0.f) - ((f & 64u) ? sB1[p - EX] : 0.f);
             const float x2v = ((f & 32u) ? hB1_2 : 0.f) - ((f & 128u) ? sB2[p - 1] : 0.f);
             const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
             float pti = ptAp, ui = uAp, ti;
             const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, UP.inv_rcc, 1.f, 0.f);
-            st(B.hpo + zb, ob, hB1_0);
-            st(B.hpo + n + zb, ob, hB1_1);
-            st(B.hpo + 2 * n + zb, ob, hB1_2);
-            st(B.pto + zb, ob, pti);
-            st(B.uo + zb, ob, ui);
-            st(B.tmpo + zb, ob, ti);
+            bst(R.hpo, ob, zp, hB1_0);
+            bst(R.hpo, ob, zp + nb, hB1_1);
+            bst(R.hpo, ob, zp + 2u * nb, hB1_2);
+            bst(R.pto, ob, zp, pti);
+            bst(R.uo, ob, zp, ui);
+            bst(R.tmpo, ob, zp, ti);
             if (check) err_acc += fabsf(err);
         }
         // level-A tmp of plane z for the next step's level-B p-step (every read
@@ -621,7 +637,7 @@ struct Runner {
           UP{p->rcc, float(1.0 / double(p->rcc)), p->rs[0], p->rs[1], p->rs[2], p->error_bound,
              float(1.0 / (double(n_) * double(p->rcc)))},
           sup(l != nullptr),
-          sweep_elig(p->const_evals && l == nullptr && int64_t(d.d1) * d.d2 < (int64_t(1) << 27)),
+          sweep_elig(p->const_evals && l == nullptr && n_ < kSweepMaxVoxels),
           limit(p->num_iter)
     {
     }
@@ -1348,8 +1364,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     // if it has not), else groups of 4 through the per-iteration kernels,
     // which share the frame and capacities in registers.
     const std::array<int, 3> key{D.d0, D.d1, D.d2};
-    const bool sweep_elig = V >= prm->num_iter && int64_t(D.d1) * D.d2 < (int64_t(1) << 27) &&
-                            ctx->batch_sweep != 1;
+    const bool sweep_elig = V >= prm->num_iter && n < kSweepMaxVoxels && ctx->batch_sweep != 1;
     bool want_sweep = sweep_elig && ctx->batch_sweep == 2;
     SweepMethod sm{0, 16};
     if (ctx->sweep_mode == 2) {
