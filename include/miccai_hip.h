@@ -279,7 +279,8 @@ int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
  * the plain kernels and a few sweep tile shapes on its own iterations and the
  * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
  * sweeps when the solve is eligible, with tile (1-based index of the built-in
- * tile shapes, 0 the first) and zchunk (planes per work-group, 0: 16).
+ * tile shapes, 0 the first) and zchunk (planes per work-group, 0: the plane
+ * chunk a cost model of the launch favours for that tile shape).
  * Results do not depend on any of them, bit for bit. */
 int mh_cmf_set_sweep(mh_ctx *ctx, int mode, int tile, int zchunk);
 /* Graph replay in mh_cmf_solve and the resident solves (no counterpart in the

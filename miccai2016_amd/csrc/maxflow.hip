@@ -220,8 +220,6 @@ constexpr int kNumSweepShapes = 5;
 struct SweepMethod {
     int shape, zchunk;
 };
-// candidates the tuner times on a solve of new dimensions
-constexpr SweepMethod kSweepCandidates[] = {{-1, 0}, {0, 16}, {0, 32}, {1, 8}, {1, 16}, {2, 16}, {4, 16}};
 
 struct SweepBufs {
     const float *hp, *tmp, *pt, *u;  // iteration t
@@ -300,7 +298,53 @@ __device__ __forceinline__ void sweep_load(SweepIn &L, const SweepRsrc &R, unsig
     }
 }
 
-template <int TY, int TX, int NT, bool BATCH>
+// Thread t -> point (yy, xx) of the widened tile (TY + 3) x (TX + 3).
+// MAP 0: row-major. MAP 1: whole rows (a wave's loads stay contiguous), the
+// rows of the output region O (yy = 2 .. TY+1) first, then the ring rows that
+// only the earlier levels need: yy = 1 (level-B p-step, level-A u-step),
+// yy = TY + 2 (level-A u-step), yy = 0 (level-A p-step only). MAP 2: O
+// row-major first, then each ring (rows and columns) by the last phase it
+// takes part in: E1 \ O, E2 \ E1, E \ E2. Waves made only of ring points
+// skip the later phases; MAP 2 skips more but loads the ring columns one
+// point per lane.
+template <int TY, int TX, int MAP>
+__device__ __forceinline__ void sweep_point(int t, int &yy, int &xx)
+{
+    constexpr int EX = TX + 3;
+    if (MAP == 0) {
+        yy = t / EX;
+        xx = t % EX;
+        return;
+    }
+    if (MAP == 1) {
+        const int r = t / EX;
+        xx = t % EX;
+        yy = r < TY ? r + 2 : (r == TY ? 1 : (r == TY + 1 ? TY + 2 : 0));
+        return;
+    }
+    yy = xx = 0;
+    if (t < TY * TX) {
+        yy = 2 + t / TX;
+        xx = 2 + t % TX;
+        return;
+    }
+    t -= TY * TX;
+    if (t < TX + 1 + TY) {  // E1 \ O: row 1, then column 1
+        if (t < TX + 1) { yy = 1; xx = 1 + t; } else { yy = 2 + (t - (TX + 1)); xx = 1; }
+        return;
+    }
+    t -= TX + 1 + TY;
+    if (t < TX + 2 + TY + 1) {  // E2 \ E1: row TY + 2, then column TX + 2
+        if (t < TX + 2) { yy = TY + 2; xx = 1 + t; } else { yy = 1 + (t - (TX + 2)); xx = TX + 2; }
+        return;
+    }
+    t -= TX + 2 + TY + 1;
+    if (t < TX + 3 + TY + 2) {  // E \ E2: row 0, then column 0
+        if (t < TX + 3) { yy = 0; xx = t; } else { yy = 1 + (t - (TX + 3)); xx = 0; }
+    }
+}
+
+template <int TY, int TX, int NT, int MAP, bool BATCH>
 __global__ void __launch_bounds__(NT)
 cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__restrict__ alpha,
                  const float *__restrict__ cs, const float *__restrict__ ct, const Dims D,
@@ -350,11 +394,14 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                       rsrc_of(B.tmpo, 4 * n), rsrc_of(B.pto, 4 * n), rsrc_of(B.uo, 4 * n)};
     const unsigned nb = unsigned(4 * n), pb = unsigned(4 * st0), d2b = 4u * unsigned(D.d2);
 
-    // this thread's point of the widened tile
-    const int p = int(threadIdx.x);
-    const int yy = p / EX, xx = p % EX;
+    // this thread's point of the widened tile; LDS is indexed by the point
+    // p = yy EX + xx
+    int yy, xx;
+    sweep_point<TY, TX, MAP>(int(threadIdx.x), yy, xx);
+    const bool live = int(threadIdx.x) < NE;
+    const int p = yy * EX + xx;
     const int y = y0 - 2 + yy, x = x0 - 2 + xx;
-    const bool inv = p < NE && y >= 0 && y < D.d1 && x >= 0 && x < D.d2;
+    const bool inv = live && y >= 0 && y < D.d1 && x >= 0 && x < D.d2;
     const bool inE2 = inv && yy >= 1 && xx >= 1;
     const bool inE1 = inv && yy >= 1 && yy < TY + 2 && xx >= 1 && xx < TX + 2;
     const bool inO = inv && yy >= 2 && yy < TY + 2 && xx >= 2 && xx < TX + 2;
@@ -407,7 +454,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                 bst(R.hpo, ob, zb + 2u * nb, hA0_2);
             }
         }
-        if (p < NE) {
+        if (live) {
             sA1[p] = hA0_1;
             sA2[p] = hA0_2;
         }
@@ -446,7 +493,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             mh::cmf_p_voxel_m(hB1_0, hB1_1, hB1_2, g0, g1, g2, RP, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
                               PP.rq1, PP.rq2, aP);
         }
-        if (p < NE) {
+        if (live) {
             sB1[p] = hB1_1;
             sB2[p] = hB1_2;
         }
@@ -470,7 +517,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         }
         // level-A tmp of plane z for the next step's level-B p-step (every read
         // of plane z-1's values finished before the barrier above)
-        if (p < NE) sT[p] = tA0;
+        if (live) sT[p] = tA0;
         hB2_0 = hB1_0;
         hA1_0 = hA0_0;
         hA1_1 = hA0_1;
@@ -496,19 +543,50 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     }
 }
 
-template <int TY, int TX, int NT>
+template <int TY, int TX, int NT, int MAP>
 void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const float4 *q4,
                     const float *alpha, const float *cs, const float *ct, const Dims &D,
                     const PParams &PP, const UParams &UP, int zchunk, int lv, int chk, int out_set,
                     int *ctrl, float *partials, const SweepBatch *batch)
 {
     if (batch)
-        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, true>), dim3(grid), dim3(NT), 0, s, SBf, q4,
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, true>), dim3(grid), dim3(NT), 0, s, SBf, q4,
                            alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, *batch);
     else
-        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, false>), dim3(grid), dim3(NT), 0, s, SBf, q4,
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, false>), dim3(grid), dim3(NT), 0, s, SBf, q4,
                            alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials,
                            SweepBatch{});
+}
+
+// thread-to-point mapping of each tile shape (sweep_point)
+constexpr int kMap0 = 2, kMap1 = 1, kMap2 = 2, kMap3 = 2, kMap4 = 0;
+
+template <int TY, int TX, int NT, int MAP>
+int sweep_occupancy_t(int &blocks)
+{
+    return int(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, reinterpret_cast<const void *>(&cmf_sweep_kernel<TY, TX, NT, MAP, false>), NT, 0));
+}
+
+// Work-groups of one sweep launch with this tile shape that the device holds
+// at once (CU count x resident work-groups per CU), cached in the context.
+int sweep_slots(mh_ctx *ctx, int shape_ix)
+{
+    int &slots = ctx->sweep_slots[size_t(shape_ix)];
+    if (slots > 0) return slots;
+    int cus = 0, per = 0, rc = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+        cus = 0;
+    switch (shape_ix) {
+    case 1: rc = sweep_occupancy_t<11, 32, 512, kMap1>(per); break;
+    case 2: rc = sweep_occupancy_t<4, 64, 512, kMap2>(per); break;
+    case 3: rc = sweep_occupancy_t<26, 32, 1024, kMap3>(per); break;
+    case 4: rc = sweep_occupancy_t<13, 61, 1024, kMap4>(per); break;
+    default: rc = sweep_occupancy_t<12, 64, 1024, kMap0>(per); break;
+    }
+    (void)hipGetLastError();
+    slots = (rc == 0 && cus > 0 && per > 0) ? cus * per : 256;
+    return slots;
 }
 
 // grid: work-groups per solve times the number of solves (batch->ns)
@@ -518,11 +596,11 @@ void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &S
                   int out_set, int *ctrl, float *partials, const SweepBatch *batch = nullptr)
 {
     switch (shape_ix) {
-    case 1: launch_sweep_t<11, 32, 512>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 2: launch_sweep_t<4, 64, 512>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 3: launch_sweep_t<26, 32, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 4: launch_sweep_t<13, 61, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    default: launch_sweep_t<12, 64, 1024>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 1: launch_sweep_t<11, 32, 512, kMap1>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 2: launch_sweep_t<4, 64, 512, kMap2>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 3: launch_sweep_t<26, 32, 1024, kMap3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 4: launch_sweep_t<13, 61, 1024, kMap4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    default: launch_sweep_t<12, 64, 1024, kMap0>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
     }
 }
 
@@ -663,6 +741,29 @@ struct Runner {
         return m.shape < 0 || (sweep_elig && sweep_grid_of(m) <= int64_t(nb) + 64);
     }
 
+    // The plane chunk of a tile shape that a launch's cost model favours:
+    // work-groups run in ceil(groups / slots) waves, each group sweeping its
+    // chunk plus the two planes below it that the levels recompute, so the
+    // launch takes about waves x (chunk + 2) plane steps.
+    int model_zchunk(int shape_ix) const
+    {
+        const SweepShape &S = kSweepShapes[shape_ix];
+        const int64_t tiles = int64_t((D.d2 + S.tx - 1) / S.tx) * ((D.d1 + S.ty - 1) / S.ty);
+        const int64_t slots = sweep_slots(ctx, shape_ix);
+        int best = std::min(16, D.d0);
+        int64_t best_cost = -1;
+        for (int z = std::min(4, D.d0); z <= std::min(64, D.d0); ++z) {
+            const int64_t groups = tiles * ((D.d0 + z - 1) / z);
+            if (groups > int64_t(nb) + 64) continue;
+            const int64_t cost = ((groups + slots - 1) / slots) * (z + 2);
+            if (best_cost < 0 || cost < best_cost || (cost == best_cost && z > best)) {
+                best_cost = cost;
+                best = z;
+            }
+        }
+        return best;
+    }
+
     // one iteration (plain kernels) or one pass of up to two (sweep) that
     // does not go past `limit`; returns the iterations advanced
     int step(const SweepMethod &m)
@@ -726,8 +827,8 @@ struct Runner {
         const std::array<int, 3> key{D.d0, D.d1, D.d2};
         if (ctx->sweep_mode == 1 || !sweep_elig) return MH_OK;
         if (ctx->sweep_mode == 2) {
-            SweepMethod m{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
-                          ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+            const int k = ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0;
+            SweepMethod m{k, ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : model_zchunk(k)};
             if (usable(m)) use = m;
             return MH_OK;
         }
@@ -737,17 +838,23 @@ struct Runner {
             return MH_OK;
         }
         if (!allow_tune || prm->num_iter < 96) {
-            const SweepMethod m{0, 16};
-            if (usable(m) && sweep_grid_of(m) >= 512) use = m;
+            // the sweep wins from about 128^3 on
+            const SweepMethod m{0, model_zchunk(0)};
+            if (usable(m) && n >= (int64_t(1) << 21)) use = m;
             return MH_OK;
         }
+        // candidates: the per-iteration kernels and every tile shape with its
+        // modelled plane chunk
+        SweepMethod cands[1 + kNumSweepShapes];
+        cands[0] = SweepMethod{-1, 0};
+        for (int k = 0; k < kNumSweepShapes; ++k) cands[1 + k] = SweepMethod{k, model_zchunk(k)};
         // time each candidate on this solve's own iterations: one warm-up
         // pass, then >= 8 iterations
         hipEvent_t e0, e1;
         MH_CHECK(ctx, hipEventCreate(&e0));
         MH_CHECK(ctx, hipEventCreate(&e1));
         float best = 0.f;
-        for (const SweepMethod &m : kSweepCandidates) {
+        for (const SweepMethod &m : cands) {
             if (!usable(m) || it + 12 > prm->num_iter) continue;
             step(m);
             MH_CHECK(ctx, hipEventRecord(e0, s));

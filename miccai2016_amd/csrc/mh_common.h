@@ -40,13 +40,16 @@ struct mh_ctx {
     // two-level sweep of mh_cmf_solve: 0 automatic, 1 off, 2 whenever eligible
     int sweep_mode = 0;
     int sweep_shape = 0;   // tile shape (1-based; 0: default)
-    int sweep_zchunk = 0;  // planes per work-group (0: 16)
+    int sweep_zchunk = 0;  // planes per work-group (0: modelled)
     // graph replay of the per-iteration kernels' criterion cycles: 1 on, 0 off
     int graph_mode = 0;
     hipStream_t graph_stream = nullptr;  // capture stream, created on first use
     // method the tuner chose per volume dims: (tile shape or -1 for the plain
     // kernels, plane chunk)
     std::map<std::array<int, 3>, std::pair<int, int>> sweep_tuned;
+    // resident work-groups of the sweep kernel on the device, per tile shape
+    // (CU count x occupancy; 0 until queried)
+    std::array<int, 8> sweep_slots{};
 };
 
 namespace mh {
