@@ -46,8 +46,8 @@ TRAFFIC_GRID_PER_VOXEL_ITER = 67.6     # batched p + u kernels, groups of 4 solv
 TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 # two-level sweep by tile shape (1-based, as mh_cmf_sweep_choice reports it), at 128^3
 # (the grid) and at 256^3
-TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 41.7, 2: 43.8, 3: 42.1, 5: 43.3}
-TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.5, 2: 49.0, 3: 46.1, 5: 48.6}
+TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 44.1, 2: 49.3, 3: 44.5, 4: 45.4, 5: 41.9}
+TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.3, 2: 58.4, 3: 51.4, 4: 45.9, 5: 47.2}
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
 PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu", "sigmas")
@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--iters-256", type=int, default=1000)
     ap.add_argument("--sweep", type=int, default=0,
                     help="two-level sweep of single solves: 0 tuned, 1 off, 2 on")
+    ap.add_argument("--sweep-tile", type=int, default=0,
+                    help="with --sweep 2: the tile shape (1-5; 0: the first)")
     ap.add_argument("--batched-sweep", type=int, default=0,
                     help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
     ap.add_argument("--cpu-iters", type=int, default=150)
@@ -166,7 +168,7 @@ def main():
         CmfCutQAnisotropicGpu, initial_u)
 
     dev = default_context(0)
-    dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, args.sweep, 0, 0), "mh_cmf_set_sweep")
+    dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, args.sweep, args.sweep_tile, 0), "mh_cmf_set_sweep")
     dev.check(dev.lib.mh_cmf_batched_set_sweep(dev.handle, args.batched_sweep),
               "mh_cmf_batched_set_sweep")
 

@@ -570,6 +570,32 @@ int sweep_occupancy_t(int &blocks)
 
 // Work-groups of one sweep launch with this tile shape that the device holds
 // at once (CU count x resident work-groups per CU), cached in the context.
+int sweep_slots(mh_ctx *ctx, int shape_ix);
+
+// The plane chunk of a tile shape that a launch's cost model favours:
+// work-groups run in ceil(groups / slots) waves, each group sweeping its chunk
+// plus the two planes below it that the levels recompute, so the launch takes
+// about waves x (chunk + 2) plane steps. Chunks giving more than max_groups
+// work-groups are skipped.
+int sweep_model_zchunk(mh_ctx *ctx, int shape_ix, const Dims &D, int64_t max_groups)
+{
+    const SweepShape &S = kSweepShapes[shape_ix];
+    const int64_t tiles = int64_t((D.d2 + S.tx - 1) / S.tx) * ((D.d1 + S.ty - 1) / S.ty);
+    const int64_t slots = sweep_slots(ctx, shape_ix);
+    int best = std::min(16, D.d0);
+    int64_t best_cost = -1;
+    for (int z = std::min(4, D.d0); z <= std::min(64, D.d0); ++z) {
+        const int64_t groups = tiles * ((D.d0 + z - 1) / z);
+        if (groups > max_groups) continue;
+        const int64_t cost = ((groups + slots - 1) / slots) * (z + 2);
+        if (best_cost < 0 || cost < best_cost || (cost == best_cost && z > best)) {
+            best_cost = cost;
+            best = z;
+        }
+    }
+    return best;
+}
+
 int sweep_slots(mh_ctx *ctx, int shape_ix)
 {
     int &slots = ctx->sweep_slots[size_t(shape_ix)];
@@ -741,28 +767,7 @@ struct Runner {
         return m.shape < 0 || (sweep_elig && sweep_grid_of(m) <= int64_t(nb) + 64);
     }
 
-    // The plane chunk of a tile shape that a launch's cost model favours:
-    // work-groups run in ceil(groups / slots) waves, each group sweeping its
-    // chunk plus the two planes below it that the levels recompute, so the
-    // launch takes about waves x (chunk + 2) plane steps.
-    int model_zchunk(int shape_ix) const
-    {
-        const SweepShape &S = kSweepShapes[shape_ix];
-        const int64_t tiles = int64_t((D.d2 + S.tx - 1) / S.tx) * ((D.d1 + S.ty - 1) / S.ty);
-        const int64_t slots = sweep_slots(ctx, shape_ix);
-        int best = std::min(16, D.d0);
-        int64_t best_cost = -1;
-        for (int z = std::min(4, D.d0); z <= std::min(64, D.d0); ++z) {
-            const int64_t groups = tiles * ((D.d0 + z - 1) / z);
-            if (groups > int64_t(nb) + 64) continue;
-            const int64_t cost = ((groups + slots - 1) / slots) * (z + 2);
-            if (best_cost < 0 || cost < best_cost || (cost == best_cost && z > best)) {
-                best_cost = cost;
-                best = z;
-            }
-        }
-        return best;
-    }
+    int model_zchunk(int shape_ix) const { return sweep_model_zchunk(ctx, shape_ix, D, int64_t(nb) + 64); }
 
     // one iteration (plain kernels) or one pass of up to two (sweep) that
     // does not go past `limit`; returns the iterations advanced
@@ -1475,8 +1480,15 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     bool want_sweep = sweep_elig && ctx->batch_sweep == 2;
     SweepMethod sm{0, 16};
     if (ctx->sweep_mode == 2) {
-        sm = SweepMethod{ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0,
-                         ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk : 16};
+        // the forced single-solve method, applied to every solve in turn
+        // unless the batched sweep or the group kernels are asked for
+        const int k = ctx->sweep_shape > 0 ? std::min(ctx->sweep_shape, kNumSweepShapes) - 1 : 0;
+        sm = SweepMethod{k, ctx->sweep_zchunk > 0 ? ctx->sweep_zchunk
+                                                   : sweep_model_zchunk(ctx, k, D, int64_t(nb))};
+        if (sweep_elig && ctx->batch_sweep == 0) {
+            want_sweep = true;
+            if (G <= 0) G = 1;
+        }
     } else {
         if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0 && prm->num_iter >= 96 &&
             ctx->sweep_tuned.find(key) == ctx->sweep_tuned.end()) {
