@@ -457,6 +457,32 @@ def test_batched_solves_equal_plain_solves(dev, rng):
         assert np.array_equal(ub[s], up)
 
 
+@pytest.mark.parametrize("tile,zchunk", [(1, 0), (2, 5), (5, 0)])
+def test_forced_sweep_applies_to_batched_solves(dev, rng, tile, zchunk):
+    """With the single-solve sweep forced (tile shape, modelled or given plane
+    chunk), the batched solver runs every solve through it; each equals the
+    per-iteration kernels' solve bit for bit."""
+    img, q, _, cs, ct, gms = _cut_case(rng, (21, 30, 70))
+    w = np.array([0.05, 0.3, 2.0], np.float32)
+    d = np.array([3.0, 30.0, 300.0], np.float32)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    g32 = gms.astype(np.float32)
+    try:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, tile, zchunk))
+        c = _cutter(dev, 61, update=7)
+        ub = c.execute_batched(img, q, eigs, w, d, g32, cs, ct)
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 1, 0, 0))
+        g_t, a_t = dev.upload(g32), dev.empty(g32.size)
+        for s in range(3):
+            dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(g_t), g32.size, float(w[s]),
+                                                    float(d[s]), dev.ptr(a_t), dev.stream))
+            a_dev = dev.download(a_t).reshape(img.shape)
+            up = _cutter(dev, 61, update=7).execute(img, q, eigs, a_dev, cs, ct, verbose=False)
+            assert np.array_equal(ub[s], up), s
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+
+
 def test_batched_schedules_give_identical_results(dev, rng):
     """Every (group, visit) schedule, with and without the two-level sweep of
     the groups, computes the same iterations, bit for bit."""
