@@ -148,6 +148,22 @@ int mh_complete_frames(mh_ctx *ctx, float *out3_dev, const float *dirs3_dev, int
 int mh_hysteresis_host(const uint8_t *cand_host, const uint8_t *sure_host, uint8_t *out_host,
                        const int64_t dims[3]);
 
+/* The ridge's hysteresis and its 26-connected segments on the device:
+ * union-find over the candidate voxels (cand != 0) and their 26 neighbours.
+ * With use_upper, a segment is kept when one of its voxels has resp >= upper
+ * (the flood fill from the seeds of hysteresis_threshold,
+ * processing/ridge_detection.py:80-131, with seeds = candidate and response
+ * >= the upper threshold, adjust_directions.py:258-288); without, every
+ * segment is kept. ridge (may alias cand) receives the kept voxels as 1.f,
+ * labels (int32, N) the segment of every kept voxel as its smallest voxel
+ * index (so segments in increasing label order are scipy.ndimage.label's
+ * numbering, as the cleaning at adjust_directions.py:290-339 uses) and -1
+ * elsewhere. work: mh_ridge_segments_work_bytes(dims) bytes. */
+int64_t mh_ridge_segments_work_bytes(const int64_t dims[3]);
+int mh_ridge_segments(mh_ctx *ctx, const float *cand_dev, const float *resp_dev, float upper,
+                      int use_upper, const int64_t dims[3], float *ridge_dev, int *labels_dev,
+                      void *work_dev, int64_t work_bytes, void *stream);
+
 /* Host: a minimum spanning tree of the complete Euclidean graph over n points
  * (pts_host: n x 3 doubles, e.g. voxel-size scaled ridge coordinates), as
  * n - 1 index pairs in edges_host (2(n - 1) int64). r0 is the longest

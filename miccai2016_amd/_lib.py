@@ -70,6 +70,8 @@ SIGNATURES = {
     "mh_nonridge_signs": (i32, [vp, vp, I64x3, vp, vp, i64, F32x3, vp, i64, vp]),
     "mh_complete_frames": (i32, [vp, vp, vp, i64, vp]),
     "mh_hysteresis_host": (i32, [vp, vp, vp, I64x3]),
+    "mh_ridge_segments_work_bytes": (i64, [I64x3]),
+    "mh_ridge_segments": (i32, [vp, vp, vp, f32, i32, I64x3, vp, vp, vp, i64, vp]),
     "mh_spanning_tree_host": (i32, [vp, i64, C.c_double, vp]),
     "mh_ridge_tree_host": (i32, [vp, i64, I64x3, vp, vp]),
     "mh_ridge_forest_host": (i64, [vp, i64, I64x3, vp, vp]),

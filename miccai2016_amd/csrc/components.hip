@@ -175,6 +175,67 @@ constexpr unsigned kParts = 1024;
 
 int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
+// ----------------------------------------------- ridge segments (26-conn.) --
+// The ridge's hysteresis and its segments for the cleaning step, as union-find
+// over the 26-neighbourhood: every candidate voxel joins its 13 backward
+// neighbours. A component is kept when it holds a seed (a candidate whose
+// response reaches the upper threshold), which is the flood fill from the
+// seeds through the candidates of ridge_detection.py:80-131.
+
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_init_kernel(const float *__restrict__ cand, int64_t n, int *__restrict__ lab,
+                  int *__restrict__ seeded)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lab[i] = cand[i] != 0.f ? int(i) : kBg;
+    seeded[i] = 0;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_merge26_kernel(int *__restrict__ lab, int d0, int d1, int d2)
+{
+    const int64_t n = int64_t(d0) * d1 * d2;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || lab[i] == kBg) return;
+    const int i2 = int(i % d2), i1 = int((i / d2) % d1), i0 = int(i / (int64_t(d2) * d1));
+    // backward neighbours: (-1, *, *), (0, -1, *), (0, 0, -1)
+    for (int a = -1; a <= 0; ++a)
+        for (int b = -1; b <= 1; ++b)
+            for (int c = -1; c <= 1; ++c) {
+                if (a == 0 && (b > 0 || (b == 0 && c >= 0))) continue;
+                const int j0 = i0 + a, j1 = i1 + b, j2 = i2 + c;
+                if (j0 < 0 || j1 < 0 || j1 >= d1 || j2 < 0 || j2 >= d2) continue;
+                const int64_t j = (int64_t(j0) * d1 + j1) * d2 + j2;
+                if (lab[j] != kBg) unite(lab, int(i), int(j));
+            }
+}
+
+// final labels (smallest voxel index of the component); a component holding
+// a candidate whose response is >= upper is marked at its root
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_compress_kernel(int *__restrict__ lab, const float *__restrict__ resp, float upper,
+                      int use_upper, int64_t n, int *__restrict__ seeded)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || lab[i] == kBg) return;
+    const int r = find_root(lab, int(i));
+    lab[i] = r;  // parents stay ancestors, so concurrent finds remain valid
+    if (!use_upper || resp[i] >= upper) seeded[r] = 1;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+ridge_keep_kernel(int *__restrict__ lab, const int *__restrict__ seeded, int64_t n,
+                  float *__restrict__ ridge)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = lab[i];
+    const bool keep = r != kBg && seeded[r] != 0;
+    ridge[i] = keep ? 1.f : 0.f;
+    lab[i] = keep ? r : kBg;
+}
+
 }  // namespace
 
 MH_EXPORT int64_t mh_largest_component_work_bytes(const int64_t dims[3])
@@ -212,5 +273,32 @@ MH_EXPORT int mh_largest_component_dice(mh_ctx *ctx, const float *u, float thres
                        cpart);
     hipLaunchKernelGGL(cc_dice_final_kernel, dim3(1), dim3(64), 0, s, cpart, int(np), out_dev);
     MH_CHECK_LAUNCH(ctx, "largest component");
+    return MH_OK;
+}
+
+MH_EXPORT int64_t mh_ridge_segments_work_bytes(const int64_t dims[3])
+{
+    if (!mh::dims_ok(dims)) return -1;
+    return align256(sizeof(int) * dims[0] * dims[1] * dims[2]);
+}
+
+MH_EXPORT int mh_ridge_segments(mh_ctx *ctx, const float *cand, const float *resp, float upper,
+                                int use_upper, const int64_t dims[3], float *ridge, int *labels,
+                                void *work, int64_t work_bytes, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, cand && ridge && labels && work && (resp || !use_upper), "null pointer");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    MH_REQUIRE(ctx, work_bytes >= mh_ridge_segments_work_bytes(dims), "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    int *seeded = static_cast<int *>(work);
+    const unsigned nb = mh::blocks_for(n);
+    hipLaunchKernelGGL(ridge_init_kernel, dim3(nb), dim3(mh::kBlock), 0, s, cand, n, labels, seeded);
+    hipLaunchKernelGGL(ridge_merge26_kernel, dim3(nb), dim3(mh::kBlock), 0, s, labels, int(dims[0]),
+                       int(dims[1]), int(dims[2]));
+    hipLaunchKernelGGL(ridge_compress_kernel, dim3(nb), dim3(mh::kBlock), 0, s, labels, resp, upper,
+                       use_upper, n, seeded);
+    hipLaunchKernelGGL(ridge_keep_kernel, dim3(nb), dim3(mh::kBlock), 0, s, labels, seeded, n, ridge);
+    MH_CHECK_LAUNCH(ctx, "ridge segment kernels");
     return MH_OK;
 }

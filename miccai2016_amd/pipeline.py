@@ -138,7 +138,7 @@ def prepare_device(img_t, shape, noise_level, cfg=None, loader=None, verbose=Fal
     dev.check(dev.lib.mh_grad_mag_sq(dev.handle, dev.ptr(img_t), _lib.dims3(shape), dev.ptr(gms),
                                      dev.stream), "mh_grad_mag_sq")
     lap("fields")
-    return PreparedDevice(quats, cs, ct, gms, int(np.count_nonzero(adj.ridge)))
+    return PreparedDevice(quats, cs, ct, gms, adj.ridge_voxels)
 
 
 def cut_parameters(cfg):

@@ -29,6 +29,7 @@ class AdvancedAdjuster(TreePrintable):
         self.__response = None
         self.__voxel_size = None
         self.__ridge = None
+        self.__ridge_coords = None
         self.__main_dirs_unadjusted = None
         self._ridge_detector = None
         self._hysteresis = None
@@ -89,6 +90,12 @@ class AdvancedAdjuster(TreePrintable):
     @property
     def ridge(self):
         """The detected (and, if requested, cleaned) ridge after execute()."""
+        if self.__ridge is None and self.__ridge_coords is not None:
+            coords, shape = self.__ridge_coords
+            mask = np.zeros(shape, np.float32)
+            if len(coords):
+                mask[tuple(np.asarray(coords).T)] = 1
+            self.__ridge = mask
         return self.__ridge
 
     @property
@@ -121,9 +128,11 @@ class AdvancedAdjuster(TreePrintable):
     def execute_tensor(self, qeigs_t, resp_t, shape, verbose=False):
         """Device form of execute(): qeigs_t (squeezed frames, 3N) and resp_t
         (N) device tensors; returns the adjusted directions (AoS x3 device
-        tensor). Only the ridge mask, the response and the ridge voxels'
-        directions visit the host (hysteresis, spanning tree, cleaning)."""
+        tensor). The ridge's hysteresis and its 26-connected segments run on
+        the device (mh_ridge_segments); only the ridge voxels' indices,
+        segments and directions visit the host (spanning tree, cleaning)."""
         dev = self._loader
+        torch = dev.torch
         self._verbose = verbose
         shape = tuple(shape)
         n = int(np.prod(shape))
@@ -131,38 +140,65 @@ class AdvancedAdjuster(TreePrintable):
         self._d_resp = resp_t.reshape(-1)
         d_ridge = dev.empty(n)
         self._ridge_detector.execute(d_ridge, self._d_resp, self._d_qeigs, shape)
-        ridge = dev.download(d_ridge).reshape(shape)
-        if self._hysteresis:
-            resp = dev.download(self._d_resp).reshape(shape)
-            sure = np.logical_and(ridge, resp >= self.threshold[1])
-            ridge[:] = hysteresis_threshold(ridge, sure.astype(np.float32))
-        self.__ridge = ridge
-        rc = dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(self._d_qeigs), n,
-                                  dev.stream)
-        dev.check(rc, "mh_main_dirs")
-        md = self._d_qeigs.reshape(n, 3)
-
-        def ridge_dirs(mask):
-            """Coordinates of the mask's voxels and their main directions
-            (gathered on the device)."""
-            coords = np.argwhere(mask != 0)
-            if len(coords) == 0:
-                return coords, np.zeros((0, 3), np.float32)
-            lin = np.ravel_multi_index(tuple(coords.T), shape).astype(np.int64)
-            idx = dev.torch.from_numpy(lin).to(dev.device)
-            return coords, dev.download(md.index_select(0, idx))
-
-        coords, dirs = ridge_dirs(ridge)
+        labels = dev.empty(n, dtype=torch.int32)
+        wbytes = int(dev.lib.mh_ridge_segments_work_bytes(_lib.dims3(shape)))
+        work = dev.bytes(wbytes)
+        upper = float(self.threshold[1]) if self._hysteresis else 0.0
+        dev.check(dev.lib.mh_ridge_segments(dev.handle, dev.ptr(d_ridge), dev.ptr(self._d_resp),
+                                            upper, int(self._hysteresis), _lib.dims3(shape),
+                                            dev.ptr(d_ridge), dev.ptr(labels), dev.ptr(work),
+                                            wbytes, dev.stream), "mh_ridge_segments")
+        dev.check(dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(self._d_qeigs),
+                                       n, dev.stream), "mh_main_dirs")
+        # ridge voxels in C order, their segments and main directions
+        lin_t = torch.nonzero(d_ridge).reshape(-1)
+        lin = dev.download(lin_t)
+        seg = dev.download(labels.index_select(0, lin_t))
+        dirs = dev.download(self._d_qeigs.reshape(n, 3).index_select(0, lin_t))
+        del labels, work, d_ridge
+        coords = np.stack(np.unravel_index(lin, shape), 1) if len(lin) else np.zeros((0, 3), np.int64)
+        self._set_ridge(coords, shape)
         if self.clean_ridge:
-            coords, adjusted = self._cleaning_signs(coords, dirs, shape)
-        else:
-            coords, adjusted = self._tree_signs(coords, dirs, shape)
-        if self.clean_ridge:
-            ridge = self._do_clean_ridge(ridge, coords, adjusted)
-            coords, dirs = ridge_dirs(ridge)
-            coords, adjusted = self._tree_signs(coords, dirs, shape)
+            _, adjusted = self._cleaning_signs(coords, dirs, shape)
+            keep = self._segments_to_keep(seg, adjusted)
+            coords, dirs = coords[keep], dirs[keep]
+            self._set_ridge(coords, shape)
+        coords, adjusted = self._tree_signs(coords, dirs, shape)
         self._flip_nonridge(coords, adjusted, shape)
         return self._d_qeigs
+
+    def _set_ridge(self, coords, shape):
+        """The ridge as voxel coordinates; the mask is built when asked for."""
+        self.__ridge = None
+        self.__ridge_coords = (coords, tuple(shape))
+
+    @property
+    def ridge_voxels(self):
+        """Number of voxels of the (cleaned) ridge after execute()."""
+        if self.__ridge is None and self.__ridge_coords is not None:
+            return len(self.__ridge_coords[0])
+        return int(np.count_nonzero(self.ridge)) if self.ridge is not None else 0
+
+    def _segments_to_keep(self, seg, ridge_dirs):
+        """_do_clean_ridge on segment labels: seg holds each ridge point's
+        segment (the segment's smallest voxel index, so increasing labels are
+        scipy's label numbering). Returns the points of the kept segments."""
+        roots, lab_of = np.unique(seg, return_inverse=True)
+        ncomp = len(roots)
+        if ncomp == 0:
+            return np.zeros(0, bool)
+        scaled = ridge_dirs * np.asarray(self.voxel_size)
+        sums = np.zeros((ncomp, 3))
+        for k in range(3):
+            sums[:, k] = np.bincount(lab_of, weights=scaled[:, k], minlength=ncomp)
+        lengths = np.linalg.norm(sums, axis=1)
+        order = np.argsort(-lengths, kind="stable")
+        keep = int(np.ceil(3 * np.log(np.e + ncomp)))
+        if getattr(self, "_verbose", False):
+            print("Keeping %s of %s labels." % (min(keep, ncomp), ncomp))
+        kept = np.zeros(ncomp, bool)
+        kept[order[:keep]] = True
+        return kept[lab_of]
 
     def _cleaning_signs(self, coords, dirs, shape):
         """Ridge directions good for the cleaning scores, which only need

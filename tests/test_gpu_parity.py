@@ -203,6 +203,60 @@ def test_feature_filter_roi_crop(dev):
 
 # -------------------------------------------------------------- directions --
 
+@pytest.mark.parametrize("shape,density,hyst", [((20, 24, 28), 0.08, True), ((33, 17, 40), 0.2, True),
+                                                ((16, 16, 16), 0.05, False), ((1, 40, 50), 0.3, True)])
+def test_ridge_segments_match_scipy(dev, rng, shape, density, hyst):
+    """GPU hysteresis and 26-connected segments against scipy's labelling:
+    the kept voxels are the candidate segments holding a seed, and every kept
+    voxel's label is the smallest voxel index of its scipy component."""
+    from miccai2016_amd import _lib
+    cand = (rng.random(shape) < density).astype(np.float32)
+    resp = rng.random(shape).astype(np.float32)
+    lab, ncomp = ndi.label(cand != 0, structure=np.ones((3, 3, 3), int))
+    if hyst:
+        want = fp64.hysteresis(cand != 0, (cand != 0) & (resp >= np.float32(0.9)))
+    else:
+        want = cand != 0
+    n = cand.size
+    c_t, r_t = dev.upload(cand).reshape(-1), dev.upload(resp).reshape(-1)
+    labels = dev.empty(n, dtype=dev.torch.int32)
+    wb = int(dev.lib.mh_ridge_segments_work_bytes(_lib.dims3(shape)))
+    work = dev.bytes(wb)
+    dev.check(dev.lib.mh_ridge_segments(dev.handle, dev.ptr(c_t), dev.ptr(r_t), 0.9, int(hyst),
+                                        _lib.dims3(shape), dev.ptr(c_t), dev.ptr(labels),
+                                        dev.ptr(work), wb, dev.stream))
+    got = dev.download(c_t).reshape(shape)
+    np.testing.assert_array_equal(got != 0, want)
+    L = dev.download(labels).reshape(-1)
+    lin = np.arange(n)
+    first = np.full(ncomp + 1, n)
+    np.minimum.at(first, lab.reshape(-1), lin)
+    expect = np.where(want.reshape(-1), first[lab.reshape(-1)], -1)
+    np.testing.assert_array_equal(L, expect)
+
+
+def test_adjuster_device_path_equals_host_path(dev):
+    """execute_tensor (GPU hysteresis and segments, forest signs for the
+    cleaning scores) against execute (host flood fill, scipy labelling, full
+    tree): the same ridge and the same adjusted directions, bit for bit."""
+    from miccai2016_amd import phantom as ph
+    from miccai2016_amd.helpers.misc import normalize
+    from miccai2016_amd.processing import ridge_graph
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    img, _ = ph.make_helix(ph.HelixSpec((40, 36, 44), 3, 9, 18, 0.8, seed=5))
+    wv, wq = ref32.vesselness(img, fp64.scales_voxels((1.0, 2.0, 3), (1, 1, 1)))
+    ves = normalize(wv).astype(np.float32)
+    host = AdvancedAdjuster((0.01, 0.3), wq, ves, (1, 1, 1), loader=dev)
+    out_h = host.execute(verbose=False)
+    devadj = AdvancedAdjuster((0.01, 0.3), None, None, (1, 1, 1), loader=dev)
+    out_d = devadj.execute_tensor(dev.upload(wq).reshape(-1), dev.upload(ves).reshape(-1),
+                                  img.shape)
+    assert host.ridge.sum() > ridge_graph.DENSE_LIMIT   # the host C++ tree, not the dense one
+    np.testing.assert_array_equal(host.ridge, devadj.ridge)
+    assert devadj.ridge_voxels == int(host.ridge.sum())
+    np.testing.assert_array_equal(out_h, dev.download(out_d).reshape(img.shape + (3,)))
+
+
 def test_ridges_and_main_dirs_match_oracle(dev, rng):
     from miccai2016_amd.processing.ridge_detection import RawRidgeDetector
     shape = (20, 24, 28)
