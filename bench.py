@@ -460,7 +460,8 @@ def main():
                             "as resident solves, smaller ones as packed jobs"})
             log(f"serving at {qps} QPS: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s,"
                 f" peak {srv.peak_resident} resident")
-            del srv
+            del srv, tickets
+            torch.cuda.empty_cache()   # the next server's budget starts from the free memory
 
     # ---- the experiment's grid at every noise level (config 3 in full)
     if run["sigmas"]:
