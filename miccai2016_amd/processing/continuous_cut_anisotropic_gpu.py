@@ -193,9 +193,11 @@ class CmfCutQAnisotropicGpu(TreePrintable):
         self.last_result = list(res)
         return u
 
-    def execute_ragged(self, imgs, quats, eigs, alphas, css, cts, scaling=None):
+    def execute_ragged(self, imgs, quats, eigs, alphas, css, cts, scaling=None,
+                       sweep_min_voxels=1 << 20):
         """One unsupervised solve per volume, volumes of different shapes packed
-        into one launch sequence. Lists in, list of u arrays out."""
+        into one launch sequence (volumes of sweep_min_voxels or more on their
+        own, see execute_ragged_device). Lists in, list of u arrays out."""
         dev = self.loader
         shapes = [np.shape(a) for a in imgs]
         sizes = [int(np.prod(s)) for s in shapes]
@@ -207,12 +209,39 @@ class CmfCutQAnisotropicGpu(TreePrintable):
         t_u = dev.upload(u0)
         self.execute_ragged_device(t_u, dev.upload(pack(quats)), dev.upload(pack(alph)),
                                    dev.upload(pack(css)), dev.upload(pack(cts)), shapes, offs,
-                                   eigs, scaling)
+                                   eigs, scaling, sweep_min_voxels)
         u = dev.download(t_u)
         return [u[o:o + sz].reshape(s) for o, sz, s in zip(offs, sizes, shapes)]
 
-    def execute_ragged_device(self, u, quats, alpha, cs, ct, shapes, offsets, eigs, scaling=None):
+    def execute_ragged_device(self, u, quats, alpha, cs, ct, shapes, offsets, eigs, scaling=None,
+                              sweep_min_voxels=1 << 20):
+        """Volumes of at least sweep_min_voxels voxels are solved one at a
+        time with the tuned single-solve method (the two-level sweep where it
+        wins), the others together in one packed launch sequence; both give
+        the same iterations bit for bit (None: all packed)."""
         dev = self.loader
+        shapes = [tuple(int(v) for v in s) for s in shapes]
+        offsets = [int(o) for o in offsets]
+        sizes = [int(np.prod(s)) for s in shapes]
+        big = [v for v in range(len(shapes))
+               if sweep_min_voxels is not None and sizes[v] >= sweep_min_voxels]
+        if big:
+            sq = np.sqrt(np.asarray(eigs, dtype=np.float64))
+            results = [None] * len(shapes)
+            for v in big:
+                o, m = offsets[v], sizes[v]
+                self.execute_device(u[o:o + m], quats[3 * o:3 * (o + m)], alpha[o:o + m],
+                                    cs[o:o + m], ct[o:o + m], scaling=scaling, const_sqevals=sq,
+                                    shape=shapes[v])
+                results[v] = self.last_result
+            small = [v for v in range(len(shapes)) if v not in set(big)]
+            if small:
+                self.execute_ragged_device(u, quats, alpha, cs, ct, [shapes[v] for v in small],
+                                           [offsets[v] for v in small], eigs, scaling, None)
+                for v, r in zip(small, self.last_result):
+                    results[v] = r
+            self.last_result = results
+            return u
         scaling = (1, 1, 1) if scaling is None else scaling
         sq = np.sqrt(np.asarray(eigs, dtype=np.float64))
         total = int(max(int(o) + int(np.prod(s)) for o, s in zip(offsets, shapes)))

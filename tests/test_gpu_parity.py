@@ -563,13 +563,18 @@ def test_batched_schedules_give_identical_results(dev, rng):
         assert np.array_equal(o, outs[0])
 
 
-def test_ragged_solves_equal_plain_solves(dev, rng):
-    shapes = [(9, 12, 15), (20, 7, 11), (16, 16, 16), (5, 33, 8)]
+@pytest.mark.parametrize("sweep_min", [None, 2000])
+def test_ragged_solves_equal_plain_solves(dev, rng, sweep_min):
+    """Packed ragged solves (and, with sweep_min, the larger volumes solved on
+    their own by the tuned method) equal plain solves bit for bit."""
+    shapes = [(9, 12, 15), (20, 7, 11), (16, 16, 16), (5, 33, 8), (14, 15, 13)]
     cases = [_cut_case(rng, s) for s in shapes]
     eigs = (1, 10 ** -1.5, 10 ** -1.5)
     c = _cutter(dev, 150)
     us = c.execute_ragged([k[0] for k in cases], [k[1] for k in cases], eigs,
-                          [k[2] for k in cases], [k[3] for k in cases], [k[4] for k in cases])
+                          [k[2] for k in cases], [k[3] for k in cases], [k[4] for k in cases],
+                          sweep_min_voxels=sweep_min)
+    assert [r.iterations for r in c.last_result] == [150] * len(shapes)
     for k, u in zip(cases, us):
         up = _cutter(dev, 150).execute(k[0], k[1], eigs, k[2], k[3], k[4], verbose=False)
         assert np.array_equal(u, up)
