@@ -824,8 +824,9 @@ struct Runner {
 
     // The method for this solve: forced by the context, tuned earlier for
     // these dims, tuned now on this solve's own iterations (allow_tune, at
-    // least 96 iterations), or the default (the sweep when eligible and the
-    // volume gives >= 512 work-groups, else the plain kernels).
+    // least 96 iterations), or the default (the sweep with the first tile
+    // shape when eligible and the volume has >= 800,000 voxels, else the
+    // plain kernels).
     int choose(bool allow_tune, SweepMethod &use)
     {
         use = SweepMethod{-1, 0};
@@ -843,9 +844,10 @@ struct Runner {
             return MH_OK;
         }
         if (!allow_tune || prm->num_iter < 96) {
-            // the sweep wins from about 128^3 on
+            // measured: the sweep wins from about 10^6 voxels (100^3) on, the
+            // per-iteration kernels at 80^3
             const SweepMethod m{0, model_zchunk(0)};
-            if (usable(m) && n >= (int64_t(1) << 21)) use = m;
+            if (usable(m) && n >= 800000) use = m;
             return MH_OK;
         }
         // candidates: the per-iteration kernels and every tile shape with its
