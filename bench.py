@@ -133,7 +133,8 @@ def main():
                     help="with --sweep 2: the tile shape (1-5; 0: the first)")
     ap.add_argument("--batched-sweep", type=int, default=0,
                     help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
-    ap.add_argument("--cpu-iters", type=int, default=150)
+    ap.add_argument("--cpu-iters", type=int, default=500)
+    ap.add_argument("--cpu-solves", type=int, default=2, help="grid cells the CPU oracle solves")
     ap.add_argument("--serve-qps", type=str, default="8,16",
                     help="comma-separated arrival rates; one serving run each")
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -522,15 +523,19 @@ def main():
         from oracle import ref32
         threads = len(os.sched_getaffinity(0))
         ref32.set_num_threads(threads)
-        a0 = ph.nonterminal_weight(prep.gms, w_all[0], d_all[0]).astype(np.float32)
+        ncpu = max(1, min(args.cpu_solves, len(w_all)))
+        alphas = [ph.nonterminal_weight(prep.gms, w_all[j], d_all[j]).astype(np.float32)
+                  for j in range(ncpu)]
         t1 = time.time()
-        ref32.cmf_solve(img, prep.quats, eigs, a0, prep.cs, prep.ct, steps=cfg.steps, cc=cfg.cc,
-                        num_iter=args.cpu_iters, update=cfg.cut_update, error_bound=0)
+        for a0 in alphas:
+            ref32.cmf_solve(img, prep.quats, eigs, a0, prep.cs, prep.ct, steps=cfg.steps,
+                            cc=cfg.cc, num_iter=args.cpu_iters, update=cfg.cut_update,
+                            error_bound=0)
         tc = time.time() - t1
         line["cpu_baseline"] = {
-            "value": n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
+            "value": ncpu * n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
             "kind": "port",
-            "sample": f"1 of the 425 solves, {args.cpu_iters} of 500 iterations, 128^3, "
+            "sample": f"{ncpu} of the 425 solves, {args.cpu_iters} of 500 iterations each, 128^3, "
                       f"C/OpenMP restatement of the reference kernels ({tc:.1f} s)"}
         log(f"cpu oracle: {line['cpu_baseline']['value']:.4e} voxel-iters/s on {threads} threads")
 
