@@ -114,6 +114,17 @@ def device_pipeline(dev, spec, weighting, decay, cfg, timings, seed=42):
     return u0, prep.quats, al, prep.cs, prep.ct, truth
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -534,10 +545,24 @@ def main():
         tc = time.time() - t1
         line["cpu_baseline"] = {
             "value": ncpu * n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{ncpu} of the 425 solves, {args.cpu_iters} of 500 iterations each, 128^3, "
-                      f"C/OpenMP restatement of the reference kernels ({tc:.1f} s)"}
+            "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"{ncpu} of the 425 solves, {args.cpu_iters} of 500 iterations each, 128^3: "
+                      f"CPU restatement (reference algorithm), C/OpenMP ({tc:.1f} s)"}
         log(f"cpu oracle: {line['cpu_baseline']['value']:.4e} voxel-iters/s on {threads} threads")
+        # the 64^3 case in full: its 500 iterations on the CPU restatement (inputs
+        # prepared by the library)
+        spec = ph.PLAIN_64
+        img64, _ = ph.make_helix(spec)
+        prep64h = prepare(img64, spec.sigma, cfg, dev)
+        a64 = ph.nonterminal_weight(prep64h.gms, 10 ** -1.625, 10 ** 1.5).astype(np.float32)
+        t1 = time.time()
+        ref32.cmf_solve(img64, prep64h.quats, eigs, a64, prep64h.cs, prep64h.ct, steps=cfg.steps,
+                        cc=cfg.cc, num_iter=500, update=cfg.cut_update, error_bound=0)
+        t64c = time.time() - t1
+        line["cpu_baseline"]["plain_64"] = {
+            "voxel_iters_per_s": img64.size * 500 / t64c, "solve_s": t64c,
+            "sample": "the whole 64^3 solve, 500 iterations"}
+        log(f"cpu oracle 64^3: {t64c:.2f} s")
 
     print(json.dumps(line), flush=True)
 

@@ -269,11 +269,29 @@ __device__ __forceinline__ void bst(Rsrc r, unsigned voff, unsigned soff, float 
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, int(voff), int(soff), 0);
 }
+// the same store through a 64-bit base address (two scalar registers instead
+// of a four-register descriptor)
+__device__ __forceinline__ void pst(float *base, unsigned voff, unsigned soff, float v)
+{
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + soff + size_t(voff)) = v;
+}
 
 struct SweepRsrc {
     Rsrc tmp, hp, q, a, pt, u, cs, ct;  // inputs
     Rsrc hpo, tmpo, pto, uo;            // outputs
 };
+
+// Stores of the sweep. The 1024-thread shapes hold one work-group per CU at
+// any register count up to 128, so they store through addresses: the
+// descriptors they save would otherwise be spilled scalar registers. The
+// 512-thread shapes keep the descriptors, which need fewer vector registers.
+template <bool PTR>
+__device__ __forceinline__ void sweep_store(Rsrc r, float *base, unsigned voff, unsigned soff,
+                                            float v)
+{
+    if constexpr (PTR) pst(base, voff, soff, v);
+    else bst(r, voff, soff, v);
+}
 
 __device__ __forceinline__ void sweep_load(SweepIn &L, const SweepRsrc &R, unsigned nb,
                                            unsigned pb, unsigned d2b, unsigned zb, unsigned ob,
@@ -375,6 +393,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     }
     constexpr int EY = TY + 3, EX = TX + 3, NE = EY * EX;
     static_assert(NE <= NT, "one point per thread");
+    constexpr bool PST = NT >= 1024;
     // axis-1 and axis-2 components of the hp planes (the axis-0 component is
     // only needed at the thread's own voxel), and level-A tmp of plane z-1
     __shared__ float sA1[NE], sA2[NE];  // level-A hp at plane z
@@ -428,7 +447,10 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     L.q = make_float4(1.f, 0.f, 0.f, 0.f);
     if (zs <= zend)
         sweep_load(L, R, nb, pb, d2b, unsigned(zs) * pb, ob, p_on(zs), u_on(zs), zs < D.d0 - 1, f);
-    for (int z = zs; z <= zend; ++z) {
+    // one plane step; the loop below takes two per turn, so that the values
+    // carried between steps alternate between two register sets instead of
+    // being copied back at every step
+    auto step = [&](const int z) {
         const bool ph1 = p_on(z), ph2 = u_on(z);
         const bool ph3 = two && inE1 && z >= max(z0, 1);
         const bool ph4 = two && inO && z - 1 >= z0 && z - 1 < z1;
@@ -449,9 +471,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             mh::cmf_p_voxel_m(hA0_0, hA0_1, hA0_2, g0, g1, g2, RN, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
                               PP.rq1, PP.rq2, L.a);
             if (!two && inO && z >= z0) {
-                bst(R.hpo, ob, zb, hA0_0);
-                bst(R.hpo, ob, zb + nb, hA0_1);
-                bst(R.hpo, ob, zb + 2u * nb, hA0_2);
+                sweep_store<PST>(R.hpo, B.hpo, ob, zb, hA0_0);
+                sweep_store<PST>(R.hpo, B.hpo, ob, zb + nb, hA0_1);
+                sweep_store<PST>(R.hpo, B.hpo, ob, zb + 2u * nb, hA0_2);
             }
         }
         if (live) {
@@ -470,9 +492,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             uA0 = L.u;
             const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, UP.inv_rcc, 1.f, 0.f);
             if (!two) {
-                bst(R.pto, ob, zb, ptA0);
-                bst(R.uo, ob, zb, uA0);
-                bst(R.tmpo, ob, zb, tA0);
+                sweep_store<PST>(R.pto, B.pto, ob, zb, ptA0);
+                sweep_store<PST>(R.uo, B.uo, ob, zb, uA0);
+                sweep_store<PST>(R.tmpo, B.tmpo, ob, zb, tA0);
                 if (check) err_acc += fabsf(err);
             }
         }
@@ -507,12 +529,12 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
             float pti = ptAp, ui = uAp, ti;
             const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, UP.inv_rcc, 1.f, 0.f);
-            bst(R.hpo, ob, zp, hB1_0);
-            bst(R.hpo, ob, zp + nb, hB1_1);
-            bst(R.hpo, ob, zp + 2u * nb, hB1_2);
-            bst(R.pto, ob, zp, pti);
-            bst(R.uo, ob, zp, ui);
-            bst(R.tmpo, ob, zp, ti);
+            sweep_store<PST>(R.hpo, B.hpo, ob, zp, hB1_0);
+            sweep_store<PST>(R.hpo, B.hpo, ob, zp + nb, hB1_1);
+            sweep_store<PST>(R.hpo, B.hpo, ob, zp + 2u * nb, hB1_2);
+            sweep_store<PST>(R.pto, B.pto, ob, zp, pti);
+            sweep_store<PST>(R.uo, B.uo, ob, zp, ui);
+            sweep_store<PST>(R.tmpo, B.tmpo, ob, zp, ti);
             if (check) err_acc += fabsf(err);
         }
         // level-A tmp of plane z for the next step's level-B p-step (every read
@@ -529,6 +551,11 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         aP = aN;
         csP = csN;
         ctP = ctN;
+    };
+    for (int z = zs; z <= zend; z += 2) {
+        step(z);
+        if (z + 1 > zend) break;  // uniform: zs and zend are the work-group's
+        step(z + 1);
     }
     if (check) {
         const float v = mh::block_reduce<SumOp>(err_acc);
