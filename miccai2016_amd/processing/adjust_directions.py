@@ -205,16 +205,21 @@ class AdvancedAdjuster(TreePrintable):
         signs that agree with the spanning tree within each 26-connected
         segment, up to one flip per segment (the score is the norm of the
         segment's summed directions). With an isotropic voxel size the tree
-        restricted to a segment is the forest of its neighbour edges, so the
-        forest's signs suffice, unless a forest edge's dot product is zero or
-        NaN (such a decision does not flip with its parent); then, and for
-        anisotropic voxels, the full tree is used."""
+        restricted to a segment is the forest of its neighbour edges, and a
+        sign decision flips exactly with its parent's sign (negating a float
+        vector negates its dot products exactly), so the forest's signs
+        suffice, unless a forest edge's dot product is zero or NaN (such a
+        decision keeps the child whatever its parent's sign, so it depends on
+        the direction the tree is walked); then, and for anisotropic voxels,
+        the full tree is used. The products are formed as propagate_signs
+        forms them: float32, ((a0 b0 + a1 b1) + a2 b2)."""
         vs = self.voxel_size
         if len(coords) > ridge_graph.DENSE_LIMIT and vs[0] == vs[1] == vs[2]:
             edges = ridge_graph.ridge_forest_edges(coords, shape, vs)
-            d = np.asarray(dirs, np.float64)
-            dots = np.einsum("ij,ij->i", d[edges[:, 0]], d[edges[:, 1]])
-            if not (np.abs(dots) < 1e-6).any() and not np.isnan(dots).any():
+            d = np.require(dirs, np.float32).reshape(-1, 3)
+            a, b = d[edges[:, 0]], d[edges[:, 1]]
+            dots = (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+            if not (dots == 0).any() and not np.isnan(dots).any():
                 return (np.require(coords, np.uint16, ["C"]),
                         ridge_graph.propagate_signs(dirs, edges, 0))
         return self._tree_signs(coords, dirs, shape)

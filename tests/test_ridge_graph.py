@@ -110,10 +110,11 @@ class _NoDevice:
     """Stand-in loader for the adjuster's host steps."""
 
 
-def _segments(rng, shape, count, zero_dot=False):
+def _segments(rng, shape, count, pair=None):
     """Random 26-connected ridge segments (random walks) and random
-    directions; with zero_dot, one segment is two neighbours with orthogonal
-    axis directions."""
+    directions; with pair "zero", one segment is two neighbours with
+    orthogonal axis directions, with "tiny" two neighbours whose float32 dot
+    product is 1e-7 (small, but its sign flips with the parent's)."""
     ridge = np.zeros(shape, np.float32)
     for _ in range(count):
         p = rng.integers(2, np.array(shape) - 2)
@@ -122,29 +123,35 @@ def _segments(rng, shape, count, zero_dot=False):
             p = np.clip(p + rng.integers(-1, 2, size=3), 0, np.array(shape) - 1)
     dirs = rng.normal(size=shape + (3,)).astype(np.float32)
     dirs /= np.linalg.norm(dirs, axis=-1, keepdims=True)
-    if zero_dot:                      # a segment of two voxels, one forest edge
+    if pair:                          # a segment of two voxels, one forest edge
         ridge[:4, :4, :4] = 0
         ridge[1, 1, 1] = ridge[1, 1, 2] = 1
         dirs[1, 1, 1] = (1, 0, 0)
-        dirs[1, 1, 2] = (0, 1, 0)
+        dirs[1, 1, 2] = (0, 1, 0) if pair == "zero" else (1e-7, 1, 0)
     return ridge, dirs
 
 
-@pytest.mark.parametrize("seed,zero_dot", [(1, False), (2, False), (3, True)])
-def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, zero_dot):
+@pytest.mark.parametrize("seed,pair", [(1, None), (2, None), (3, "zero"), (4, "tiny")])
+def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, pair):
     """The cleaning keeps the same segments whether its scores come from the
-    neighbour forest's signs or from the full spanning tree's."""
+    neighbour forest's signs or from the full spanning tree's; an exactly
+    zero forest product falls back to the tree, a tiny one does not."""
     from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
     monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
     rng = np.random.default_rng(seed)
     shape = (64, 64, 64)
-    ridge, dirs = _segments(rng, shape, 80, zero_dot)
+    ridge, dirs = _segments(rng, shape, 80, pair)
     adj = AdvancedAdjuster((0.01, 0.3), None, None, voxel_size=1, loader=_NoDevice())
     c_full, d_full = adj._adjust_ridge_directions(ridge, dirs)
+    trees = []
+    tree_edges = ridge_graph.ridge_tree_edges
+    monkeypatch.setattr(ridge_graph, "ridge_tree_edges",
+                        lambda *a: trees.append(1) or tree_edges(*a))
     c_forest, d_forest = adj._adjust_ridge_directions(ridge, dirs, for_cleaning=True)
+    assert len(trees) == (1 if pair == "zero" else 0)
     np.testing.assert_array_equal(c_full, c_forest)
     np.testing.assert_array_equal(np.abs(d_full), np.abs(d_forest))
-    if zero_dot:
+    if pair == "zero":
         np.testing.assert_array_equal(d_full, d_forest)       # fell back to the tree
     keep_full = adj._do_clean_ridge(ridge, c_full, d_full)
     keep_forest = adj._do_clean_ridge(ridge, c_forest, d_forest)
