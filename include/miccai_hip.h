@@ -276,7 +276,9 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * advances `visit` iterations before the next group takes its turn (visit >=
  * num_iter runs the groups one after another). 0 selects the default: groups
  * of 4 run one after another, which keeps a 128^3 group's state within the
- * last-level cache. Results do not depend on the schedule, bit for bit. */
+ * last-level cache; when the two-level sweep runs the solves, groups of 2,
+ * which the 1024-thread tile shapes advance two solves per thread (group 1:
+ * one per thread). Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
 /* Two-level sweep of mh_cmf_solve_batched's groups (no counterpart in the
  * reference). 0 (default): when the groups run one after another and the

@@ -215,6 +215,19 @@ constexpr SweepShape kSweepShapes[] = {{12, 64, 1024}, {11, 32, 512}, {4, 64, 51
                                        {13, 61, 1024}};
 constexpr int kNumSweepShapes = 5;
 
+// Two solves per thread: the descriptors span both solves' fields (hp: 24 B
+// per voxel), so the volume must stay below half the single-solve limit.
+constexpr int64_t kSweepPairMaxVoxels = int64_t(1) << 27;
+
+// Batched solves advance two per thread with the 1024-thread tile shapes
+// (measured 5 % faster at 128^3: one work-group per CU either way, the pair
+// sharing frames, capacities and barriers); the 512-thread shapes would drop
+// from three resident work-groups per CU to two, and stay one per thread.
+bool sweep_pairs(int shape_ix, int64_t n)
+{
+    return kSweepShapes[shape_ix].threads >= 1024 && n < kSweepPairMaxVoxels;
+}
+
 // a way to run iterations: shape < 0 the plain kernels, else a two-level
 // sweep with that tile shape and plane chunk
 struct SweepMethod {
@@ -236,12 +249,16 @@ struct SweepBatch {
     int *parity;
 };
 
-// Inputs of one point at one plane, loaded a step ahead of their use.
+// Inputs of one point at one plane, loaded a step ahead of their use: those
+// of one solve, and those the solves of one volume share.
 struct SweepIn {
     float t, t0, t1, t2;  // tmp at the voxel and its +axis-0/1/2 neighbours
     float h0, h1, h2;     // hp
+    float a, pt, u;
+};
+struct SweepInShared {
     float4 q;
-    float a, pt, u, cs, ct;
+    float cs, ct;
 };
 
 // The sweep addresses its fields through buffer resources: a uniform byte
@@ -293,26 +310,38 @@ __device__ __forceinline__ void sweep_store(Rsrc r, float *base, unsigned voff, 
     else bst(r, voff, soff, v);
 }
 
-__device__ __forceinline__ void sweep_load(SweepIn &L, const SweepRsrc &R, unsigned nb,
-                                           unsigned pb, unsigned d2b, unsigned zb, unsigned ob,
-                                           bool p_on, bool u_on, bool e0, unsigned f)
+// The inputs of NS solves (solve j's scalar fields at byte offset j nb within
+// the buffers, its hp at 3 j nb) and those they share, in one branch per
+// phase.
+template <int NS>
+__device__ __forceinline__ void sweep_load(SweepIn (&L)[NS], SweepInShared &C, const SweepRsrc &R,
+                                           unsigned nb, unsigned pb, unsigned d2b, unsigned zb,
+                                           unsigned ob, bool p_on, bool u_on, bool e0, unsigned f)
 {
     if (p_on) {
-        L.t = bld(R.tmp, ob, zb);
-        L.t0 = e0 ? bld(R.tmp, ob, zb + pb) : 0.f;
-        L.t1 = (f & 16u) ? bld(R.tmp, ob + d2b, zb) : 0.f;
-        L.t2 = (f & 32u) ? bld(R.tmp, ob + 4u, zb) : 0.f;
-        L.h0 = bld(R.hp, ob, zb);
-        L.h1 = bld(R.hp, ob, zb + nb);
-        L.h2 = bld(R.hp, ob, zb + 2u * nb);
-        L.q = bld4(R.q, 4u * ob, 4u * zb);
-        L.a = bld(R.a, ob, zb);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const unsigned zs = zb + unsigned(j) * nb, zh = zb + 3u * unsigned(j) * nb;
+            L[j].t = bld(R.tmp, ob, zs);
+            L[j].t0 = e0 ? bld(R.tmp, ob, zs + pb) : 0.f;
+            L[j].t1 = (f & 16u) ? bld(R.tmp, ob + d2b, zs) : 0.f;
+            L[j].t2 = (f & 32u) ? bld(R.tmp, ob + 4u, zs) : 0.f;
+            L[j].h0 = bld(R.hp, ob, zh);
+            L[j].h1 = bld(R.hp, ob, zh + nb);
+            L[j].h2 = bld(R.hp, ob, zh + 2u * nb);
+        }
+        C.q = bld4(R.q, 4u * ob, 4u * zb);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) L[j].a = bld(R.a, ob, zb + unsigned(j) * nb);
     }
     if (u_on) {
-        L.pt = bld(R.pt, ob, zb);
-        L.u = bld(R.u, ob, zb);
-        L.cs = bld(R.cs, ob, zb);
-        L.ct = bld(R.ct, ob, zb);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            L[j].pt = bld(R.pt, ob, zb + unsigned(j) * nb);
+            L[j].u = bld(R.u, ob, zb + unsigned(j) * nb);
+        }
+        C.cs = bld(R.cs, ob, zb);
+        C.ct = bld(R.ct, ob, zb);
     }
 }
 
@@ -362,7 +391,12 @@ __device__ __forceinline__ void sweep_point(int t, int &yy, int &xx)
     }
 }
 
-template <int TY, int TX, int NT, int MAP, bool BATCH>
+// NS solves per thread (NS = 2 only in a batch: the pair shares the frames,
+// capacities, masks and barriers, and gives each wave two independent chains
+// of arithmetic). Solve j of the pair is at element offset j n of every
+// per-solve buffer (hp: 3 j n), so the descriptors span NS solves and a
+// solve is a uniform byte offset.
+template <int TY, int TX, int NT, int MAP, bool BATCH, int NS>
 __global__ void __launch_bounds__(NT)
 cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__restrict__ alpha,
                  const float *__restrict__ cs, const float *__restrict__ ct, const Dims D,
@@ -370,14 +404,24 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                  int out_set, int *__restrict__ ctrl, float *__restrict__ partials,
                  const SweepBatch SB)
 {
-    // block -> (solve of the batch, tile)
+    static_assert(NS == 1 || (BATCH && NS == 2), "two solves per thread only in a batch");
+    // block -> (first of the NS solves of the batch it serves, tile)
     unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
     int ks = 0;
     if (BATCH) {
-        ks = int(b % unsigned(SB.ns));
-        b /= unsigned(SB.ns);
+        const unsigned units = unsigned(SB.ns / NS);
+        ks = int(b % units) * NS;
+        b /= units;
     }
-    if (BATCH ? SB.stop[SB.s0 + ks] : ctrl[kStop]) return;
+    // a stopped solve is neither advanced nor reported
+    bool act[NS];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        act[j] = !(BATCH ? SB.stop[SB.s0 + ks + j] : ctrl[kStop]);
+        any = any || act[j];
+    }
+    if (!any) return;
     const int64_t nvox = int64_t(D.d0) * D.d1 * D.d2;
     if (BATCH) {
         const int64_t o = nvox * ks;
@@ -394,23 +438,25 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     constexpr int EY = TY + 3, EX = TX + 3, NE = EY * EX;
     static_assert(NE <= NT, "one point per thread");
     constexpr bool PST = NT >= 1024;
-    // axis-1 and axis-2 components of the hp planes (the axis-0 component is
-    // only needed at the thread's own voxel), and level-A tmp of plane z-1
-    __shared__ float sA1[NE], sA2[NE];  // level-A hp at plane z
-    __shared__ float sT[NE];            // level-A tmp at plane z-1
-    __shared__ float sB1[NE], sB2[NE];  // level-B hp at plane z-1
+    // per solve: axis-1 and axis-2 components of the hp planes (the axis-0
+    // component is only needed at the thread's own voxel), and level-A tmp
+    // of plane z-1
+    __shared__ float sA1[NS][NE], sA2[NS][NE];  // level-A hp at plane z
+    __shared__ float sT[NS][NE];                // level-A tmp at plane z-1
+    __shared__ float sB1[NS][NE], sB2[NS][NE];  // level-B hp at plane z-1
 
     const int ntx = (D.d2 + TX - 1) / TX, nty = (D.d1 + TY - 1) / TY;
     const int tx = int(b % unsigned(ntx)), ty = int((b / unsigned(ntx)) % unsigned(nty));
     const int tz = int(b / unsigned(ntx * nty));
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zchunk;
     const int z1 = min(z0 + zchunk, D.d0);
-    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    const int64_t n = nvox;
     const int64_t st0 = int64_t(D.d1) * D.d2;
-    const SweepRsrc R{rsrc_of(B.tmp, 4 * n), rsrc_of(B.hp, 12 * n), rsrc_of(q4, 16 * n),
-                      rsrc_of(alpha, 4 * n), rsrc_of(B.pt, 4 * n), rsrc_of(B.u, 4 * n),
-                      rsrc_of(cs, 4 * n), rsrc_of(ct, 4 * n), rsrc_of(B.hpo, 12 * n),
-                      rsrc_of(B.tmpo, 4 * n), rsrc_of(B.pto, 4 * n), rsrc_of(B.uo, 4 * n)};
+    const SweepRsrc R{rsrc_of(B.tmp, 4 * n * NS), rsrc_of(B.hp, 12 * n * NS), rsrc_of(q4, 16 * n),
+                      rsrc_of(alpha, 4 * n * NS), rsrc_of(B.pt, 4 * n * NS), rsrc_of(B.u, 4 * n * NS),
+                      rsrc_of(cs, 4 * n), rsrc_of(ct, 4 * n), rsrc_of(B.hpo, 12 * n * NS),
+                      rsrc_of(B.tmpo, 4 * n * NS), rsrc_of(B.pto, 4 * n * NS),
+                      rsrc_of(B.uo, 4 * n * NS)};
     const unsigned nb = unsigned(4 * n), pb = unsigned(4 * st0), d2b = 4u * unsigned(D.d2);
 
     // this thread's point of the widened tile; LDS is indexed by the point
@@ -435,18 +481,28 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     auto u_on = [&](int z) { return (two ? inE2 : inO) && z < D.d0 && z >= (two ? z0 - 1 : z0); };
 
     // values carried from step z-1 to step z
-    float hA1_0 = 0.f, hA1_1 = 0.f, hA1_2 = 0.f;  // level-A hp at z-1
-    float hB2_0 = 0.f;                             // level-B hp (axis 0) at z-2
-    float ptAp = 0.f, uAp = 0.f, tAp = 0.f;       // level-A pt, u, tmp at z-1
-    float aP = 0.f, csP = 0.f, ctP = 0.f;
+    float hA1_0[NS], hA1_1[NS], hA1_2[NS];  // level-A hp at z-1
+    float hB2_0[NS];                        // level-B hp (axis 0) at z-2
+    float ptAp[NS], uAp[NS], tAp[NS];       // level-A pt, u, tmp at z-1
+    float aP[NS];
+    float err_acc[NS];
+    float csP = 0.f, ctP = 0.f;
     mh::Mat3 RP = mh::quat_to_mat(mh::Quat{1.f, 0.f, 0.f, 0.f});  // frame at z-1
-    float err_acc = 0.f;
-
-    SweepIn L;
-    L.t = L.t0 = L.t1 = L.t2 = L.h0 = L.h1 = L.h2 = L.a = L.pt = L.u = L.cs = L.ct = 0.f;
-    L.q = make_float4(1.f, 0.f, 0.f, 0.f);
-    if (zs <= zend)
-        sweep_load(L, R, nb, pb, d2b, unsigned(zs) * pb, ob, p_on(zs), u_on(zs), zs < D.d0 - 1, f);
+    SweepIn L[NS];
+    SweepInShared LC;
+    LC.q = make_float4(1.f, 0.f, 0.f, 0.f);
+    LC.cs = LC.ct = 0.f;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        hA1_0[j] = hA1_1[j] = hA1_2[j] = hB2_0[j] = 0.f;
+        ptAp[j] = uAp[j] = tAp[j] = aP[j] = err_acc[j] = 0.f;
+        L[j].t = L[j].t0 = L[j].t1 = L[j].t2 = L[j].h0 = L[j].h1 = L[j].h2 = 0.f;
+        L[j].a = L[j].pt = L[j].u = 0.f;
+    }
+    auto load = [&](const int z, const unsigned zb) {
+        sweep_load<NS>(L, LC, R, nb, pb, d2b, zb, ob, p_on(z), u_on(z), z < D.d0 - 1, f);
+    };
+    if (zs <= zend) load(zs, unsigned(zs) * pb);
     // one plane step; the loop below takes two per turn, so that the values
     // carried between steps alternate between two register sets instead of
     // being copied back at every step
@@ -458,97 +514,149 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         const bool e0b = z - 1 < D.d0 - 1, f0b = z - 1 > 0;  // plane z-1
         const unsigned zb = unsigned(z) * pb;  // plane z, bytes
         // 1. level-A p-step at plane z on the widened tile
-        float hA0_0 = 0.f, hA0_1 = 0.f, hA0_2 = 0.f;
-        // the frame of plane z, built once for both levels
-        const mh::Mat3 RN = mh::quat_to_mat(mh::Quat{L.q.x, L.q.y, L.q.z, L.q.w});
+        float hA0_0[NS], hA0_1[NS], hA0_2[NS];
+        // the frame of plane z, built once for both levels and every solve
+        const mh::Mat3 RN = mh::quat_to_mat(mh::Quat{LC.q.x, LC.q.y, LC.q.z, LC.q.w});
+#pragma unroll
+        for (int j = 0; j < NS; ++j) hA0_0[j] = hA0_1[j] = hA0_2[j] = 0.f;
         if (ph1) {
-            const float g0 = e0 ? (L.t0 - L.t) * PP.ss0 : 0.f;
-            const float g1 = (f & 16u) ? (L.t1 - L.t) * PP.ss1 : 0.f;
-            const float g2 = (f & 32u) ? (L.t2 - L.t) * PP.ss2 : 0.f;
-            hA0_0 = L.h0;
-            hA0_1 = L.h1;
-            hA0_2 = L.h2;
-            mh::cmf_p_voxel_m(hA0_0, hA0_1, hA0_2, g0, g1, g2, RN, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
-                              PP.rq1, PP.rq2, L.a);
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const float g0 = e0 ? (L[j].t0 - L[j].t) * PP.ss0 : 0.f;
+                const float g1 = (f & 16u) ? (L[j].t1 - L[j].t) * PP.ss1 : 0.f;
+                const float g2 = (f & 32u) ? (L[j].t2 - L[j].t) * PP.ss2 : 0.f;
+                hA0_0[j] = L[j].h0;
+                hA0_1[j] = L[j].h1;
+                hA0_2[j] = L[j].h2;
+                mh::cmf_p_voxel_m(hA0_0[j], hA0_1[j], hA0_2[j], g0, g1, g2, RN, PP.sq0, PP.sq1,
+                                  PP.sq2, PP.rq0, PP.rq1, PP.rq2, L[j].a);
+            }
             if (!two && inO && z >= z0) {
-                sweep_store<PST>(R.hpo, B.hpo, ob, zb, hA0_0);
-                sweep_store<PST>(R.hpo, B.hpo, ob, zb + nb, hA0_1);
-                sweep_store<PST>(R.hpo, B.hpo, ob, zb + 2u * nb, hA0_2);
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    if (NS > 1 && !act[j]) continue;
+                    const unsigned zh = zb + 3u * unsigned(j) * nb;
+                    sweep_store<PST>(R.hpo, B.hpo, ob, zh, hA0_0[j]);
+                    sweep_store<PST>(R.hpo, B.hpo, ob, zh + nb, hA0_1[j]);
+                    sweep_store<PST>(R.hpo, B.hpo, ob, zh + 2u * nb, hA0_2[j]);
+                }
             }
         }
         if (live) {
-            sA1[p] = hA0_1;
-            sA2[p] = hA0_2;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                sA1[j][p] = hA0_1[j];
+                sA2[j][p] = hA0_2[j];
+            }
         }
         __syncthreads();
         // 2. level-A u-step at plane z (stores when it is the only level)
-        float tA0 = 0.f, ptA0 = 0.f, uA0 = 0.f;
+        float tA0[NS], ptA0[NS], uA0[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) tA0[j] = ptA0[j] = uA0[j] = 0.f;
         if (ph2) {
-            const float x0v = (e0 ? hA0_0 : 0.f) - (f0 ? hA1_0 : 0.f);
-            const float x1v = ((f & 16u) ? hA0_1 : 0.f) - ((f & 64u) ? sA1[p - EX] : 0.f);
-            const float x2v = ((f & 32u) ? hA0_2 : 0.f) - ((f & 128u) ? sA2[p - 1] : 0.f);
-            const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
-            ptA0 = L.pt;
-            uA0 = L.u;
-            const float err = mh::cmf_u_voxel(div, ptA0, uA0, tA0, L.cs, L.ct, UP.rcc, UP.inv_rcc, 1.f, 0.f);
+            float err[NS];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const float x0v = (e0 ? hA0_0[j] : 0.f) - (f0 ? hA1_0[j] : 0.f);
+                const float x1v = ((f & 16u) ? hA0_1[j] : 0.f) - ((f & 64u) ? sA1[j][p - EX] : 0.f);
+                const float x2v = ((f & 32u) ? hA0_2[j] : 0.f) - ((f & 128u) ? sA2[j][p - 1] : 0.f);
+                const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
+                ptA0[j] = L[j].pt;
+                uA0[j] = L[j].u;
+                err[j] = mh::cmf_u_voxel(div, ptA0[j], uA0[j], tA0[j], LC.cs, LC.ct, UP.rcc,
+                                         UP.inv_rcc, 1.f, 0.f);
+            }
             if (!two) {
-                sweep_store<PST>(R.pto, B.pto, ob, zb, ptA0);
-                sweep_store<PST>(R.uo, B.uo, ob, zb, uA0);
-                sweep_store<PST>(R.tmpo, B.tmpo, ob, zb, tA0);
-                if (check) err_acc += fabsf(err);
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    if (check) err_acc[j] += fabsf(err[j]);
+                    if (NS > 1 && !act[j]) continue;
+                    const unsigned zj = zb + unsigned(j) * nb;
+                    sweep_store<PST>(R.pto, B.pto, ob, zj, ptA0[j]);
+                    sweep_store<PST>(R.uo, B.uo, ob, zj, uA0[j]);
+                    sweep_store<PST>(R.tmpo, B.tmpo, ob, zj, tA0[j]);
+                }
             }
         }
         // constants of plane z, needed at plane z-1's turn in the next step
-        const float aN = L.a, csN = L.cs, ctN = L.ct;
+        float aN[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) aN[j] = L[j].a;
+        const float csN = LC.cs, ctN = LC.ct;
         // inputs of the next step, in flight while the level-B work runs
-        if (z + 1 <= zend)
-            sweep_load(L, R, nb, pb, d2b, zb + pb, ob, p_on(z + 1), u_on(z + 1), z + 1 < D.d0 - 1, f);
+        if (z + 1 <= zend) load(z + 1, zb + pb);
         // 3. level-B p-step at plane z-1 on E1
-        float hB1_0 = 0.f, hB1_1 = 0.f, hB1_2 = 0.f;
+        float hB1_0[NS], hB1_1[NS], hB1_2[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) hB1_0[j] = hB1_1[j] = hB1_2[j] = 0.f;
         if (ph3) {
-            const float g0 = e0b ? (tA0 - tAp) * PP.ss0 : 0.f;
-            const float g1 = (f & 16u) ? (sT[p + EX] - tAp) * PP.ss1 : 0.f;
-            const float g2 = (f & 32u) ? (sT[p + 1] - tAp) * PP.ss2 : 0.f;
-            hB1_0 = hA1_0;
-            hB1_1 = hA1_1;
-            hB1_2 = hA1_2;
-            mh::cmf_p_voxel_m(hB1_0, hB1_1, hB1_2, g0, g1, g2, RP, PP.sq0, PP.sq1, PP.sq2, PP.rq0,
-                              PP.rq1, PP.rq2, aP);
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const float g0 = e0b ? (tA0[j] - tAp[j]) * PP.ss0 : 0.f;
+                const float g1 = (f & 16u) ? (sT[j][p + EX] - tAp[j]) * PP.ss1 : 0.f;
+                const float g2 = (f & 32u) ? (sT[j][p + 1] - tAp[j]) * PP.ss2 : 0.f;
+                hB1_0[j] = hA1_0[j];
+                hB1_1[j] = hA1_1[j];
+                hB1_2[j] = hA1_2[j];
+                mh::cmf_p_voxel_m(hB1_0[j], hB1_1[j], hB1_2[j], g0, g1, g2, RP, PP.sq0, PP.sq1,
+                                  PP.sq2, PP.rq0, PP.rq1, PP.rq2, aP[j]);
+            }
         }
         if (live) {
-            sB1[p] = hB1_1;
-            sB2[p] = hB1_2;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                sB1[j][p] = hB1_1[j];
+                sB2[j][p] = hB1_2[j];
+            }
         }
         __syncthreads();
         // 4. level-B u-step at plane z-1 on O; store iteration t + 2
         if (ph4) {
             const unsigned zp = zb - pb;  // plane z-1
-            const float x0v = (e0b ? hB1_0 : 0.f) - (f0b ? hB2_0 : 0.f);
-            const float x1v = ((f & 16u) ? hB1_1 : 0.f) - ((f & 64u) ? sB1[p - EX] : 0.f);
-            const float x2v = ((f & 32u) ? hB1_2 : 0.f) - ((f & 128u) ? sB2[p - 1] : 0.f);
-            const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
-            float pti = ptAp, ui = uAp, ti;
-            const float err = mh::cmf_u_voxel(div, pti, ui, ti, csP, ctP, UP.rcc, UP.inv_rcc, 1.f, 0.f);
-            sweep_store<PST>(R.hpo, B.hpo, ob, zp, hB1_0);
-            sweep_store<PST>(R.hpo, B.hpo, ob, zp + nb, hB1_1);
-            sweep_store<PST>(R.hpo, B.hpo, ob, zp + 2u * nb, hB1_2);
-            sweep_store<PST>(R.pto, B.pto, ob, zp, pti);
-            sweep_store<PST>(R.uo, B.uo, ob, zp, ui);
-            sweep_store<PST>(R.tmpo, B.tmpo, ob, zp, ti);
-            if (check) err_acc += fabsf(err);
+            float pti[NS], ui[NS], ti[NS], err[NS];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const float x0v = (e0b ? hB1_0[j] : 0.f) - (f0b ? hB2_0[j] : 0.f);
+                const float x1v = ((f & 16u) ? hB1_1[j] : 0.f) - ((f & 64u) ? sB1[j][p - EX] : 0.f);
+                const float x2v = ((f & 32u) ? hB1_2[j] : 0.f) - ((f & 128u) ? sB2[j][p - 1] : 0.f);
+                const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
+                pti[j] = ptAp[j];
+                ui[j] = uAp[j];
+                err[j] = mh::cmf_u_voxel(div, pti[j], ui[j], ti[j], csP, ctP, UP.rcc, UP.inv_rcc,
+                                         1.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (check) err_acc[j] += fabsf(err[j]);
+                if (NS > 1 && !act[j]) continue;
+                const unsigned zj = zp + unsigned(j) * nb, zh = zp + 3u * unsigned(j) * nb;
+                sweep_store<PST>(R.hpo, B.hpo, ob, zh, hB1_0[j]);
+                sweep_store<PST>(R.hpo, B.hpo, ob, zh + nb, hB1_1[j]);
+                sweep_store<PST>(R.hpo, B.hpo, ob, zh + 2u * nb, hB1_2[j]);
+                sweep_store<PST>(R.pto, B.pto, ob, zj, pti[j]);
+                sweep_store<PST>(R.uo, B.uo, ob, zj, ui[j]);
+                sweep_store<PST>(R.tmpo, B.tmpo, ob, zj, ti[j]);
+            }
         }
         // level-A tmp of plane z for the next step's level-B p-step (every read
         // of plane z-1's values finished before the barrier above)
-        if (live) sT[p] = tA0;
-        hB2_0 = hB1_0;
-        hA1_0 = hA0_0;
-        hA1_1 = hA0_1;
-        hA1_2 = hA0_2;
-        ptAp = ptA0;
-        uAp = uA0;
-        tAp = tA0;
+        if (live) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) sT[j][p] = tA0[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            hB2_0[j] = hB1_0[j];
+            hA1_0[j] = hA0_0[j];
+            hA1_1[j] = hA0_1[j];
+            hA1_2[j] = hA0_2[j];
+            ptAp[j] = ptA0[j];
+            uAp[j] = uA0[j];
+            tAp[j] = tA0[j];
+            aP[j] = aN[j];
+        }
         RP = RN;
-        aP = aN;
         csP = csN;
         ctP = ctN;
     };
@@ -558,31 +666,44 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         step(z + 1);
     }
     if (check) {
-        const float v = mh::block_reduce<SumOp>(err_acc);
-        if (threadIdx.x == 0) {
-            if (BATCH) partials[int64_t(SB.s0 + ks) * SB.nper + b] = v;
-            else partials[blockIdx.x] = v;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const float v = mh::block_reduce<SumOp>(err_acc[j]);
+            if (threadIdx.x == 0 && (NS == 1 || act[j])) {
+                if (BATCH) partials[int64_t(SB.s0 + ks + j) * SB.nper + b] = v;
+                else partials[blockIdx.x] = v;
+            }
         }
     }
     if (threadIdx.x == 0) {
-        if (BATCH && b == 0) SB.parity[SB.s0 + ks] = out_set;
-        else if (!BATCH && blockIdx.x == 0) ctrl[kParity] = out_set;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (NS > 1 && !act[j]) continue;
+            if (BATCH && b == 0) SB.parity[SB.s0 + ks + j] = out_set;
+            else if (!BATCH && blockIdx.x == 0) ctrl[kParity] = out_set;
+        }
     }
 }
 
+// batch: NS solves per thread, grid = work-groups per solve x batch->ns / NS
 template <int TY, int TX, int NT, int MAP>
 void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const float4 *q4,
                     const float *alpha, const float *cs, const float *ct, const Dims &D,
                     const PParams &PP, const UParams &UP, int zchunk, int lv, int chk, int out_set,
-                    int *ctrl, float *partials, const SweepBatch *batch)
+                    int *ctrl, float *partials, const SweepBatch *batch, int ns_thread)
 {
-    if (batch)
-        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, true>), dim3(grid), dim3(NT), 0, s, SBf, q4,
-                           alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, *batch);
+    if (batch && ns_thread == 2)
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, true, 2>), dim3(grid), dim3(NT), 0, s,
+                           SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl,
+                           partials, *batch);
+    else if (batch)
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, true, 1>), dim3(grid), dim3(NT), 0, s,
+                           SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl,
+                           partials, *batch);
     else
-        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, false>), dim3(grid), dim3(NT), 0, s, SBf, q4,
-                           alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials,
-                           SweepBatch{});
+        hipLaunchKernelGGL((cmf_sweep_kernel<TY, TX, NT, MAP, false, 1>), dim3(grid), dim3(NT), 0, s,
+                           SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl,
+                           partials, SweepBatch{});
 }
 
 // thread-to-point mapping of each tile shape (sweep_point)
@@ -592,7 +713,7 @@ template <int TY, int TX, int NT, int MAP>
 int sweep_occupancy_t(int &blocks)
 {
     return int(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocks, reinterpret_cast<const void *>(&cmf_sweep_kernel<TY, TX, NT, MAP, false>), NT, 0));
+        &blocks, reinterpret_cast<const void *>(&cmf_sweep_kernel<TY, TX, NT, MAP, false, 1>), NT, 0));
 }
 
 // Work-groups of one sweep launch with this tile shape that the device holds
@@ -646,14 +767,15 @@ int sweep_slots(mh_ctx *ctx, int shape_ix)
 void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &SBf,
                   const float4 *q4, const float *alpha, const float *cs, const float *ct,
                   const Dims &D, const PParams &PP, const UParams &UP, int zchunk, int lv, int chk,
-                  int out_set, int *ctrl, float *partials, const SweepBatch *batch = nullptr)
+                  int out_set, int *ctrl, float *partials, const SweepBatch *batch = nullptr,
+                  int ns_thread = 1)
 {
     switch (shape_ix) {
-    case 1: launch_sweep_t<11, 32, 512, kMap1>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 2: launch_sweep_t<4, 64, 512, kMap2>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 3: launch_sweep_t<26, 32, 1024, kMap3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    case 4: launch_sweep_t<13, 61, 1024, kMap4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
-    default: launch_sweep_t<12, 64, 1024, kMap0>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch); break;
+    case 1: launch_sweep_t<11, 32, 512, kMap1>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 2: launch_sweep_t<4, 64, 512, kMap2>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 3: launch_sweep_t<26, 32, 1024, kMap3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 4: launch_sweep_t<13, 61, 1024, kMap4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    default: launch_sweep_t<12, 64, 1024, kMap0>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
     }
 }
 
@@ -1484,8 +1606,6 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     const int nctrl = 2 * S + groups;
     hipLaunchKernelGGL(zero_ints, dim3((nctrl + 255) / 256), dim3(256), 0, s, W.stop, nctrl);
     hipLaunchKernelGGL(quat_kernel, dim3(nb), dim3(mh::kBlock), 0, s, W.q4, eigs, n);
-    hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, S), dim3(mh::kBlock), 0, s, W.hp, W.pt, W.tmp,
-                       u, cs, ct, n, prm->rcc);
     MH_CHECK_LAUNCH(ctx, "batched init");
 
     const PParams PP = p_params(prm);
@@ -1516,7 +1636,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                                                    : sweep_model_zchunk(ctx, k, D, int64_t(nb))};
         if (sweep_elig && ctx->batch_sweep == 0) {
             want_sweep = true;
-            if (G <= 0) G = 1;
+            if (G <= 0) G = sweep_pairs(sm.shape, n) ? 2 : 1;
         }
     } else {
         if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0 && prm->num_iter >= 96 &&
@@ -1529,7 +1649,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
             sm = SweepMethod{f->second.first, f->second.second};
             if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0) {
                 want_sweep = true;
-                if (G <= 0) G = 1;
+                if (G <= 0) G = sweep_pairs(sm.shape, n) ? 2 : 1;
             }
         }
     }
@@ -1548,21 +1668,65 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     const int64_t nper = int64_t((D.d2 + SS.tx - 1) / SS.tx) * ((D.d1 + SS.ty - 1) / SS.ty) *
                          ((D.d0 + sm.zchunk - 1) / sm.zchunk);
     const bool use_sweep = want_sweep && nper <= int64_t(nb);
+    if (!use_sweep) {
+        hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, S), dim3(mh::kBlock), 0, s, W.hp, W.pt,
+                           W.tmp, u, cs, ct, n, prm->rcc);
+        MH_CHECK_LAUNCH(ctx, "batched init");
+    }
+    // The sweep runs the groups one after another, so only the running
+    // group's state is live: it is set up just before the group runs, in one
+    // block at the start of the state arrays when they have room for four
+    // groups (the state of all S solves would put one group's buffers
+    // gigabytes apart: 5 % slower on the 425-solve grid at 128^3).
+    const bool compact = int64_t(S) >= 4 * int64_t(G);
     for (int g = 0; use_sweep && g < ngroups; ++g) {
         const int s0 = g * G, ns = std::min(G, S - s0);
-        const SweepBatch SBt{s0, ns, int(nper), W.stop, W.parity};
+        const int64_t o = n * s0, nG = n * G;
+        float *hpA = W.hp + 3 * o, *ptA = W.pt + o, *tmpA = W.tmp + o;
+        float *hpB = W.hp2, *ptB = W.pt2, *tmpB = W.tmp2, *uB = W.u2, *alB = W.alpha2;
+        if (compact) {  // 12 n G <= 3 n S floats of the hp array
+            hpA = W.hp;
+            ptA = hpA + 3 * nG;
+            tmpA = ptA + nG;
+            hpB = tmpA + nG;
+            ptB = hpB + 3 * nG;
+            tmpB = ptB + nG;
+            uB = tmpB + nG;
+            alB = uB + nG;
+        }
+        hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, hpA, ptA, tmpA,
+                           u + o, cs, ct, n, prm->rcc);
         hipLaunchKernelGGL(cmf_group_alpha_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, gms, n, gp[g],
-                           W.alpha2);
+                           alB);
         int cur = 0;
         for (int it = 0; it < prm->num_iter;) {
             const bool chk_a = (it % prm->update) == 0;
             const int lv = (!chk_a && it + 1 < prm->num_iter) ? 2 : 1;
             const bool chk = ((it + lv - 1) % prm->update) == 0;
-            const int64_t o = n * s0;
-            SweepBufs B0{W.hp + 3 * o, W.tmp + o, W.pt + o, u + o, W.hp2, W.tmp2, W.pt2, W.u2};
-            SweepBufs B1{W.hp2, W.tmp2, W.pt2, W.u2, W.hp + 3 * o, W.tmp + o, W.pt + o, u + o};
-            launch_sweep(sm.shape, unsigned(nper * ns), s, cur == 0 ? B0 : B1, W.q4, W.alpha2, cs, ct, D,
-                         PP, UP, sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBt);
+            SweepBufs B0{hpA, tmpA, ptA, u + o, hpB, tmpB, ptB, uB};
+            SweepBufs B1{hpB, tmpB, ptB, uB, hpA, tmpA, ptA, u + o};
+            // the group's solves two per thread, an odd last one alone
+            const int npair = sweep_pairs(sm.shape, n) ? ns / 2 : 0, rest = ns - 2 * npair;
+            if (npair) {
+                const SweepBatch SBp{s0, 2 * npair, int(nper), W.stop, W.parity};
+                launch_sweep(sm.shape, unsigned(nper * npair), s, cur == 0 ? B0 : B1, W.q4, alB, cs, ct, D,
+                             PP, UP, sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBp, 2);
+            }
+            if (rest) {
+                const int64_t r = n * (2 * npair);
+                SweepBufs Br = cur == 0 ? B0 : B1;
+                Br.hp += 3 * r;
+                Br.tmp += r;
+                Br.pt += r;
+                Br.u += r;
+                Br.hpo += 3 * r;
+                Br.tmpo += r;
+                Br.pto += r;
+                Br.uo += r;
+                const SweepBatch SBr{s0 + 2 * npair, rest, int(nper), W.stop, W.parity};
+                launch_sweep(sm.shape, unsigned(nper * rest), s, Br, W.q4, alB + r, cs, ct, D, PP, UP,
+                             sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBr, 1);
+            }
             if (chk)
                 hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(ns), dim3(mh::kBlock), 0, s, W.partials,
                                    int(nper), UP.inv_n_rcc, UP.error_bound, C, s0);
@@ -1570,8 +1734,8 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
             cur ^= 1;
             it += lv;
         }
-        hipLaunchKernelGGL(cmf_batch_sweep_final_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, u, W.u2,
-                           n, s0, W.parity);
+        hipLaunchKernelGGL(cmf_batch_sweep_final_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, u, uB, n,
+                           s0, W.parity);
         MH_CHECK_LAUNCH(ctx, "batched sweep final");
     }
     for (int base = 0; !use_sweep && base < prm->num_iter; base += V) {
