@@ -48,6 +48,10 @@ TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 # (the grid) and at 256^3
 TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 44.1, 2: 49.3, 3: 44.5, 4: 45.4, 5: 41.9}
 TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.1, 2: 58.4, 3: 51.4, 4: 45.9, 5: 47.2}
+# the batched grid's sweep, two solves per thread (the 1024-thread tile shapes 1, 4
+# and 5), at 128^3; shapes 4 and 5 unmeasured
+TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER = {1: 37.5}
+PAIR_TILES = (1, 4, 5)
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
 PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu", "sigmas")
@@ -262,16 +266,21 @@ def main():
                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
         tile = sweep_choice(dev, img.shape)
         if tile:
-            # solves ran one at a time through the two-level sweep: one launch
-            # is one pass of up to two iterations over one solve
+            # the groups ran one after another through the two-level sweep:
+            # one launch is one pass of up to two iterations over a pair of
+            # solves (the 1024-thread tile shapes) or over one solve
             passes = sweep_passes(args.iters, cfg.cut_update)
-            per_vi = TRAFFIC_SWEEP_PER_VOXEL_ITER.get(tile)
+            pair = tile in PAIR_TILES and n < (1 << 27)
+            per_launch = 2 if pair else 1
+            per_vi = (TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER if pair
+                      else TRAFFIC_SWEEP_PER_VOXEL_ITER).get(tile)
             line["roofline"].update(
-                kernel=f"cmf_sweep_kernel (tile shape {tile}): one pass = two max-flow iterations of "
-                       f"one solve (84 algorithmic B per voxel-iteration; traffic = measured HBM "
-                       f"bytes per pass)",
-                launch_us=secs / args.steps / (S * passes) * 1e6,
-                traffic=None if per_vi is None else per_vi * 2 * n,
+                kernel=f"cmf_sweep_kernel (tile shape {tile}, {per_launch} solve(s) per thread): one "
+                       f"launch = one pass of two max-flow iterations over {per_launch} solve(s) "
+                       f"(84 algorithmic B per voxel-iteration; traffic = measured HBM bytes per "
+                       f"launch)",
+                launch_us=secs / args.steps / (S * passes) * per_launch * 1e6,
+                traffic=None if per_vi is None else per_vi * 2 * per_launch * n,
                 traffic_per_voxel_iteration=per_vi)
         else:
             groups = -(-S // GRID_GROUP)
