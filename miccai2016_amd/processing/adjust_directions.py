@@ -10,6 +10,8 @@ into squeezed frames (GPU).
 """
 import numpy as np
 from scipy.ndimage import label
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
 
 from .. import _lib
 from ..device import default_context
@@ -115,13 +117,12 @@ class AdvancedAdjuster(TreePrintable):
         adjusted = None
         if (not self.ridge_detection_only) or self.clean_ridge:
             main_dirs = self._qeigs_to_main_dirs()
-            coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs,
-                                                          for_cleaning=self.clean_ridge)
             if self.clean_ridge:
-                labeled = self._do_clean_ridge(labeled, coords, rdirs)
-                if not self.ridge_detection_only:
-                    coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
+                coords, rdirs, parts = self._adjust_ridge_directions(labeled, main_dirs,
+                                                                     for_cleaning=True)
+                labeled = self._do_clean_ridge(labeled, coords, rdirs, parts)
             if not self.ridge_detection_only:
+                coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
                 adjusted = self._adjust_all_directions(coords, rdirs, shape)
         return adjusted
 
@@ -159,8 +160,11 @@ class AdvancedAdjuster(TreePrintable):
         coords = np.stack(np.unravel_index(lin, shape), 1) if len(lin) else np.zeros((0, 3), np.int64)
         self._set_ridge(coords, shape)
         if self.clean_ridge:
-            _, adjusted = self._cleaning_signs(coords, dirs, shape)
-            keep = self._segments_to_keep(seg, adjusted)
+            _, adjusted, parts = self._cleaning_signs(coords, dirs, shape)
+            keep = self._segments_to_keep(seg, adjusted, parts)
+            if keep is None:          # the forest's bounds did not decide: the full tree
+                _, adjusted = self._tree_signs(coords, dirs, shape)
+                keep = self._segments_to_keep(seg, adjusted)
             coords, dirs = coords[keep], dirs[keep]
             self._set_ridge(coords, shape)
         coords, adjusted = self._tree_signs(coords, dirs, shape)
@@ -179,26 +183,61 @@ class AdvancedAdjuster(TreePrintable):
             return len(self.__ridge_coords[0])
         return int(np.count_nonzero(self.ridge)) if self.ridge is not None else 0
 
-    def _segments_to_keep(self, seg, ridge_dirs):
+    def _segments_to_keep(self, seg, ridge_dirs, parts=None):
         """_do_clean_ridge on segment labels: seg holds each ridge point's
         segment (the segment's smallest voxel index, so increasing labels are
-        scipy's label numbering). Returns the points of the kept segments."""
+        scipy's label numbering). Returns the points of the kept segments, or
+        None when `parts` leaves the choice open (see _kept_segments)."""
         roots, lab_of = np.unique(seg, return_inverse=True)
         ncomp = len(roots)
         if ncomp == 0:
             return np.zeros(0, bool)
+        kept = self._kept_segments(lab_of, ncomp, ridge_dirs, parts)
+        return None if kept is None else kept[lab_of]
+
+    def _kept_segments(self, lab_of, ncomp, ridge_dirs, parts=None):
+        """Mask over the ncomp segments: the ceil(3 ln(e + n)) whose summed
+        (voxel-size scaled) directions are longest, the lower label first on
+        ties (adjust_directions.py:290-339). With `parts` (each point's piece
+        of its segment between zero-product forest edges, whose relative
+        signs the forest leaves open) a segment's length is only known to lie
+        in [2 max|P| - sum|P|, sum|P|] over its pieces' sums P (triangle
+        inequality); the mask is returned when those bounds settle every
+        segment in or out, else None."""
         scaled = ridge_dirs * np.asarray(self.voxel_size)
         sums = np.zeros((ncomp, 3))
         for k in range(3):
             sums[:, k] = np.bincount(lab_of, weights=scaled[:, k], minlength=ncomp)
         lengths = np.linalg.norm(sums, axis=1)
-        order = np.argsort(-lengths, kind="stable")
         keep = int(np.ceil(3 * np.log(np.e + ncomp)))
         if getattr(self, "_verbose", False):
             print("Keeping %s of %s labels." % (min(keep, ncomp), ncomp))
-        kept = np.zeros(ncomp, bool)
-        kept[order[:keep]] = True
-        return kept[lab_of]
+        if parts is None:
+            order = np.argsort(-lengths, kind="stable")
+            kept = np.zeros(ncomp, bool)
+            kept[order[:keep]] = True
+            return kept
+        pid, p_of = np.unique(parts, return_inverse=True)
+        psum = np.zeros((len(pid), 3))
+        for k in range(3):
+            psum[:, k] = np.bincount(p_of, weights=scaled[:, k], minlength=len(pid))
+        plen = np.linalg.norm(psum, axis=1)
+        pseg = np.zeros(len(pid), np.int64)
+        pseg[p_of] = lab_of                          # a piece lies within one segment
+        tot = np.bincount(pseg, weights=plen, minlength=ncomp)
+        big = np.zeros(ncomp)
+        np.maximum.at(big, pseg, plen)
+        multi = np.bincount(pseg, minlength=ncomp) > 1
+        margin = 1e-9 * (tot + 1.0)                  # summation-order rounding
+        lo = np.where(multi, np.maximum(0.0, 2 * big - tot) - margin, lengths)
+        hi = np.where(multi, tot + margin, lengths)
+        # others that may reach x's lower bound; others sure to beat x's upper one
+        reach = ncomp - np.searchsorted(np.sort(hi), lo, "left") - 1
+        beat = ncomp - np.searchsorted(np.sort(lo), hi, "right")
+        sure_in, sure_out = reach < keep, beat >= keep
+        if not (sure_in | sure_out).all() or sure_in.sum() != min(keep, ncomp):
+            return None
+        return sure_in
 
     def _cleaning_signs(self, coords, dirs, shape):
         """Ridge directions good for the cleaning scores, which only need
@@ -210,19 +249,30 @@ class AdvancedAdjuster(TreePrintable):
         vector negates its dot products exactly), so the forest's signs
         suffice, unless a forest edge's dot product is zero or NaN (such a
         decision keeps the child whatever its parent's sign, so it depends on
-        the direction the tree is walked); then, and for anisotropic voxels,
-        the full tree is used. The products are formed as propagate_signs
-        forms them: float32, ((a0 b0 + a1 b1) + a2 b2)."""
+        the direction the tree is walked). Returns (coords, signed directions,
+        parts): parts is None when the signs are the tree's up to one flip
+        per segment; with zero products it labels each point's piece between
+        them (the pieces' relative signs stay open, see _kept_segments). A
+        NaN product, or an anisotropic voxel size, takes the full tree. The
+        products are formed as propagate_signs forms them: float32,
+        ((a0 b0 + a1 b1) + a2 b2)."""
         vs = self.voxel_size
         if len(coords) > ridge_graph.DENSE_LIMIT and vs[0] == vs[1] == vs[2]:
             edges = ridge_graph.ridge_forest_edges(coords, shape, vs)
             d = np.require(dirs, np.float32).reshape(-1, 3)
             a, b = d[edges[:, 0]], d[edges[:, 1]]
             dots = (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
-            if not (dots == 0).any() and not np.isnan(dots).any():
-                return (np.require(coords, np.uint16, ["C"]),
-                        ridge_graph.propagate_signs(dirs, edges, 0))
-        return self._tree_signs(coords, dirs, shape)
+            if not np.isnan(dots).any():
+                signs = ridge_graph.propagate_signs(dirs, edges, 0)
+                parts = None
+                zero = dots == 0
+                if zero.any():
+                    e = edges[~zero]
+                    graph = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])),
+                                       shape=(len(d), len(d)))
+                    parts = connected_components(graph, directed=False)[1]
+                return np.require(coords, np.uint16, ["C"]), signs, parts
+        return self._tree_signs(coords, dirs, shape) + (None,)
 
     def _tree_signs(self, coords, dirs, shape):
         if len(coords) == 0:
@@ -258,31 +308,31 @@ class AdvancedAdjuster(TreePrintable):
         distances, adjust_directions.py:365-417); signs propagated from ridge
         voxel 0 so that every tree edge has a non-negative dot product.
         Returns (uint16 coords R x 3 in C order, float32 adjusted dirs).
-        for_cleaning: the directions only feed the cleaning scores."""
+        for_cleaning: the directions only feed the cleaning scores, and the
+        pieces of _cleaning_signs come third."""
         coords = np.argwhere(labeled_ridge != 0)
         dirs = main_dirs[tuple(coords.T)] if len(coords) else None
-        if for_cleaning and len(coords):
-            return self._cleaning_signs(coords, dirs, labeled_ridge.shape)
+        if for_cleaning:
+            if len(coords):
+                return self._cleaning_signs(coords, dirs, labeled_ridge.shape)
+            return self._tree_signs(coords, dirs, labeled_ridge.shape) + (None,)
         return self._tree_signs(coords, dirs, labeled_ridge.shape)
 
-    def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs):
+    def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs, parts=None):
         """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed
-        (voxel-size scaled) directions are longest."""
+        (voxel-size scaled) directions are longest (parts: see
+        _cleaning_signs)."""
         comps, ncomp = label(labeled_ridge, structure=np.ones((3, 3, 3), dtype=int))
         if ncomp == 0:
             self.__ridge = labeled_ridge
             return labeled_ridge
-        scaled = ridge_dirs * np.asarray(self.voxel_size)
-        lab_of = comps[tuple(np.asarray(ridge_coords, dtype=np.intp).T)]
-        sums = np.zeros((ncomp + 1, 3))
-        for k in range(3):
-            sums[:, k] = np.bincount(lab_of, weights=scaled[:, k], minlength=ncomp + 1)
-        lengths = np.linalg.norm(sums[1:], axis=1)
-        order = np.argsort(-lengths, kind="stable") + 1
-        keep = int(np.ceil(3 * np.log(np.e + ncomp)))
-        if getattr(self, "_verbose", False):
-            print("Keeping %s of %s labels." % (min(keep, ncomp), ncomp))
-        cleaned = np.isin(comps, order[:keep]).astype(labeled_ridge.dtype)
+        lab_of = comps[tuple(np.asarray(ridge_coords, dtype=np.intp).T)] - 1
+        kept = self._kept_segments(lab_of, ncomp, ridge_dirs, parts)
+        if kept is None:              # the forest's bounds did not decide: the full tree
+            _, tree_dirs = self._tree_signs(np.asarray(ridge_coords, np.int64), ridge_dirs,
+                                            labeled_ridge.shape)
+            kept = self._kept_segments(lab_of, ncomp, tree_dirs)
+        cleaned = np.isin(comps, np.nonzero(kept)[0] + 1).astype(labeled_ridge.dtype)
         self.__ridge = cleaned
         return cleaned
 

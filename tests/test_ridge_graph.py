@@ -114,7 +114,9 @@ def _segments(rng, shape, count, pair=None):
     """Random 26-connected ridge segments (random walks) and random
     directions; with pair "zero", one segment is two neighbours with
     orthogonal axis directions, with "tiny" two neighbours whose float32 dot
-    product is 1e-7 (small, but its sign flips with the parent's)."""
+    product is 1e-7 (small, but its sign flips with the parent's); with
+    "straddle", one segment is a straight chain of 40 voxels whose halves
+    point along two orthogonal axes (one zero product, pieces of 20)."""
     ridge = np.zeros(shape, np.float32)
     for _ in range(count):
         p = rng.integers(2, np.array(shape) - 2)
@@ -123,19 +125,27 @@ def _segments(rng, shape, count, pair=None):
             p = np.clip(p + rng.integers(-1, 2, size=3), 0, np.array(shape) - 1)
     dirs = rng.normal(size=shape + (3,)).astype(np.float32)
     dirs /= np.linalg.norm(dirs, axis=-1, keepdims=True)
-    if pair:                          # a segment of two voxels, one forest edge
+    if pair in ("zero", "tiny"):      # a segment of two voxels, one forest edge
         ridge[:4, :4, :4] = 0
         ridge[1, 1, 1] = ridge[1, 1, 2] = 1
         dirs[1, 1, 1] = (1, 0, 0)
         dirs[1, 1, 2] = (0, 1, 0) if pair == "zero" else (1e-7, 1, 0)
+    if pair == "straddle":
+        ridge[:, :4, :4] = 0
+        ridge[5:45, 1, 1] = 1
+        dirs[5:25, 1, 1] = (0, 0, 1)
+        dirs[25:45, 1, 1] = (0, 1, 0)
     return ridge, dirs
 
 
-@pytest.mark.parametrize("seed,pair", [(1, None), (2, None), (3, "zero"), (4, "tiny")])
+@pytest.mark.parametrize("seed,pair", [(1, None), (2, None), (3, "zero"), (4, "tiny"),
+                                       (5, "straddle")])
 def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, pair):
     """The cleaning keeps the same segments whether its scores come from the
-    neighbour forest's signs or from the full spanning tree's; an exactly
-    zero forest product falls back to the tree, a tiny one does not."""
+    neighbour forest's signs or from the full spanning tree's. A tiny forest
+    product keeps the forest's signs; an exactly zero one leaves its pieces'
+    relative sign open, which the score bounds settle for a short segment
+    (no tree) and not for one whose bounds straddle the cut (the tree)."""
     from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
     monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
     rng = np.random.default_rng(seed)
@@ -147,14 +157,13 @@ def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, pair):
     tree_edges = ridge_graph.ridge_tree_edges
     monkeypatch.setattr(ridge_graph, "ridge_tree_edges",
                         lambda *a: trees.append(1) or tree_edges(*a))
-    c_forest, d_forest = adj._adjust_ridge_directions(ridge, dirs, for_cleaning=True)
-    assert len(trees) == (1 if pair == "zero" else 0)
+    c_forest, d_forest, parts = adj._adjust_ridge_directions(ridge, dirs, for_cleaning=True)
+    assert (parts is None) == (pair not in ("zero", "straddle"))
     np.testing.assert_array_equal(c_full, c_forest)
     np.testing.assert_array_equal(np.abs(d_full), np.abs(d_forest))
-    if pair == "zero":
-        np.testing.assert_array_equal(d_full, d_forest)       # fell back to the tree
     keep_full = adj._do_clean_ridge(ridge, c_full, d_full)
-    keep_forest = adj._do_clean_ridge(ridge, c_forest, d_forest)
+    keep_forest = adj._do_clean_ridge(ridge, c_forest, d_forest, parts)
+    assert len(trees) == (1 if pair == "straddle" else 0)
     assert 0 < keep_full.sum() < ridge.sum()
     np.testing.assert_array_equal(keep_full, keep_forest)
 
@@ -184,3 +193,44 @@ def test_lattice_tree_rejects_bad_coordinates(monkeypatch):
         ridge_graph.ridge_tree_edges(np.array([[0, 0, 0], [0, 0, 0]]), (2, 2, 2), 1)
     with pytest.raises(RuntimeError):
         ridge_graph.ridge_tree_edges(np.array([[0, 0, 0], [0, 0, 2]]), (2, 2, 2), 1)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_kept_segments_bounds_hold_for_every_piece_sign(seed):
+    """When the piece bounds decide the cleaning, the kept segments are the
+    top ceil(3 ln(e + n)) (stable order) under every choice of the pieces'
+    relative signs, checked by enumerating them; when some choice would
+    change the set, the bounds must not decide."""
+    import itertools
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    rng = np.random.default_rng(seed)
+    adj = AdvancedAdjuster((0.01, 0.3), None, None, voxel_size=1, loader=_NoDevice())
+    ncomp = 14
+    npieces = rng.integers(1, 4, size=ncomp)
+    npieces[rng.choice(ncomp, size=ncomp - 4, replace=False)] = 1   # at most 4 open segments
+    seg_of, vecs = [], []
+    for s_, k in enumerate(npieces):
+        scale = 10 ** rng.uniform(-1, 2)
+        for j in range(k):            # open segments: a leading piece, smaller others
+            seg_of.append(s_)
+            vecs.append(rng.normal(size=3) * scale * (1.0 if j == 0 else rng.uniform(0.01, 0.6)))
+    lab_of, vecs = np.array(seg_of), np.array(vecs)
+    parts = np.arange(len(vecs))                 # one point per piece
+    keep = int(np.ceil(3 * np.log(np.e + ncomp)))
+    open_pieces = [i for i in range(len(vecs)) if npieces[lab_of[i]] > 1
+                   and i != np.nonzero(lab_of == lab_of[i])[0][0]]
+    sets = set()
+    for flips in itertools.product((1.0, -1.0), repeat=len(open_pieces)):
+        v = vecs.copy()
+        v[open_pieces] *= np.array(flips)[:, None]
+        lengths = np.linalg.norm(np.stack([np.bincount(lab_of, v[:, k], ncomp)
+                                           for k in range(3)], 1), axis=1)
+        order = np.argsort(-lengths, kind="stable")
+        sets.add(tuple(sorted(order[:keep])))
+    got = adj._kept_segments(lab_of, ncomp, vecs, parts)
+    if got is not None:
+        assert sets == {tuple(np.nonzero(got)[0])}
+    # the bounds decide whenever the set cannot change, up to their slack:
+    # with no open segment they always do
+    if len(open_pieces) == 0:
+        assert got is not None
