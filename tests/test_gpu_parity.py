@@ -619,6 +619,79 @@ def test_two_level_sweep_equals_plain_kernels(dev, rng, shape, iters, update, bo
         np.testing.assert_allclose(out[1][3], out[mode][3], rtol=1e-5)
 
 
+def _device_cut_case(dev, shape, seed=3):
+    """A helix phantom, its centre-line frames and its capacities built on
+    the device (full-size volumes without host-side generation)."""
+    import ctypes
+    from miccai2016_amd import _lib
+    L, h, st = dev.lib, dev.handle, dev.stream
+    n = int(np.prod(shape))
+    img, q = dev.empty(n), dev.empty(3 * n)
+    dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), 6.0, shape[1] / 4, 48.0, 1.0,
+                                 ctypes.c_uint64(seed), dev.ptr(img), None, dev.ptr(q), st))
+    dev.check(L.mh_complete_frames(h, dev.ptr(q), dev.ptr(q), n, st))
+    cs, ct, gms, a, u0 = (dev.empty(n) for _ in range(5))
+    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, 1.0, 1e-9, dev.ptr(cs), dev.ptr(ct), st))
+    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(shape), dev.ptr(gms), st))
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(gms), n, 10 ** -1.625, 10 ** 1.5, dev.ptr(a), st))
+    wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
+    dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), st))
+    return q, cs, ct, gms, a, u0
+
+
+@pytest.mark.parametrize("side,tile", [(128, 1), (256, 1), (256, 3)])
+def test_sweep_equals_plain_kernels_at_full_size(dev, side, tile):
+    """At the full volume sizes of the grid (128^3) and of the plain solve
+    (256^3) the two-level sweep computes the plain kernels' iterations bit
+    for bit (single- and two-level passes, criterion checks included)."""
+    shape = (side,) * 3
+    q, cs, ct, _, a, u0 = _device_cut_case(dev, shape)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    outs = []
+    try:
+        for mode, t in ((1, 0), (2, tile)):
+            dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, mode, t, 0))
+            ud = u0.clone()
+            c = _cutter(dev, 21, update=7)
+            c.execute_device(ud, q, a, cs, ct, const_sqevals=np.sqrt(eigs), shape=shape)
+            outs.append((ud, c.last_result.iterations))
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    dev.torch.cuda.synchronize()
+    assert outs[0][1] == outs[1][1] == 21
+    assert dev.torch.equal(outs[0][0], outs[1][0])
+
+
+def test_batched_pairs_equal_lone_solves_at_full_size(dev):
+    """The grid's volume size (128^3): nine solves through the batched sweep
+    (four pairs advanced two per thread and a lone ninth, each group's state
+    in the workspace's compact block) equal the same solves run one at a
+    time, bit for bit."""
+    shape = (128, 128, 128)
+    n = int(np.prod(shape))
+    q, cs, ct, gms, _, u0 = _device_cut_case(dev, shape, seed=5)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    S = 9
+    w = np.geomspace(0.01, 3.0, S).astype(np.float32)
+    d = np.geomspace(1.0, 1000.0, S)[::-1].astype(np.float32)
+    torch = dev.torch
+    try:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, 1, 0))
+        ub = u0.reshape(1, -1).repeat(S, 1)
+        _cutter(dev, 33, update=7).execute_batched_device(ub, S, w, d, q, gms, cs, ct, shape, eigs)
+        a = dev.empty(n)
+        for k in range(S):
+            dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(gms), n, float(w[k]),
+                                                    float(d[k]), dev.ptr(a), dev.stream))
+            ud = u0.clone()
+            _cutter(dev, 33, update=7).execute_device(ud, q, a, cs, ct,
+                                                      const_sqevals=np.sqrt(eigs), shape=shape)
+            torch.cuda.synchronize()
+            assert torch.equal(ub[k], ud.reshape(-1)), k
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+
+
 @pytest.mark.parametrize("shape,iters,update,bound,labelled", [
     ((20, 17, 23), 200, 7, 0.0, False), ((16, 16, 16), 3000, 7, 1e-3, False),
     ((12, 14, 16), 95, 30, 0.0, True), ((5, 6, 7), 64, 1, 0.0, False)])
