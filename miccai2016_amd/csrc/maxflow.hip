@@ -322,10 +322,13 @@ __device__ __forceinline__ void sweep_load(SweepIn (&L)[NS], SweepInShared &C, c
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const unsigned zs = zb + unsigned(j) * nb, zh = zb + 3u * unsigned(j) * nb;
+            // the forward neighbours only inside the volume (the uniform plane
+            // offset is outside the descriptor's range check); the p-step
+            // masks the faces with the same conditions
             L[j].t = bld(R.tmp, ob, zs);
-            L[j].t0 = e0 ? bld(R.tmp, ob, zs + pb) : 0.f;
-            L[j].t1 = (f & 16u) ? bld(R.tmp, ob + d2b, zs) : 0.f;
-            L[j].t2 = (f & 32u) ? bld(R.tmp, ob + 4u, zs) : 0.f;
+            if (e0) L[j].t0 = bld(R.tmp, ob, zs + pb);
+            if (f & 16u) L[j].t1 = bld(R.tmp, ob + d2b, zs);
+            if (f & 32u) L[j].t2 = bld(R.tmp, ob + 4u, zs);
             L[j].h0 = bld(R.hp, ob, zh);
             L[j].h1 = bld(R.hp, ob, zh + nb);
             L[j].h2 = bld(R.hp, ob, zh + 2u * nb);
@@ -514,20 +517,22 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         const bool e0b = z - 1 < D.d0 - 1, f0b = z - 1 > 0;  // plane z-1
         const unsigned zb = unsigned(z) * pb;  // plane z, bytes
         // 1. level-A p-step at plane z on the widened tile
+        // (lanes outside a phase's region keep values no valid lane reads)
         float hA0_0[NS], hA0_1[NS], hA0_2[NS];
         // the frame of plane z, built once for both levels and every solve
         const mh::Mat3 RN = mh::quat_to_mat(mh::Quat{LC.q.x, LC.q.y, LC.q.z, LC.q.w});
 #pragma unroll
-        for (int j = 0; j < NS; ++j) hA0_0[j] = hA0_1[j] = hA0_2[j] = 0.f;
+        for (int j = 0; j < NS; ++j) {
+            hA0_0[j] = L[j].h0;
+            hA0_1[j] = L[j].h1;
+            hA0_2[j] = L[j].h2;
+        }
         if (ph1) {
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
                 const float g0 = e0 ? (L[j].t0 - L[j].t) * PP.ss0 : 0.f;
                 const float g1 = (f & 16u) ? (L[j].t1 - L[j].t) * PP.ss1 : 0.f;
                 const float g2 = (f & 32u) ? (L[j].t2 - L[j].t) * PP.ss2 : 0.f;
-                hA0_0[j] = L[j].h0;
-                hA0_1[j] = L[j].h1;
-                hA0_2[j] = L[j].h2;
                 mh::cmf_p_voxel_m(hA0_0[j], hA0_1[j], hA0_2[j], g0, g1, g2, RN, PP.sq0, PP.sq1,
                                   PP.sq2, PP.rq0, PP.rq1, PP.rq2, L[j].a);
             }
@@ -553,17 +558,21 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         // 2. level-A u-step at plane z (stores when it is the only level)
         float tA0[NS], ptA0[NS], uA0[NS];
 #pragma unroll
-        for (int j = 0; j < NS; ++j) tA0[j] = ptA0[j] = uA0[j] = 0.f;
+        for (int j = 0; j < NS; ++j) {
+            tA0[j] = L[j].t;
+            ptA0[j] = L[j].pt;
+            uA0[j] = L[j].u;
+        }
         if (ph2) {
             float err[NS];
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
+                // E2 holds p - EX and p - 1; the faces mask what they read
+                const float n1 = sA1[j][p - EX], n2 = sA2[j][p - 1];
                 const float x0v = (e0 ? hA0_0[j] : 0.f) - (f0 ? hA1_0[j] : 0.f);
-                const float x1v = ((f & 16u) ? hA0_1[j] : 0.f) - ((f & 64u) ? sA1[j][p - EX] : 0.f);
-                const float x2v = ((f & 32u) ? hA0_2[j] : 0.f) - ((f & 128u) ? sA2[j][p - 1] : 0.f);
+                const float x1v = ((f & 16u) ? hA0_1[j] : 0.f) - ((f & 64u) ? n1 : 0.f);
+                const float x2v = ((f & 32u) ? hA0_2[j] : 0.f) - ((f & 128u) ? n2 : 0.f);
                 const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
-                ptA0[j] = L[j].pt;
-                uA0[j] = L[j].u;
                 err[j] = mh::cmf_u_voxel(div, ptA0[j], uA0[j], tA0[j], LC.cs, LC.ct, UP.rcc,
                                          UP.inv_rcc, 1.f, 0.f);
             }
@@ -589,16 +598,19 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         // 3. level-B p-step at plane z-1 on E1
         float hB1_0[NS], hB1_1[NS], hB1_2[NS];
 #pragma unroll
-        for (int j = 0; j < NS; ++j) hB1_0[j] = hB1_1[j] = hB1_2[j] = 0.f;
+        for (int j = 0; j < NS; ++j) {
+            hB1_0[j] = hA1_0[j];
+            hB1_1[j] = hA1_1[j];
+            hB1_2[j] = hA1_2[j];
+        }
         if (ph3) {
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
+                // E holds p + EX and p + 1 of every point of E1
+                const float t1 = sT[j][p + EX], t2 = sT[j][p + 1];
                 const float g0 = e0b ? (tA0[j] - tAp[j]) * PP.ss0 : 0.f;
-                const float g1 = (f & 16u) ? (sT[j][p + EX] - tAp[j]) * PP.ss1 : 0.f;
-                const float g2 = (f & 32u) ? (sT[j][p + 1] - tAp[j]) * PP.ss2 : 0.f;
-                hB1_0[j] = hA1_0[j];
-                hB1_1[j] = hA1_1[j];
-                hB1_2[j] = hA1_2[j];
+                const float g1 = (f & 16u) ? (t1 - tAp[j]) * PP.ss1 : 0.f;
+                const float g2 = (f & 32u) ? (t2 - tAp[j]) * PP.ss2 : 0.f;
                 mh::cmf_p_voxel_m(hB1_0[j], hB1_1[j], hB1_2[j], g0, g1, g2, RP, PP.sq0, PP.sq1,
                                   PP.sq2, PP.rq0, PP.rq1, PP.rq2, aP[j]);
             }
@@ -617,9 +629,10 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             float pti[NS], ui[NS], ti[NS], err[NS];
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
+                const float n1 = sB1[j][p - EX], n2 = sB2[j][p - 1];  // O: inside E1
                 const float x0v = (e0b ? hB1_0[j] : 0.f) - (f0b ? hB2_0[j] : 0.f);
-                const float x1v = ((f & 16u) ? hB1_1[j] : 0.f) - ((f & 64u) ? sB1[j][p - EX] : 0.f);
-                const float x2v = ((f & 32u) ? hB1_2[j] : 0.f) - ((f & 128u) ? sB2[j][p - 1] : 0.f);
+                const float x1v = ((f & 16u) ? hB1_1[j] : 0.f) - ((f & 64u) ? n1 : 0.f);
+                const float x2v = ((f & 32u) ? hB1_2[j] : 0.f) - ((f & 128u) ? n2 : 0.f);
                 const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
                 pti[j] = ptAp[j];
                 ui[j] = uAp[j];
