@@ -194,7 +194,7 @@ class CmfCutQAnisotropicGpu(TreePrintable):
         return u
 
     def execute_ragged(self, imgs, quats, eigs, alphas, css, cts, scaling=None,
-                       sweep_min_voxels=1 << 20):
+                       sweep_min_voxels=1 << 19):
         """One unsupervised solve per volume, volumes of different shapes packed
         into one launch sequence (volumes of sweep_min_voxels or more on their
         own, see execute_ragged_device). Lists in, list of u arrays out."""
@@ -214,7 +214,7 @@ class CmfCutQAnisotropicGpu(TreePrintable):
         return [u[o:o + sz].reshape(s) for o, sz, s in zip(offs, sizes, shapes)]
 
     def execute_ragged_device(self, u, quats, alpha, cs, ct, shapes, offsets, eigs, scaling=None,
-                              sweep_min_voxels=1 << 20):
+                              sweep_min_voxels=1 << 19):
         """Volumes of at least sweep_min_voxels voxels are solved one at a
         time with the tuned single-solve method (the two-level sweep where it
         wins), the others together in one packed launch sequence; both give
