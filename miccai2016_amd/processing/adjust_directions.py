@@ -154,7 +154,10 @@ class AdvancedAdjuster(TreePrintable):
         # ridge voxels in C order, their segments and main directions
         lin_t = torch.nonzero(d_ridge).reshape(-1)
         lin = dev.download(lin_t)
-        seg = dev.download(labels.index_select(0, lin_t))
+        # each ridge point's segment as an index into the sorted segment labels
+        roots_t, lab_t = torch.unique(labels.index_select(0, lin_t), sorted=True,
+                                      return_inverse=True)
+        seg = (len(roots_t), dev.download(lab_t))
         dirs = dev.download(self._d_qeigs.reshape(n, 3).index_select(0, lin_t))
         del labels, work, d_ridge
         coords = np.stack(np.unravel_index(lin, shape), 1) if len(lin) else np.zeros((0, 3), np.int64)
@@ -186,10 +189,15 @@ class AdvancedAdjuster(TreePrintable):
     def _segments_to_keep(self, seg, ridge_dirs, parts=None):
         """_do_clean_ridge on segment labels: seg holds each ridge point's
         segment (the segment's smallest voxel index, so increasing labels are
-        scipy's label numbering). Returns the points of the kept segments, or
-        None when `parts` leaves the choice open (see _kept_segments)."""
-        roots, lab_of = np.unique(seg, return_inverse=True)
-        ncomp = len(roots)
+        scipy's label numbering), or is (number of segments, each point's
+        index into the sorted labels). Returns the points of the kept
+        segments, or None when `parts` leaves the choice open (see
+        _kept_segments)."""
+        if isinstance(seg, tuple):
+            ncomp, lab_of = seg
+        else:
+            roots, lab_of = np.unique(seg, return_inverse=True)
+            ncomp = len(roots)
         if ncomp == 0:
             return np.zeros(0, bool)
         kept = self._kept_segments(lab_of, ncomp, ridge_dirs, parts)
@@ -199,7 +207,8 @@ class AdvancedAdjuster(TreePrintable):
         """Mask over the ncomp segments: the ceil(3 ln(e + n)) whose summed
         (voxel-size scaled) directions are longest, the lower label first on
         ties (adjust_directions.py:290-339). With `parts` (each point's piece
-        of its segment between zero-product forest edges, whose relative
+        of its segment between zero-product forest edges, labelled 0 .. P-1,
+        whose relative
         signs the forest leaves open) a segment's length is only known to lie
         in [2 max|P| - sum|P|, sum|P|] over its pieces' sums P (triangle
         inequality); the mask is returned when those bounds settle every
@@ -217,12 +226,13 @@ class AdvancedAdjuster(TreePrintable):
             kept = np.zeros(ncomp, bool)
             kept[order[:keep]] = True
             return kept
-        pid, p_of = np.unique(parts, return_inverse=True)
-        psum = np.zeros((len(pid), 3))
+        p_of = np.asarray(parts, np.int64)           # piece labels 0 .. P-1
+        npiece = int(p_of.max()) + 1                 # (an unused label adds a zero piece)
+        psum = np.zeros((npiece, 3))
         for k in range(3):
-            psum[:, k] = np.bincount(p_of, weights=scaled[:, k], minlength=len(pid))
+            psum[:, k] = np.bincount(p_of, weights=scaled[:, k], minlength=npiece)
         plen = np.linalg.norm(psum, axis=1)
-        pseg = np.zeros(len(pid), np.int64)
+        pseg = np.zeros(npiece, np.int64)
         pseg[p_of] = lab_of                          # a piece lies within one segment
         tot = np.bincount(pseg, weights=plen, minlength=ncomp)
         big = np.zeros(ncomp)
