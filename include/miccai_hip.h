@@ -184,6 +184,17 @@ int mh_ridge_tree_host(const int64_t *coords_host, int64_t n, const int64_t dims
  * scores need no more than this (processing/adjust_directions.py:290-339). */
 int64_t mh_ridge_forest_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
                              const double voxel_size[3], int64_t *edges_host);
+/* Host: the pieces of a forest between its zero-product edges and signed
+ * directions within each piece (every non-zero edge joins points whose
+ * signed directions have a non-negative product; a piece's smallest point
+ * keeps its direction), by a union-find carrying sign parities. pieces_host:
+ * n labels 0 .. P-1; counts_host[0]: zero products, counts_host[1]: NaN
+ * products (then the tree walk itself is needed). Feeds the cleaning scores
+ * of _do_clean_ridge (processing/adjust_directions.py:290-339), which only
+ * need the walk's signs up to one flip per piece. n < 2^31. */
+int mh_forest_pieces_host(const float *dirs_host, int64_t n, const int64_t *edges_host,
+                          int64_t nedges, float *out_host, int32_t *pieces_host,
+                          int64_t counts_host[2]);
 /* Host: directions (n x 3 floats) with signs propagated along a tree from
  * root: a child keeps its direction when its dot product with its adjusted
  * parent is >= 0, else flips (NaN flips). Trees of a forest not reached from

@@ -532,6 +532,86 @@ MH_EXPORT int64_t mh_ridge_forest_host(const int64_t *coords, int64_t n, const i
     return rc ? rc : ne;
 }
 
+// Pieces of a forest between its edges whose direction product is zero, and
+// signs within each piece: a union-find whose links carry the parity of the
+// product (negative: opposite signs), so that every edge of a piece joins
+// points whose signed directions have a non-negative product, as the
+// breadth-first walk below makes them. Within a tree without zero products
+// the result is that walk's up to one flip of the whole tree. Products are
+// formed as the walk forms them (dot3). counts[0]: zero products, counts[1]:
+// NaN products (the caller then needs the walk itself). Pieces are labelled
+// 0 .. P-1 in order of their smallest point.
+MH_EXPORT int mh_forest_pieces_host(const float *dirs, int64_t n, const int64_t *edges, int64_t ne,
+                                    float *out, int32_t *pieces, int64_t counts[2])
+{
+    if (n < 0 || ne < 0 || (n > 0 && (!dirs || !out || !pieces)) || (ne > 0 && !edges) || !counts)
+        return MH_ERR_ARG;
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    counts[0] = counts[1] = 0;
+    std::vector<int32_t> parent(static_cast<size_t>(n));
+    std::vector<uint8_t> par(static_cast<size_t>(n), 0);  // parity to the parent
+    for (int64_t i = 0; i < n; ++i) parent[size_t(i)] = int32_t(i);
+    // root of x, with x's parity to it; compresses the path
+    auto find = [&](int32_t x, uint8_t &px) {
+        int32_t r = x;
+        uint8_t p = 0;
+        while (parent[size_t(r)] != r) {
+            p ^= par[size_t(r)];
+            r = parent[size_t(r)];
+        }
+        px = p;
+        // second pass: point every node on the path at r with its parity
+        uint8_t q = p;
+        while (parent[size_t(x)] != r && parent[size_t(x)] != x) {
+            const int32_t nx = parent[size_t(x)];
+            const uint8_t pnx = q ^ par[size_t(x)];
+            parent[size_t(x)] = r;
+            par[size_t(x)] = q;
+            x = nx;
+            q = pnx;
+        }
+        return r;
+    };
+    for (int64_t k = 0; k < ne; ++k) {
+        const int64_t a = edges[2 * k], b = edges[2 * k + 1];
+        if (a < 0 || a >= n || b < 0 || b >= n) return MH_ERR_ARG;
+        const float d = dot3(dirs + 3 * a, dirs + 3 * b);
+        if (d != d) {
+            ++counts[1];
+            continue;
+        }
+        if (d == 0.f) {
+            ++counts[0];
+            continue;
+        }
+        uint8_t pa, pb;
+        const int32_t ra = find(int32_t(a), pa), rb = find(int32_t(b), pb);
+        if (ra == rb) continue;  // not a forest edge after all: keep the first link
+        // parity(a, b) = (d < 0) = pa ^ pb ^ parity(ra, rb)
+        const uint8_t link = uint8_t((d < 0.f) ? 1 : 0) ^ pa ^ pb;
+        if (ra < rb) {  // the smaller index stays the root
+            parent[size_t(rb)] = ra;
+            par[size_t(rb)] = link;
+        } else {
+            parent[size_t(ra)] = rb;
+            par[size_t(ra)] = link;
+        }
+    }
+    std::vector<int32_t> label(static_cast<size_t>(n), -1);
+    int32_t next = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint8_t p;
+        const int32_t r = find(int32_t(i), p);
+        if (label[size_t(r)] < 0) label[size_t(r)] = next++;
+        pieces[i] = label[size_t(r)];
+        const float s = p ? -1.f : 1.f;  // the root keeps its direction
+        out[3 * i] = s * dirs[3 * i];
+        out[3 * i + 1] = s * dirs[3 * i + 1];
+        out[3 * i + 2] = s * dirs[3 * i + 2];
+    }
+    return MH_OK;
+}
+
 // Signs along the trees of a forest: breadth first from `root`, then from
 // every point not reached yet, in index order (such a point keeps its own
 // direction). A child keeps its direction when its dot product with its

@@ -10,8 +10,6 @@ into squeezed frames (GPU).
 """
 import numpy as np
 from scipy.ndimage import label
-from scipy.sparse import coo_matrix
-from scipy.sparse.csgraph import connected_components
 
 from .. import _lib
 from ..device import default_context
@@ -264,24 +262,16 @@ class AdvancedAdjuster(TreePrintable):
         per segment; with zero products it labels each point's piece between
         them (the pieces' relative signs stay open, see _kept_segments). A
         NaN product, or an anisotropic voxel size, takes the full tree. The
-        products are formed as propagate_signs forms them: float32,
-        ((a0 b0 + a1 b1) + a2 b2)."""
+        forest's signs and pieces come from one union-find pass
+        (ridge_graph.forest_pieces), whose products are formed as
+        propagate_signs forms them: float32, ((a0 b0 + a1 b1) + a2 b2)."""
         vs = self.voxel_size
         if len(coords) > ridge_graph.DENSE_LIMIT and vs[0] == vs[1] == vs[2]:
             edges = ridge_graph.ridge_forest_edges(coords, shape, vs)
-            d = np.require(dirs, np.float32).reshape(-1, 3)
-            a, b = d[edges[:, 0]], d[edges[:, 1]]
-            dots = (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
-            if not np.isnan(dots).any():
-                signs = ridge_graph.propagate_signs(dirs, edges, 0)
-                parts = None
-                zero = dots == 0
-                if zero.any():
-                    e = edges[~zero]
-                    graph = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])),
-                                       shape=(len(d), len(d)))
-                    parts = connected_components(graph, directed=False)[1]
-                return np.require(coords, np.uint16, ["C"]), signs, parts
+            signs, pieces, nzero, nnan = ridge_graph.forest_pieces(dirs, edges)
+            if not nnan:
+                return (np.require(coords, np.uint16, ["C"]), signs,
+                        pieces if nzero else None)
         return self._tree_signs(coords, dirs, shape) + (None,)
 
     def _tree_signs(self, coords, dirs, shape):

@@ -79,6 +79,26 @@ def ridge_forest_edges(coords, shape, voxel_size):
     return edges[:ne]
 
 
+def forest_pieces(dirs, edges):
+    """Pieces of a forest between its zero-product edges and signs within
+    them (mh_forest_pieces_host): (signed float32 directions, int32 piece
+    labels 0 .. P-1, number of zero products, number of NaN products). In a
+    tree without zero or NaN products the signs are propagate_signs' up to
+    one flip of the whole tree."""
+    d32 = np.require(dirs, np.float32, ["C"]).reshape(-1, 3)
+    n = len(d32)
+    out = np.empty((n, 3), np.float32)
+    pieces = np.empty(n, np.int32)
+    counts = np.zeros(2, np.int64)
+    e = np.require(edges, np.int64, ["C"]).reshape(-1, 2)
+    rc = _lib.lib().mh_forest_pieces_host(d32.ctypes.data, n, e.ctypes.data, len(e),
+                                          out.ctypes.data, pieces.ctypes.data,
+                                          counts.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"mh_forest_pieces_host failed (status {rc})")
+    return out, pieces, int(counts[0]), int(counts[1])
+
+
 def propagate_signs(dirs, edges, root=0):
     """Signs along the tree from `root`: a child keeps its direction if its
     dot product with its (already adjusted) parent is >= 0, else flips

@@ -234,3 +234,54 @@ def test_kept_segments_bounds_hold_for_every_piece_sign(seed):
     # with no open segment they always do
     if len(open_pieces) == 0:
         assert got is not None
+
+
+@pytest.mark.parametrize("seed,zeros,nans", [(1, 0, 0), (2, 5, 0), (3, 40, 0), (4, 3, 1)])
+def test_forest_pieces_against_the_walk_and_scipy(seed, zeros, nans):
+    """The union-find pieces and signs: without zero products the signs are
+    the breadth-first walk's up to one flip per tree; every non-zero edge
+    joins points with non-negative signed products; the pieces are scipy's
+    connected components of the forest without its zero-product edges, in
+    order of their smallest point; zero and NaN products are counted."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    rng = np.random.default_rng(seed)
+    n = 3000
+    # a random forest: each point links to an earlier one, except tree roots
+    parent = np.array([-1] + [int(rng.integers(0, i)) if rng.random() > 0.02 else -1
+                              for i in range(1, n)])
+    child = np.nonzero(parent >= 0)[0]
+    edges = np.stack([parent[child], child], 1)
+    rng.shuffle(edges)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    for k in rng.choice(len(edges), size=zeros + nans, replace=False)[:zeros]:
+        a, b = edges[k]
+        dirs[b] = np.cross(dirs[a], rng.normal(size=3)).astype(np.float32)
+        dirs[b] -= dirs[a] * (np.dot(dirs[a], dirs[b]) / np.dot(dirs[a], dirs[a]))
+        dirs[b] = (0.0, 0.0, 0.0) if rng.random() < 0.5 else dirs[b]
+    if nans:
+        dirs[edges[rng.integers(len(edges)), 1]] = (np.nan, 0.0, 0.0)
+    a32, b32 = dirs[edges[:, 0]], dirs[edges[:, 1]]
+    dots = (a32[:, 0] * b32[:, 0] + a32[:, 1] * b32[:, 1]) + a32[:, 2] * b32[:, 2]
+    signs, pieces, nzero, nnan = ridge_graph.forest_pieces(dirs, edges)
+    assert nzero == int((dots == 0).sum()) and nnan == int(np.isnan(dots).sum())
+    keep = ~((dots == 0) | np.isnan(dots))
+    e = edges[keep]
+    ncc, lab = connected_components(coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])),
+                                               shape=(n, n)), directed=False)
+    # scipy's labels renumbered by smallest point equal ours
+    first = np.full(ncc, n)
+    np.minimum.at(first, lab, np.arange(n))
+    order = np.argsort(first)
+    renum = np.empty(ncc, np.int64)
+    renum[order] = np.arange(ncc)
+    np.testing.assert_array_equal(pieces, renum[lab])
+    s = signs[e[:, 0]], signs[e[:, 1]]
+    prod = (s[0][:, 0] * s[1][:, 0] + s[0][:, 1] * s[1][:, 1]) + s[0][:, 2] * s[1][:, 2]
+    assert (prod >= 0).all()
+    np.testing.assert_array_equal(np.abs(signs), np.abs(dirs))
+    if zeros == 0 and nans == 0:
+        walk = ridge_graph.propagate_signs(dirs, edges, 0)
+        flip = np.sign((walk * signs).sum(1))       # +-1 per point, constant per tree
+        assert (flip[edges[:, 0]] == flip[edges[:, 1]]).all()
+        np.testing.assert_array_equal(walk, signs * flip[:, None].astype(np.float32))
