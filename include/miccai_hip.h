@@ -191,7 +191,9 @@ int64_t mh_ridge_forest_host(const int64_t *coords_host, int64_t n, const int64_
  * n labels 0 .. P-1; counts_host[0]: zero products, counts_host[1]: NaN
  * products (then the tree walk itself is needed). Feeds the cleaning scores
  * of _do_clean_ridge (processing/adjust_directions.py:290-339), which only
- * need the walk's signs up to one flip per piece. n < 2^31. */
+ * need the walk's signs up to one flip per piece; without zero or NaN
+ * products the signs are mh_tree_signs_host's from root 0 bit for bit.
+ * n < 2^31. */
 int mh_forest_pieces_host(const float *dirs_host, int64_t n, const int64_t *edges_host,
                           int64_t nedges, float *out_host, int32_t *pieces_host,
                           int64_t counts_host[2]);
@@ -199,7 +201,9 @@ int mh_forest_pieces_host(const float *dirs_host, int64_t n, const int64_t *edge
  * root: a child keeps its direction when its dot product with its adjusted
  * parent is >= 0, else flips (NaN flips). Trees of a forest not reached from
  * root start at their smallest index, which keeps its direction. Replaces
- * the traversal of _adjust_ridge_directions (processing/adjust_directions.py:384-417). */
+ * the traversal of _adjust_ridge_directions (processing/adjust_directions.py:384-417),
+ * computed without walking the points: parity sets within the pieces between
+ * zero and NaN products, a walk over the pieces for their anchors. n < 2^31. */
 int mh_tree_signs_host(const float *dirs_host, int64_t n, const int64_t *edges_host, int64_t nedges,
                        int64_t root, float *out_host);
 

@@ -532,27 +532,18 @@ MH_EXPORT int64_t mh_ridge_forest_host(const int64_t *coords, int64_t n, const i
     return rc ? rc : ne;
 }
 
-// Pieces of a forest between its edges whose direction product is zero, and
-// signs within each piece: a union-find whose links carry the parity of the
-// product (negative: opposite signs), so that every edge of a piece joins
-// points whose signed directions have a non-negative product, as the
-// breadth-first walk below makes them. Within a tree without zero products
-// the result is that walk's up to one flip of the whole tree. Products are
-// formed as the walk forms them (dot3). counts[0]: zero products, counts[1]:
-// NaN products (the caller then needs the walk itself). Pieces are labelled
-// 0 .. P-1 in order of their smallest point.
-MH_EXPORT int mh_forest_pieces_host(const float *dirs, int64_t n, const int64_t *edges, int64_t ne,
-                                    float *out, int32_t *pieces, int64_t counts[2])
-{
-    if (n < 0 || ne < 0 || (n > 0 && (!dirs || !out || !pieces)) || (ne > 0 && !edges) || !counts)
-        return MH_ERR_ARG;
-    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
-    counts[0] = counts[1] = 0;
-    std::vector<int32_t> parent(static_cast<size_t>(n));
-    std::vector<uint8_t> par(static_cast<size_t>(n), 0);  // parity to the parent
-    for (int64_t i = 0; i < n; ++i) parent[size_t(i)] = int32_t(i);
-    // root of x, with x's parity to it; compresses the path
-    auto find = [&](int32_t x, uint8_t &px) {
+// Disjoint sets whose links carry a sign parity: parity(x, y) = 1 when x and
+// y take opposite signs. A set's root is its smallest index.
+struct ParitySets {
+    std::vector<int32_t> parent;
+    std::vector<uint8_t> par;  // parity to the parent
+    explicit ParitySets(int64_t n) : parent(static_cast<size_t>(n)), par(static_cast<size_t>(n), 0)
+    {
+        for (int64_t i = 0; i < n; ++i) parent[size_t(i)] = int32_t(i);
+    }
+    // root of x, with x's parity to it; points the path at the root
+    int32_t find(int32_t x, uint8_t &px)
+    {
         int32_t r = x;
         uint8_t p = 0;
         while (parent[size_t(r)] != r) {
@@ -560,36 +551,25 @@ MH_EXPORT int mh_forest_pieces_host(const float *dirs, int64_t n, const int64_t 
             r = parent[size_t(r)];
         }
         px = p;
-        // second pass: point every node on the path at r with its parity
-        uint8_t q = p;
+        uint8_t q = p;  // parity of x to r, then of each next node on the path
         while (parent[size_t(x)] != r && parent[size_t(x)] != x) {
             const int32_t nx = parent[size_t(x)];
-            const uint8_t pnx = q ^ par[size_t(x)];
+            const uint8_t qn = q ^ par[size_t(x)];
             parent[size_t(x)] = r;
             par[size_t(x)] = q;
             x = nx;
-            q = pnx;
+            q = qn;
         }
         return r;
-    };
-    for (int64_t k = 0; k < ne; ++k) {
-        const int64_t a = edges[2 * k], b = edges[2 * k + 1];
-        if (a < 0 || a >= n || b < 0 || b >= n) return MH_ERR_ARG;
-        const float d = dot3(dirs + 3 * a, dirs + 3 * b);
-        if (d != d) {
-            ++counts[1];
-            continue;
-        }
-        if (d == 0.f) {
-            ++counts[0];
-            continue;
-        }
+    }
+    // joins the sets of a and b so that parity(a, b) = p (no-op once joined)
+    void join(int32_t a, int32_t b, uint8_t p)
+    {
         uint8_t pa, pb;
-        const int32_t ra = find(int32_t(a), pa), rb = find(int32_t(b), pb);
-        if (ra == rb) continue;  // not a forest edge after all: keep the first link
-        // parity(a, b) = (d < 0) = pa ^ pb ^ parity(ra, rb)
-        const uint8_t link = uint8_t((d < 0.f) ? 1 : 0) ^ pa ^ pb;
-        if (ra < rb) {  // the smaller index stays the root
+        const int32_t ra = find(a, pa), rb = find(b, pb);
+        if (ra == rb) return;
+        const uint8_t link = p ^ pa ^ pb;  // parity(ra, rb)
+        if (ra < rb) {
             parent[size_t(rb)] = ra;
             par[size_t(rb)] = link;
         } else {
@@ -597,69 +577,143 @@ MH_EXPORT int mh_forest_pieces_host(const float *dirs, int64_t n, const int64_t 
             par[size_t(ra)] = link;
         }
     }
-    std::vector<int32_t> label(static_cast<size_t>(n), -1);
-    int32_t next = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        uint8_t p;
-        const int32_t r = find(int32_t(i), p);
-        if (label[size_t(r)] < 0) label[size_t(r)] = next++;
-        pieces[i] = label[size_t(r)];
-        const float s = p ? -1.f : 1.f;  // the root keeps its direction
-        out[3 * i] = s * dirs[3 * i];
-        out[3 * i + 1] = s * dirs[3 * i + 1];
-        out[3 * i + 2] = s * dirs[3 * i + 2];
+};
+
+// The forest's edges with a non-zero, non-NaN direction product joined in
+// parity sets (negative product: opposite signs); the others returned as
+// cuts, with the sign the walk below gives the child across them (zero: the
+// child keeps its direction, NaN: it flips). Products are formed as the walk
+// forms them (dot3).
+struct Cut {
+    int32_t a, b;
+    int8_t sign;
+};
+int join_forest(const float *dirs, int64_t n, const int64_t *edges, int64_t ne, ParitySets &S,
+                std::vector<Cut> &cuts)
+{
+    for (int64_t k = 0; k < ne; ++k) {
+        const int64_t a = edges[2 * k], b = edges[2 * k + 1];
+        if (a < 0 || a >= n || b < 0 || b >= n) return MH_ERR_ARG;
+        const float d = dot3(dirs + 3 * a, dirs + 3 * b);
+        if (d != d) cuts.push_back(Cut{int32_t(a), int32_t(b), -1});
+        else if (d == 0.f) cuts.push_back(Cut{int32_t(a), int32_t(b), 1});
+        else S.join(int32_t(a), int32_t(b), uint8_t(d < 0.f ? 1 : 0));
     }
     return MH_OK;
 }
 
-// Signs along the trees of a forest: breadth first from `root`, then from
-// every point not reached yet, in index order (such a point keeps its own
-// direction). A child keeps its direction when its dot product with its
-// adjusted parent is >= 0 and flips otherwise (a NaN product flips).
+// Pieces of a forest between its edges whose direction product is zero or
+// NaN, and signs within each piece: every edge of a piece joins points whose
+// signed directions have a non-negative product, a piece's smallest point
+// keeping its direction. In a forest without zero or NaN products this is
+// the walk of mh_tree_signs_host from root 0, bit for bit. counts[0]: zero
+// products, counts[1]: NaN products. Pieces are labelled 0 .. P-1 in order
+// of their smallest point.
+MH_EXPORT int mh_forest_pieces_host(const float *dirs, int64_t n, const int64_t *edges, int64_t ne,
+                                    float *out, int32_t *pieces, int64_t counts[2])
+{
+    if (n < 0 || ne < 0 || (n > 0 && (!dirs || !out || !pieces)) || (ne > 0 && !edges) || !counts)
+        return MH_ERR_ARG;
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    ParitySets S(n);
+    std::vector<Cut> cuts;
+    const int rc = join_forest(dirs, n, edges, ne, S, cuts);
+    if (rc) return rc;
+    counts[0] = counts[1] = 0;
+    for (const Cut &c : cuts) ++counts[c.sign > 0 ? 0 : 1];
+    std::vector<int32_t> label(static_cast<size_t>(n), -1);
+    int32_t next = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint8_t p;
+        const int32_t r = S.find(int32_t(i), p);
+        if (label[size_t(r)] < 0) label[size_t(r)] = next++;
+        pieces[i] = label[size_t(r)];
+        const float sg = p ? -1.f : 1.f;
+        out[3 * i] = sg * dirs[3 * i];
+        out[3 * i + 1] = sg * dirs[3 * i + 1];
+        out[3 * i + 2] = sg * dirs[3 * i + 2];
+    }
+    return MH_OK;
+}
+
+// Signs along the trees of a forest, as a breadth-first walk from `root`
+// and then from every point not reached yet, in index order, gives them: the
+// start keeps its direction, and a child keeps its direction when its dot
+// product with its adjusted parent is >= 0 and flips otherwise (a NaN
+// product flips). Negating the parent negates that product exactly, so
+// within a piece between zero and NaN products every sign is the parity of
+// its path (parity sets), and the walk only decides each piece's anchor: the
+// start, or the child end of the zero or NaN edge it is entered through
+// (which keeps, or flips, its own direction). The pieces are walked instead
+// of the points.
 MH_EXPORT int mh_tree_signs_host(const float *dirs, int64_t n, const int64_t *edges, int64_t ne,
                                  int64_t root, float *out)
 {
     if (n < 0 || ne < 0 || (n > 0 && (!dirs || !out || root < 0 || root >= n)) || (ne > 0 && !edges))
         return MH_ERR_ARG;
     if (n == 0) return MH_OK;
-    std::vector<int64_t> start(static_cast<size_t>(n) + 1, 0), adj(static_cast<size_t>(2 * ne));
-    for (int64_t k = 0; k < 2 * ne; ++k) {
-        if (edges[k] < 0 || edges[k] >= n) return MH_ERR_ARG;
-        ++start[size_t(edges[k]) + 1];
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    ParitySets S(n);
+    std::vector<Cut> cuts;
+    const int rc = join_forest(dirs, n, edges, ne, S, cuts);
+    if (rc) return rc;
+    // the cuts as adjacency between pieces (keyed by their roots)
+    std::vector<int32_t> start(static_cast<size_t>(n) + 1, 0);
+    std::vector<int32_t> rootA(cuts.size()), rootB(cuts.size());
+    for (size_t k = 0; k < cuts.size(); ++k) {
+        uint8_t p;
+        rootA[k] = S.find(cuts[k].a, p);
+        rootB[k] = S.find(cuts[k].b, p);
+        ++start[size_t(rootA[k]) + 1];
+        ++start[size_t(rootB[k]) + 1];
     }
     for (int64_t i = 0; i < n; ++i) start[size_t(i) + 1] += start[size_t(i)];
-    std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    for (int64_t k = 0; k < ne; ++k) {
-        const int64_t a = edges[2 * k], b = edges[2 * k + 1];
-        adj[size_t(fill[size_t(a)]++)] = b;
-        adj[size_t(fill[size_t(b)]++)] = a;
+    struct Entry {
+        int32_t piece, entry;
+        int8_t sign;
+    };
+    std::vector<Entry> adj(2 * cuts.size());
+    {
+        std::vector<int32_t> fill(start.begin(), start.end() - 1);
+        for (size_t k = 0; k < cuts.size(); ++k) {
+            adj[size_t(fill[size_t(rootA[k])]++)] = Entry{rootB[k], cuts[k].b, cuts[k].sign};
+            adj[size_t(fill[size_t(rootB[k])]++)] = Entry{rootA[k], cuts[k].a, cuts[k].sign};
+        }
     }
-    std::vector<char> seen(static_cast<size_t>(n), 0);
-    std::vector<int64_t> queue;
-    queue.reserve(size_t(n));
-    auto grow = [&](int64_t r) {
-        std::memcpy(out + 3 * r, dirs + 3 * r, 3 * sizeof(float));
-        seen[size_t(r)] = 1;
+    std::vector<int32_t> anchor(static_cast<size_t>(n), -1);  // by piece root
+    std::vector<int8_t> asign(static_cast<size_t>(n), 0);
+    std::vector<int32_t> queue;
+    auto settle = [&](int32_t from) {
+        uint8_t p;
+        const int32_t r0 = S.find(from, p);
+        anchor[size_t(r0)] = from;
+        asign[size_t(r0)] = 1;
         queue.clear();
-        queue.push_back(r);
+        queue.push_back(r0);
         for (size_t h = 0; h < queue.size(); ++h) {
-            const int64_t p = queue[h];
-            const float *pp = out + 3 * p;
-            for (int64_t t = start[size_t(p)]; t < start[size_t(p) + 1]; ++t) {
-                const int64_t c = adj[size_t(t)];
-                if (seen[size_t(c)]) continue;
-                seen[size_t(c)] = 1;
-                const float *cd = dirs + 3 * c;
-                const float s = (dot3(cd, pp) >= 0.f) ? 1.f : -1.f;
-                out[3 * c] = s * cd[0];
-                out[3 * c + 1] = s * cd[1];
-                out[3 * c + 2] = s * cd[2];
-                queue.push_back(c);
+            const int32_t r = queue[h];
+            for (int32_t t = start[size_t(r)]; t < start[size_t(r) + 1]; ++t) {
+                const Entry &e = adj[size_t(t)];
+                if (anchor[size_t(e.piece)] >= 0) continue;
+                anchor[size_t(e.piece)] = e.entry;
+                asign[size_t(e.piece)] = e.sign;
+                queue.push_back(e.piece);
             }
         }
     };
-    grow(root);
-    for (int64_t i = 0; i < n; ++i)
-        if (!seen[size_t(i)]) grow(i);
+    settle(int32_t(root));
+    for (int64_t i = 0; i < n; ++i) {
+        uint8_t p;
+        if (anchor[size_t(S.find(int32_t(i), p))] < 0) settle(int32_t(i));
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        uint8_t p, pa;
+        const int32_t r = S.find(int32_t(i), p);
+        (void)S.find(anchor[size_t(r)], pa);
+        const float sg = ((p ^ pa) ? -1.f : 1.f) * float(asign[size_t(r)]);
+        out[3 * i] = sg * dirs[3 * i];
+        out[3 * i + 1] = sg * dirs[3 * i + 1];
+        out[3 * i + 2] = sg * dirs[3 * i + 2];
+    }
     return MH_OK;
 }

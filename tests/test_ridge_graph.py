@@ -239,7 +239,7 @@ def test_kept_segments_bounds_hold_for_every_piece_sign(seed):
 @pytest.mark.parametrize("seed,zeros,nans", [(1, 0, 0), (2, 5, 0), (3, 40, 0), (4, 3, 1)])
 def test_forest_pieces_against_the_walk_and_scipy(seed, zeros, nans):
     """The union-find pieces and signs: without zero products the signs are
-    the breadth-first walk's up to one flip per tree; every non-zero edge
+    the breadth-first walk's, bit for bit; every non-zero edge
     joins points with non-negative signed products; the pieces are scipy's
     connected components of the forest without its zero-product edges, in
     order of their smallest point; zero and NaN products are counted."""
@@ -281,7 +281,74 @@ def test_forest_pieces_against_the_walk_and_scipy(seed, zeros, nans):
     assert (prod >= 0).all()
     np.testing.assert_array_equal(np.abs(signs), np.abs(dirs))
     if zeros == 0 and nans == 0:
+        # every tree's smallest point keeps its direction in both (the walk
+        # starts the tree of point 0 there), so the signs are the walk's bit
+        # for bit
         walk = ridge_graph.propagate_signs(dirs, edges, 0)
-        flip = np.sign((walk * signs).sum(1))       # +-1 per point, constant per tree
-        assert (flip[edges[:, 0]] == flip[edges[:, 1]]).all()
-        np.testing.assert_array_equal(walk, signs * flip[:, None].astype(np.float32))
+        np.testing.assert_array_equal(walk.view(np.uint32), signs.view(np.uint32))
+
+
+def _walk_signs(dirs, edges, root):
+    """The reference's traversal restated in plain Python: breadth first
+    from root, then from every unreached point in index order; a child keeps
+    its direction when its float32 product ((c0 p0 + c1 p1) + c2 p2) with
+    its adjusted parent is >= 0, else flips (a NaN product flips)."""
+    from collections import deque
+    n = len(dirs)
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    out = np.array(dirs, np.float32)
+    seen = np.zeros(n, bool)
+
+    def grow(r):
+        seen[r] = True
+        q = deque([r])
+        while q:
+            p = q.popleft()
+            for c in adj[p]:
+                if seen[c]:
+                    continue
+                seen[c] = True
+                cd, pp = dirs[c], out[p]
+                d = (cd[0] * pp[0] + cd[1] * pp[1]) + cd[2] * pp[2]
+                out[c] = cd * np.float32(1.0 if d >= 0 else -1.0)   # a sign times the vector
+                q.append(c)
+    grow(root)
+    for i in range(n):
+        if not seen[i]:
+            grow(i)
+    return out
+
+
+@pytest.mark.parametrize("seed,cuts,root", [(1, 0, 0), (2, 12, 0), (3, 60, 57), (4, 20, 1999)])
+def test_tree_signs_equal_the_walk(seed, cuts, root):
+    """mh_tree_signs_host (pieces between zero and NaN products, anchored by
+    the walk over the pieces) gives the reference traversal's directions bit
+    for bit on forests with zero and NaN products and several trees."""
+    rng = np.random.default_rng(seed)
+    n = 2000
+    parent = np.array([-1] + [int(rng.integers(0, i)) if rng.random() > 0.01 else -1
+                              for i in range(1, n)])
+    child = np.nonzero(parent >= 0)[0]
+    edges = np.stack([parent[child], child], 1)
+    rng.shuffle(edges)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    for k, idx in enumerate(rng.choice(len(edges), size=cuts, replace=False)):
+        b = edges[idx, 1]
+        if k % 3 == 0:
+            dirs[b] = (0.0, 0.0, 0.0)                     # zero products
+        elif k % 3 == 1:
+            a = edges[idx, 0]
+            v = np.zeros(3, np.float32)
+            j = int(np.argmax(np.abs(dirs[a])))
+            v[(j + 1) % 3] = 1.0                          # exactly orthogonal to an axis
+            dirs[a] = 0.0
+            dirs[a, j] = 1.0
+            dirs[b] = v
+        else:
+            dirs[b, 0] = np.nan                           # NaN products
+    got = ridge_graph.propagate_signs(dirs, edges, root)
+    want = _walk_signs(dirs, edges, root)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
