@@ -229,6 +229,12 @@ int64_t mh_gvf_work_floats(const int64_t dims[3]);
 int mh_gvf(mh_ctx *ctx, float *g3_dev, float *swap3_dev, const float *v3_dev, const float *r_dev,
            const int64_t dims[3], const mh_gvf_params *params, int iters, float *mag_dev,
            float *work_dev, void *stream);
+/* How mh_gvf runs its steps on this context (no counterpart in the
+ * reference): 0 (default) two steps per launch, blocked along axis 0, for
+ * volumes of 2^20 voxels or more; 1 one step per launch; 2 two per launch
+ * whenever the volume allows. Results do not depend on it, bit for bit (the
+ * magnitude sums of the checks may differ in their summation order). */
+int mh_gvf_set_sweep(mh_ctx *ctx, int mode);
 
 /* Normalise non-zero vectors of g (SoA x3) to unit length
  * (clcode/gvf.cl:215-232). */

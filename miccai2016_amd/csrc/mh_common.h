@@ -37,6 +37,7 @@ struct mh_ctx {
     int batch_group = 0;
     int batch_visit = 0;
     int batch_sweep = 0;  // two-level sweep of batched groups: 0 automatic, 1 off, 2 on
+    int gvf_sweep = 0;    // two GVF steps per launch: 0 automatic, 1 off, 2 whenever possible
     // two-level sweep of mh_cmf_solve: 0 automatic, 1 off, 2 whenever eligible
     int sweep_mode = 0;
     int sweep_shape = 0;   // tile shape (1-based; 0: default)

@@ -399,6 +399,33 @@ def test_gvf_callback_checks_and_normalize(dev, rng):
     np.testing.assert_allclose(g2, np.where(nrm > 0, one / nrm, one), atol=1e-5)
 
 
+@pytest.mark.parametrize("shape,iters,update", [
+    ((2, 2, 2), 7, 0), ((3, 5, 7), 10, 3), ((18, 20, 22), 40, 0), ((5, 70, 130), 13, 4),
+    ((40, 23, 65), 25, 0), ((128, 128, 128), 21, 10)])
+def test_gvf_two_step_passes_equal_one_step(dev, rng, shape, iters, update):
+    """Two GVF steps per launch (blocked along axis 0) compute the one-step
+    kernel's iterations bit for bit: odd counts, several tiles and plane
+    chunks, the volume faces, and the checks' magnitudes (summed in another
+    order) included."""
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    v = rng.uniform(-0.5, 0.5, size=shape + (3,)).astype(np.float32)
+    outs, mags = [], []
+    try:
+        for mode in (1, 2):
+            dev.check(dev.lib.mh_gvf_set_sweep(dev.handle, mode))
+            seen = []
+            p = GvfParameters()
+            p.mu, p.tol, p.maxiter, p.update, p.normalize = 0.3, 0, iters, update, False
+            p.callback = (lambda g, i, m: seen.append((i, m))) if update else None
+            outs.append(GvfGpu(v, p, cl_program_loader=dev).execute())
+            mags.append(seen)
+    finally:
+        dev.check(dev.lib.mh_gvf_set_sweep(dev.handle, 0))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    assert [i for i, _ in mags[0]] == [i for i, _ in mags[1]]
+    np.testing.assert_allclose([m for _, m in mags[0]], [m for _, m in mags[1]], rtol=1e-5)
+
+
 # ---------------------------------------------------------------- max-flow --
 
 def _cut_case(rng, shape, sigma=0.8):
