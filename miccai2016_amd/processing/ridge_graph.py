@@ -83,8 +83,9 @@ def forest_pieces(dirs, edges):
     """Pieces of a forest between its zero-product edges and signs within
     them (mh_forest_pieces_host): (signed float32 directions, int32 piece
     labels 0 .. P-1, number of zero products, number of NaN products). In a
-    tree without zero or NaN products the signs are propagate_signs' up to
-    one flip of the whole tree."""
+    forest without zero or NaN products the signs are propagate_signs(dirs,
+    edges, 0)'s bit for bit: each tree's smallest point keeps its direction
+    and every other sign is the parity of its path."""
     d32 = np.require(dirs, np.float32, ["C"]).reshape(-1, 3)
     n = len(d32)
     out = np.empty((n, 3), np.float32)
