@@ -184,6 +184,16 @@ int mh_ridge_tree_host(const int64_t *coords_host, int64_t n, const int64_t dims
  * scores need no more than this (processing/adjust_directions.py:290-339). */
 int64_t mh_ridge_forest_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
                              const double voxel_size[3], int64_t *edges_host);
+/* Host: mh_ridge_tree_host with its first phase given (replaces the same
+ * pdist + minimum_spanning_tree, processing/adjust_directions.py:365-379, for
+ * the cleaned ridge): forest_host (nf x 2) is mh_ridge_forest_host's output
+ * for these points, or that output for a superset restricted to a union of
+ * its trees and renumbered in order, as after the cleaning keeps whole
+ * segments (adjust_directions.py:290-339). edges_host receives exactly
+ * mh_ridge_tree_host's edges. */
+int mh_ridge_tree_from_forest_host(const int64_t *coords_host, int64_t n, const int64_t dims[3],
+                                   const double voxel_size[3], const int64_t *forest_host,
+                                   int64_t nf, int64_t *edges_host);
 /* Host: the pieces of a forest between its zero-product edges and signed
  * directions within each piece (every non-zero edge joins points whose
  * signed directions have a non-negative product; a piece's smallest point

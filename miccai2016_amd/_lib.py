@@ -75,6 +75,7 @@ SIGNATURES = {
     "mh_spanning_tree_host": (i32, [vp, i64, C.c_double, vp]),
     "mh_ridge_tree_host": (i32, [vp, i64, I64x3, vp, vp]),
     "mh_ridge_forest_host": (i64, [vp, i64, I64x3, vp, vp]),
+    "mh_ridge_tree_from_forest_host": (i32, [vp, i64, I64x3, vp, vp, i64, vp]),
     "mh_forest_pieces_host": (i32, [vp, i64, vp, i64, vp, vp, vp]),
     "mh_gvf_set_sweep": (i32, [vp, i32]),
     "mh_tree_signs_host": (i32, [vp, i64, vp, i64, i64, vp]),

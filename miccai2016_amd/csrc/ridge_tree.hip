@@ -532,6 +532,50 @@ MH_EXPORT int64_t mh_ridge_forest_host(const int64_t *coords, int64_t n, const i
     return rc ? rc : ne;
 }
 
+// mh_ridge_tree_host with its first phase given: `forest` (nf edges) is
+// mh_ridge_forest_host's output for these points, or that output for a
+// superset restricted to a union of its trees and renumbered in order (the
+// forest of the subset: a tree's Kruskal decisions only involve its own
+// edges, and renumbering in order keeps their order). The forest is copied
+// to the front of `edges` and the Boruvka rounds add the rest, so the edges
+// equal mh_ridge_tree_host's. Checked: indices in range, coordinates inside
+// dims, no edge closing a cycle and every edge shorter than twice the
+// smallest voxel size; the points must be distinct, as there.
+MH_EXPORT int mh_ridge_tree_from_forest_host(const int64_t *coords, int64_t n, const int64_t dims[3],
+                                             const double voxel_size[3], const int64_t *forest,
+                                             int64_t nf, int64_t *edges)
+{
+    if (n < 0 || nf < 0 || nf > std::max<int64_t>(n - 1, 0) || !mh::dims_ok(dims) ||
+        !voxel_size_ok(voxel_size) || (n > 0 && !coords) || (n > 1 && !edges) || (nf > 0 && !forest))
+        return MH_ERR_ARG;
+    if (n < 2) return MH_OK;
+    if (n >= (int64_t(1) << 31)) return MH_ERR_ARG;
+    std::vector<double> pts(size_t(3 * n));
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t c = coords[3 * i + k];
+            if (c < 0 || c >= dims[k]) return MH_ERR_ARG;
+            pts[size_t(3 * i + k)] = double(c) * voxel_size[k];
+        }
+    const double vmin = std::min(voxel_size[0], std::min(voxel_size[1], voxel_size[2]));
+    const double limit = (2.0 * vmin) * (2.0 * vmin) * (1.0 - 1e-9);
+    DisjointSets sets(n);
+    int64_t ne = 0;
+    for (int64_t k = 0; k < nf; ++k) {
+        const int64_t a = forest[2 * k], b = forest[2 * k + 1];
+        if (a < 0 || a >= n || b < 0 || b >= n) return MH_ERR_ARG;
+        const double *p = &pts[size_t(3 * a)], *q = &pts[size_t(3 * b)];
+        const double x0 = p[0] - q[0], x1 = p[1] - q[1], x2 = p[2] - q[2];
+        if (!(x0 * x0 + x1 * x1 + x2 * x2 < limit)) return MH_ERR_ARG;
+        const int64_t before = ne;
+        add_edge(Edge{0.0, a, b}, sets, edges, ne);
+        if (ne == before) return MH_ERR_ARG;  // closes a cycle
+    }
+    if (ne == n - 1) return MH_OK;
+    KdTree tree(pts.data(), n);
+    return complete_tree(tree, n, sets, edges, ne);
+}
+
 // Disjoint sets whose links carry a sign parity: parity(x, y) = 1 when x and
 // y take opposite signs. A set's root is its smallest index.
 struct ParitySets {

@@ -33,6 +33,9 @@ class AdvancedAdjuster(TreePrintable):
         self.__main_dirs_unadjusted = None
         self._ridge_detector = None
         self._hysteresis = None
+        # (ridge points, their neighbour forest): the cleaning's forest, then
+        # its restriction to the kept points, the final tree's first phase
+        self._forest = None
         self._loader = loader if loader is not None else default_context()
         if threshold is not None:
             self.threshold = threshold
@@ -113,6 +116,7 @@ class AdvancedAdjuster(TreePrintable):
         self._d_resp = dev.upload(self.response)
         labeled = self._label_ridge()
         adjusted = None
+        self._forest = None
         if (not self.ridge_detection_only) or self.clean_ridge:
             main_dirs = self._qeigs_to_main_dirs()
             if self.clean_ridge:
@@ -122,6 +126,7 @@ class AdvancedAdjuster(TreePrintable):
             if not self.ridge_detection_only:
                 coords, rdirs = self._adjust_ridge_directions(labeled, main_dirs)
                 adjusted = self._adjust_all_directions(coords, rdirs, shape)
+        self._forest = None
         return adjusted
 
     def execute_tensor(self, qeigs_t, resp_t, shape, verbose=False):
@@ -160,15 +165,20 @@ class AdvancedAdjuster(TreePrintable):
         del labels, work, d_ridge
         coords = np.stack(np.unravel_index(lin, shape), 1) if len(lin) else np.zeros((0, 3), np.int64)
         self._set_ridge(coords, shape)
+        forest = None
         if self.clean_ridge:
+            self._forest = None
             _, adjusted, parts = self._cleaning_signs(coords, dirs, shape)
             keep = self._segments_to_keep(seg, adjusted, parts)
             if keep is None:          # the forest's bounds did not decide: the full tree
                 _, adjusted = self._tree_signs(coords, dirs, shape)
                 keep = self._segments_to_keep(seg, adjusted)
             coords, dirs = coords[keep], dirs[keep]
+            if self._forest is not None:
+                forest = ridge_graph.restrict_forest(self._forest[1], keep)
+            self._forest = None
             self._set_ridge(coords, shape)
-        coords, adjusted = self._tree_signs(coords, dirs, shape)
+        coords, adjusted = self._tree_signs(coords, dirs, shape, forest)
         self._flip_nonridge(coords, adjusted, shape)
         return self._d_qeigs
 
@@ -268,16 +278,24 @@ class AdvancedAdjuster(TreePrintable):
         vs = self.voxel_size
         if len(coords) > ridge_graph.DENSE_LIMIT and vs[0] == vs[1] == vs[2]:
             edges = ridge_graph.ridge_forest_edges(coords, shape, vs)
+            # the forest's trees are the 26-connected segments: the kept
+            # segments' share is the first phase of the final tree
+            self._forest = (coords, edges)
             signs, pieces, nzero, nnan = ridge_graph.forest_pieces(dirs, edges)
             if not nnan:
                 return (np.require(coords, np.uint16, ["C"]), signs,
                         pieces if nzero else None)
         return self._tree_signs(coords, dirs, shape) + (None,)
 
-    def _tree_signs(self, coords, dirs, shape):
+    def _tree_signs(self, coords, dirs, shape, forest=None):
+        """Tree signs from ridge point 0; forest: the first phase of the
+        tree (ridge_graph.ridge_tree_from_forest), when already known."""
         if len(coords) == 0:
             return coords.astype(np.uint16), np.zeros((0, 3), np.float32)
-        edges = ridge_graph.ridge_tree_edges(coords, shape, self.voxel_size)
+        if forest is not None:
+            edges = ridge_graph.ridge_tree_from_forest(coords, shape, self.voxel_size, forest)
+        else:
+            edges = ridge_graph.ridge_tree_edges(coords, shape, self.voxel_size)
         return np.require(coords, np.uint16, ["C"]), ridge_graph.propagate_signs(dirs, edges, 0)
 
     def _label_ridge(self):
@@ -316,7 +334,12 @@ class AdvancedAdjuster(TreePrintable):
             if len(coords):
                 return self._cleaning_signs(coords, dirs, labeled_ridge.shape)
             return self._tree_signs(coords, dirs, labeled_ridge.shape) + (None,)
-        return self._tree_signs(coords, dirs, labeled_ridge.shape)
+        # the cleaning's forest, restricted by _do_clean_ridge to this ridge
+        stash, self._forest = self._forest, None
+        forest = None
+        if stash is not None and np.array_equal(stash[0], coords):
+            forest = stash[1]
+        return self._tree_signs(coords, dirs, labeled_ridge.shape, forest)
 
     def _do_clean_ridge(self, labeled_ridge, ridge_coords, ridge_dirs, parts=None):
         """Keep the ceil(3 ln(e + n)) 26-connected ridge segments whose summed
@@ -333,6 +356,11 @@ class AdvancedAdjuster(TreePrintable):
                                             labeled_ridge.shape)
             kept = self._kept_segments(lab_of, ncomp, tree_dirs)
         cleaned = np.isin(comps, np.nonzero(kept)[0] + 1).astype(labeled_ridge.dtype)
+        stash, self._forest = self._forest, None
+        if stash is not None and np.array_equal(stash[0], ridge_coords):
+            # the kept points, in order: the cleaned ridge's argwhere
+            keep = kept[lab_of]
+            self._forest = (stash[0][keep], ridge_graph.restrict_forest(stash[1], keep))
         self.__ridge = cleaned
         return cleaned
 

@@ -79,6 +79,41 @@ def ridge_forest_edges(coords, shape, voxel_size):
     return edges[:ne]
 
 
+def ridge_tree_from_forest(coords, shape, voxel_size, forest):
+    """ridge_tree_edges(coords, shape, voxel_size) from its first phase:
+    `forest` is ridge_forest_edges of these points, or of a superset
+    restricted to a union of its trees (whole 26-connected segments, for an
+    isotropic voxel size) and renumbered in order (restrict_forest). The
+    edges equal ridge_tree_edges' (mh_ridge_tree_from_forest_host)."""
+    coords = np.asarray(coords)
+    n = len(coords)
+    if n <= DENSE_LIMIT:
+        return ridge_tree_edges(coords, shape, voxel_size)
+    c = np.require(coords, np.int64, ["C"])
+    vs = np.require(np.multiply(np.ones(3), voxel_size), np.float64, ["C"])
+    f = np.require(forest, np.int64, ["C"]).reshape(-1, 2)
+    edges = np.empty((n - 1, 2), np.int64)
+    rc = _lib.lib().mh_ridge_tree_from_forest_host(c.ctypes.data, n, _lib.dims3(shape),
+                                                   vs.ctypes.data, f.ctypes.data, len(f),
+                                                   edges.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"mh_ridge_tree_from_forest_host failed (status {rc})")
+    return edges
+
+
+def restrict_forest(forest, keep):
+    """The edges of `forest` (E x 2) between points kept by the boolean mask
+    `keep`, renumbered to the kept points' positions in order. Kept points
+    must hold whole trees of the forest (an edge with one end kept raises)."""
+    keep = np.asarray(keep, bool)
+    f = np.asarray(forest, np.int64).reshape(-1, 2)
+    ka, kb = keep[f[:, 0]], keep[f[:, 1]]
+    if np.any(ka != kb):
+        raise ValueError("restrict_forest: the kept points split a tree of the forest")
+    new = np.cumsum(keep, dtype=np.int64) - 1
+    return new[f[ka]]
+
+
 def forest_pieces(dirs, edges):
     """Pieces of a forest between its zero-product edges and signs within
     them (mh_forest_pieces_host): (signed float32 directions, int32 piece

@@ -168,6 +168,96 @@ def test_cleaning_with_forest_signs_equals_full_tree(monkeypatch, seed, pair):
     np.testing.assert_array_equal(keep_full, keep_forest)
 
 
+def _forest_components(n, forest):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    f = np.asarray(forest).reshape(-1, 2)
+    g = coo_matrix((np.ones(len(f)), (f[:, 0], f[:, 1])), shape=(n, n))
+    return connected_components(g, directed=False)
+
+
+@pytest.mark.parametrize("vs", [(1, 1, 1), (2.0, 0.5, 1.25), (1, 1, 3)])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_tree_from_forest_equals_ridge_tree(vs, seed):
+    """mh_ridge_tree_from_forest_host, given the neighbour forest of the
+    points or of a superset restricted to whole trees of it, returns
+    mh_ridge_tree_host's edges, in order."""
+    rng = np.random.default_rng(seed)
+    shape = (40, 40, 40)
+    ridge, _ = _segments(rng, shape, 60)
+    coords = np.argwhere(ridge != 0)
+    forest = ridge_graph.ridge_forest_edges(coords, shape, vs)
+    np.testing.assert_array_equal(ridge_graph.ridge_tree_from_forest(coords, shape, vs, forest),
+                                  ridge_graph.ridge_tree_edges(coords, shape, vs))
+    ncomp, comp = _forest_components(len(coords), forest)
+    assert ncomp > 4
+    keep = (rng.random(ncomp) < 0.6)[comp]
+    sub = ridge_graph.restrict_forest(forest, keep)
+    assert len(coords[keep]) > ridge_graph.DENSE_LIMIT
+    np.testing.assert_array_equal(
+        ridge_graph.ridge_tree_from_forest(coords[keep], shape, vs, sub),
+        ridge_graph.ridge_tree_edges(coords[keep], shape, vs))
+
+
+def test_tree_from_forest_rejects_other_forests(monkeypatch):
+    """A forest that is not a forest of short neighbour edges of the points
+    is refused; restrict_forest refuses a mask that splits a tree."""
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 1)
+    shape = (8, 8, 8)
+    coords = np.array([[1, 1, 1], [1, 1, 2], [1, 2, 2], [5, 5, 5]])
+    vs = (1, 1, 1)
+    forest = ridge_graph.ridge_forest_edges(coords, shape, vs)
+    assert forest.tolist() == [[0, 1], [1, 2]]
+    want = ridge_graph.ridge_tree_edges(coords, shape, vs)
+    np.testing.assert_array_equal(ridge_graph.ridge_tree_from_forest(coords, shape, vs, forest), want)
+    np.testing.assert_array_equal(
+        ridge_graph.ridge_tree_from_forest(coords, shape, vs, np.zeros((0, 2), np.int64)), want)
+    for bad in ([[0, 1], [1, 2], [0, 2]],     # more edges than a forest holds
+                [[0, 1], [0, 1]],             # a cycle
+                [[0, 3]],                     # a long edge
+                [[0, 4]], [[-1, 0]]):         # out of range
+        with pytest.raises(RuntimeError):
+            ridge_graph.ridge_tree_from_forest(coords, shape, vs, np.array(bad))
+    with pytest.raises(RuntimeError):         # a voxel outside the volume
+        ridge_graph.ridge_tree_from_forest(coords, (4, 8, 8), vs, forest)
+    with pytest.raises(ValueError):
+        ridge_graph.restrict_forest(forest, [True, False, True, True])
+    np.testing.assert_array_equal(ridge_graph.restrict_forest(forest, [True, True, True, False]),
+                                  [[0, 1], [1, 2]])
+    np.testing.assert_array_equal(
+        ridge_graph.restrict_forest([[1, 3], [3, 4]], [False, True, False, True, True]),
+        [[0, 1], [1, 2]])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 6])
+def test_final_tree_from_the_cleaning_forest(monkeypatch, seed):
+    """The numpy API's adjustment builds the cleaned ridge's tree from the
+    cleaning's neighbour forest, restricted to the kept segments: the same
+    coordinates and signed directions, bit for bit, as building it anew."""
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    monkeypatch.setattr(ridge_graph, "DENSE_LIMIT", 10)
+    rng = np.random.default_rng(seed)
+    shape = (64, 64, 64)
+    ridge, dirs = _segments(rng, shape, 80)
+    adj = AdvancedAdjuster((0.01, 0.3), None, None, voxel_size=1, loader=_NoDevice())
+    c, d, parts = adj._adjust_ridge_directions(ridge, dirs, for_cleaning=True)
+    assert adj._forest is not None
+    cleaned = adj._do_clean_ridge(ridge, c, d, parts)
+    assert 0 < cleaned.sum() < ridge.sum()
+    assert len(adj._forest[0]) == cleaned.sum()
+    trees = []
+    tree_edges = ridge_graph.ridge_tree_edges
+    monkeypatch.setattr(ridge_graph, "ridge_tree_edges",
+                        lambda *a: trees.append(1) or tree_edges(*a))
+    c2, d2 = adj._adjust_ridge_directions(cleaned, dirs)
+    assert trees == [] and adj._forest is None
+    fresh = AdvancedAdjuster((0.01, 0.3), None, None, voxel_size=1, loader=_NoDevice())
+    c3, d3 = fresh._adjust_ridge_directions(cleaned, dirs)
+    assert trees == [1]
+    np.testing.assert_array_equal(c2, c3)
+    np.testing.assert_array_equal(d2.view(np.uint32), d3.view(np.uint32))
+
+
 @pytest.mark.parametrize("vs", [(1, 1, 1), (2.0, 0.5, 1.25), (1, 1, 3)])
 def test_lattice_tree_is_the_point_tree(monkeypatch, vs):
     """The dense-lookup lattice construction gives the k-d tree
