@@ -287,10 +287,15 @@ __device__ __forceinline__ void bst(Rsrc r, unsigned voff, unsigned soff, float 
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, int(voff), int(soff), 0);
 }
 // the same store through a 64-bit base address (two scalar registers instead
-// of a four-register descriptor)
+// of a four-register descriptor). The byte offset voff + soff is summed in 32
+// bits (it stays below 2^32 for every volume the sweep takes, as the
+// descriptors' offsets do), so the store takes the base in scalar registers
+// and one 32-bit offset, which stores at the same offset into different
+// fields share; adding soff to a base + voff address instead kept a 64-bit
+// address per field in vector registers and a 64-bit add per store.
 __device__ __forceinline__ void pst(float *base, unsigned voff, unsigned soff, float v)
 {
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + soff + size_t(voff)) = v;
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + size_t(voff + soff)) = v;
 }
 
 struct SweepRsrc {
