@@ -301,6 +301,9 @@ __device__ __forceinline__ void pst(float *base, unsigned voff, unsigned soff, f
 struct SweepRsrc {
     Rsrc tmp, hp, q, a, pt, u, cs, ct;  // inputs
     Rsrc hpo, tmpo, pto, uo;            // outputs
+    // the inputs as addresses, for loads through 64-bit bases
+    const float *ptmp, *php, *pa, *ppt, *pu, *pcs, *pct;
+    const float4 *pq;
 };
 
 // Stores of the sweep. The 1024-thread shapes hold one work-group per CU at
@@ -318,7 +321,30 @@ __device__ __forceinline__ void sweep_store(Rsrc r, float *base, unsigned voff, 
 // The inputs of NS solves (solve j's scalar fields at byte offset j nb within
 // the buffers, its hp at 3 j nb) and those they share, in one branch per
 // phase.
-template <int NS>
+// Loads of the sweep: through a descriptor (plane offset in a scalar
+// register, for free), or through a 64-bit base with the 32-bit byte offset
+// voff + soff (which stays below 2^32, as the descriptors' offsets do). One
+// solve per thread with 1024 threads loads through bases: the descriptors'
+// 32 scalar registers were spilled and reloaded in every plane step (84 -> 74
+// VGPRs, 37 -> 21 lane reloads per two plane steps); with two solves per
+// thread the bases measured more instructions, so the pair kernel keeps the
+// descriptors.
+template <bool PTR>
+__device__ __forceinline__ float sweep_ld(Rsrc r, const float *p, unsigned voff, unsigned soff)
+{
+    if constexpr (PTR)
+        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + size_t(voff + soff));
+    else return bld(r, voff, soff);
+}
+template <bool PTR>
+__device__ __forceinline__ float4 sweep_ld4(Rsrc r, const float4 *p, unsigned voff, unsigned soff)
+{
+    if constexpr (PTR)
+        return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(p) + size_t(voff + soff));
+    else return bld4(r, voff, soff);
+}
+
+template <int NS, bool PTR>
 __device__ __forceinline__ void sweep_load(SweepIn (&L)[NS], SweepInShared &C, const SweepRsrc &R,
                                            unsigned nb, unsigned pb, unsigned d2b, unsigned zb,
                                            unsigned ob, bool p_on, bool u_on, bool e0, unsigned f)
@@ -330,26 +356,26 @@ __device__ __forceinline__ void sweep_load(SweepIn (&L)[NS], SweepInShared &C, c
             // the forward neighbours only inside the volume (the uniform plane
             // offset is outside the descriptor's range check); the p-step
             // masks the faces with the same conditions
-            L[j].t = bld(R.tmp, ob, zs);
-            if (e0) L[j].t0 = bld(R.tmp, ob, zs + pb);
-            if (f & 16u) L[j].t1 = bld(R.tmp, ob + d2b, zs);
-            if (f & 32u) L[j].t2 = bld(R.tmp, ob + 4u, zs);
-            L[j].h0 = bld(R.hp, ob, zh);
-            L[j].h1 = bld(R.hp, ob, zh + nb);
-            L[j].h2 = bld(R.hp, ob, zh + 2u * nb);
+            L[j].t = sweep_ld<PTR>(R.tmp, R.ptmp, ob, zs);
+            if (e0) L[j].t0 = sweep_ld<PTR>(R.tmp, R.ptmp, ob, zs + pb);
+            if (f & 16u) L[j].t1 = sweep_ld<PTR>(R.tmp, R.ptmp, ob + d2b, zs);
+            if (f & 32u) L[j].t2 = sweep_ld<PTR>(R.tmp, R.ptmp, ob + 4u, zs);
+            L[j].h0 = sweep_ld<PTR>(R.hp, R.php, ob, zh);
+            L[j].h1 = sweep_ld<PTR>(R.hp, R.php, ob, zh + nb);
+            L[j].h2 = sweep_ld<PTR>(R.hp, R.php, ob, zh + 2u * nb);
         }
-        C.q = bld4(R.q, 4u * ob, 4u * zb);
+        C.q = sweep_ld4<PTR>(R.q, R.pq, 4u * ob, 4u * zb);
 #pragma unroll
-        for (int j = 0; j < NS; ++j) L[j].a = bld(R.a, ob, zb + unsigned(j) * nb);
+        for (int j = 0; j < NS; ++j) L[j].a = sweep_ld<PTR>(R.a, R.pa, ob, zb + unsigned(j) * nb);
     }
     if (u_on) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            L[j].pt = bld(R.pt, ob, zb + unsigned(j) * nb);
-            L[j].u = bld(R.u, ob, zb + unsigned(j) * nb);
+            L[j].pt = sweep_ld<PTR>(R.pt, R.ppt, ob, zb + unsigned(j) * nb);
+            L[j].u = sweep_ld<PTR>(R.u, R.pu, ob, zb + unsigned(j) * nb);
         }
-        C.cs = bld(R.cs, ob, zb);
-        C.ct = bld(R.ct, ob, zb);
+        C.cs = sweep_ld<PTR>(R.cs, R.pcs, ob, zb);
+        C.ct = sweep_ld<PTR>(R.ct, R.pct, ob, zb);
     }
 }
 
@@ -464,7 +490,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                       rsrc_of(alpha, 4 * n * NS), rsrc_of(B.pt, 4 * n * NS), rsrc_of(B.u, 4 * n * NS),
                       rsrc_of(cs, 4 * n), rsrc_of(ct, 4 * n), rsrc_of(B.hpo, 12 * n * NS),
                       rsrc_of(B.tmpo, 4 * n * NS), rsrc_of(B.pto, 4 * n * NS),
-                      rsrc_of(B.uo, 4 * n * NS)};
+                      rsrc_of(B.uo, 4 * n * NS), B.tmp, B.hp, alpha, B.pt, B.u, cs, ct, q4};
     const unsigned nb = unsigned(4 * n), pb = unsigned(4 * st0), d2b = 4u * unsigned(D.d2);
 
     // this thread's point of the widened tile; LDS is indexed by the point
@@ -508,7 +534,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         L[j].a = L[j].pt = L[j].u = 0.f;
     }
     auto load = [&](const int z, const unsigned zb) {
-        sweep_load<NS>(L, LC, R, nb, pb, d2b, zb, ob, p_on(z), u_on(z), z < D.d0 - 1, f);
+        sweep_load<NS, PST && NS == 1>(L, LC, R, nb, pb, d2b, zb, ob, p_on(z), u_on(z), z < D.d0 - 1, f);
     };
     if (zs <= zend) load(zs, unsigned(zs) * pb);
     // one plane step; the loop below takes two per turn, so that the values
