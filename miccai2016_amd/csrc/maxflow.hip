@@ -381,22 +381,22 @@ __device__ __forceinline__ void sweep_load_all(SweepIn (&L)[NS], SweepInShared &
 }
 
 // The inputs of NS solves, in one branch per phase: only the lanes of each
-// phase's region load, and the forward neighbours only inside the volume
-// (the uniform plane offset is outside the descriptor's range check); the
-// phases mask the same faces.
+// phase's region load. The forward neighbours' offsets fall back to the
+// point itself on the far faces (ob1, ob2; e0 false), as in sweep_load_all,
+// which the phases mask.
 template <int NS>
 __device__ __forceinline__ void sweep_load(SweepIn (&L)[NS], SweepInShared &C, const SweepRsrc &R,
-                                           unsigned nb, unsigned pb, unsigned d2b, unsigned zb,
-                                           unsigned ob, bool p_on, bool u_on, bool e0, unsigned f)
+                                           unsigned nb, unsigned pb, unsigned zb, unsigned ob,
+                                           unsigned ob1, unsigned ob2, bool p_on, bool u_on, bool e0)
 {
     if (p_on) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const unsigned zs = zb + unsigned(j) * nb, zh = zb + 3u * unsigned(j) * nb;
             L[j].t = bld(R.tmp, ob, zs);
-            if (e0) L[j].t0 = bld(R.tmp, ob, zs + pb);
-            if (f & 16u) L[j].t1 = bld(R.tmp, ob + d2b, zs);
-            if (f & 32u) L[j].t2 = bld(R.tmp, ob + 4u, zs);
+            L[j].t0 = bld(R.tmp, ob, zs + (e0 ? pb : 0u));
+            L[j].t1 = bld(R.tmp, ob1, zs);
+            L[j].t2 = bld(R.tmp, ob2, zs);
             L[j].h0 = bld(R.hp, ob, zh);
             L[j].h1 = bld(R.hp, ob, zh + nb);
             L[j].h2 = bld(R.hp, ob, zh + 2u * nb);
@@ -580,8 +580,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             sweep_load_all<NS, PST>(L, LC, R, nb, pb, unsigned(zc) * pb, ob, ob1, ob2,
                                     zc < D.d0 - 1);
         } else {
-            sweep_load<NS>(L, LC, R, nb, pb, d2b, unsigned(z) * pb, ob, p_on(z), u_on(z),
-                           z < D.d0 - 1, f);
+            sweep_load<NS>(L, LC, R, nb, pb, unsigned(z) * pb, ob, ob1, ob2, p_on(z), u_on(z),
+                           z < D.d0 - 1);
         }
     };
     if (zs <= zend) load(zs);
