@@ -725,6 +725,39 @@ ORC_API void orc_gvf_response(float *r, const float *v0, const float *v1, const 
     for (int64_t k = 0; k < n; ++k) r[k] = v0[k] * v0[k] + v1[k] * v1[k] + v2[k] * v2[k];
 }
 
+/* Weighted 7-point Laplacian of one field at voxel (i0, i1, i2) with the
+ * neighbour across a face mirrored (index -1 reads index 1, index n reads
+ * n - 2), the reference's gvfDelta stencil (gvf.cl:77-92): per axis
+ * (g[i-1] + g[i+1] - 2 g[i]) * RSS_k, accumulated in float after each axis,
+ * RSS double as the -D macros. Requires every dim >= 2. Pinned by
+ * scipy.ndimage.correlate1d([1, -2, 1], mode='mirror') per axis. */
+static inline float gvf_lap_at(const float *gc, int64_t i0, int64_t i1, int64_t i2,
+                               const int64_t *dims, const double *rss)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t i = lin3(i0, i1, i2, d1, d2);
+    const int64_t s0 = d1 * d2, s1 = d2;
+    const int64_t b0 = i + (i0 > 0 ? -1 : 1) * s0, a0 = i + (i0 < d0 - 1 ? 1 : -1) * s0;
+    const int64_t b1 = i + (i1 > 0 ? -1 : 1) * s1, a1 = i + (i1 < d1 - 1 ? 1 : -1) * s1;
+    const int64_t b2 = i + (i2 > 0 ? -1 : 1), a2 = i + (i2 < d2 - 1 ? 1 : -1);
+    const float gi = gc[i];
+    float lap = (float)((gc[b0] + gc[a0] - 2 * gi) * rss[0]);
+    lap = (float)(lap + (gc[b1] + gc[a1] - 2 * gi) * rss[1]);
+    lap = (float)(lap + (gc[b2] + gc[a2] - 2 * gi) * rss[2]);
+    return lap;
+}
+
+/* The Laplacian of gvf_lap_at over a whole field (the stencil orc_gvf uses). */
+ORC_API void orc_gvf_laplacian(const float *g, const int64_t *dims, const double *rss, float *out)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2)
+                out[lin3(i0, i1, i2, d1, d2)] = gvf_lap_at(g, i0, i1, i2, dims, rss);
+}
+
 /* maxiter explicit Euler steps from g = 0: delta = r (g - v) - MU lap(g)
  * (7-point, mirrored neighbour at edges), g -= DT delta. MU, DT, RSS are
  * double as the reference's -D macros. If mag_out != NULL, mag_out[k] holds
@@ -747,17 +780,9 @@ ORC_API void orc_gvf(float *g0, float *g1, float *g2, const float *v0, const flo
             for (int64_t i1 = 0; i1 < d1; ++i1)
                 for (int64_t i2 = 0; i2 < d2; ++i2) {
                     const int64_t i = lin3(i0, i1, i2, d1, d2);
-                    const int64_t s0 = d1 * d2, s1 = d2;
-                    const int64_t b0 = i + (i0 > 0 ? -1 : 1) * s0, a0 = i + (i0 < d0 - 1 ? 1 : -1) * s0;
-                    const int64_t b1 = i + (i1 > 0 ? -1 : 1) * s1, a1 = i + (i1 < d1 - 1 ? 1 : -1) * s1;
-                    const int64_t b2 = i + (i2 > 0 ? -1 : 1), a2 = i + (i2 < d2 - 1 ? 1 : -1);
                     for (int c = 0; c < 3; ++c) {
-                        const float *gc = g[c];
-                        const float gi = gc[i];
-                        float lap = (float)((gc[b0] + gc[a0] - 2 * gi) * rss[0]);
-                        lap = (float)(lap + (gc[b1] + gc[a1] - 2 * gi) * rss[1]);
-                        lap = (float)(lap + (gc[b2] + gc[a2] - 2 * gi) * rss[2]);
-                        dl[c][i] = (float)(r[i] * (gi - v[c][i]) - mu * lap);
+                        const float lap = gvf_lap_at(g[c], i0, i1, i2, dims, rss);
+                        dl[c][i] = (float)(r[i] * (g[c][i] - v[c][i]) - mu * lap);
                     }
                 }
 #pragma omp parallel for schedule(static)
@@ -788,6 +813,63 @@ typedef struct {
     int supervised;   /* labels given */
 } orc_cut_params;
 
+/* Forward difference of x at voxel (i0, i1, i2) along each axis k, scaled by
+ * w[k] (double, as the SS -D macros) and 0 on the axis's last index
+ * (update_qp.cl:179-181). Pinned by numpy.diff and by the adjoint identity
+ * with div_at. */
+static inline void fwd_grad_at(const float *x, int64_t i0, int64_t i1, int64_t i2,
+                               const int64_t *dims, const double *w, float g[3])
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t i = lin3(i0, i1, i2, d1, d2);
+    g[0] = i0 < d0 - 1 ? (float)((x[i + d2 * d1] - x[i]) * w[0]) : 0.f;
+    g[1] = i1 < d1 - 1 ? (float)((x[i + d2] - x[i]) * w[1]) : 0.f;
+    g[2] = i2 < d2 - 1 ? (float)((x[i + 1] - x[i]) * w[2]) : 0.f;
+}
+
+/* Divergence of the vector field (y0, y1, y2) at voxel (i0, i1, i2), each axis
+ * k weighted by w[k] (double, as the RS -D macros): the negative adjoint of
+ * the forward difference, y_k[i] [i_k < D_k - 1] - y_k[i - e_k] [i_k > 0]
+ * (update_qp.cl:270-272), summed in double and rounded once. With equal
+ * weights, <fwd_grad x, y> = -<x, div y>. */
+static inline float div_at(const float *y0, const float *y1, const float *y2, int64_t i0,
+                           int64_t i1, int64_t i2, const int64_t *dims, const double *w)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    const int64_t i = lin3(i0, i1, i2, d1, d2);
+    return (float)(((i0 < d0 - 1 ? y0[i] : 0.f) - (i0 > 0 ? y0[i - d2 * d1] : 0.f)) * w[0] +
+                   ((i1 < d1 - 1 ? y1[i] : 0.f) - (i1 > 0 ? y1[i - d2] : 0.f)) * w[1] +
+                   ((i2 < d2 - 1 ? y2[i] : 0.f) - (i2 > 0 ? y2[i - 1] : 0.f)) * w[2]);
+}
+
+/* fwd_grad_at over a whole field: g0, g1, g2 receive the three components. */
+ORC_API void orc_grad_fwd(const float *x, const int64_t *dims, const double *w, float *g0,
+                          float *g1, float *g2)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                float g[3];
+                fwd_grad_at(x, i0, i1, i2, dims, w, g);
+                const int64_t i = lin3(i0, i1, i2, d1, d2);
+                g0[i] = g[0]; g1[i] = g[1]; g2[i] = g[2];
+            }
+}
+
+/* div_at over a whole field. */
+ORC_API void orc_divergence(const float *y0, const float *y1, const float *y2, const int64_t *dims,
+                            const double *w, float *out)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2)
+                out[lin3(i0, i1, i2, d1, d2)] = div_at(y0, y1, y2, i0, i1, i2, dims, w);
+}
+
 /* update_p: p = H^-1 hp + H^T (SS * grad+ tmp); project |p| <= alpha;
  * hp = H p, with H = R(q) diag(sqrt(lambda)). */
 ORC_API void orc_update_p(float *hp0, float *hp1, float *hp2, const float *tmp, const float *eigs,
@@ -799,10 +881,8 @@ ORC_API void orc_update_p(float *hp0, float *hp1, float *hp2, const float *tmp, 
         for (int64_t i1 = 0; i1 < d1; ++i1)
             for (int64_t i2 = 0; i2 < d2; ++i2) {
                 const int64_t i = lin3(i0, i1, i2, d1, d2);
-                float g[3] = {
-                    i0 < d0 - 1 ? (float)((tmp[i + d2 * d1] - tmp[i]) * P->ss[0]) : 0.f,
-                    i1 < d1 - 1 ? (float)((tmp[i + d2] - tmp[i]) * P->ss[1]) : 0.f,
-                    i2 < d2 - 1 ? (float)((tmp[i + 1] - tmp[i]) * P->ss[2]) : 0.f};
+                float g[3];
+                fwd_grad_at(tmp, i0, i1, i2, dims, P->ss, g);
                 const float *e = P->const_evals ? eigs + 3 * i : eigs + 6 * i + 3;
                 double sq[3];
                 if (P->const_evals) { sq[0] = P->sqev[0]; sq[1] = P->sqev[1]; sq[2] = P->sqev[2]; }
@@ -844,10 +924,7 @@ ORC_API void orc_update_u(const float *hp0, const float *hp1, const float *hp2, 
         for (int64_t i1 = 0; i1 < d1; ++i1)
             for (int64_t i2 = 0; i2 < d2; ++i2) {
                 const int64_t i = lin3(i0, i1, i2, d1, d2);
-                const float divhp = (float)(
-                    ((i0 < d0 - 1 ? hp0[i] : 0.f) - (i0 > 0 ? hp0[i - d2 * d1] : 0.f)) * P->rs[0] +
-                    ((i1 < d1 - 1 ? hp1[i] : 0.f) - (i1 > 0 ? hp1[i - d2] : 0.f)) * P->rs[1] +
-                    ((i2 < d2 - 1 ? hp2[i] : 0.f) - (i2 > 0 ? hp2[i - 1] : 0.f)) * P->rs[2]);
+                const float divhp = div_at(hp0, hp1, hp2, i0, i1, i2, dims, P->rs);
                 float ui = u[i];
                 float ps, pti;
                 if (labels) {

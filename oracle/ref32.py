@@ -66,6 +66,9 @@ def lib():
         L.orc_sum_abs_reduce.restype = C.c_float
         L.orc_gvf.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i64p, C.c_double,
                               C.c_double, _f64p, C.c_int, C.c_int, vp]
+        L.orc_gvf_laplacian.argtypes = [_f32p, _i64p, _f64p, _f32p]
+        L.orc_grad_fwd.argtypes = [_f32p, _i64p, _f64p, _f32p, _f32p, _f32p]
+        L.orc_divergence.argtypes = [_f32p, _f32p, _f32p, _i64p, _f64p, _f32p]
         L.orc_update_p.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i64p,
                                    C.POINTER(CutParams)]
         L.orc_update_u.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, vp,
@@ -265,7 +268,31 @@ def gvf(vfield, mu, maxiter, voxel_size=(1.0, 1.0, 1.0), response=None, update=N
     return np.stack(g, axis=-1), mags
 
 
+def gvf_laplacian(g, rss=(1.0, 1.0, 1.0)):
+    """The mirrored, RSS-weighted 7-point Laplacian orc_gvf applies (gvf.cl:77-92)."""
+    g = _f32(g)
+    out = np.empty_like(g)
+    lib().orc_gvf_laplacian(g, _dims(g.shape), np.ascontiguousarray(rss, np.float64), out)
+    return out
+
+
 # -------------------------------------------------------------- max-flow ----
+
+def grad_fwd(x, w=(1.0, 1.0, 1.0)):
+    """The weighted forward difference of update_p (update_qp.cl:179-181), (3, D0, D1, D2)."""
+    x = _f32(x)
+    g = [np.empty_like(x) for _ in range(3)]
+    lib().orc_grad_fwd(x, _dims(x.shape), np.ascontiguousarray(w, np.float64), *g)
+    return np.stack(g)
+
+
+def divergence(y, w=(1.0, 1.0, 1.0)):
+    """The weighted divergence of update_u (update_qp.cl:270-272); y is (3, D0, D1, D2)."""
+    y = [_f32(c) for c in y]
+    out = np.empty_like(y[0])
+    lib().orc_divergence(*y, _dims(y[0].shape), np.ascontiguousarray(w, np.float64), out)
+    return out
+
 
 def cut_params(scaling=(1, 1, 1), steps=0.11, cc=0.2, eigs=(1.0, 1.0, 1.0), supervised=False):
     """Derived constants as in continuous_cut_anisotropic_gpu.py:104-129."""
