@@ -108,6 +108,7 @@ class Ticket:
         self.dice = None          # fast Dice against the request's truth, when it has one
         self.error = None
         self._done = threading.Event()
+        self._released = False    # its reservation was returned (set once, under the lock)
 
     @property
     def latency(self):
@@ -317,9 +318,7 @@ class MaxFlowServer:
                         self._step()
                         continue
                 except Exception as e:     # a failed launch fails every job it touched
-                    for job in set(new) | set(self._active):
-                        self._fail(job.ticket, job.reserved, e)
-                    self._active, self._jobs_arr = [], None
+                    self._fail_all(new, e)
                     continue
                 with self._wake:
                     if self._stop:
@@ -362,13 +361,25 @@ class MaxFlowServer:
         return new
 
     def _fail(self, t, need, e):
+        """Fail one request and return its reservation, unless it has already
+        finished or failed (each request is released exactly once)."""
         with self._lock:
+            if t._released:
+                return
+            t._released = True
             self.reserved -= need
             self._inflight -= 1
         t.error = e
         t.t_done = time.perf_counter()
         t._done.set()
         self.completed.append(t)
+
+    def _fail_all(self, new, e):
+        """After a failed launch or finish: fail the jobs being initialised and
+        every resident job that has not finished, and empty the resident set."""
+        for job in list(new) + [j for j in self._active if j not in new]:
+            self._fail(job.ticket, job.reserved, e)
+        self._active, self._jobs_arr = [], None
 
     def _params_for(self, req, shape):
         """The solver constants of one job (unit voxel size)."""
@@ -571,12 +582,17 @@ class MaxFlowServer:
                                                      dev.stream), "mh_cmf_resident_status")
             if st.value:
                 done.append(job)
-        if done:
-            for job in done:
-                self._finish(job)
-            gone = set(id(j) for j in done)
-            self._active = [j for j in self._active if id(j) not in gone]
-            if any(not j.resident for j in done):
+        self._retire(done)
+
+    def _retire(self, done):
+        """Finish the done jobs one by one, each leaving the resident set once
+        it has finished. If a finish raises, that job and the ones after it
+        stay resident and the exception reaches the loop, which fails them
+        (_fail_all); the finished ones are not failed again."""
+        for job in done:
+            self._finish(job)
+            self._active = [j for j in self._active if j is not job]
+            if not job.resident:
                 self._jobs_arr = None
 
     def _finish(self, job):
@@ -602,6 +618,9 @@ class MaxFlowServer:
         job.t = {}
         job.state = None
         with self._lock:
+            if t._released:
+                raise RuntimeError("request finished after it was released")
+            t._released = True
             self.reserved -= job.reserved
             self._inflight -= 1
         t.t_done = time.perf_counter()

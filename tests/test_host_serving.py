@@ -87,3 +87,55 @@ def test_ragged_set_follows_the_workload():
     assert set(w) <= set(ph.GRID_WEIGHTINGS) and set(d) <= set(ph.GRID_DECAYS)
     again = ph.ragged_set(64, seed=9)[0]
     assert [sp.shape for sp in again] == [sp.shape for sp in specs]
+
+
+def _bare_server():
+    """A MaxFlowServer with its bookkeeping only (no device)."""
+    import threading
+    srv = object.__new__(serving.MaxFlowServer)
+    srv._lock = threading.Lock()
+    srv.reserved = 0
+    srv._inflight = 0
+    srv.completed = []
+    srv._active = []
+    srv._jobs_arr = None
+    return srv
+
+
+def test_failed_finish_releases_each_request_once():
+    # two resident jobs finish in one step; the second one's finish raises:
+    # the first stays finished, the second fails, and the budget and the
+    # in-flight count return to zero exactly once each
+    srv = _bare_server()
+    jobs = []
+    for k, need in enumerate((1000, 2500)):
+        t = serving.Ticket(serving.PhantomCutRequest((4, 4, 4), 1, 1, 10, 0.5, 0.1, 1.0))
+        job = serving._Job(t, (4, 4, 4), None, need)
+        jobs.append(job)
+        srv.reserved += need
+        srv._inflight += 1
+    srv._active = list(jobs)
+
+    def finish(job):
+        if job is jobs[1]:
+            raise RuntimeError("download failed")
+        with srv._lock:
+            job.ticket._released = True
+            srv.reserved -= job.reserved
+            srv._inflight -= 1
+        job.ticket._done.set()
+        srv.completed.append(job.ticket)
+
+    srv._finish = finish
+    with pytest.raises(RuntimeError):
+        srv._retire(jobs)
+    assert srv._active == [jobs[1]]
+    srv._fail_all([], RuntimeError("download failed"))
+    assert srv.reserved == 0 and srv._inflight == 0 and srv._active == []
+    assert jobs[0].ticket.error is None and jobs[0].ticket.done()
+    assert isinstance(jobs[1].ticket.error, RuntimeError) and jobs[1].ticket.done()
+    assert [t is jobs[0].ticket for t in srv.completed] == [True, False]
+    # failing again (a later loop turn) changes nothing
+    srv._fail(jobs[0].ticket, jobs[0].reserved, RuntimeError("again"))
+    srv._fail(jobs[1].ticket, jobs[1].reserved, RuntimeError("again"))
+    assert srv.reserved == 0 and srv._inflight == 0 and len(srv.completed) == 2
