@@ -3,15 +3,13 @@ reading of the reference: numpy.diff for the forward difference, the adjoint
 identity and an explicit sparse transpose for the divergence, scipy's mirrored
 Laplacian for GVF, the linear part of the p-step against R diag(l) R^T, and the
 converged cut of 1-D chains against a brute-force minimum cut. CPU only."""
-import itertools
-
 import numpy as np
 import pytest
 import scipy.ndimage as ndi
 import scipy.sparse as sp
 from scipy.spatial.transform import Rotation
 
-from oracle import fp64, ref32
+from oracle import fp64, mincut, ref32
 
 SHAPES = [(5, 7, 9), (1, 6, 4), (3, 1, 1), (2, 2, 2), (1, 1, 11), (4, 3, 2)]
 WEIGHTS = [(1.0, 1.0, 1.0), (0.3, 1.7, 2.5)]
@@ -215,53 +213,7 @@ def test_gvf_second_step_applies_the_mirrored_laplacian(rng, shape):
 
 # ------------------------------------------------- brute-force 1-D min cut ----
 
-def _chain_problem(seed, n, axis, atv):
-    """Random 1-D chain along `axis` whose binary minimum cut beats every other
-    labelling by a margin. Energy of a labelling b (u = 1 pays ct, u = 0 pays
-    cs; edge (i, i+1) pays the flow bound of voxel i, alpha_i |H_i^T e_axis|):
-    sum (1 - b) cs + b ct + sum_i w_i |b_{i+1} - b_i|."""
-    rng = np.random.default_rng(seed)
-    shape = [1, 1, 1]
-    shape[axis] = n
-    shape = tuple(shape)
-    cs = rng.uniform(0.0, 1.0, n)
-    ct = rng.uniform(0.0, 1.0, n)
-    alpha = rng.uniform(0.05, 0.6, n)
-    eigs = (1.0, 10 ** -1.5, 10 ** -1.5) if atv else (1.0, 1.0, 1.0)
-    rot = Rotation.random(n, random_state=seed)
-    q = _squeezed(rot)
-    R = fp64.squeezed_to_mat(q)
-    w = alpha * np.sqrt((np.asarray(eigs) * R[:, axis, :] ** 2).sum(1))
-    cs32, ct32, a32 = (x.astype(np.float32) for x in (cs, ct, alpha))
-    best = []
-    for bits in itertools.product((0, 1), repeat=n):
-        b = np.asarray(bits, np.float64)
-        e = ((1 - b) * cs32 + b * ct32).sum() + (w[:-1] * np.abs(np.diff(b))).sum()
-        best.append((e, bits))
-    best.sort()
-    margin = best[1][0] - best[0][0]
-    return (shape, q.reshape(shape + (3,)), eigs, cs32.reshape(shape), ct32.reshape(shape),
-            a32.reshape(shape), np.asarray(best[0][1], bool).reshape(shape), margin)
-
-
-def chain_cases():
-    """(shape, frames, eigs, cs, ct, alpha, min-cut labelling) of 1-D chains
-    along each axis, TV and ATV, with a margin of at least 0.03."""
-    cases = []
-    seed = 0
-    for axis in range(3):
-        for atv in (False, True):
-            for n in (6, 9, 12):
-                while True:
-                    seed += 1
-                    case = _chain_problem(seed, n, axis, atv)
-                    if case[-1] >= 0.03:
-                        cases.append(case[:-1])
-                        break
-    return cases
-
-
-@pytest.mark.parametrize("case", chain_cases(), ids=lambda c: f"{c[0]}-{'atv' if c[2][1] < 1 else 'tv'}")
+@pytest.mark.parametrize("case", mincut.chain_cases(), ids=mincut.case_id)
 def test_converged_cut_is_the_brute_force_min_cut(case):
     shape, q, eigs, cs, ct, alpha, want = case
     img = np.linspace(0.2, 1.0, int(np.prod(shape))).reshape(shape).astype(np.float32)
