@@ -2,27 +2,37 @@
 import numpy as np
 
 
-def normalize(data, dst_min=0.0, dst_max=1.0, perc_low=None, perc_up=None, clip=True):
-    """Map data linearly to [dst_min, dst_max] in float64 (helpers/misc.py:63-114).
+def _source_range(x, perc_low, perc_up):
+    """(lo, hi, from_percentiles): the interval of x that is mapped onto the
+    destination, either x's extremes (in x's own dtype) or the two requested
+    percentiles, a missing one standing for the 0th or the 100th, taken in a
+    single np.percentile call so that x is sorted once."""
+    if perc_low is None and perc_up is None:
+        return np.min(x), np.max(x), False
+    q = np.percentile(x, [0.0 if perc_low is None else perc_low,
+                          100.0 if perc_up is None else perc_up])
+    return q[0], q[1], True
 
-    The source range is [min, max] of data, or the given percentiles. If the
-    range is empty, an array of dst_min is returned. With percentiles and
-    clip, values outside the range are clipped.
+
+def normalize(data, dst_min=0.0, dst_max=1.0, perc_low=None, perc_up=None, clip=True):
+    """Affine map of data onto [dst_min, dst_max], as a new float64 array
+    (the reference's normalize, helpers/misc.py:63-114).
+
+    The interval [lo, hi] that lands on [dst_min, dst_max] is the data's
+    minimum and maximum, or its perc_low / perc_up percentiles. Each value x
+    goes to unit * (dst_max - dst_min) + dst_min with unit = (1 / (hi - lo)) *
+    (x - lo), the width hi - lo taken in the data's dtype and the rest in
+    float64 (so the unit position rounds as the reference's does; scaling
+    onto [0, 1] is exact). An empty interval gives an array of dst_min. With
+    percentiles and clip, the result is clipped to [dst_min, dst_max].
     """
-    use_percentiles = perc_low is not None or perc_up is not None
-    if use_percentiles:
-        lo = 0. if perc_low is None else perc_low
-        hi = 100. if perc_up is None else perc_up
-        src_min, src_max = np.percentile(data, [lo, hi])
-    else:
-        src_min, src_max = np.min(data), np.max(data)
-    if src_max - src_min == 0:
-        return np.full(np.shape(data), dst_min, dtype=np.float64)
-    result = (1. / (src_max - src_min)) * (np.asarray(data, dtype=np.float64) - src_min)
-    if dst_max != 1 and dst_min == 0:
-        result = result * dst_max
-    elif dst_min != 0:
-        result = result * (dst_max - dst_min) + dst_min
-    if clip and use_percentiles:
-        np.clip(result, dst_min, dst_max, out=result)
-    return result
+    x = np.asarray(data)
+    lo, hi, from_percentiles = _source_range(x, perc_low, perc_up)
+    width = hi - lo
+    if width == 0:
+        return np.full(x.shape, dst_min, dtype=np.float64)
+    unit = (1.0 / width) * (x.astype(np.float64) - lo)
+    out = unit * (dst_max - dst_min) + dst_min
+    if clip and from_percentiles:
+        np.clip(out, dst_min, dst_max, out=out)
+    return out
