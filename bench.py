@@ -36,8 +36,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-BYTES_PER_VOXEL_ITER = 84.0      # algorithmic bytes of one max-flow iteration
+BYTES_PER_VOXEL_ITER = 84.0      # bytes of one max-flow iteration as two stencil passes
 HBM_PEAK_GBS = 8000.0            # MI355X
+
+# Algorithmic bytes of one pass of the two-level sweep per voxel: each solve's
+# state (hp x3, pt, tmp, u) read once and written once (48 B) and its alpha
+# read (4 B), plus the fields a pass reads once for every solve it advances
+# (frame as 4 floats, cs, ct: 24 B). A pass applies one or two iterations.
+SWEEP_STATE_BYTES = 52.0
+SWEEP_SHARED_BYTES = 24.0
 
 # HBM bytes the same kernels moved per voxel-iteration, from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE passes scaled by a 512 MiB copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0);
@@ -150,14 +157,20 @@ def main():
                     help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
     ap.add_argument("--cpu-iters", type=int, default=500)
     ap.add_argument("--cpu-solves", type=int, default=2, help="grid cells the CPU oracle solves")
-    ap.add_argument("--serve-qps", type=str, default="8,16",
+    ap.add_argument("--serve-qps", type=str, default="5,10,20,40,80,160,200",
                     help="comma-separated arrival rates; one serving run each")
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="arrival span per rate: requests = rate x seconds, within "
+                         "[--serve-min-requests, --serve-max-requests]")
+    ap.add_argument("--serve-min-requests", type=int, default=20)
+    ap.add_argument("--serve-max-requests", type=int, default=120)
+    ap.add_argument("--serve-max-queue", type=int, default=48,
+                    help="requests waiting for admission beyond which new ones are refused")
     ap.add_argument("--serve-pipeline-qps", type=str, default="2",
                     help="QPS of requests served through the full preparation ('' skips)")
     ap.add_argument("--serve-pipeline-requests", type=int, default=20)
     ap.add_argument("--serve-prepare-workers", type=int, default=4)
-    ap.add_argument("--ragged-volumes", type=int, default=16)
+    ap.add_argument("--ragged-volumes", type=int, default=64)
     ap.add_argument("--sigmas", type=int, default=15,
                     help="noise levels of --only sigmas (the first of linspace(0.1, 1.5, 15))")
     ap.add_argument("--ragged-workers", type=int, default=8,
@@ -243,13 +256,19 @@ def main():
             torch.cuda.synchronize()
             log(f"warmup step {k + 1}/{args.warmup} done")
         secs = 0.0
+        done_iters = 0
         for k in range(args.steps):
             secs += timed(torch, step)
+            # the iterations every solve actually ran (a NaN stop ends a solve early)
+            done_iters += sum(int(r.iterations) for r in cutter.last_result)
             log(f"timed step {k + 1}/{args.steps}: {secs / (k + 1):.3f} s/step")
-        value = float(S) * n * args.iters * args.steps / secs
+        value = float(done_iters) * n / secs
         line["value"] = value
         line["ms_per_step"] = secs / args.steps * 1e3
-        log(f"batched grid: {value:.4e} voxel-iters/s ({line['ms_per_step']:.1f} ms/step)")
+        line["config"]["iterations_run"] = done_iters // args.steps
+        line["config"]["stopped_solves"] = sum(int(r.stopped != 0) for r in cutter.last_result)
+        log(f"batched grid: {value:.4e} voxel-iters/s ({line['ms_per_step']:.1f} ms/step, "
+            f"{done_iters // args.steps} solve-iterations per step)")
         from miccai2016_amd.evaluation import DeviceDice
         grid_dice = DeviceDice(truth, dev)(u, S)
         best = int(np.nanargmax(grid_dice))
@@ -261,36 +280,56 @@ def main():
                                 "best_decay": float(d_all[best])}
         log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "
             f"decay={d_all[best]:.4g}")
-        achieved = value * BYTES_PER_VOXEL_ITER / 1e9
-        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
         tile = sweep_choice(dev, img.shape)
+        step_s = secs / args.steps
+        effective = value * BYTES_PER_VOXEL_ITER / 1e9
         if tile:
             # the groups ran one after another through the two-level sweep:
-            # one launch is one pass of up to two iterations over a pair of
+            # one launch is one pass of one or two iterations over a pair of
             # solves (the 1024-thread tile shapes) or over one solve
             passes = sweep_passes(args.iters, cfg.cut_update)
-            pair = tile in PAIR_TILES and n < (1 << 27)
+            pair = tile in PAIR_TILES and n < (1 << 26)
             per_launch = 2 if pair else 1
+            launches = (S // per_launch + S % per_launch) * passes
+            pass_bytes = (per_launch * SWEEP_STATE_BYTES + SWEEP_SHARED_BYTES) * n
+            step_bytes = (S // per_launch) * passes * pass_bytes
+            if S % per_launch:
+                step_bytes += passes * (SWEEP_STATE_BYTES + SWEEP_SHARED_BYTES) * n
+            achieved = step_bytes / step_s / 1e9
             per_vi = (TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER if pair
                       else TRAFFIC_SWEEP_PER_VOXEL_ITER).get(tile)
-            line["roofline"].update(
-                kernel=f"cmf_sweep_kernel (tile shape {tile}, {per_launch} solve(s) per thread): one "
-                       f"launch = one pass of two max-flow iterations over {per_launch} solve(s) "
-                       f"(84 algorithmic B per voxel-iteration; traffic = measured HBM bytes per "
-                       f"launch)",
-                launch_us=secs / args.steps / (S * passes) * per_launch * 1e6,
-                traffic=None if per_vi is None else per_vi * 2 * per_launch * n,
-                traffic_per_voxel_iteration=per_vi)
+            line["roofline"] = {
+                "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None if per_vi is None else per_vi * 2 * per_launch * n,
+                "kernel": f"cmf_sweep_kernel (tile shape {tile}, {per_launch} solve(s) per "
+                          f"thread): one launch = one pass of up to two max-flow iterations over "
+                          f"{per_launch} solve(s)",
+                "algorithmic_bytes_per_launch": pass_bytes,
+                "algorithmic_bytes": "the blocked scheme's minimum: per pass and voxel, each "
+                                     "solve's state (hp x3, pt, tmp, u) read and written once and "
+                                     "its alpha read (52 B), the frame (16 B) and cs, ct read once "
+                                     "for the pair",
+                "launch_us": step_s / launches * 1e6,
+                "traffic_per_voxel_iteration": per_vi,
+                "traffic_note": "measured HBM bytes per launch of two iterations (rocprofv3 "
+                                "FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); at 128^3 one "
+                                "pair's working set (~250 MB) mostly stays in the 256 MB "
+                                "last-level cache",
+                "effective_GBs_84B": effective,
+                "effective_frac_84B": effective / HBM_PEAK_GBS}
         else:
             groups = -(-S // GRID_GROUP)
-            line["roofline"].update(
-                kernel="one max-flow iteration of a group of 4 solves: cmf_batch_p_kernel<4> + "
-                       "cmf_batch_u_kernel<4> (84 algorithmic B per solve-voxel-iteration; traffic "
-                       "= measured HBM bytes per such launch pair)",
-                launch_us=secs / args.steps / (groups * args.iters) * 1e6,
-                traffic=TRAFFIC_GRID_PER_VOXEL_ITER * GRID_GROUP * n,
-                traffic_per_voxel_iteration=TRAFFIC_GRID_PER_VOXEL_ITER)
+            achieved = effective
+            line["roofline"] = {
+                "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": TRAFFIC_GRID_PER_VOXEL_ITER * GRID_GROUP * n,
+                "kernel": "one max-flow iteration of a group of 4 solves: cmf_batch_p_kernel<4> + "
+                          "cmf_batch_u_kernel<4> (84 algorithmic B per solve-voxel-iteration; "
+                          "traffic = measured HBM bytes per such launch pair)",
+                "launch_us": step_s / (groups * args.iters) * 1e6,
+                "traffic_per_voxel_iteration": TRAFFIC_GRID_PER_VOXEL_ITER}
         ub = u.detach().cpu().numpy()
         del u
 
@@ -457,12 +496,16 @@ def main():
         from miccai2016_amd.serving import MaxFlowServer, poisson_load, synthetic_request
         torch.cuda.empty_cache()
         line["serving"] = []
-        runs = [(float(q), False, args.serve_requests) for q in args.serve_qps.split(",") if q]
+        runs = [(float(q), False,
+                 int(min(args.serve_max_requests,
+                         max(args.serve_min_requests, round(float(q) * args.serve_seconds)))))
+                for q in args.serve_qps.split(",") if q]
         runs += [(float(q), True, args.serve_pipeline_requests)
                  for q in args.serve_pipeline_qps.split(",") if q]
         for qps, full, nreq in runs:
             srv = MaxFlowServer(cut_parameters(cfg), chunk=50,
-                                prepare_workers=args.serve_prepare_workers).start()
+                                prepare_workers=args.serve_prepare_workers,
+                                max_queue=None if full else args.serve_max_queue).start()
             log(f"serving {nreq} {'prepared ' if full else ''}requests at {qps} QPS "
                 f"(budget {srv.budget / 2 ** 30:.1f} GiB)")
             tickets = poisson_load(srv, lambda rng: synthetic_request(rng, prepare=full), qps,
@@ -472,7 +515,9 @@ def main():
             dice = [t.dice for t in tickets if t.dice is not None and not np.isnan(t.dice)]
             line["serving"].append({
                 "qps": qps, "requests": nreq, "completed": st["completed"],
-                "errors": sum(t.error is not None for t in tickets),
+                "rejected_queue_full": st["rejected"], "max_queue": srv.max_queue,
+                "peak_queue": st["peak_queue"], "queue_wait_p99_s": st.get("queue_wait_p99_s"),
+                "errors": sum(t.error is not None for t in tickets) - st["rejected"],
                 "p50_s": st.get("p50_s"), "p99_s": st.get("p99_s"), "mean_s": st.get("mean_s"),
                 "voxel_iters_per_s": st.get("voxel_iters_per_s"),
                 "offered_voxel_iters_per_s": qps * float(np.mean(
@@ -488,9 +533,25 @@ def main():
                             "as the ragged set, (w, decay) from the grid; cuts of >= 2^20 voxels "
                             "as resident solves, smaller ones as packed jobs"})
             log(f"serving at {qps} QPS: p50 {st.get('p50_s', 0):.3f} s, p99 {st.get('p99_s', 0):.3f} s,"
-                f" peak {srv.peak_resident} resident")
+                f" {st.get('voxel_iters_per_s') or 0:.3e} voxel-iters/s, peak {srv.peak_resident} "
+                f"resident, {st['rejected']} refused")
             del srv, tickets
             torch.cuda.empty_cache()   # the next server's budget starts from the free memory
+        sweep = [r for r in line["serving"] if not r["frames"].startswith("full")]
+        if sweep:
+            # capacity: the most this workload was served at; saturation: the
+            # first rate whose served throughput fell below 90 % of the offered
+            cap = max((r["voxel_iters_per_s"] or 0.0) for r in sweep)
+            sat = next((r["qps"] for r in sweep
+                        if (r["voxel_iters_per_s"] or 0.0) < 0.9 * r["offered_voxel_iters_per_s"]
+                        or r["rejected_queue_full"]), None)
+            line["serving_summary"] = {
+                "capacity_voxel_iters_per_s": cap, "saturation_qps": sat,
+                "mean_request_voxel_iters": float(np.mean(
+                    [r["offered_voxel_iters_per_s"] / r["qps"] for r in sweep])),
+                "beyond_saturation": "arrivals beyond capacity queue up to max_queue and later "
+                                     "ones are refused at submission (OverloadError), so p99 of "
+                                     "the served requests stays bounded"}
 
     # ---- the experiment's grid at every noise level (config 3 in full)
     if run["sigmas"]:
@@ -546,40 +607,77 @@ def main():
                         "adjustment and GVF once per level on the device"}
         del u
 
-    # ---- CPU oracle on a bounded sample of the same workload
+    # ---- CPU oracle on a bounded sample of the same workload, and the
+    # oracle's results beside the GPU's on the same inputs
     if run["cpu"]:
-        from oracle import ref32
+        from oracle import chain, ref32
+        try:
+            build_flags = ref32.use_native_build()
+        except Exception as e:          # no compiler: the portable build is timed
+            build_flags = f"portable x86-64-v3 build (native build failed: {e})"
         threads = len(os.sched_getaffinity(0))
         ref32.set_num_threads(threads)
         ncpu = max(1, min(args.cpu_solves, len(w_all)))
-        alphas = [ph.nonterminal_weight(prep.gms, w_all[j], d_all[j]).astype(np.float32)
-                  for j in range(ncpu)]
-        t1 = time.time()
-        for a0 in alphas:
-            ref32.cmf_solve(img, prep.quats, eigs, a0, prep.cs, prep.ct, steps=cfg.steps,
-                            cc=cfg.cc, num_iter=args.cpu_iters, update=cfg.cut_update,
-                            error_bound=0)
-        tc = time.time() - t1
+        # the reference's own best cell (w 10^-1.625, decay 10^1.5) first, then
+        # the grid's best cell in this run, then cells in grid order
+        cells = [int(np.argmin(np.abs(np.log10(w_all) + 1.625) + np.abs(np.log10(d_all) - 1.5)))]
+        if "grid_quality" in line:
+            cells.append(int(np.argmin(np.abs(w_all - line["grid_quality"]["best_weighting"]) +
+                                       np.abs(d_all - line["grid_quality"]["best_decay"]))))
+        cells += [j for j in range(len(w_all)) if j not in cells]
+        cells = [j for j in dict.fromkeys(cells) if j < S][:ncpu]
+        parity = []
+        tc = 0.0
+        for j in cells:
+            a0 = ph.nonterminal_weight(prep.gms, w_all[j], d_all[j]).astype(np.float32)
+            t1 = time.time()
+            uo, done, _ = ref32.cmf_solve(img, prep.quats, eigs, a0, prep.cs, prep.ct,
+                                          steps=cfg.steps, cc=cfg.cc, num_iter=args.cpu_iters,
+                                          update=cfg.cut_update, error_bound=0)
+            tc += time.time() - t1
+            if ub is not None and args.cpu_iters == args.iters:
+                parity.append({"cell": j, "weighting": float(w_all[j]), "decay": float(d_all[j]),
+                               "max_abs_u": float(np.abs(ub[j].reshape(img.shape) - uo).max()),
+                               "dice_cut": float(ph.dice_between(ub[j].reshape(img.shape), uo)),
+                               "iterations": int(done)})
         line["cpu_baseline"] = {
-            "value": ncpu * n * args.cpu_iters / tc, "unit": "voxel-iters/s", "cores": threads,
-            "kind": "port", "cpu_model": _cpu_model(),
-            "sample": f"{ncpu} of the 425 solves, {args.cpu_iters} of 500 iterations each, 128^3: "
-                      f"CPU restatement (reference algorithm), C/OpenMP ({tc:.1f} s)"}
+            "value": len(cells) * n * args.cpu_iters / tc, "unit": "voxel-iters/s",
+            "cores": threads, "kind": "port", "cpu_model": _cpu_model(), "build": build_flags,
+            "sample": f"{len(cells)} of the 425 solves, {args.cpu_iters} of 500 iterations each, "
+                      f"128^3: CPU restatement (reference algorithm), C/OpenMP ({tc:.1f} s)"}
+        if parity:
+            line["parity_grid_cells"] = parity
+            log("oracle vs GPU grid cells: " + ", ".join(
+                f"cell {p['cell']}: max|du| {p['max_abs_u']:.2e}, Dice {p['dice_cut']:.6f}"
+                for p in parity))
         log(f"cpu oracle: {line['cpu_baseline']['value']:.4e} voxel-iters/s on {threads} threads")
-        # the 64^3 case in full: its 500 iterations on the CPU restatement (inputs
-        # prepared by the library)
+        # the 64^3 workload end to end: the oracle's chain (every stage) against
+        # the GPU pipeline through the numpy API, and the CPU time of each
         spec = ph.PLAIN_64
-        img64, _ = ph.make_helix(spec)
-        prep64h = prepare(img64, spec.sigma, cfg, dev)
-        a64 = ph.nonterminal_weight(prep64h.gms, 10 ** -1.625, 10 ** 1.5).astype(np.float32)
+        img64, truth64 = ph.make_helix(spec)
+        w64, d64 = 10 ** -1.625, 10 ** 1.5
         t1 = time.time()
-        ref32.cmf_solve(img64, prep64h.quats, eigs, a64, prep64h.cs, prep64h.ct, steps=cfg.steps,
-                        cc=cfg.cc, num_iter=500, update=cfg.cut_update, error_bound=0)
-        t64c = time.time() - t1
+        cprep = chain.prepare(img64, spec.sigma)
+        t2 = time.time()
+        u_or, _ = chain.cut(img64, cprep, w64, d64)
+        t3 = time.time()
+        gprep = prepare(img64, spec.sigma, cfg, dev)
+        from miccai2016_amd.pipeline import segment
+        u_gpu = segment(img64, gprep, w64, d64, cfg, dev)
         line["cpu_baseline"]["plain_64"] = {
-            "voxel_iters_per_s": img64.size * 500 / t64c, "solve_s": t64c,
-            "sample": "the whole 64^3 solve, 500 iterations"}
-        log(f"cpu oracle 64^3: {t64c:.2f} s")
+            "voxel_iters_per_s": img64.size * 500 / (t3 - t2), "solve_s": t3 - t2,
+            "pipeline_s": t3 - t1, "prepare_s": t2 - t1,
+            "sample": "the whole 64^3 workload on the CPU restatement: Frangi (3 scales), "
+                      "adjustment, GVF (200), frames, 500 max-flow iterations"}
+        line["parity_pipeline_64"] = {
+            "max_abs_u": float(np.abs(u_gpu - u_or).max()),
+            "dice_cut": float(ph.dice_between(u_gpu, u_or)),
+            "fast_dice_vs_truth_gpu": float(ph.fast_dice(u_gpu, truth64)),
+            "fast_dice_vs_truth_oracle": float(ph.fast_dice(u_or, truth64)),
+            "oracle_ridge": cprep.info,
+            "note": "every stage on both sides (oracle chain: ref32 kernels, scipy labelling, "
+                    "the host spanning tree shared); GPU: prepare + segment"}
+        log(f"64^3 pipeline, oracle vs GPU: {line['parity_pipeline_64']}")
 
     print(json.dumps(line), flush=True)
 

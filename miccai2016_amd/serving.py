@@ -1,6 +1,8 @@
 """Continuous-batching server for ATV max-flow cuts on one GPU.
 
-Requests wait in a FIFO queue. A request is admitted when its device
+Requests wait in a FIFO queue (bounded by ``max_queue`` when given: a
+request that finds it full is refused at once with ``OverloadError``, which
+keeps latency bounded beyond the server's capacity). A request is admitted when its device
 footprint fits the unreserved part of an HBM budget. An admitted cut joins
 the resident set at the next chunk boundary; every chunk, each cut advances
 ``chunk`` iterations: cuts of at least ``resident_min_voxels`` voxels as
@@ -94,6 +96,11 @@ def _prepares(req):
     return isinstance(req, VolumeRequest) or (isinstance(req, PhantomCutRequest) and req.prepare)
 
 
+class OverloadError(RuntimeError):
+    """A request refused at submission because max_queue requests were
+    already waiting for admission (load shedding beyond capacity)."""
+
+
 class Ticket:
     """Handle of one submitted request."""
 
@@ -185,11 +192,13 @@ class MaxFlowServer:
     for requests (default: 90 % of the device's free memory at start).
     chunk: iterations per step of the resident set. prepare_workers: threads
     preparing VolumeRequests (each with its own library context and stream).
+    max_queue: most requests waiting for admission (None: unbounded); a
+    request submitted to a full queue fails at once with OverloadError.
     """
 
     def __init__(self, cut_params, budget_bytes=None, chunk=50, device=0, return_device=False,
                  max_jobs=1024, prepare_cfg=None, resident_min_voxels=RESIDENT_MIN_VOXELS,
-                 prepare_workers=2):
+                 prepare_workers=2, max_queue=None):
         import torch
         self.resident_min = int(resident_min_voxels)
         self.torch = torch
@@ -204,6 +213,9 @@ class MaxFlowServer:
             free, _ = torch.cuda.mem_get_info(self.dev.device)
             budget_bytes = int(0.9 * free)
         self.budget = int(budget_bytes)
+        self.max_queue = None if max_queue is None else int(max_queue)
+        self.rejected = 0         # refused at submission: the queue was full
+        self.peak_queue = 0       # most requests waiting for admission at once
         self.reserved = 0
         self.peak_reserved = 0
         self.peak_resident = 0
@@ -242,7 +254,15 @@ class MaxFlowServer:
             t._done.set()
             return t
         with self._wake:
+            if self.max_queue is not None and len(self._pending) >= self.max_queue:
+                self.rejected += 1
+                t.error = OverloadError(f"{len(self._pending)} requests are waiting for "
+                                        f"admission (max_queue {self.max_queue})")
+                t.t_done = time.perf_counter()
+                t._done.set()
+                return t
             self._pending.append((t, need))
+            self.peak_queue = max(self.peak_queue, len(self._pending))
             self._wake.notify()
         return t
 
@@ -285,10 +305,14 @@ class MaxFlowServer:
     def stats(self):
         lat = np.array([t.latency for t in self.completed if t.error is None])
         wall = (max(t.t_done for t in self.completed) - self.t_start) if self.completed else 0.0
-        out = dict(completed=len(lat), voxel_iterations=self.voxel_iterations, wall_s=wall)
+        out = dict(completed=len(lat), voxel_iterations=self.voxel_iterations, wall_s=wall,
+                   rejected=self.rejected, peak_queue=self.peak_queue)
         if len(lat):
+            waits = np.array([t.t_admit - t.t_submit for t in self.completed
+                              if t.error is None and t.t_admit is not None])
             out.update(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                        mean_s=float(lat.mean()),
+                       queue_wait_p99_s=float(np.percentile(waits, 99)) if len(waits) else None,
                        voxel_iters_per_s=self.voxel_iterations / wall if wall > 0 else None)
         return out
 

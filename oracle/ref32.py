@@ -34,6 +34,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native_build():
+    """Build the oracle for this host's CPU (-march=native, oracle/Makefile's
+    `native` target) and load it instead of the portable build; returns the
+    compiler command line. Both builds compute the same values (no
+    contraction); the native one is the CPU baseline's timed build."""
+    global _lib, _LIB_PATH
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True)
+    flags = subprocess.run(["make", "-s", "-C", _HERE, "native-flags"], check=True,
+                           capture_output=True, text=True).stdout.strip()
+    _LIB_PATH = os.path.join(_HERE, "liboracle_native.so")
+    _lib = None
+    lib()
+    return flags
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -139,3 +139,24 @@ def test_failed_finish_releases_each_request_once():
     srv._fail(jobs[0].ticket, jobs[0].reserved, RuntimeError("again"))
     srv._fail(jobs[1].ticket, jobs[1].reserved, RuntimeError("again"))
     assert srv.reserved == 0 and srv._inflight == 0 and len(srv.completed) == 2
+
+
+def test_full_queue_refuses_requests():
+    import collections
+    import threading
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    srv = _bare_server()
+    srv.params = CmfCutQAnisotropicGpu.CutParameters()
+    srv.resident_min = serving.RESIDENT_MIN_VOXELS
+    srv.budget = 1 << 40
+    srv.max_queue = 2
+    srv.rejected = 0
+    srv.peak_queue = 0
+    srv._pending = collections.deque()
+    srv._wake = threading.Condition(srv._lock)
+    req = serving.PhantomCutRequest((8, 8, 8), 2, 2, 10, 0.5, 0.1, 1.0, num_iter=10)
+    tickets = [srv.submit(req) for _ in range(4)]
+    assert [t.done() for t in tickets] == [False, False, True, True]
+    with pytest.raises(serving.OverloadError):
+        tickets[2].wait(0)
+    assert srv.rejected == 2 and srv.peak_queue == 2 and len(srv._pending) == 2
