@@ -287,6 +287,24 @@ int mh_cmf_solve(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const float *
                  int64_t work_bytes, mh_cmf_result *result_host, float *crit_host,
                  void *stream);
 
+/* mh_cmf_solve without a workspace argument and without any device
+ * allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The workspace lives in a static device
+ * array of 64 MiB per GPU, held from the first mh_create on that GPU until the
+ * process ends, so volumes are limited to MH_CMF_NOALLOC_MAX_VOXELS voxels
+ * (larger ones are rejected with MH_ERR_ARG). Same arguments, results (bit
+ * for bit) and blocking as mh_cmf_solve; the context's device must be
+ * current. The method is the one mh_cmf_solve tuned for these dims on this
+ * context, else the untuned default; it never tunes and never captures a
+ * graph. Every call on a GPU shares its array: calls that overlap on one GPU
+ * (other streams or threads) race and must be serialised by the caller. */
+#define MH_CMF_NOALLOC_MAX_VOXELS (1 << 20)
+int mh_cmf_solve_noalloc(mh_ctx *ctx, float *u_dev, const float *eigs_dev, const float *alpha_dev,
+                         const float *cs_dev, const float *ct_dev, const uint8_t *labels_dev,
+                         const int64_t dims[3], const mh_cmf_params *params,
+                         mh_cmf_result *result_host, float *crit_host, void *stream);
+
 /* ------------------------------------------------------ batched max-flow */
 
 /* Many independent unsupervised constant-eigenvalue solves on volumes that

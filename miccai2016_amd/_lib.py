@@ -86,6 +86,8 @@ SIGNATURES = {
     "mh_cmf_workspace_bytes": (i64, [i64]),
     "mh_cmf_solve": (i32, [vp, vp, vp, vp, vp, vp, vp, I64x3, C.POINTER(CmfParams), vp, i64,
                            C.POINTER(CmfResult), vp, vp]),
+    "mh_cmf_solve_noalloc": (i32, [vp, vp, vp, vp, vp, vp, vp, I64x3, C.POINTER(CmfParams),
+                                   C.POINTER(CmfResult), vp, vp]),
     "mh_cmf_batched_workspace_bytes": (i64, [i64, i32]),
     "mh_cmf_solve_batched": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, I64x3,
                                    C.POINTER(CmfParams), vp, i64, C.POINTER(CmfResult), vp]),

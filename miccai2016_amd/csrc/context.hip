@@ -88,6 +88,9 @@ MH_EXPORT int mh_create(int device, mh_ctx **out)
         delete ctx;
         return MH_ERR_HIP;
     }
+    // the no-allocation solver's array on this device; a failure is kept and
+    // reported by that entry point (every other entry point still works)
+    if (mh_noalloc_setup(ctx) != MH_OK) ctx->noalloc_error = ctx->last_error;
     *out = ctx;
     return MH_OK;
 }

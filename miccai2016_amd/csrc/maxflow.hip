@@ -10,6 +10,7 @@
 // one-block kernel adds them in block order and decides the stop, so the host
 // never waits inside the loop. A stopped solve turns every later launch into
 // an immediate return.
+#include <mutex>
 #include <vector>
 
 #include "cmf_voxel.h"
@@ -972,6 +973,7 @@ struct Runner {
     bool sup, sweep_elig;
     int cur = 0, it = 0, limit = 0;
     bool swept = false;
+    bool graphs_allowed = true;  // false: never capture (no stream or graph is created)
 
     Runner(mh_ctx *c, float *u_, const float *e, const float *a, const float *cs_, const float *ct_,
            const uint8_t *l, Dims d, int64_t n_, const mh_cmf_params *p, const Workspace &w,
@@ -1137,7 +1139,7 @@ struct Runner {
         // replay ends the solve the same way. Off by default: on gfx950
         // back-to-back launches on one stream already run at the dispatch
         // rate, and replayed cycles measured 1-20% slower from 32^3 to 128^3.
-        if (m.shape < 0 && ctx->graph_mode == 1) {
+        if (m.shape < 0 && ctx->graph_mode == 1 && graphs_allowed) {
             while (it < limit && it % prm->update != 0) step(m);
             const int cycles = (limit - it) / prm->update;
             if (cycles >= 2) {
@@ -1269,15 +1271,16 @@ int read_result(mh_ctx *ctx, const Workspace &W, const mh_cmf_params *prm, mh_cm
     MH_CHECK(ctx, hipMemcpyAsync(ctrl, W.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, s));
     MH_CHECK(ctx, hipStreamSynchronize(s));
     const int ncrit = std::min(ctrl[kNcrit], kCritLogMax);
-    std::vector<float> log(std::max(ncrit, 1));
+    float last = 0.f;
     if (ncrit > 0)
-        MH_CHECK(ctx, hipMemcpy(log.data(), W.crit_log, sizeof(float) * ncrit, hipMemcpyDeviceToHost));
-    if (crit_host && ncrit > 0) std::copy(log.begin(), log.begin() + ncrit, crit_host);
+        MH_CHECK(ctx, hipMemcpy(&last, W.crit_log + ncrit - 1, sizeof(float), hipMemcpyDeviceToHost));
+    if (crit_host && ncrit > 0)
+        MH_CHECK(ctx, hipMemcpy(crit_host, W.crit_log, sizeof(float) * ncrit, hipMemcpyDeviceToHost));
     if (res) {
         res->stopped = ctrl[kStop];
         res->ncrit = ncrit;
         res->iterations = ctrl[kStop] ? (ctrl[kNcrit] - 1) * prm->update + 1 : prm->num_iter;
-        res->last_crit = ncrit > 0 ? log[ncrit - 1] : 0.f;
+        res->last_crit = last;
     }
     return ctrl[kStop] == 1 ? MH_STOPPED_NAN : MH_OK;
 }
@@ -1369,6 +1372,92 @@ MH_EXPORT int mh_cmf_resident_finish(mh_ctx *ctx, mh_cmf_resident *r, mh_cmf_res
     rc = R.finish();
     if (rc) return rc;
     return read_result(ctx, W, &r->params, res, crit_host, (hipStream_t)stream);
+}
+
+// =========================================================================
+// mh_cmf_solve for callers that cannot allocate device memory. The solve's
+// workspace (control block, criterion log, partial sums, both state sets of
+// the two-level sweep and the expanded frames: mh_cmf_workspace_bytes(n),
+// about 60 B per voxel) lives in `scratch`, one file-scope device array per
+// GPU. Its device address is looked up once per device when a context is
+// created on it (mh_noalloc_setup, called by mh_create with the device
+// current) and only that address is handed to kernels. Every call on a GPU
+// shares the array: the control block is cleared by a kernel at the start of
+// each solve and every other word is written before it is read, so a call
+// never sees the previous call's values; calls that overlap on one GPU (two
+// streams, two threads) race on it.
+// =========================================================================
+
+constexpr int64_t kNoAllocMaxVoxels = MH_CMF_NOALLOC_MAX_VOXELS;
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+// Called by mh_create on ctx->device (current): resolves this device's copy
+// of `scratch` once and records it in the context.
+int mh_noalloc_setup(mh_ctx *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+        return mh::set_error(ctx, MH_ERR_ARG, "mh_create", "device index beyond the no-allocation table");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return mh::set_error(ctx, MH_ERR_HIP, "hipGetSymbolAddress(scratch)",
+                                 e != hipSuccess ? hipGetErrorString(e) : "null address");
+        g_noalloc_ptr[ctx->device] = p;
+    }
+    ctx->noalloc_scratch = g_noalloc_ptr[ctx->device];
+    return MH_OK;
+}
+
+MH_EXPORT int mh_cmf_solve_noalloc(mh_ctx *ctx, float *u, const float *eigs, const float *alpha,
+                                   const float *cs, const float *ct, const uint8_t *labels,
+                                   const int64_t dims[3], const mh_cmf_params *prm,
+                                   mh_cmf_result *res, float *crit_host, void *stream)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
+    MH_REQUIRE(ctx, u && eigs && alpha && cs && ct && prm, "null pointer");
+    MH_REQUIRE(ctx, prm->num_iter >= 0 && prm->update >= 1, "num_iter >= 0 and update >= 1 required");
+    MH_REQUIRE(ctx, prm->rcc > 0.f, "rcc must be positive");
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    MH_REQUIRE(ctx, n <= kNoAllocMaxVoxels,
+               "volume exceeds the no-allocation limit of MH_CMF_NOALLOC_MAX_VOXELS (2^20) voxels");
+    MH_REQUIRE(ctx, layout(n, nullptr, nullptr) <= int64_t(kNoAllocScratchBytes),
+               "workspace exceeds the no-allocation scratch array");
+    if (!ctx->noalloc_scratch)
+        return mh::set_error(ctx, MH_ERR_HIP, __func__,
+                             ctx->noalloc_error.empty() ? "scratch array not set up"
+                                                        : ctx->noalloc_error.c_str());
+    int cur_dev = -1;
+    MH_CHECK(ctx, hipGetDevice(&cur_dev));
+    MH_REQUIRE(ctx, cur_dev == ctx->device, "the context's device must be current");
+    Workspace W;
+    layout(n, static_cast<char *>(ctx->noalloc_scratch), &W);
+    hipStream_t s = (hipStream_t)stream;
+    const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
+    // the method mh_cmf_solve tuned for these dims on this context, else the
+    // untuned default; never tuned here (tuning creates events) and never
+    // captured as a graph
+    Runner R(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
+    R.graphs_allowed = false;
+    int rc = R.init();
+    if (rc) return rc;
+    SweepMethod use;
+    rc = R.choose(false, use);
+    if (rc) return rc;
+    rc = R.run_until(use, prm->num_iter);
+    if (rc) return rc;
+    rc = R.finish();
+    if (rc) return rc;
+    return read_result(ctx, W, prm, res, crit_host, s);
 }
 
 // =========================================================================

@@ -51,7 +51,15 @@ struct mh_ctx {
     // resident work-groups of the sweep kernel on the device, per tile shape
     // (CU count x occupancy; 0 until queried)
     std::array<int, 8> sweep_slots{};
+    // this device's copy of the no-allocation solver's scratch array (set by
+    // mh_create; never freed, it lives as long as the process)
+    void *noalloc_scratch = nullptr;
+    std::string noalloc_error;  // why it is null, if it is
 };
+
+// Resolves the no-allocation scratch array of ctx->device (current) into
+// ctx->noalloc_scratch; maxflow.hip.
+int mh_noalloc_setup(mh_ctx *ctx);
 
 namespace mh {
 
