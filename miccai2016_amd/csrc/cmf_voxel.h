@@ -62,11 +62,80 @@ __device__ __forceinline__ void cmf_p_voxel_m(float &h0, float &h1, float &h2, f
     h2 = fmaf(m[8], p2, fmaf(m[7], p1, m[6] * p0));
 }
 
-// the same with the rotation given as the unit quaternion (w, x, y, z) in qq
+// Constant eigenvalues (l0, l1, l1), the two smaller ones equal (the
+// experiment's ATV (1, a, a) and TV (1, 1, 1)): then H H^T = A =
+// l1 I + (l0 - l1) v v^T with v = R e0, the frame's first column, and the
+// p-step reduces to
+//   z = hp + A g,  |p|^2 = z^T A^-1 z = (v.z)^2 / l0 + |z - (v.z) v|^2 / l1,
+//   hp' = min(1, alpha / |p|) z,
+// the same map as cmf_p_voxel_m (p = H^-1 hp + H^T g, hp' = H proj(p)) in
+// about 30 operations instead of 47, needing v only. Both terms of |p|^2 are
+// non-negative, so nothing cancels.
+struct AxiParams {
+    int on;       // the eigenvalues are constant and l1 == l2
+    float l0, l1;
+    float dl;     // l0 - l1
+    float il0, il1;
+};
+
+// AxiParams of a solve (host): on when the eigenvalues are constant and the
+// two smaller square roots are equal; l = sq^2, formed in double
+inline AxiParams axi_params(const mh_cmf_params *p)
+{
+    AxiParams A{};
+    A.on = p->const_evals && p->sqev[1] == p->sqev[2];
+    const double l0 = double(p->sqev[0]) * double(p->sqev[0]);
+    const double l1 = double(p->sqev[1]) * double(p->sqev[1]);
+    A.l0 = float(l0);
+    A.l1 = float(l1);
+    A.dl = float(l0 - l1);
+    A.il0 = l0 > 0 ? float(1.0 / l0) : 0.f;
+    A.il1 = l1 > 0 ? float(1.0 / l1) : 0.f;
+    return A;
+}
+
+// v = first column of the rotation of the unit quaternion qq = (w, x, y, z)
+__device__ __forceinline__ void frame_axis(const float4 qq, float &v0, float &v1, float &v2)
+{
+#pragma clang fp contract(off)
+    const float w = qq.x, x = qq.y, y = qq.z, z = qq.w;
+    v0 = 1.f - 2.f * (y * y + z * z);
+    v1 = 2.f * (x * y + w * z);
+    v2 = 2.f * (x * z - w * y);
+}
+
+__device__ __forceinline__ void cmf_p_voxel_axi(float &h0, float &h1, float &h2, float g0, float g1,
+                                                float g2, float v0, float v1, float v2,
+                                                const AxiParams &A, float alpha)
+{
+#pragma clang fp contract(off)
+    const float t = A.dl * fmaf(v2, g2, fmaf(v1, g1, v0 * g0));
+    const float z0 = fmaf(t, v0, fmaf(A.l1, g0, h0));
+    const float z1 = fmaf(t, v1, fmaf(A.l1, g1, h1));
+    const float z2 = fmaf(t, v2, fmaf(A.l1, g2, h2));
+    const float c = fmaf(v2, z2, fmaf(v1, z1, v0 * z0));
+    const float e0 = fmaf(-c, v0, z0), e1 = fmaf(-c, v1, z1), e2 = fmaf(-c, v2, z2);
+    const float nrm = __builtin_amdgcn_sqrtf(
+        fmaf(A.il1, fmaf(e2, e2, fmaf(e1, e1, e0 * e0)), A.il0 * (c * c)));
+    const float sc = nrm > alpha ? alpha * __builtin_amdgcn_rcpf(nrm) : 1.f;
+    h0 = sc * z0;
+    h1 = sc * z1;
+    h2 = sc * z2;
+}
+
+// the p-step with the rotation given as the unit quaternion (w, x, y, z) in
+// qq: the axial form when A.on, else the general one
 __device__ __forceinline__ void cmf_p_voxel(float &h0, float &h1, float &h2, float g0, float g1,
                                             float g2, const float4 qq, float sq0, float sq1,
-                                            float sq2, float rq0, float rq1, float rq2, float alpha)
+                                            float sq2, float rq0, float rq1, float rq2,
+                                            const AxiParams &A, float alpha)
 {
+    if (A.on) {
+        float v0, v1, v2;
+        frame_axis(qq, v0, v1, v2);
+        cmf_p_voxel_axi(h0, h1, h2, g0, g1, g2, v0, v1, v2, A, alpha);
+        return;
+    }
     cmf_p_voxel_m(h0, h1, h2, g0, g1, g2, quat_to_mat(Quat{qq.x, qq.y, qq.z, qq.w}), sq0, sq1, sq2,
                   rq0, rq1, rq2, alpha);
 }

@@ -27,6 +27,7 @@ struct JobDesc {
     int *ctrl;
     float rcc, inv_rcc, rs0, rs1, rs2, ss0, ss1, ss2, sq0, sq1, sq2, rq0, rq1, rq2, error_bound,
         inv_n_rcc;
+    mh::AxiParams A;
     int iter0, num_iter, update, maxcheck;
 };
 
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(mh::kBlock) jobs_p_kernel(const JobDesc *__res
     const float g1 = i1 < D.d1 - 1 ? (tmp[i + D.d2] - t) * D.ss1 : 0.f;
     const float g2 = i2 < D.d2 - 1 ? (tmp[i + 1] - t) * D.ss2 : 0.f;
     float h0 = D.hp[i], h1 = D.hp[n + i], h2 = D.hp[2 * n + i];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, D.q4[i], D.sq0, D.sq1, D.sq2, D.rq0, D.rq1, D.rq2,
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, D.q4[i], D.sq0, D.sq1, D.sq2, D.rq0, D.rq1, D.rq2, D.A,
                     D.alpha[i]);
     D.hp[i] = h0;
     D.hp[n + i] = h1;
@@ -206,6 +207,7 @@ int build_descs(mh_ctx *ctx, mh_cmf_job *jobs, int nj, std::vector<JobDesc> &D, 
         d.rq0 = float(1.0 / double(P.sqev[0]));
         d.rq1 = float(1.0 / double(P.sqev[1]));
         d.rq2 = float(1.0 / double(P.sqev[2]));
+        d.A = mh::axi_params(&P);
         d.inv_rcc = float(1.0 / double(P.rcc));
         d.error_bound = P.error_bound;
         d.inv_n_rcc = float(1.0 / (double(n) * double(P.rcc)));

@@ -69,13 +69,14 @@ struct PParams {
     float ss0, ss1, ss2;
     float sq0, sq1, sq2;  // constant sqrt(lambda) (CONST_EV)
     float rq0, rq1, rq2;  // 1 / sq (CONST_EV)
+    mh::AxiParams A;      // the axial p-step (CONST_EV, l1 == l2)
 };
 
 PParams p_params(const mh_cmf_params *prm)
 {
     return PParams{prm->ss[0], prm->ss[1], prm->ss[2], prm->sqev[0], prm->sqev[1], prm->sqev[2],
                    float(1.0 / double(prm->sqev[0])), float(1.0 / double(prm->sqev[1])),
-                   float(1.0 / double(prm->sqev[2]))};
+                   float(1.0 / double(prm->sqev[2])), mh::axi_params(prm)};
 }
 
 template <bool CONST_EV>
@@ -106,7 +107,9 @@ cmf_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp, const float4
         rq2 = 1.f / sq2;
     }
     float h0 = hp[i], h1 = hp[n + i], h2 = hp[2 * n + i];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], sq0, sq1, sq2, rq0, rq1, rq2, alpha[i]);
+    mh::AxiParams A = P.A;
+    if (!CONST_EV) A.on = 0;  // pointwise eigenvalues: the general form
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], sq0, sq1, sq2, rq0, rq1, rq2, A, alpha[i]);
     hp[i] = h0;
     hp[n + i] = h1;
     hp[2 * n + i] = h2;
@@ -217,8 +220,9 @@ constexpr SweepShape kSweepShapes[] = {{12, 64, 1024}, {11, 32, 512}, {4, 64, 51
 constexpr int kNumSweepShapes = 5;
 
 // Two solves per thread: the descriptors span both solves' fields (hp: 24 B
-// per voxel), so the volume must stay below half the single-solve limit.
-constexpr int64_t kSweepPairMaxVoxels = int64_t(1) << 27;
+// per voxel), so the volume must stay below half the single-solve limit
+// (every byte offset below 2^31).
+constexpr int64_t kSweepPairMaxVoxels = int64_t(1) << 26;
 
 // Batched solves advance two per thread with the 1024-thread tile shapes
 // (measured 5 % faster at 128^3: one work-group per CU either way, the pair
@@ -265,10 +269,13 @@ struct SweepInShared {
 // The sweep addresses its fields through buffer resources: a uniform byte
 // offset (the plane, and the component of hp) plus the thread's 32-bit byte
 // offset within the plane, so no 64-bit address is formed per lane. Every
-// byte offset stays below 2^32 (the sweep takes volumes of fewer than 2^28
-// voxels), and each resource spans exactly its field.
+// byte offset stays below 2^31 (the sweep takes volumes of fewer than 2^27
+// voxels: the largest field, the AoS x4 frame, holds 16 B per voxel), so the
+// offsets are non-negative as the signed operands of the buffer builtins, and
+// each resource spans exactly its field. Larger volumes run the per-iteration
+// kernels, which index in 64 bits.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
-constexpr int64_t kSweepMaxVoxels = int64_t(1) << 28;
+constexpr int64_t kSweepMaxVoxels = int64_t(1) << 27;
 constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit data, gfx9 family
 
 __device__ __forceinline__ Rsrc rsrc_of(const void *p, int64_t bytes)
@@ -289,7 +296,7 @@ __device__ __forceinline__ void bst(Rsrc r, unsigned voff, unsigned soff, float 
 }
 // the same store through a 64-bit base address (two scalar registers instead
 // of a four-register descriptor). The byte offset voff + soff is summed in 32
-// bits (it stays below 2^32 for every volume the sweep takes, as the
+// bits (it stays below 2^31 for every volume the sweep takes, as the
 // descriptors' offsets do), so the store takes the base in scalar registers
 // and one 32-bit offset, which stores at the same offset into different
 // fields share; adding soff to a base + voff address instead kept a 64-bit
@@ -321,7 +328,7 @@ __device__ __forceinline__ void sweep_store(Rsrc r, float *base, unsigned voff, 
 
 // Loads of the sweep: through a descriptor (plane offset in a scalar
 // register, for free), or through a 64-bit base with the 32-bit byte offset
-// voff + soff (which stays below 2^32, as the descriptors' offsets do). One
+// voff + soff (which stays below 2^31, as the descriptors' offsets do). One
 // solve per thread with 1024 threads loads through bases: the descriptors'
 // 32 scalar registers were spilled and reloaded in every plane step (84 -> 74
 // VGPRs, 37 -> 21 lane reloads per two plane steps); with two solves per
@@ -559,7 +566,7 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     float aP[NS];
     float err_acc[NS];
     float csP = 0.f, ctP = 0.f;
-    mh::Mat3 RP = mh::quat_to_mat(mh::Quat{1.f, 0.f, 0.f, 0.f});  // frame at z-1
+    float vP0 = 1.f, vP1 = 0.f, vP2 = 0.f;  // frame axis at z-1
     SweepIn L[NS];
     SweepInShared LC;
     LC.q = make_float4(1.f, 0.f, 0.f, 0.f);
@@ -598,8 +605,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
         // 1. level-A p-step at plane z on the widened tile
         // (lanes outside a phase's region keep values no valid lane reads)
         float hA0_0[NS], hA0_1[NS], hA0_2[NS];
-        // the frame of plane z, built once for both levels and every solve
-        const mh::Mat3 RN = mh::quat_to_mat(mh::Quat{LC.q.x, LC.q.y, LC.q.z, LC.q.w});
+        // the frame axis of plane z, formed once for both levels and every solve
+        float vN0, vN1, vN2;
+        mh::frame_axis(LC.q, vN0, vN1, vN2);
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             hA0_0[j] = L[j].h0;
@@ -612,8 +620,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                 const float g0 = e0 ? (L[j].t0 - L[j].t) * PP.ss0 : 0.f;
                 const float g1 = (f & 16u) ? (L[j].t1 - L[j].t) * PP.ss1 : 0.f;
                 const float g2 = (f & 32u) ? (L[j].t2 - L[j].t) * PP.ss2 : 0.f;
-                mh::cmf_p_voxel_m(hA0_0[j], hA0_1[j], hA0_2[j], g0, g1, g2, RN, PP.sq0, PP.sq1,
-                                  PP.sq2, PP.rq0, PP.rq1, PP.rq2, L[j].a);
+                mh::cmf_p_voxel_axi(hA0_0[j], hA0_1[j], hA0_2[j], g0, g1, g2, vN0, vN1, vN2, PP.A,
+                                    L[j].a);
             }
             if (!two && inO && z >= z0) {
 #pragma unroll
@@ -690,8 +698,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
                 const float g0 = e0b ? (tA0[j] - tAp[j]) * PP.ss0 : 0.f;
                 const float g1 = (f & 16u) ? (t1 - tAp[j]) * PP.ss1 : 0.f;
                 const float g2 = (f & 32u) ? (t2 - tAp[j]) * PP.ss2 : 0.f;
-                mh::cmf_p_voxel_m(hB1_0[j], hB1_1[j], hB1_2[j], g0, g1, g2, RP, PP.sq0, PP.sq1,
-                                  PP.sq2, PP.rq0, PP.rq1, PP.rq2, aP[j]);
+                mh::cmf_p_voxel_axi(hB1_0[j], hB1_1[j], hB1_2[j], g0, g1, g2, vP0, vP1, vP2, PP.A,
+                                    aP[j]);
             }
         }
         if (live) {
@@ -748,7 +756,9 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
             tAp[j] = tA0[j];
             aP[j] = aN[j];
         }
-        RP = RN;
+        vP0 = vN0;
+        vP1 = vN1;
+        vP2 = vN2;
         csP = csN;
         ctP = ctN;
     };
@@ -983,7 +993,8 @@ struct Runner {
           UP{p->rcc, float(1.0 / double(p->rcc)), p->rs[0], p->rs[1], p->rs[2], p->error_bound,
              float(1.0 / (double(n_) * double(p->rcc)))},
           sup(l != nullptr),
-          sweep_elig(p->const_evals && l == nullptr && n_ < kSweepMaxVoxels),
+          sweep_elig(p->const_evals && l == nullptr && n_ < kSweepMaxVoxels &&
+                     mh::axi_params(p).on),
           limit(p->num_iter)
     {
     }
@@ -1547,7 +1558,7 @@ cmf_batch_p_kernel(float *__restrict__ hp, const float *__restrict__ tmp,
         const float g1 = e1 ? (a1[k] - t[k]) * P.ss1 : 0.f;
         const float g2 = e2 ? (a2[k] - t[k]) * P.ss2 : 0.f;
         mh::cmf_p_voxel(h0[k], h1[k], h2[k], g0, g1, g2, qq, P.sq0, P.sq1, P.sq2, P.rq0, P.rq1, P.rq2,
-                        SP.w[k] * expf(-gm * SP.k[k]));
+                        P.A, SP.w[k] * expf(-gm * SP.k[k]));
         float *h = hp + 3 * n * (s0 + k);
         h[i] = h0[k];
         h[n + i] = h1[k];
@@ -1805,7 +1816,8 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     // if it has not), else groups of 4 through the per-iteration kernels,
     // which share the frame and capacities in registers.
     const std::array<int, 3> key{D.d0, D.d1, D.d2};
-    const bool sweep_elig = V >= prm->num_iter && n < kSweepMaxVoxels && ctx->batch_sweep != 1;
+    const bool sweep_elig = V >= prm->num_iter && n < kSweepMaxVoxels && ctx->batch_sweep != 1 &&
+                            mh::axi_params(prm).on;
     bool want_sweep = sweep_elig && ctx->batch_sweep == 2;
     SweepMethod sm{0, 16};
     if (ctx->sweep_mode == 2) {
@@ -2019,7 +2031,7 @@ ragged_p_kernel(const Vol *__restrict__ vols, int nvol, float *__restrict__ hp,
     const float g2 = i2 < V.d2 - 1 ? (tmp[i + 1] - t) * P.ss2 : 0.f;
     float *h = hp + 3 * V.off;
     float h0 = h[li], h1 = h[n + li], h2 = h[2 * n + li];
-    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], P.sq0, P.sq1, P.sq2, P.rq0, P.rq1, P.rq2,
+    mh::cmf_p_voxel(h0, h1, h2, g0, g1, g2, q4[i], P.sq0, P.sq1, P.sq2, P.rq0, P.rq1, P.rq2, P.A,
                     alpha[i]);
     h[li] = h0;
     h[n + li] = h1;
