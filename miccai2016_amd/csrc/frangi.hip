@@ -13,6 +13,7 @@ namespace {
 
 constexpr int kMaxRadius = 63;
 constexpr int kMaxTaps = 2 * kMaxRadius + 1;
+constexpr int kTapsPad = 128;   // taps per order (>= kMaxTaps), zero beyond K
 
 struct ConvOut {
     float *ptr;
@@ -34,7 +35,9 @@ struct ConvPass {
     ConvGroup grp[6];
     int ngroups;
     int radius;
-    float taps[3][kMaxTaps];
+    // taps of each derivative order in convolution order: ctaps[o][j] is tap
+    // K-1-j, the weight of element p - R + j for output p; zero for j >= K
+    float ctaps[3][kTapsPad];
 };
 
 // Index of position p of an axis of length len, mirror-completed and tiled
@@ -54,9 +57,9 @@ __device__ __forceinline__ int mirror_index(int p, int len)
 
 // Both pass kernels stage a tile of the input line segments plus a halo of
 // R taps on each side in LDS (each element loaded once, coalesced along axis
-// 2), together with the taps in convolution order; every output then sums
-// its K taps from LDS in the order j = 0 .. K-1 of the true convolution (tap
-// K-1-j against element p - R + j).
+// 2); every output then sums its K taps in the order j = 0 .. K-1 of the true
+// convolution (tap K-1-j against element p - R + j). The taps are uniform
+// across the work-group and are read from the kernel arguments.
 
 constexpr int kTileX = 64;    // consecutive voxels along axis 2 per tile
 constexpr int kRunA = 16;     // outputs per thread along the pass axis (axes 0, 1)
@@ -65,10 +68,72 @@ constexpr int kRows2 = 16;    // rows (axis 1) per tile of the axis-2 pass
 
 // Pass along axis 0 or 1. Tile: 64 voxels along axis 2 x 64 along the pass
 // axis; thread (x, y) computes the 16 consecutive outputs y*16 .. y*16+15 of
-// column x. LDS: (64 + 2R) x 64 floats and the group's taps (up to three
-// orders).
+// column x. LDS: 64 x (64 + 2R + 16) floats: the tile and its halo, then 16
+// zero rows that the window's last loads read; then the group's taps.
+//
+// The thread keeps a sliding window of the 16 column elements its outputs
+// need for the current tap in registers, as a ring indexed by position mod
+// 16: at tap j, output q reads element q + j from slot (q + j) mod 16, and
+// once all 16 outputs have used element j its slot takes element j + 16.
+// Taps go in unrolled blocks of 16 (the last one cut at K by a uniform
+// test), so every slot index is a constant: one LDS read per tap for the
+// column and one broadcast read per tap and output order for the weights,
+// instead of one per tap and output.
+constexpr int kZeroRows = kRunA;
+
+template <int NOUT>
+__device__ __forceinline__ void conv_tap(const float *wts, int j, int jj, float *win,
+                                         float (*acc)[kRunA], const float *col)
+{
+    float w[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) w[k] = wts[k * kTapsPad + j];
+#pragma unroll
+    for (int q = 0; q < kRunA; ++q) {
+        const float v = win[(jj + q) & (kRunA - 1)];
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) acc[k][q] += w[k] * v;
+    }
+    // element j is used up: its slot takes element j + 16
+    win[jj] = col[(j + kRunA) * kTileX];
+}
+
+template <int NOUT>
+__device__ __forceinline__ void conv_column(const float *wts, const ConvOut *out,
+                                            const float *col, int K, bool live, int64_t base,
+                                            int64_t sa, int a0, int len, int y)
+{
+    float acc[NOUT][kRunA];
+#pragma unroll
+    for (int q = 0; q < kRunA; ++q)
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) acc[k][q] = 0.f;
+    float win[kRunA];
+#pragma unroll
+    for (int q = 0; q < kRunA; ++q) win[q] = col[q * kTileX];
+    int j0 = 0;
+    for (; j0 + kRunA <= K; j0 += kRunA) {
+#pragma unroll
+        for (int jj = 0; jj < kRunA; ++jj) conv_tap<NOUT>(wts, j0 + jj, jj, win, acc, col);
+    }
+    const int rem = K - j0;
+#pragma unroll
+    for (int jj = 0; jj < kRunA; ++jj)
+        if (jj < rem) conv_tap<NOUT>(wts, j0 + jj, jj, win, acc, col);
+    if (!live) return;
+#pragma unroll
+    for (int q = 0; q < kRunA; ++q) {
+        const int a = a0 + y * kRunA + q;
+        if (a < len) {
+            const int64_t i = base + int64_t(a) * sa;
+#pragma unroll
+            for (int k = 0; k < NOUT; ++k) out[k].ptr[i * out[k].stride + out[k].offset] = acc[k][q];
+        }
+    }
+}
+
 template <int AXIS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
 conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
 {
     extern __shared__ float lds[];
@@ -85,53 +150,52 @@ conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
     const int i2 = t2 * kTileX + x;
     const int a0 = ta * kTileA;
     const int R = P.radius, K = 2 * R + 1, rows = kTileA + 2 * R;
+    const int rows_alloc = rows + kZeroRows;
     const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : int64_t(d2);
     const int64_t base = AXIS == 0 ? int64_t(o) * d2 + i2 : int64_t(o) * d1 * d2 + i2;
     const bool live = i2 < d2;
-    float *wts = lds + rows * kTileX;
+    const float *col = lds + (y * kRunA) * kTileX + x;
+    float *wts = lds + rows_alloc * kTileX;   // the group's taps, [output][j]
     for (int g = 0; g < P.ngroups; ++g) {
         const ConvGroup &G = P.grp[g];
-        for (int r = y; r < rows; r += 4)
-            lds[r * kTileX + x] = live ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa] : 0.f;
-        for (int t = threadIdx.x; t < 3 * K; t += 256)
-            wts[t] = P.taps[G.out[t / K].order][K - 1 - t % K];
+        for (int r = y; r < rows_alloc; r += 4)
+            lds[r * kTileX + x] = (live && r < rows)
+                                      ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa]
+                                      : 0.f;
+        for (int t = threadIdx.x; t < G.nout * kTapsPad; t += 256)
+            wts[t] = P.ctaps[G.out[t / kTapsPad].order][t % kTapsPad];
         __syncthreads();
-        float acc[3][kRunA];
-#pragma unroll
-        for (int q = 0; q < kRunA; ++q) acc[0][q] = acc[1][q] = acc[2][q] = 0.f;
-        const float *col = lds + (y * kRunA) * kTileX + x;
-        for (int j = 0; j < K; ++j) {
-            const float w0 = wts[j], w1 = wts[K + j], w2 = wts[2 * K + j];
-#pragma unroll
-            for (int q = 0; q < kRunA; ++q) {
-                const float v = col[(q + j) * kTileX];
-                acc[0][q] += w0 * v;
-                if (G.nout > 1) acc[1][q] += w1 * v;
-                if (G.nout > 2) acc[2][q] += w2 * v;
-            }
-        }
-        if (live) {
-#pragma unroll
-            for (int q = 0; q < kRunA; ++q) {
-                const int a = a0 + y * kRunA + q;
-                if (a < len) {
-                    const int64_t i = base + int64_t(a) * sa;
-                    G.out[0].ptr[i * G.out[0].stride + G.out[0].offset] = acc[0][q];
-                    if (G.nout > 1) G.out[1].ptr[i * G.out[1].stride + G.out[1].offset] = acc[1][q];
-                    if (G.nout > 2) G.out[2].ptr[i * G.out[2].stride + G.out[2].offset] = acc[2][q];
-                }
-            }
-        }
+        if (G.nout == 3) conv_column<3>(wts, G.out, col, K, live, base, sa, a0, len, y);
+        else if (G.nout == 2) conv_column<2>(wts, G.out, col, K, live, base, sa, a0, len, y);
+        else conv_column<1>(wts, G.out, col, K, live, base, sa, a0, len, y);
         __syncthreads();
     }
 }
 
 // Pass along axis 2 (contiguous), one output per group. Tile: 64 outputs
-// along axis 2 x 16 rows along axis 1; thread (x, y) computes voxel x of rows
-// y, y+4, y+8, y+12; the tile rows are loaded by 16 threads each. LDS:
-// 16 x (64 + 2R) floats and the group's K taps. The results of all groups
-// stay in registers and are stored together, so a voxel's AoS components
-// leave in one burst.
+// along axis 2 x 16 rows along axis 1; the tile rows are loaded by 16
+// threads each. Lane l of wave w computes the 4 consecutive outputs
+// 4 (l mod 16) .. 4 (l mod 16) + 3 of row 4 w + l / 16. LDS: 16 rows of
+// W floats, W = 64 + 2R rounded up past 8 zero columns to a multiple of 4
+// (16-byte aligned rows). The thread's window is two groups of 4 elements,
+// A (elements j0 .. j0+3 of its run) and B (j0+4 .. j0+7), each one 16-byte
+// LDS read: taps go in blocks of 4 (the last one cut at K by a uniform
+// test), where output q at tap j0 + jj reads element q + jj of A:B, then A
+// takes B and B the next 4 elements. The results of all groups stay in
+// registers and are stored together, so a voxel's AoS components leave in
+// one burst.
+__device__ __forceinline__ float4 lds4(const float *p)
+{
+    return *reinterpret_cast<const float4 *>(p);
+}
+
+__device__ __forceinline__ void row_tap(float *a, const float4 &A, const float4 &B, int jj, float w)
+{
+    const float v[8] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] += w * v[q + jj];
+}
+
 __global__ void __launch_bounds__(256)
 conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
 {
@@ -143,47 +207,73 @@ conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
     const int t1 = int((b / unsigned(n2)) % unsigned(n1));
     const int i0 = int(b / (unsigned(n2) * unsigned(n1)));
     if (i0 >= d0) return;
-    const int x = threadIdx.x & (kTileX - 1), y = threadIdx.x / kTileX;
-    const int R = P.radius, K = 2 * R + 1, width = kTileX + 2 * R;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = wave * 4 + (lane >> 4), xq = 4 * (lane & 15);
+    const int R = P.radius, K = 2 * R + 1, valid = kTileX + 2 * R;
+    const int width = (valid + 8 + 3) & ~3;
     const int s0 = t2 * kTileX;
     // loading: row lr of the tile, columns lc, lc + 16, ...
     const int lr = threadIdx.x / 16, lc = threadIdx.x % 16;
     const int li1 = t1 * kRows2 + lr;
     const int64_t lrow = (int64_t(i0) * d1 + li1) * d2;
-    float *wts = lds + kRows2 * width;
+    const float *line = lds + row * width + xq;
     float res[4][6];
 #pragma unroll
     for (int g = 0; g < 6; ++g) {
         if (g >= P.ngroups) break;
         const ConvGroup &G = P.grp[g];
         for (int c = lc; c < width; c += 16)
-            lds[lr * width + c] = li1 < d1 ? G.in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
-        if (int(threadIdx.x) < K) wts[threadIdx.x] = P.taps[G.out[0].order][K - 1 - int(threadIdx.x)];
+            lds[lr * width + c] =
+                (li1 < d1 && c < valid) ? G.in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
         __syncthreads();
         float a[4] = {0.f, 0.f, 0.f, 0.f};
-        const float *line = lds + y * width + x;
-#pragma unroll 4
-        for (int j = 0; j < K; ++j) {
-            const float w = wts[j];
+        const float *wts = P.ctaps[G.out[0].order];
+        float4 A = lds4(line), B = lds4(line + 4);
+        int j0 = 0;
+#pragma unroll 2
+        for (; j0 + 4 <= K; j0 += 4) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] += w * line[4 * q * width + j];
+            for (int jj = 0; jj < 4; ++jj) row_tap(a, A, B, jj, wts[j0 + jj]);
+            A = B;
+            B = lds4(line + j0 + 8);
         }
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+            if (j0 + jj < K) row_tap(a, A, B, jj, wts[j0 + jj]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) res[q][g] = a[q];
         __syncthreads();
     }
-    const int i2 = s0 + x;
-    if (i2 >= d2) return;
+    const int i1 = t1 * kRows2 + row;
+    if (i1 >= d1) return;
+    const int64_t i = (int64_t(i0) * d1 + i1) * d2 + s0 + xq;
+    // the Hessian's AoS x6 output: the thread's 4 voxels are 24 consecutive
+    // floats, stored as 6 aligned 16-byte writes
+    bool aos6 = P.ngroups == 6 && s0 + xq + 3 < d2;
+#pragma unroll
+    for (int g = 0; g < 6; ++g)
+        aos6 = aos6 && P.grp[g].out[0].ptr == P.grp[0].out[0].ptr &&
+               P.grp[g].out[0].stride == 6 && P.grp[g].out[0].offset == g;
+    aos6 = aos6 && (reinterpret_cast<uintptr_t>(P.grp[0].out[0].ptr) & 15) == 0;
+    if (aos6) {
+        float4 *o = reinterpret_cast<float4 *>(P.grp[0].out[0].ptr + i * 6);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = res[(4 * k + e) / 6][(4 * k + e) % 6];
+            o[k] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int i1 = t1 * kRows2 + y + 4 * q;
-        if (i1 >= d1) continue;
-        const int64_t i = (int64_t(i0) * d1 + i1) * d2 + i2;
+        if (s0 + xq + q >= d2) continue;
 #pragma unroll
         for (int g = 0; g < 6; ++g) {
             if (g >= P.ngroups) break;
             const ConvOut &O = P.grp[g].out[0];
-            O.ptr[i * O.stride + O.offset] = res[q][g];
+            O.ptr[(i + q) * O.stride + O.offset] = res[q][g];
         }
     }
 }
@@ -205,13 +295,14 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
     if (axis == 2) {
         const int64_t blocks = n2 * ((d1 + kRows2 - 1) / kRows2) * d0;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
-        const size_t shm = sizeof(float) * (kRows2 * (kTileX + 2 * P.radius) + kMaxTaps);
+        const size_t shm = sizeof(float) * kRows2 * ((kTileX + 2 * P.radius + 8 + 3) & ~3);
         hipLaunchKernelGGL(conv_row_kernel, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
     } else {
         const int len = axis == 0 ? d0 : d1, other = axis == 0 ? d1 : d0;
         const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
-        const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius) + 3 * kMaxTaps);
+        const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius + kZeroRows) +
+                                            3 * kTapsPad);
         if (axis == 0)
             hipLaunchKernelGGL(conv_tile_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
         else
@@ -223,10 +314,12 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
 
 void set_taps(ConvPass &P, int slot, const float *taps, int r)
 {
-    // taps of a shorter arm are centred in the common radius P.radius
-    for (int k = 0; k < kMaxTaps; ++k) P.taps[slot][k] = 0.f;
-    const int shift = P.radius - r;
-    for (int k = 0; k < 2 * r + 1; ++k) P.taps[slot][k + shift] = taps[k];
+    // taps of a shorter arm are centred in the common radius P.radius, then
+    // stored in convolution order (tap K-1-j at j), zero beyond K
+    float centred[kMaxTaps] = {};
+    const int shift = P.radius - r, K = 2 * P.radius + 1;
+    for (int k = 0; k < 2 * r + 1; ++k) centred[k + shift] = taps[k];
+    for (int j = 0; j < kTapsPad; ++j) P.ctaps[slot][j] = j < K ? centred[K - 1 - j] : 0.f;
 }
 
 // ------------------------------------------------------------ eigensolver ---
