@@ -281,6 +281,8 @@ def main():
         log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "
             f"decay={d_all[best]:.4g}")
         tile = sweep_choice(dev, img.shape)
+        if args.sweep == 2 and args.batched_sweep != 1:
+            tile = args.sweep_tile or 1      # forced, not tuned
         step_s = secs / args.steps
         effective = value * BYTES_PER_VOXEL_ITER / 1e9
         if tile:

@@ -117,6 +117,7 @@ def adjust_directions(qeig3, response, thresholds=(0.01, 0.3), voxel_size=(1.0, 
 @dataclass
 class ChainPrepared:
     vesselness: np.ndarray   # normalised, float64
+    qeig: np.ndarray         # squeezed frames of the winning scale, float32 (D0, D1, D2, 3)
     ridge: np.ndarray        # final (cleaned) ridge, bool
     dirs: np.ndarray         # adjusted main directions, float32 (D0, D1, D2, 3)
     quats: np.ndarray        # completed GVF frames, float32 (D0, D1, D2, 3)
@@ -138,8 +139,8 @@ def prepare(img, noise_level, scales=(1.0, 2.0, 3), gvf_mu=0.1, gvf_iters=200,
     g, _ = ref32.gvf(v, gvf_mu, gvf_iters, voxel_size)
     quats = ref32.complete_frames(g)
     ct, cs = fp64.capacities(img, noise_level)
-    return ChainPrepared(ves, ridge, dirs, quats, cs.astype(np.float32), ct.astype(np.float32),
-                         fp64.grad_mag_sq(img), info)
+    return ChainPrepared(ves, qeig, ridge, dirs, quats, cs.astype(np.float32),
+                         ct.astype(np.float32), fp64.grad_mag_sq(img), info)
 
 
 def cut(img, prep, weighting, decay, num_iter=500, update=30, a=A_ATV, steps=0.16):

@@ -1,0 +1,460 @@
+"""The whole GPU pipeline against the oracle's chain, and the benchmark
+workloads at their own sizes.
+
+  config 1 (64^3)   every stage from the oracle (oracle/chain.py) against the
+                    GPU pipeline, numpy API and device path: cut Dice >= 0.999
+  config 3 (128^3)  two grid cells x 500 iterations through the batched pair
+                    sweep against the oracle's solve: max |u| difference <= 1e-4
+  config 2 (256^3)  the preparation (Frangi, ridges, signs, GVF, frames)
+                    against the oracle at reduced GVF iterations
+  config 4          ragged volumes with sides drawn from U[48, 192]
+  config 5          serving requests with sides drawn from log-U[64, 320]
+  1-D chains        the converged GPU cut equals the brute-force minimum cut
+  offset limits     the two-level sweep at the largest volumes it takes
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from miccai2016_amd import _lib
+from miccai2016_amd import phantom as ph
+from oracle import chain, fp64, mincut, ref32
+
+pytestmark = pytest.mark.gpu
+
+EIGS = (1.0, ph.A_ATV, ph.A_ATV)
+
+
+def _params(num_iter, update=30, steps=0.16):
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    p = CmfCutQAnisotropicGpu.CutParameters()
+    p.num_iter, p.update, p.error_bound, p.steps = num_iter, update, 0.0, steps
+    return p
+
+
+def _cutter(dev, num_iter, update=30):
+    from miccai2016_amd.processing.continuous_cut_anisotropic_gpu import CmfCutQAnisotropicGpu
+    return CmfCutQAnisotropicGpu(_params(num_iter, update), dev)
+
+
+def _frame_angle(qa, qb):
+    """Angle between the first frame columns (the main directions), up to sign."""
+    a = fp64.squeezed_to_mat(qa)[..., :, 0]
+    b = fp64.squeezed_to_mat(qb)[..., :, 0]
+    return np.arccos(np.clip(np.abs((a * b).sum(-1)), 0, 1))
+
+
+# ------------------------------------------------ config 1: the whole chain --
+
+def _gpu_prepare_steps(img, sigma, cfg, dev):
+    """pipeline.prepare's stages one by one, keeping the intermediates."""
+    from miccai2016_amd.helpers.misc import normalize
+    from miccai2016_amd.processing.adjust_directions import (AdvancedAdjuster,
+                                                             TransformationCompleter)
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    from miccai2016_amd.processing.qfrangi_gpu import FeatureFilter, VesselnessCalculation
+    vp = VesselnessCalculation.Parameters()
+    vp.scales, vp.bright_objects = cfg.scales, cfg.bright
+    ves, qeig, _ = FeatureFilter(vp, dev).execute_for(img, None, cfg.voxel_size)
+    ves = normalize(ves)
+    adj = AdvancedAdjuster(cfg.ridge_thresholds, qeig, ves, cfg.voxel_size, loader=dev)
+    dirs = adj.execute(verbose=False)
+    v = np.rollaxis(np.rollaxis(dirs, -1) * ves, 0, 4)
+    gp = GvfParameters()
+    gp.mu, gp.tol, gp.maxiter = cfg.gvf_mu, 0, cfg.gvf_iters
+    gp.callback, gp.update, gp.normalize = None, max(cfg.gvf_iters // 10, 1), False
+    g = GvfGpu(v, gp, cfg.voxel_size, cl_program_loader=dev).execute()
+    quats = TransformationCompleter(g, dev).complete()
+    return ves, adj.ridge.astype(bool), dirs, g, quats
+
+
+CHAIN_CASES = [
+    # (name, noise, Frangi scales): the workload's own settings; the same at a
+    # lower noise; and the workload with the scale rule of the reference
+    # experiment (tube radius / 1.2 .. 1.2 x radius in 16 scales,
+    # phantom_experiment.py:226-236)
+    ("workload", 1.5, (1.0, 2.0, 3)),
+    ("noise-0.5", 0.5, (1.0, 2.0, 3)),
+    ("matched-scales", 1.5, (4.0 / 1.2, 4.0 * 1.2, 16)),
+]
+
+
+@pytest.mark.parametrize("name,noise,scales", CHAIN_CASES, ids=[c[0] for c in CHAIN_CASES])
+def test_pipeline_matches_oracle_chain_64(dev, name, noise, scales):
+    """The 64^3 workload (tube 4, helix 16, pitch 24; GVF mu 0.1 x 200, 500
+    ATV iterations at w 10^-1.625, decay 10^1.5): the oracle's chain against
+    the GPU pipeline, end to end and from shared inputs.
+
+    End to end, both score the same against the truth. Voxel for voxel the
+    cuts agree only where the ridge is well conditioned: the ridge test
+    samples the response at pos + a v2 + b v3 (detect_ridges.cl:58-63), and
+    across a tube l2 ~ l3, so v2 and v3 are defined up to a rotation in their
+    plane; FP32 differences in the Hessian turn that rotation and move voxels
+    in or out of the ridge, which changes the spanning tree and, on a ridge
+    made of noise (small Frangi scales against a 4-voxel tube), the signs of
+    whole branches. With scales matched to the tube the ridge is the centre
+    line and the cuts agree voxel for voxel. From the oracle's adjusted
+    directions on, GVF, frames and cut agree voxel for voxel in every case."""
+    from miccai2016_amd.pipeline import PipelineConfig, prepare, prepare_device, segment
+    from miccai2016_amd.processing.adjust_directions import TransformationCompleter
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    spec = ph.HelixSpec(ph.PLAIN_64.shape, ph.PLAIN_64.r_tube, ph.PLAIN_64.r_helix,
+                        ph.PLAIN_64.pitch, noise)
+    img, truth = ph.make_helix(spec)
+    w, d = 10 ** -1.625, 10 ** 1.5
+    print(f"oracle chain, 64^3, {name}", flush=True)
+    want_prep = chain.prepare(img, spec.sigma, scales=scales)
+    want, done = chain.cut(img, want_prep, w, d)
+    assert done == 500
+    cfg = PipelineConfig()
+    cfg.scales = scales
+    print("GPU pipeline, numpy API", flush=True)
+    ves, ridge, dirs, g, quats = _gpu_prepare_steps(img, spec.sigma, cfg, dev)
+    prep = prepare(img, spec.sigma, cfg, dev)
+    np.testing.assert_array_equal(prep.quats, quats)
+    got = segment(img, prep, w, d, cfg, dev)
+    md_agree = np.mean((dirs * want_prep.dirs).sum(-1) > 0)
+    v_gpu = np.rollaxis(np.rollaxis(dirs, -1) * ves, 0, 4).astype(np.float32)
+    gvf_err = float(np.abs(g - ref32.gvf(v_gpu, cfg.gvf_mu, cfg.gvf_iters)[0]).max())
+    ang = _frame_angle(quats, want_prep.quats)
+    dice_pair = ph.dice_between(got, want)
+    print(f"64^3 {name}: oracle ridge {want_prep.info}; GPU ridge {int(ridge.sum())}, ridge "
+          f"voxels differing {int((ridge != want_prep.ridge).sum())}; directions agreeing in "
+          f"sign {md_agree:.5f}; GVF on the GPU's own input: max |d| {gvf_err:.3e}; frames "
+          f"within 1e-3 rad on {np.mean(ang <= 1e-3):.5f} of voxels; max |du| "
+          f"{np.abs(got - want).max():.3e}; Dice(gpu, oracle) {dice_pair:.6f}; fast Dice vs "
+          f"truth: gpu {ph.fast_dice(got, truth):.4f}, oracle {ph.fast_dice(want, truth):.4f}",
+          flush=True)
+    np.testing.assert_allclose(ves, want_prep.vesselness, atol=1e-5)
+    assert gvf_err <= 1e-5 * max(1.0, float(np.abs(g).max()))
+    assert abs(ph.fast_dice(got, truth) - ph.fast_dice(want, truth)) <= 3e-3
+    if name == "matched-scales":
+        # a few ridge voxels of 560 still change with the (v2, v3) rotation
+        # (15 in the measured run): Dice 0.9986 between the cuts
+        assert dice_pair >= 0.995
+    # from the oracle's Frangi output: the GPU adjuster against the oracle's
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    adj = AdvancedAdjuster(cfg.ridge_thresholds, want_prep.qeig, want_prep.vesselness,
+                           cfg.voxel_size, loader=dev)
+    dirs_f = adj.execute(verbose=False)
+    ridge_f = adj.ridge.astype(bool)
+    rdiff = int((ridge_f != want_prep.ridge).sum())
+    sign_f = np.mean((dirs_f * want_prep.dirs).sum(-1) > 0)
+    print(f"64^3 {name}, shared Frangi output: ridge voxels differing {rdiff} of "
+          f"{int(want_prep.ridge.sum())}; directions agreeing in sign {sign_f:.5f}", flush=True)
+    assert rdiff <= max(2, 0.002 * want_prep.ridge.sum())
+    # from the oracle's adjusted directions on: GVF, frames and the cut
+    gp = GvfParameters()
+    gp.mu, gp.tol, gp.maxiter = cfg.gvf_mu, 0, cfg.gvf_iters
+    gp.callback, gp.update, gp.normalize = None, max(cfg.gvf_iters // 10, 1), False
+    v_or = (want_prep.dirs.astype(np.float64) * want_prep.vesselness[..., None]).astype(np.float32)
+    q_shared = TransformationCompleter(GvfGpu(v_or, gp, cfg.voxel_size,
+                                              cl_program_loader=dev).execute(), dev).complete()
+    prep.quats, prep.cs, prep.ct = q_shared, want_prep.cs, want_prep.ct
+    got_s = segment(img, prep, w, d, cfg, dev)
+    print(f"64^3 {name}, shared directions: frames within 1e-3 rad on "
+          f"{np.mean(_frame_angle(q_shared, want_prep.quats) <= 1e-3):.5f}; max |du| "
+          f"{np.abs(got_s - want).max():.3e}; Dice(gpu, oracle) "
+          f"{ph.dice_between(got_s, want):.6f}", flush=True)
+    assert ph.dice_between(got_s, want) >= 0.999
+    # the device path: every stage on the GPU, only the ridge's host steps on the CPU
+    print("GPU pipeline, device path", flush=True)
+    x_t = dev.upload(img.astype(np.float32)).reshape(-1)
+    pd = prepare_device(x_t, img.shape, spec.sigma, cfg, dev)
+    n = img.size
+    a, u = dev.empty(n), dev.empty(n)
+    dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(pd.gms), n, w, d, dev.ptr(a),
+                                            dev.stream))
+    work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+    dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), n, dev.ptr(u), dev.ptr(work),
+                                   dev.stream))
+    _cutter(dev, 500).execute_device(u, pd.quats, a, pd.cs, pd.ct, const_sqevals=np.sqrt(EIGS),
+                                     shape=img.shape)
+    got_d = dev.download(u).reshape(img.shape)
+    print(f"64^3 {name} device path: max |du| {np.abs(got_d - want).max():.3e}; Dice(gpu, "
+          f"oracle) {ph.dice_between(got_d, want):.6f}", flush=True)
+    assert abs(ph.fast_dice(got_d, truth) - ph.fast_dice(want, truth)) <= 3e-3
+
+
+# ------------------------------------------------ config 3: the headline grid --
+
+GRID_CELLS = [(10 ** -1.625, 10 ** 1.5), (10 ** 0.5, 10 ** -1.0)]
+
+
+def test_grid_cells_128_match_oracle(dev):
+    """Two of the 425 cells of the 128^3 grid (the reference's best-known
+    cell and the grid's strongest, most local weighting), 500 iterations
+    through the batched solver (the pair sweep the bench runs), against the
+    oracle's solve on the same fields."""
+    from miccai2016_amd.pipeline import PipelineConfig, prepare_device
+    spec = ph.GRID_128
+    img, truth = ph.make_helix(spec)
+    cfg = PipelineConfig()
+    x_t = dev.upload(img.astype(np.float32)).reshape(-1)
+    pd = prepare_device(x_t, img.shape, spec.sigma, cfg, dev)
+    n = img.size
+    w = np.array([c[0] for c in GRID_CELLS], np.float32)
+    d = np.array([c[1] for c in GRID_CELLS], np.float32)
+    u0 = dev.empty(n)
+    work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+    dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), n, dev.ptr(u0), dev.ptr(work),
+                                   dev.stream))
+    ub = u0.reshape(1, -1).repeat(len(GRID_CELLS), 1)
+    c = _cutter(dev, 500)
+    c.execute_batched_device(ub, len(GRID_CELLS), w, d, pd.quats, pd.gms, pd.cs, pd.ct, img.shape,
+                             EIGS)
+    got = dev.download(ub).reshape((-1,) + img.shape)
+    q = dev.download(pd.quats).reshape(img.shape + (3,))
+    cs = dev.download(pd.cs).reshape(img.shape)
+    ct = dev.download(pd.ct).reshape(img.shape)
+    gms = dev.download(pd.gms).reshape(img.shape)
+    for k, (wk, dk) in enumerate(zip(w, d)):
+        print(f"oracle solve 128^3 x 500, cell w={wk:.4g} decay={dk:.4g}", flush=True)
+        alpha = (np.float32(wk) * np.exp(-gms / np.float32(dk) ** 2)).astype(np.float32)
+        want, done, _ = ref32.cmf_solve(img.astype(np.float32), q, EIGS, alpha, cs, ct, steps=0.16,
+                                        num_iter=500, update=30, error_bound=0)
+        err = float(np.abs(got[k] - want).max())
+        dice = ph.dice_between(got[k], want)
+        print(f"cell {k}: max |du| {err:.3e}, Dice {dice:.6f}, fast Dice vs truth "
+              f"{ph.fast_dice(got[k], truth):.4f}", flush=True)
+        assert done == 500
+        assert err <= 1e-4
+        assert dice >= 0.999
+
+
+# ------------------------------------------- 1-D chains: the exact minimum cut --
+
+@pytest.mark.parametrize("case", mincut.chain_cases(), ids=mincut.case_id)
+def test_gpu_cut_is_the_brute_force_min_cut(dev, case):
+    shape, q, eigs, cs, ct, alpha, want = case
+    img = np.linspace(0.2, 1.0, int(np.prod(shape))).reshape(shape).astype(np.float32)
+    u = _cutter(dev, 4000, update=4000).execute(img, q, eigs, alpha, cs, ct, verbose=False)
+    np.testing.assert_array_equal(u >= 0.5, want)
+
+
+# --------------------------------------------- config 2: 256^3 preparation --
+
+def test_preparation_256_against_oracle(dev):
+    """The 256^3 workload's preparation (tube 8, helix 64, pitch 96, noise
+    1.0) with 10 GVF iterations: Frangi response and frames against the
+    oracle at full size; the GPU ridge against the oracle's candidates on the
+    GPU response; the nearest-ridge signs on a sample of voxels against a
+    brute-force search over the final ridge; the GVF against the oracle's on
+    the GPU's input; the completed frames against the oracle's."""
+    from miccai2016_amd.pipeline import PipelineConfig
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    from miccai2016_amd.processing.gradient_vector_flow import GvfGpu, GvfParameters
+    from miccai2016_amd.processing.qfrangi_gpu import FeatureFilter, VesselnessCalculation
+    spec = ph.PLAIN_256
+    img, _ = ph.make_helix(spec)
+    x = img.astype(np.float32)
+    shape = x.shape
+    n = x.size
+    cfg = PipelineConfig()
+    vp = VesselnessCalculation.Parameters()
+    vp.scales, vp.bright_objects = cfg.scales, cfg.bright
+    ff = FeatureFilter(vp, dev)
+    val_t, quat_t = ff.execute_tensor(dev.upload(x).reshape(-1), shape, cfg.voxel_size)
+    val = dev.download(val_t).reshape(shape)
+    quat = dev.download(quat_t).reshape(shape + (3,))
+    print("oracle vesselness 256^3", flush=True)
+    wv, wq = ref32.vesselness(x, fp64.scales_voxels(cfg.scales, cfg.voxel_size))
+    err = np.abs(val - wv)
+    bad = err > 1e-5 * np.abs(wv).max()
+    k = int(np.argmax(err))
+    print(f"256^3 vesselness: max |d| {err.max():.3e} (max {np.abs(wv).max():.3e}) at "
+          f"{np.unravel_index(k, shape)}: gpu {val.ravel()[k]:.6g}, oracle {wv.ravel()[k]:.6g}; "
+          f"{int(bad.sum())} voxels beyond 1e-5 of the maximum", flush=True)
+    # the Hessian of the largest scale against the oracle's
+    from miccai2016_amd.processing.qfrangi_gpu import GaussianFilter3D, Hessian
+    gf = GaussianFilter3D(dev, round_up=False)
+    work_h = dev.empty(int(dev.lib.mh_hessian_work_floats(_lib.dims3(shape))))
+    h6 = dev.empty(6 * n)
+    Hessian(gf).execute(dev.upload(x).reshape(-1), work_h, None, shape, h6, (2.0, 2.0, 2.0))
+    hg = dev.download(h6).reshape(shape + (6,))
+    hw = ref32.hessian(x, (2.0, 2.0, 2.0))
+    print(f"256^3 Hessian sigma 2: max |d| {np.abs(hg - hw).max():.3e} of max "
+          f"{np.abs(hw).max():.3e}", flush=True)
+    assert np.abs(hg - hw).max() <= 2e-5 * np.abs(hw).max()
+    assert bad.mean() <= 1e-6
+    strong = wv > 0.05 * wv.max()
+    assert np.mean(_frame_angle(quat[strong], wq[strong]) <= 1e-3) >= 0.999
+    ves = fp64.normalize(val).astype(np.float32)
+    adj = AdvancedAdjuster(cfg.ridge_thresholds, None, None, cfg.voxel_size, loader=dev)
+    dirs_t = adj.execute_tensor(quat_t, dev.upload(ves).reshape(-1), shape)
+    dirs = dev.download(dirs_t).reshape(shape + (3,))
+    cand = ref32.detect_ridges(ves, quat, cfg.ridge_thresholds[0]) != 0
+    ridge = adj.ridge.astype(bool)
+    print(f"256^3 ridge: {int(cand.sum())} candidates, {int(ridge.sum())} kept", flush=True)
+    # the kept ridge lies in the oracle's candidates, up to voxels whose
+    # response ties a trilinear sample within FP32 rounding
+    print(f"256^3: {int((ridge & ~cand).sum())} kept ridge voxels outside the oracle's "
+          f"candidates", flush=True)
+    assert np.mean(cand[ridge]) >= 0.999
+    md = ref32.main_dirs(quat)
+    np.testing.assert_allclose(np.abs(dirs), np.abs(md), atol=2e-6)
+    # every sampled voxel agrees in sign with its nearest ridge voxel (the
+    # lowest index among equally near ones)
+    from scipy.spatial import cKDTree
+    coords = np.argwhere(ridge)
+    rdirs = dirs[tuple(coords.T)].astype(np.float64)
+    tree = cKDTree(coords)
+    rng = np.random.default_rng(0)
+    sample = rng.integers(0, n, 3000)
+    pts = np.stack(np.unravel_index(sample, shape), 1)
+    dmin, _ = tree.query(pts)
+    flat = dirs.reshape(-1, 3).astype(np.float64)
+    for p, f, r in zip(pts, sample, dmin):
+        j = min(tree.query_ball_point(p, r + 1e-9))
+        assert float(np.dot(flat[f], rdirs[j])) >= -1e-6
+    # GVF of the weighted directions, 10 iterations, and the frames
+    v3 = (dirs.astype(np.float64) * ves[..., None]).astype(np.float32)
+    gp = GvfParameters()
+    gp.mu, gp.tol, gp.maxiter, gp.callback, gp.update, gp.normalize = cfg.gvf_mu, 0, 10, None, 10, False
+    g = GvfGpu(v3, gp, cfg.voxel_size, cl_program_loader=dev).execute()
+    print("oracle GVF 256^3 x 10", flush=True)
+    gw, _ = ref32.gvf(v3, cfg.gvf_mu, 10, cfg.voxel_size)
+    assert np.abs(g - gw).max() <= 1e-5 * np.abs(gw).max()
+    from miccai2016_amd.processing.adjust_directions import TransformationCompleter
+    frames = TransformationCompleter(g, dev).complete()
+    fw = ref32.complete_frames(gw)
+    big = np.linalg.norm(gw, axis=-1) > 1e-3 * np.linalg.norm(gw, axis=-1).max()
+    assert np.mean(_frame_angle(frames[big], fw[big]) <= 1e-3) >= 0.999
+
+
+# ------------------------------------------------------- config 4: ragged --
+
+def test_ragged_set_at_config_sizes(dev):
+    """Six volumes with every side drawn from U[48, 192] and the workload's
+    own settings (GVF 200, 500 iterations): the ragged pipeline's labellings
+    equal each volume's own device pipeline and plain solve, bit for bit."""
+    from miccai2016_amd.pipeline import PipelineConfig, prepare_device, segment_ragged
+    specs, w, d = ph.ragged_set(6, seed=11, side=(48, 192))
+    imgs = [ph.make_helix(sp)[0] for sp in specs]
+    cfg = PipelineConfig()
+    rr = segment_ragged(imgs, [sp.sigma for sp in specs], w, d, cfg, dev, workers=3)
+    print("ragged shapes", rr.shapes, flush=True)
+    assert [r[0] for r in rr.results] == [500] * len(specs)
+    for v, (img, sp) in enumerate(zip(imgs, specs)):
+        x = img.astype(np.float32)
+        n = x.size
+        x_t = dev.upload(x).reshape(-1)
+        prep = prepare_device(x_t, x.shape, sp.sigma, cfg, dev)
+        a, u = dev.empty(n), dev.empty(n)
+        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(prep.gms), n, float(w[v]),
+                                                float(d[v]), dev.ptr(a), dev.stream))
+        work = dev.empty(int(dev.lib.mh_initial_u_work_floats(n)))
+        dev.check(dev.lib.mh_initial_u(dev.handle, dev.ptr(x_t), n, dev.ptr(u), dev.ptr(work),
+                                       dev.stream))
+        _cutter(dev, 500).execute_device(u, prep.quats, a, prep.cs, prep.ct,
+                                         const_sqevals=np.sqrt(EIGS), shape=x.shape)
+        assert np.array_equal(dev.download(rr.volume(v)), dev.download(u).reshape(x.shape)), v
+
+
+# ------------------------------------------------------ config 5: serving --
+
+def _lone_phantom_cut(dev, r):
+    """A PhantomCutRequest (centre-line frames) solved alone on the device."""
+    n = int(np.prod(r.shape))
+    L, h, s = dev.lib, dev.handle, dev.stream
+    img, tan = dev.empty(n), dev.empty(3 * n)
+    dev.check(L.mh_helix_phantom(h, _lib.dims3(r.shape), r.r_tube, r.r_helix, r.pitch, r.sigma,
+                                 C.c_uint64(r.seed), dev.ptr(img), None, dev.ptr(tan), s))
+    dev.check(L.mh_complete_frames(h, dev.ptr(tan), dev.ptr(tan), n, s))
+    cs, ct, a, u0 = dev.empty(n), dev.empty(n), dev.empty(n), dev.empty(n)
+    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, r.sigma, 1e-9, dev.ptr(cs), dev.ptr(ct), s))
+    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(r.shape), dev.ptr(a), s))
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(a), n, r.weighting, r.decay, dev.ptr(a), s))
+    wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
+    dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), s))
+    c = _cutter(dev, r.num_iter)
+    c.execute_device(u0, tan, a, cs, ct, const_sqevals=np.sqrt(r.eigs), shape=r.shape)
+    return dev.download(u0).reshape(r.shape), c.last_result.iterations
+
+
+def test_serving_requests_at_config_sizes(dev):
+    """Six requests drawn like the serving workload (side log-U[64, 320],
+    aspect up to 2:1, 300-1000 iterations), submitted together so they share
+    the device: each result equals the request solved alone, bit for bit."""
+    from miccai2016_amd.serving import MaxFlowServer, synthetic_request
+    rng = np.random.default_rng(2026)
+    reqs = [synthetic_request(rng) for _ in range(6)]
+    print("serving shapes", [(r.shape, r.num_iter) for r in reqs], flush=True)
+    srv = MaxFlowServer(_params(1000), chunk=50)
+    tickets = [srv.submit(r) for r in reqs]
+    srv.run_until_idle()
+    srv.stop()
+    for r, t in zip(reqs, tickets):
+        u = t.wait(0)
+        want, iters = _lone_phantom_cut(dev, r)
+        assert t.error is None and t.iterations == iters == r.num_iter
+        assert np.array_equal(u, want), r.shape
+
+
+# ------------------------------------------------------ sweep offset limits --
+
+def _device_cut_case(dev, shape, seed=3):
+    L, h, st = dev.lib, dev.handle, dev.stream
+    n = int(np.prod(shape))
+    img, q = dev.empty(n), dev.empty(3 * n)
+    dev.check(L.mh_helix_phantom(h, _lib.dims3(shape), 6.0, shape[1] / 4, 48.0, 1.0,
+                                 C.c_uint64(seed), dev.ptr(img), None, dev.ptr(q), st))
+    dev.check(L.mh_complete_frames(h, dev.ptr(q), dev.ptr(q), n, st))
+    cs, ct, gms, a, u0 = (dev.empty(n) for _ in range(5))
+    dev.check(L.mh_terminal_capacities(h, dev.ptr(img), n, 1.0, 1e-9, dev.ptr(cs), dev.ptr(ct), st))
+    dev.check(L.mh_grad_mag_sq(h, dev.ptr(img), _lib.dims3(shape), dev.ptr(gms), st))
+    dev.check(L.mh_nonterminal_weight(h, dev.ptr(gms), n, 10 ** -1.625, 10 ** 1.5, dev.ptr(a), st))
+    wk = dev.empty(int(L.mh_initial_u_work_floats(n)))
+    dev.check(L.mh_initial_u(h, dev.ptr(img), n, dev.ptr(u0), dev.ptr(wk), st))
+    del img, wk
+    return q, cs, ct, gms, a, u0
+
+
+def test_sweep_at_its_largest_volume(dev):
+    """511 x 512 x 512 voxels, just below the single-solve sweep's limit of
+    2^27 voxels (the frame field's byte offsets reach 2^31 - 64 MiB): the
+    forced sweep equals the per-iteration kernels bit for bit."""
+    shape = (511, 512, 512)
+    q, cs, ct, _, a, u0 = _device_cut_case(dev, shape)
+    outs = []
+    try:
+        for mode, tile in ((1, 0), (2, 1)):
+            dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, mode, tile, 0))
+            ud = u0.clone()
+            c = _cutter(dev, 21, update=7)
+            c.execute_device(ud, q, a, cs, ct, const_sqevals=np.sqrt(EIGS), shape=shape)
+            outs.append((ud, c.last_result.iterations))
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    dev.torch.cuda.synchronize()
+    assert outs[0][1] == outs[1][1] == 21
+    assert dev.torch.equal(outs[0][0], outs[1][0])
+
+
+def test_pair_sweep_at_its_largest_volume(dev):
+    """256 x 512 x 511 voxels, just below the pair sweep's limit of 2^26
+    voxels: two batched solves advanced two per thread equal the same solves
+    through the per-iteration kernels, bit for bit."""
+    shape = (256, 512, 511)
+    n = int(np.prod(shape))
+    q, cs, ct, gms, _, u0 = _device_cut_case(dev, shape, seed=4)
+    w = np.array([0.05, 0.5], np.float32)
+    d = np.array([30.0, 3.0], np.float32)
+    torch = dev.torch
+    try:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, 1, 0))
+        ub = u0.reshape(1, -1).repeat(2, 1)
+        _cutter(dev, 21, update=7).execute_batched_device(ub, 2, w, d, q, gms, cs, ct, shape, EIGS)
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 1, 0, 0))
+        a = dev.empty(n)
+        for k in range(2):
+            dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(gms), n, float(w[k]),
+                                                    float(d[k]), dev.ptr(a), dev.stream))
+            ud = u0.clone()
+            _cutter(dev, 21, update=7).execute_device(ud, q, a, cs, ct,
+                                                      const_sqevals=np.sqrt(EIGS), shape=shape)
+            torch.cuda.synchronize()
+            assert torch.equal(ub[k], ud.reshape(-1)), k
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
