@@ -30,6 +30,7 @@ extern "C" {
 #define MH_ERR_ARG -1
 #define MH_ERR_HIP -2
 #define MH_STOPPED_NAN 1
+#define MH_DEFER 2 /* the device form does not apply here: nothing was changed */
 
 typedef struct mh_ctx mh_ctx;
 
@@ -163,6 +164,32 @@ int64_t mh_ridge_segments_work_bytes(const int64_t dims[3]);
 int mh_ridge_segments(mh_ctx *ctx, const float *cand_dev, const float *resp_dev, float upper,
                       int use_upper, const int64_t dims[3], float *ridge_dev, int *labels_dev,
                       void *work_dev, int64_t work_bytes, void *stream);
+
+/* The direction adjustment after the ridge's hysteresis, on the device:
+ * dirs_dev holds every voxel's main direction (AoS x3, mh_main_dirs) and
+ * receives the adjusted directions; ridge_dev and labels_dev are
+ * mh_ridge_segments' outputs. The ridge's minimum spanning forest (the
+ * 26-neighbour pairs by squared length, then point indices), the signs along
+ * it, with clean the ceil(3 ln(e + S)) segments of longest summed signed
+ * direction kept, the minimum spanning tree between the kept segments, the
+ * signs from the smallest kept point along it, and every voxel's sign from
+ * its nearest kept ridge point. kept_dev (N floats) receives the kept ridge
+ * (1.f). stats_host (8 int64): ridge points, segments, kept segments, kept
+ * points, zero-product forest edges, -, -, forest rounds. Replaces
+ * AdvancedAdjuster's ridge steps (processing/adjust_directions.py:213-256,
+ * :290-339, :365-446) for an isotropic voxel size that is a power of two;
+ * the results equal the host construction (mh_ridge_forest_host,
+ * mh_forest_pieces_host, mh_ridge_tree_from_forest_host, mh_tree_signs_host,
+ * mh_nonridge_signs) bit for bit. Returns MH_DEFER, with dirs_dev unchanged,
+ * where that cannot be guaranteed or the host's own rule differs: another
+ * voxel size, at most 256 ridge or kept points (the dense tree), a NaN
+ * direction product, segment lengths the bounds do not separate, more than
+ * 65536 kept points outside the largest kept segment, more than 4096 kept
+ * segments. Blocks until done. */
+int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs_dev, const float *ridge_dev,
+                           const int *labels_dev, const int64_t dims[3],
+                           const double voxel_size_host[3], int clean, float *kept_dev,
+                           int64_t *stats_host, void *stream);
 
 /* Host: a minimum spanning tree of the complete Euclidean graph over n points
  * (pts_host: n x 3 doubles, e.g. voxel-size scaled ridge coordinates), as

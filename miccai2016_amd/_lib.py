@@ -13,6 +13,7 @@ MH_OK = 0
 MH_ERR_ARG = -1
 MH_ERR_HIP = -2
 MH_STOPPED_NAN = 1
+MH_DEFER = 2
 
 _lib = None
 
@@ -74,6 +75,7 @@ SIGNATURES = {
     "mh_ridge_segments": (i32, [vp, vp, vp, f32, i32, I64x3, vp, vp, vp, i64, vp]),
     "mh_spanning_tree_host": (i32, [vp, i64, C.c_double, vp]),
     "mh_ridge_tree_host": (i32, [vp, i64, I64x3, vp, vp]),
+    "mh_adjust_ridge_device": (i32, [vp, vp, vp, vp, I64x3, vp, i32, vp, vp, vp]),
     "mh_ridge_forest_host": (i64, [vp, i64, I64x3, vp, vp]),
     "mh_ridge_tree_from_forest_host": (i32, [vp, i64, I64x3, vp, vp, i64, vp]),
     "mh_forest_pieces_host": (i32, [vp, i64, vp, i64, vp, vp, vp]),

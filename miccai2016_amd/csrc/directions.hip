@@ -12,10 +12,14 @@ namespace {
 // second and third columns, at offsets a*v2 + b*v3 with (a, b) in
 // {-1, 0, 1}^2 minus the origin. A sample's unit-cube origin is the position
 // truncated toward zero; corners outside the volume contribute the value -1.
+// One rounding per operation (no contraction), in the order of the
+// reference's kernel, so that only the frame's trigonometry can move a
+// decision, and only where the response ties a sample within rounding.
 __global__ void __launch_bounds__(mh::kBlock)
 ridge_kernel(float *__restrict__ ridge, const float *__restrict__ resp,
              const float *__restrict__ qeig3, int d0, int d1, int d2, float thr)
 {
+#pragma clang fp contract(off)
     const int64_t n = int64_t(d0) * d1 * d2;
     const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
     const int64_t i = int64_t(b) * blockDim.x + threadIdx.x;
@@ -82,7 +86,10 @@ main_dir_kernel(float *dirs, const float *qeig3, int64_t n)
 // Brute-force nearest ridge point per voxel, ridge tiles staged in LDS. The
 // scan visits ridge points in index order with a strict '<', so the lowest
 // index wins ties; then the voxel's direction is flipped if it points away
-// from that ridge point's direction.
+// from that ridge point's direction. Distances and the product round once
+// per operation, left to right (no fused multiply-adds), so a direction
+// nearly perpendicular to its ridge point's is decided the same way by every
+// build.
 constexpr int kRidgeTile = 1024;
 
 __global__ void __launch_bounds__(mh::kBlock)
@@ -90,6 +97,7 @@ nonridge_kernel(float *__restrict__ dirs, int d0, int d1, int d2,
                 const uint16_t *__restrict__ coords, const float *__restrict__ rdirs, int nr,
                 float s0, float s1, float s2)
 {
+#pragma clang fp contract(off)
     __shared__ uint16_t tc[kRidgeTile][3];
     const int64_t n = int64_t(d0) * d1 * d2;
     const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
@@ -241,6 +249,7 @@ __global__ void __launch_bounds__(mh::kBlock)
 flip_by_nearest_kernel(float *__restrict__ dirs, const unsigned *__restrict__ best,
                        const float *__restrict__ rdirs, int64_t n)
 {
+#pragma clang fp contract(off)
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned j = best[i];
@@ -380,18 +389,36 @@ MH_EXPORT int mh_nonridge_signs(mh_ctx *ctx, float *dirs, const int64_t dims[3],
         return MH_OK;
     }
     unsigned *ridx = static_cast<unsigned *>(work);
-    unsigned *dA = ridx + n, *bA = dA + n, *dB = bA + n, *bB = dB + n;
-    const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
+    unsigned *dA = ridx + n, *bA = dA + n;
     const unsigned nb = mh::blocks_for(n);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(nb), dim3(mh::kBlock), 0, s, ridx, n, kNoRidge);
     hipLaunchKernelGGL(ridge_scatter_kernel, dim3(mh::blocks_for(nr)), dim3(mh::kBlock), 0, s, ridx,
-                       coords, int(nr), d0, d1, d2);
+                       coords, int(nr), int(dims[0]), int(dims[1]), int(dims[2]));
+    MH_CHECK_LAUNCH(ctx, "ridge_scatter_kernel");
+    const int rc = mh_nearest_marked(ctx, dims, static_cast<unsigned *>(work), s);
+    if (rc != MH_OK) return rc;
+    hipLaunchKernelGGL(flip_by_nearest_kernel, dim3(nb), dim3(mh::kBlock), 0, s, dirs, bA, rdirs, n);
+    MH_CHECK_LAUNCH(ctx, "nearest-ridge transform");
+    return MH_OK;
+}
+
+// The separable transform over work = [ridx | dA | bA | dB | bB] (N unsigned
+// each): ridx holds every marked voxel's index (kNoRidge elsewhere); on
+// return dA holds each voxel's squared distance (voxel units) to the nearest
+// marked voxel and bA that voxel's index, the lowest among equidistant ones
+// (kNoRidge where nothing is marked).
+int mh_nearest_marked(mh_ctx *ctx, const int64_t dims[3], unsigned *work, hipStream_t s)
+{
+    const int64_t n = dims[0] * dims[1] * dims[2];
+    unsigned *ridx = work;
+    unsigned *dA = ridx + n, *bA = dA + n, *dB = bA + n, *bB = dB + n;
+    const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
+    const unsigned nb = mh::blocks_for(n);
     hipLaunchKernelGGL(edt_axis2_kernel, dim3(mh::blocks_for(int64_t(d0) * d1)), dim3(mh::kBlock), 0, s,
                        ridx, dA, bA, d0, d1, d2);
     hipLaunchKernelGGL(edt_scan_kernel<1>, dim3(nb), dim3(mh::kBlock), 0, s, dA, bA, dB, bB, d0, d1, d2);
     hipLaunchKernelGGL(edt_scan_kernel<0>, dim3(nb), dim3(mh::kBlock), 0, s, dB, bB, dA, bA, d0, d1, d2);
-    hipLaunchKernelGGL(flip_by_nearest_kernel, dim3(nb), dim3(mh::kBlock), 0, s, dirs, bA, rdirs, n);
-    MH_CHECK_LAUNCH(ctx, "nearest-ridge transform");
+    MH_CHECK_LAUNCH(ctx, "separable nearest-voxel transform");
     return MH_OK;
 }
 

@@ -61,6 +61,12 @@ struct mh_ctx {
 // ctx->noalloc_scratch; maxflow.hip.
 int mh_noalloc_setup(mh_ctx *ctx);
 
+// Nearest marked voxel by the separable distance transform, isotropic voxel
+// units (directions.hip): work holds 5 x N unsigned, [ridx | dA | bA | dB | bB];
+// ridx marks voxels with their indices (0xffffffff elsewhere); dA receives the
+// squared distances, bA the nearest index (lowest among equidistant ones).
+int mh_nearest_marked(mh_ctx *ctx, const int64_t dims[3], unsigned *work, hipStream_t s);
+
 namespace mh {
 
 constexpr int kBlock = 256;  // threads per work-group for 1-D kernels (4 waves)

@@ -30,7 +30,13 @@ class AdvancedAdjuster(TreePrintable):
         self.__voxel_size = None
         self.__ridge = None
         self.__ridge_coords = None
+        self.__ridge_device = None
         self.__main_dirs_unadjusted = None
+        # execute_tensor: the device form of the ridge steps
+        # (mh_adjust_ridge_device) where it applies, else the host construction
+        self.device_adjust = True
+        self.device_stats = None
+        self.device_used = False
         self._ridge_detector = None
         self._hysteresis = None
         # (ridge points, their neighbour forest): the cleaning's forest, then
@@ -93,6 +99,9 @@ class AdvancedAdjuster(TreePrintable):
     @property
     def ridge(self):
         """The detected (and, if requested, cleaned) ridge after execute()."""
+        if self.__ridge is None and self.__ridge_device is not None:
+            mask_t, shape, _ = self.__ridge_device
+            self.__ridge = self._loader.download(mask_t).reshape(shape)
         if self.__ridge is None and self.__ridge_coords is not None:
             coords, shape = self.__ridge_coords
             mask = np.zeros(shape, np.float32)
@@ -129,21 +138,34 @@ class AdvancedAdjuster(TreePrintable):
         self._forest = None
         return adjusted
 
-    def execute_tensor(self, qeigs_t, resp_t, shape, verbose=False):
+    def execute_tensor(self, qeigs_t, resp_t, shape, verbose=False, candidates_t=None,
+                       main_dirs_t=None):
         """Device form of execute(): qeigs_t (squeezed frames, 3N) and resp_t
         (N) device tensors; returns the adjusted directions (AoS x3 device
-        tensor). The ridge's hysteresis and its 26-connected segments run on
-        the device (mh_ridge_segments); only the ridge voxels' indices,
-        segments and directions visit the host (spanning tree, cleaning)."""
+        tensor). The ridge steps run on the device (mh_adjust_ridge_device)
+        where that form applies; otherwise the ridge's hysteresis and
+        segments run on the device (mh_ridge_segments) and only the ridge
+        voxels' indices, segments and directions visit the host (spanning
+        tree, cleaning). candidates_t: ridge candidates (N floats, nonzero =
+        candidate) detected elsewhere, in place of this adjuster's own ridge
+        test; main_dirs_t: the frames' first columns (AoS x3) computed
+        elsewhere, in place of mh_main_dirs (qeigs_t may then be None when
+        candidates_t is given)."""
         dev = self._loader
         torch = dev.torch
         self._verbose = verbose
         shape = tuple(shape)
         n = int(np.prod(shape))
-        self._d_qeigs = qeigs_t.reshape(-1).clone()
+        src = main_dirs_t if main_dirs_t is not None else qeigs_t
+        self._d_qeigs = src.reshape(-1).clone()
         self._d_resp = resp_t.reshape(-1)
         d_ridge = dev.empty(n)
-        self._ridge_detector.execute(d_ridge, self._d_resp, self._d_qeigs, shape)
+        if candidates_t is None:
+            self._ridge_detector.execute(d_ridge, self._d_resp, qeigs_t.reshape(-1), shape)
+        else:
+            if candidates_t.numel() != n:
+                raise ValueError("candidates_t must hold one value per voxel")
+            d_ridge.copy_(candidates_t.reshape(-1))
         labels = dev.empty(n, dtype=torch.int32)
         wbytes = int(dev.lib.mh_ridge_segments_work_bytes(_lib.dims3(shape)))
         work = dev.bytes(wbytes)
@@ -152,8 +174,26 @@ class AdvancedAdjuster(TreePrintable):
                                             upper, int(self._hysteresis), _lib.dims3(shape),
                                             dev.ptr(d_ridge), dev.ptr(labels), dev.ptr(work),
                                             wbytes, dev.stream), "mh_ridge_segments")
-        dev.check(dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(self._d_qeigs),
-                                       n, dev.stream), "mh_main_dirs")
+        if main_dirs_t is None:
+            dev.check(dev.lib.mh_main_dirs(dev.handle, dev.ptr(self._d_qeigs),
+                                           dev.ptr(self._d_qeigs), n, dev.stream), "mh_main_dirs")
+        if self.device_adjust:
+            kept = dev.empty(n)
+            stats = np.zeros(8, np.int64)
+            vs = np.require(self.voxel_size, np.float64, ["C"])
+            rc = dev.lib.mh_adjust_ridge_device(dev.handle, dev.ptr(self._d_qeigs), dev.ptr(d_ridge),
+                                                dev.ptr(labels), _lib.dims3(shape), vs.ctypes.data,
+                                                int(bool(self.clean_ridge)), dev.ptr(kept),
+                                                stats.ctypes.data, dev.stream)
+            self.device_stats = stats
+            self.device_used = rc == _lib.MH_OK
+            if rc == _lib.MH_OK:
+                if getattr(self, "_verbose", False) and self.clean_ridge:
+                    print("Keeping %s of %s labels." % (stats[2], stats[1]))
+                self._set_ridge_device(kept, shape, int(stats[3]))
+                return self._d_qeigs
+            if rc != _lib.MH_DEFER:
+                dev.check(rc, "mh_adjust_ridge_device")
         # ridge voxels in C order, their segments and main directions
         lin_t = torch.nonzero(d_ridge).reshape(-1)
         lin = dev.download(lin_t)
@@ -186,10 +226,19 @@ class AdvancedAdjuster(TreePrintable):
         """The ridge as voxel coordinates; the mask is built when asked for."""
         self.__ridge = None
         self.__ridge_coords = (coords, tuple(shape))
+        self.__ridge_device = None
+
+    def _set_ridge_device(self, mask_t, shape, count):
+        """The ridge as a device mask (N floats); downloaded when asked for."""
+        self.__ridge = None
+        self.__ridge_coords = None
+        self.__ridge_device = (mask_t, tuple(shape), count)
 
     @property
     def ridge_voxels(self):
         """Number of voxels of the (cleaned) ridge after execute()."""
+        if self.__ridge is None and self.__ridge_device is not None:
+            return self.__ridge_device[2]
         if self.__ridge is None and self.__ridge_coords is not None:
             return len(self.__ridge_coords[0])
         return int(np.count_nonzero(self.ridge)) if self.ridge is not None else 0

@@ -534,9 +534,52 @@ ORC_API int orc_vesselness(const float *img, const int64_t *dims, const double *
 /* Ridges and directions (detect_ridges.cl:34-102, adjust_directions.cl).     */
 /* ------------------------------------------------------------------------- */
 
-/* Ridge iff resp > threshold (double compare) and resp > max of 8 trilinear
- * samples at pos + a*v2 + b*v3, (a,b) in {-1,0,1}^2 \ (0,0); corner outside
- * the volume counts -1; origin by truncation toward zero. */
+/* The ridge test's comparison value: the maximum of 8 trilinear samples at
+ * pos + a*v2 + b*v3, (a,b) in {-1,0,1}^2 \ (0,0), v2 and v3 the frame's
+ * second and third columns; a corner outside the volume counts -1; the unit
+ * cube's origin by truncation toward zero (detect_ridges.cl:46-96). */
+static float ridge_sample_max(const float *resp, const float *qeig3, int64_t i0, int64_t i1,
+                              int64_t i2, int64_t d0, int64_t d1, int64_t d2, int *fragile)
+{
+    const int64_t i = lin3(i0, i1, i2, d1, d2);
+    float q[4], m[9];
+    q_expand(qeig3 + 3 * i, q);
+    q_to_mat(q, m);
+    const float c1[3] = {m[1], m[4], m[7]};
+    const float c2[3] = {m[2], m[5], m[8]};
+    const float pos[3] = {(float)i0, (float)i1, (float)i2};
+    float maxv = -INFINITY;
+    for (int j = 0; j < 8; ++j) {
+        const float a = (float)((-1) * (j < 3) + (j > 4));
+        const float b = (float)((j % 3) - 1 + (j == 4));
+        float pj[3];
+        int oj[3];
+        for (int d = 0; d < 3; ++d) {
+            pj[d] = pos[d] + a * c1[d] + b * c2[d];
+            /* truncation toward zero makes the sample jump where a coordinate
+             * crosses a negative integer (a corner leaves the volume) */
+            if (fragile && pj[d] < -0.5f && fabsf(pj[d] - roundf(pj[d])) < 1e-5f) *fragile = 1;
+            oj[d] = (int)pj[d]; /* truncation toward zero */
+            pj[d] = pj[d] - (float)oj[d];
+        }
+        float vj = 0;
+        for (int k = 0; k < 8; ++k) {
+            const int cr[3] = {k >> 2, (k >> 1) & 1, k & 1};
+            const int ca[3] = {oj[0] + cr[0], oj[1] + cr[1], oj[2] + cr[2]};
+            float w[3];
+            for (int d = 0; d < 3; ++d) w[d] = cr[d] == 1 ? pj[d] : 1 - pj[d];
+            const int outside = ca[0] < 0 || ca[1] < 0 || ca[2] < 0 ||
+                                ca[0] >= (int)d0 || ca[1] >= (int)d1 || ca[2] >= (int)d2;
+            const float s = outside ? -1.0f : resp[lin3(ca[0], ca[1], ca[2], d1, d2)];
+            vj += s * w[0] * w[1] * w[2];
+        }
+        maxv = vj > maxv ? vj : maxv;
+    }
+    return maxv;
+}
+
+/* Ridge iff resp > threshold (double compare) and resp > the sample maximum
+ * (detect_ridges.cl:34-102). */
 ORC_API void orc_detect_ridges(float *out, const float *resp, const float *qeig3,
                                const int64_t *dims, double threshold)
 {
@@ -546,37 +589,28 @@ ORC_API void orc_detect_ridges(float *out, const float *resp, const float *qeig3
         for (int64_t i1 = 0; i1 < d1; ++i1)
             for (int64_t i2 = 0; i2 < d2; ++i2) {
                 const int64_t i = lin3(i0, i1, i2, d1, d2);
-                float q[4], m[9];
-                q_expand(qeig3 + 3 * i, q);
-                q_to_mat(q, m);
-                const float c1[3] = {m[1], m[4], m[7]};
-                const float c2[3] = {m[2], m[5], m[8]};
-                const float pos[3] = {(float)i0, (float)i1, (float)i2};
-                float maxv = -INFINITY;
-                for (int j = 0; j < 8; ++j) {
-                    const float a = (float)((-1) * (j < 3) + (j > 4));
-                    const float b = (float)((j % 3) - 1 + (j == 4));
-                    float pj[3];
-                    int oj[3];
-                    for (int d = 0; d < 3; ++d) {
-                        pj[d] = pos[d] + a * c1[d] + b * c2[d];
-                        oj[d] = (int)pj[d]; /* truncation toward zero */
-                        pj[d] = pj[d] - (float)oj[d];
-                    }
-                    float vj = 0;
-                    for (int k = 0; k < 8; ++k) {
-                        const int cr[3] = {k >> 2, (k >> 1) & 1, k & 1};
-                        const int ca[3] = {oj[0] + cr[0], oj[1] + cr[1], oj[2] + cr[2]};
-                        float w[3];
-                        for (int d = 0; d < 3; ++d) w[d] = cr[d] == 1 ? pj[d] : 1 - pj[d];
-                        const int outside = ca[0] < 0 || ca[1] < 0 || ca[2] < 0 ||
-                                            ca[0] >= (int)d0 || ca[1] >= (int)d1 || ca[2] >= (int)d2;
-                        const float s = outside ? -1.0f : resp[lin3(ca[0], ca[1], ca[2], d1, d2)];
-                        vj += s * w[0] * w[1] * w[2];
-                    }
-                    maxv = vj > maxv ? vj : maxv;
-                }
+                const float maxv = ridge_sample_max(resp, qeig3, i0, i1, i2, d0, d1, d2, NULL);
                 out[i] = (float)(resp[i] > threshold && resp[i] > maxv);
+            }
+}
+
+/* resp - (the ridge test's sample maximum), in double: how far each voxel is
+ * from the test's decision (a checker of ridge decisions that other builds
+ * may take the other way within rounding). 0 where a sample coordinate lies
+ * within 1e-5 of a negative integer: there the truncated interpolation jumps
+ * (a corner leaves the volume), so rounding in the frame decides. */
+ORC_API void orc_ridge_margin(double *out, const float *resp, const float *qeig3,
+                              const int64_t *dims)
+{
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i0 = 0; i0 < d0; ++i0)
+        for (int64_t i1 = 0; i1 < d1; ++i1)
+            for (int64_t i2 = 0; i2 < d2; ++i2) {
+                const int64_t i = lin3(i0, i1, i2, d1, d2);
+                int fragile = 0;
+                const float maxv = ridge_sample_max(resp, qeig3, i0, i1, i2, d0, d1, d2, &fragile);
+                out[i] = fragile ? 0.0 : (double)resp[i] - (double)maxv;
             }
 }
 

@@ -68,6 +68,7 @@ def lib():
         L.orc_vesselness.argtypes = [_f32p, _i64p, _f64p, C.c_int, C.c_float, C.c_float, C.c_float,
                                      C.c_int, _f32p, _f32p]
         L.orc_detect_ridges.argtypes = [_f32p, _f32p, _f32p, _i64p, C.c_double]
+        L.orc_ridge_margin.argtypes = [_f64p, _f32p, _f32p, _i64p]
         L.orc_main_dirs.argtypes = [_f32p, _f32p, C.c_int64]
         L.orc_adjust_nonridge_signs.argtypes = [_f32p, _i64p, _u16p, _f32p, C.c_int64, _f32p]
         L.orc_complete_frames.argtypes = [_f32p, _f32p, C.c_int64]
@@ -179,6 +180,15 @@ def detect_ridges(resp, qeig3, threshold):
     resp = _f32(resp)
     out = np.empty_like(resp)
     lib().orc_detect_ridges(out, resp, _f32(qeig3), _dims(resp.shape), float(threshold))
+    return out
+
+
+def ridge_margin(resp, qeig3):
+    """resp - the ridge test's sample maximum (float64), per voxel: a ridge
+    decision is a near-tie where this is within rounding of zero."""
+    resp = _f32(resp)
+    out = np.empty(resp.shape, np.float64)
+    lib().orc_ridge_margin(out, resp, _f32(qeig3), _dims(resp.shape))
     return out
 
 
