@@ -175,7 +175,8 @@ int mh_ridge_segments(mh_ctx *ctx, const float *cand_dev, const float *resp_dev,
  * signs from the smallest kept point along it, and every voxel's sign from
  * its nearest kept ridge point. kept_dev (N floats) receives the kept ridge
  * (1.f). stats_host (8 int64): ridge points, segments, kept segments, kept
- * points, zero-product forest edges, -, -, forest rounds. Replaces
+ * points, zero-product forest edges, microseconds in the cleaning's full-tree
+ * walk (0 when the bounds decide), microseconds in the call, forest rounds. Replaces
  * AdvancedAdjuster's ridge steps (processing/adjust_directions.py:213-256,
  * :290-339, :365-446) for an isotropic voxel size that is a power of two;
  * the results equal the host construction (mh_ridge_forest_host,

@@ -31,6 +31,7 @@
 //     sign on the device, and the nearest-ridge signs of all voxels
 //     (mh_nonridge_signs).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -49,6 +50,7 @@ constexpr unsigned kCutCap = 1u << 16;      // zero-product forest edges kept fo
 constexpr int64_t kMaxOthers = 1 << 16;     // kept points outside the largest kept segment
 constexpr int kMaxRounds = 64;
 constexpr int kMaxKept = 4096;              // kept segments (their pair table is K x K)
+constexpr int kHostPrimMax = 64;            // larger segment trees by Boruvka on the device
 constexpr int kDensePoints = 256;           // the host's dense spanning tree up to this many
 
 struct Grid {
@@ -63,6 +65,19 @@ __device__ __forceinline__ unsigned ld_acq(const unsigned *p)
 __device__ __forceinline__ void st_rel(unsigned *p, unsigned v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A flag set by many waves: one store per wave that asks, into one of 64
+// slots 128 bytes apart chosen by the work-group (stores from thousands of
+// waves to one address serialise); the host reads all slots.
+constexpr int kFlagSlots = 64;
+constexpr int kFlagStride = 32;
+
+__device__ __forceinline__ void wave_flag(bool pred, unsigned *flags)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m && __lane_id() == unsigned(__builtin_ctzll(m)))
+        st_rel(flags + (blockIdx.x & (kFlagSlots - 1)) * kFlagStride, 1u);
 }
 
 // float dot product, one rounding per operation, left to right
@@ -127,7 +142,7 @@ scan_counts_kernel(unsigned *__restrict__ counts, unsigned nb, unsigned *__restr
 template <class T>
 __global__ void __launch_bounds__(mh::kBlock)
 compact_kernel(const T *__restrict__ flag, int64_t n, const unsigned *__restrict__ offs,
-               int *__restrict__ out, int *__restrict__ rank_of)
+               int *__restrict__ out, int *__restrict__ rank_of, const int *__restrict__ src = nullptr)
 {
     __shared__ unsigned scan[mh::kBlock];
     const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
@@ -150,7 +165,7 @@ compact_kernel(const T *__restrict__ flag, int64_t n, const unsigned *__restrict
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k >= n) break;
-        if (f[k]) out[r] = int(base + k);
+        if (f[k]) out[r] = src ? src[base + k] : int(base + k);
         if (rank_of) rank_of[base + k] = f[k] ? int(r) : -1;
         r += f[k] ? 1u : 0u;
     }
@@ -169,20 +184,23 @@ bv_init_kernel(int R, int *__restrict__ comp, unsigned long long *__restrict__ b
     fkey[p] = kNoKey;
 }
 
-// every point's lightest pair with a 26-neighbour of another component,
-// minimised per component
+// every active point's lightest pair with a 26-neighbour of another
+// component, minimised per component; still[i] says whether active point i
+// had one (a point whose neighbours all share its component never has one
+// again: components only merge). act: the active points (nullptr: all).
 __global__ void __launch_bounds__(mh::kBlock)
-bv_min_edge_kernel(const int *__restrict__ pts, const int *__restrict__ at,
-                   const int *__restrict__ comp, int R, Grid g,
-                   unsigned long long *__restrict__ best, unsigned *__restrict__ found)
+bv_min_edge_kernel(const int *__restrict__ act_list, int nact, const int *__restrict__ pts,
+                   const int *__restrict__ at, const int *__restrict__ comp, Grid g,
+                   unsigned long long *__restrict__ best, unsigned char *__restrict__ still)
 {
-    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= R) return;
-    const int c = comp[p];
-    const int v = pts[p];
-    const int i2 = v % g.d2, i1 = (v / g.d2) % g.d1, i0 = v / (g.d2 * g.d1);
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     uint64_t m = kNoKey;
-    for (int a = -1; a <= 1; ++a) {
+    const bool act = i < nact;
+    const int p = act ? (act_list ? act_list[i] : i) : 0;
+    const int c = act ? comp[p] : 0;
+    const int v = act ? pts[p] : 0;
+    const int i2 = v % g.d2, i1 = (v / g.d2) % g.d1, i0 = v / (g.d2 * g.d1);
+    for (int a = act ? -1 : 2; a <= 1; ++a) {
         const int j0 = i0 + a;
         if (j0 < 0 || j0 >= g.d0) continue;
         for (int b = -1; b <= 1; ++b) {
@@ -200,10 +218,8 @@ bv_min_edge_kernel(const int *__restrict__ pts, const int *__restrict__ at,
             }
         }
     }
-    if (m != kNoKey) {
-        atomicMin(&best[c], (unsigned long long)m);
-        st_rel(found, 1u);
-    }
+    if (m != kNoKey) atomicMin(&best[c], (unsigned long long)m);
+    if (act) still[i] = m != kNoKey ? 1 : 0;
 }
 
 // each root hooks to the component at the other end of its lightest edge;
@@ -231,11 +247,14 @@ bv_jump_kernel(const int *__restrict__ comp, const int *__restrict__ up_in, int 
                int R, unsigned *__restrict__ changed)
 {
     const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p >= R || comp[p] != p) return;
-    const int u = up_in[p];
-    const int uu = up_in[u];
-    up_out[p] = uu;
-    if (uu != u) st_rel(changed, 1u);
+    bool moved = false;
+    if (p < R && comp[p] == p) {
+        const int u = up_in[p];
+        const int uu = up_in[u];
+        up_out[p] = uu;
+        moved = uu != u;
+    }
+    wave_flag(moved, changed);
 }
 
 __global__ void __launch_bounds__(mh::kBlock)
@@ -340,15 +359,26 @@ pu_flatten_kernel(unsigned *pw, int R, unsigned *__restrict__ err)
 }
 
 // per piece: the sum of its points' signed directions (float64) and its
-// point count; each thread sums a run of 64 points, flushing on a change of
-// piece
+// point count; each thread sums a run of kSumRun points, flushing on a change
+// of piece. The device's own float64 atomic add (the memory is the device's,
+// coarse-grained), not a compare-and-swap loop.
+constexpr int kSumRun = 16;
+
+__device__ __forceinline__ void flush_piece(double *psum, unsigned *pcnt, int piece, double s0,
+                                            double s1, double s2, unsigned n)
+{
+    unsafeAtomicAdd(psum + 3 * int64_t(piece), s0);
+    unsafeAtomicAdd(psum + 3 * int64_t(piece) + 1, s1);
+    unsafeAtomicAdd(psum + 3 * int64_t(piece) + 2, s2);
+    atomicAdd(pcnt + piece, n);
+}
 __global__ void __launch_bounds__(mh::kBlock)
 piece_sums_kernel(const unsigned *__restrict__ pw, const int *__restrict__ pts,
                   const float *__restrict__ dirs, int R, double *__restrict__ psum,
                   unsigned *__restrict__ pcnt)
 {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t begin = t * 64, end = min(int64_t(R), begin + 64);
+    const int64_t begin = t * kSumRun, end = min(int64_t(R), begin + kSumRun);
     int cur = -1;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     unsigned cnt = 0;
@@ -356,12 +386,7 @@ piece_sums_kernel(const unsigned *__restrict__ pw, const int *__restrict__ pts,
         const unsigned w = pw[p];
         const int r = int(w >> 1);
         if (r != cur) {
-            if (cur >= 0) {
-                atomicAdd(psum + 3 * int64_t(cur), s0);
-                atomicAdd(psum + 3 * int64_t(cur) + 1, s1);
-                atomicAdd(psum + 3 * int64_t(cur) + 2, s2);
-                atomicAdd(pcnt + cur, cnt);
-            }
+            if (cur >= 0) flush_piece(psum, pcnt, cur, s0, s1, s2, cnt);
             cur = r;
             s0 = s1 = s2 = 0.0;
             cnt = 0;
@@ -373,11 +398,26 @@ piece_sums_kernel(const unsigned *__restrict__ pw, const int *__restrict__ pts,
         s2 += double(sg * d[2]);
         ++cnt;
     }
-    if (cur >= 0) {
-        atomicAdd(psum + 3 * int64_t(cur), s0);
-        atomicAdd(psum + 3 * int64_t(cur) + 1, s1);
-        atomicAdd(psum + 3 * int64_t(cur) + 2, s2);
-        atomicAdd(pcnt + cur, cnt);
+    // the last runs: for each distinct piece among the wave's lanes, the
+    // lanes ending in it add their sums by shuffles and one lane flushes (a
+    // long piece spans most lanes of most waves)
+    bool pending = cur >= 0;
+    for (int guard = 0; guard < 64; ++guard) {
+        const unsigned long long m = __ballot(pending);
+        if (!m) break;
+        const int leader = __builtin_ctzll(m);
+        const int key = __shfl(cur, leader);
+        const bool mine = pending && cur == key;
+        double a0 = mine ? s0 : 0.0, a1 = mine ? s1 : 0.0, a2 = mine ? s2 : 0.0;
+        unsigned an = mine ? cnt : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            a0 += __shfl_xor(a0, off);
+            a1 += __shfl_xor(a1, off);
+            a2 += __shfl_xor(a2, off);
+            an += __shfl_xor(an, off);
+        }
+        if (int(__lane_id()) == leader) flush_piece(psum, pcnt, key, a0, a1, a2, an);
+        pending = pending && !mine;
     }
 }
 
@@ -513,6 +553,108 @@ pair_kernel(const int *__restrict__ others, int m, const int *__restrict__ pts,
     }
     if (PASS == 0 && best != kNone) atomicMin(w2min + slot, best);
     if (PASS == 1 && bk != kNoKey) atomicMin(idxmin + slot, (unsigned long long)bk);
+}
+
+// ---- the segments' minimum spanning tree by Boruvka rounds on the table --
+// Entry (a, b), a < b, of the K x K tables holds segments a and b's lightest
+// point pair: squared distance w2min and pair idxmin. Keys compare as
+// (w2, pair); per round every component takes its lightest pair to another
+// component in three passes (smallest w2, smallest pair at it, its entry).
+
+__global__ void __launch_bounds__(mh::kBlock)
+sm_w2_kernel(const unsigned *__restrict__ w2min, int K, const int *__restrict__ scomp,
+             unsigned *__restrict__ cw)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(K) * K) return;
+    const int a = int(t / K), b = int(t % K);
+    if (a >= b) return;
+    const int ca = scomp[a], cb = scomp[b];
+    const unsigned w = w2min[t];
+    if (ca == cb || w == kNone) return;
+    atomicMin(cw + ca, w);
+    atomicMin(cw + cb, w);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+sm_pair_kernel(const unsigned *__restrict__ w2min, const unsigned long long *__restrict__ idxmin,
+               int K, const int *__restrict__ scomp, const unsigned *__restrict__ cw,
+               unsigned long long *__restrict__ cx)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(K) * K) return;
+    const int a = int(t / K), b = int(t % K);
+    if (a >= b) return;
+    const int ca = scomp[a], cb = scomp[b];
+    const unsigned w = w2min[t];
+    if (ca == cb || w == kNone) return;
+    if (w == cw[ca]) atomicMin(cx + ca, idxmin[t]);
+    if (w == cw[cb]) atomicMin(cx + cb, idxmin[t]);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+sm_entry_kernel(const unsigned *__restrict__ w2min, const unsigned long long *__restrict__ idxmin,
+                int K, const int *__restrict__ scomp, const unsigned *__restrict__ cw,
+                const unsigned long long *__restrict__ cx, int64_t *__restrict__ centry)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(K) * K) return;
+    const int a = int(t / K), b = int(t % K);
+    if (a >= b) return;
+    const int ca = scomp[a], cb = scomp[b];
+    const unsigned w = w2min[t];
+    if (ca == cb || w == kNone) return;
+    const unsigned long long x = idxmin[t];
+    if (w == cw[ca] && x == cx[ca]) centry[ca] = t;  // the keys are distinct: one entry
+    if (w == cw[cb] && x == cx[cb]) centry[cb] = t;
+}
+
+// each root hooks to the component at the other end of its lightest pair;
+// of two roots that chose the same pair the larger hooks
+__global__ void __launch_bounds__(mh::kBlock)
+sm_hook_kernel(int K, const int *__restrict__ scomp, const unsigned *__restrict__ cw,
+               const unsigned long long *__restrict__ cx, const int64_t *__restrict__ centry,
+               int *__restrict__ up, unsigned long long *__restrict__ edge, unsigned *__restrict__ flags)
+{
+    const int c = int(blockIdx.x * blockDim.x + threadIdx.x);
+    bool hooked = false;
+    if (c < K && scomp[c] == c) {
+        up[c] = c;
+        if (cw[c] != kNone) {
+            const int64_t t = centry[c];
+            const int a = int(t / K), b = int(t % K);
+            const int other = scomp[a] == c ? scomp[b] : scomp[a];
+            if (!(cw[other] == cw[c] && cx[other] == cx[c] && c < other)) {
+                up[c] = other;
+                edge[c] = cx[c];
+                hooked = true;
+            }
+        }
+    }
+    wave_flag(hooked, flags);
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+sm_relabel_kernel(int *__restrict__ scomp, const int *__restrict__ up, int K, unsigned *__restrict__ cw,
+                  unsigned long long *__restrict__ cx)
+{
+    const int c = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= K) return;
+    scomp[c] = up[scomp[c]];
+    cw[c] = kNone;
+    cx[c] = kNoKey;
+}
+
+__global__ void __launch_bounds__(mh::kBlock)
+sm_init_kernel(int K, int *__restrict__ scomp, unsigned *__restrict__ cw,
+               unsigned long long *__restrict__ cx, unsigned long long *__restrict__ edge)
+{
+    const int c = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= K) return;
+    scomp[c] = c;
+    cw[c] = kNone;
+    cx[c] = kNoKey;
+    edge[c] = kNoKey;
 }
 
 __global__ void __launch_bounds__(mh::kBlock)
@@ -652,6 +794,11 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     MH_REQUIRE(ctx, mh::dims_ok(dims), "bad dims");
     MH_REQUIRE(ctx, dirs && ridge && labels && kept_out && stats, "null pointer");
     for (int k = 0; k < 8; ++k) stats[k] = 0;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto usec_since = [](std::chrono::steady_clock::time_point t) {
+        return int64_t(std::chrono::duration_cast<std::chrono::microseconds>(
+                           std::chrono::steady_clock::now() - t).count());
+    };
     if (!pow2_isotropic(voxel_size)) return MH_DEFER;
     const int64_t n = dims[0] * dims[1] * dims[2];
     if (n >= (int64_t(1) << kPtBits) || dims[0] > 16384 || dims[1] > 16384 || dims[2] > 16384)
@@ -683,9 +830,11 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
 
     // per-point arrays
     DevMem m1;
+    const unsigned rtile = unsigned((int64_t(R) + kScanTile - 1) / kScanTile);
     const int64_t bytes1 = 4 * al(4 * int64_t(R)) + 2 * al(8 * int64_t(R)) + al(4 * int64_t(R)) +
                            al(24 * int64_t(R)) + 2 * al(4 * int64_t(R)) + al(8 * int64_t(kCutCap)) +
-                           al(4 * 8) + al(4 * int64_t(R)) + al(int64_t(R));
+                           al(4 * 8) + al(4 * int64_t(R)) + al(int64_t(R)) + 2 * al(4 * int64_t(R)) +
+                           al(4 * kFlagSlots * kFlagStride) + al(4 * int64_t(rtile) + 4 * 8);
     MH_CHECK(ctx, hipMalloc(&m1.p, bytes1));
     Arena A1{static_cast<char *>(m1.p), 0, bytes1};
     int *comp = A1.take<int>(R);
@@ -702,7 +851,19 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     unsigned *cnt = A1.take<unsigned>(8);  // [0] zero, [1] NaN, [2] found, [3] changed, [4] err, [5] roots
     float *fsign = A1.take<float>(R);
     unsigned char *kflag = A1.take<unsigned char>(R);
-    if (!kflag) return mh::set_error(ctx, MH_ERR_ARG, __func__, "workspace layout");
+    int *actA = A1.take<int>(R);
+    int *actB = A1.take<int>(R);
+    unsigned *flags = A1.take<unsigned>(kFlagSlots * kFlagStride);
+    unsigned *rcount = A1.take<unsigned>(int64_t(rtile) + 8);
+    if (!rcount) return mh::set_error(ctx, MH_ERR_ARG, __func__, "workspace layout");
+    auto any_flag = [&](bool &out) -> int {
+        unsigned h[kFlagSlots * kFlagStride];
+        MH_CHECK(ctx, hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
+        MH_CHECK(ctx, hipStreamSynchronize(s));
+        out = false;
+        for (int k = 0; k < kFlagSlots; ++k) out = out || h[k * kFlagStride] != 0;
+        return MH_OK;
+    };
     const int *pts = pts_all;
     const unsigned nbR = mh::blocks_for(R);
 
@@ -710,27 +871,37 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     MH_CHECK(ctx, hipMemsetAsync(cnt, 0, 8 * sizeof(unsigned), s));
     hipLaunchKernelGGL(bv_init_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, R, comp, best, fkey);
     int rounds = 0;
+    const int *cur = nullptr;  // the active points: all of them in the first round
+    int ncur = R;
     for (;; ++rounds) {
         if (rounds >= kMaxRounds) return mh::set_error(ctx, MH_ERR_HIP, __func__, "forest rounds");
-        MH_CHECK(ctx, hipMemsetAsync(cnt + 2, 0, sizeof(unsigned), s));
-        hipLaunchKernelGGL(bv_min_edge_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, pts, at, comp, R, g,
-                           best, cnt + 2);
+        hipLaunchKernelGGL(bv_min_edge_kernel, dim3(mh::blocks_for(ncur)), dim3(mh::kBlock), 0, s, cur, ncur,
+                           pts, at, comp, g, best, kflag);
+        // the points that still have an outgoing pair stay active
+        int *nxt = cur == actA ? actB : actA;
+        const unsigned ctile = unsigned((int64_t(ncur) + kScanTile - 1) / kScanTile);
+        hipLaunchKernelGGL(flag_count_kernel<unsigned char>, dim3(ctile), dim3(mh::kBlock), 0, s, kflag,
+                           int64_t(ncur), rcount);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, rcount, ctile, rcount + rtile);
+        hipLaunchKernelGGL(compact_kernel<unsigned char>, dim3(ctile), dim3(mh::kBlock), 0, s, kflag,
+                           int64_t(ncur), rcount, nxt, (int *)nullptr, cur);
         MH_CHECK_LAUNCH(ctx, "bv_min_edge_kernel");
-        unsigned found = 0;
-        MH_CHECK(ctx, hipMemcpyAsync(&found, cnt + 2, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        unsigned nnext = 0;
+        MH_CHECK(ctx, hipMemcpyAsync(&nnext, rcount + rtile, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         MH_CHECK(ctx, hipStreamSynchronize(s));
-        if (!found) break;
+        if (nnext == 0) break;
+        cur = nxt;
+        ncur = int(nnext);
         hipLaunchKernelGGL(bv_hook_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, best, R, up, fkey);
         // doubling until no pointer moves (at most log2(R) + 1 steps)
         for (int step = 0;; ++step) {
             if (step > 34) return mh::set_error(ctx, MH_ERR_HIP, __func__, "hook doubling");
-            MH_CHECK(ctx, hipMemsetAsync(cnt + 3, 0, sizeof(unsigned), s));
-            hipLaunchKernelGGL(bv_jump_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, up, up2, R,
-                               cnt + 3);
+            MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
+            hipLaunchKernelGGL(bv_jump_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, up, up2, R, flags);
             std::swap(up, up2);
-            unsigned changed = 0;
-            MH_CHECK(ctx, hipMemcpyAsync(&changed, cnt + 3, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-            MH_CHECK(ctx, hipStreamSynchronize(s));
+            bool changed = false;
+            const int rc = any_flag(changed);
+            if (rc != MH_OK) return rc;
             if (!changed) break;
         }
         hipLaunchKernelGGL(bv_relabel_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, up, R, best);
@@ -748,7 +919,8 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     hipLaunchKernelGGL(fill_kernel<double>, dim3(mh::blocks_for(3 * int64_t(R))), dim3(mh::kBlock), 0, s,
                        psum, 3 * int64_t(R), 0.0);
     hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(nbR), dim3(mh::kBlock), 0, s, pcnt, int64_t(R), 0u);
-    hipLaunchKernelGGL(piece_sums_kernel, dim3(mh::blocks_for((int64_t(R) + 63) / 64)), dim3(mh::kBlock),
+    hipLaunchKernelGGL(piece_sums_kernel, dim3(mh::blocks_for((int64_t(R) + kSumRun - 1) / kSumRun)),
+                       dim3(mh::kBlock),
                        0, s, pw, pts, dirs, R, psum, pcnt);
     hipLaunchKernelGGL(piece_roots_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, pw, R, roots, cnt + 5);
     MH_CHECK_LAUNCH(ctx, "pieces");
@@ -828,6 +1000,62 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     auto piece_node = [&](unsigned w) {
         const int r = int(w >> 1);
         return int(std::lower_bound(proot.begin(), proot.end(), r) - proot.begin());
+    };
+
+    // the segments' tree from the K x K tables by Boruvka rounds on the
+    // device (the same unique tree as Prim's: the keys are distinct)
+    auto segment_mst_device = [&](int K, const unsigned *w2min, const unsigned long long *idxmin,
+                                  std::vector<std::array<int64_t, 2>> &inter) -> int {
+        DevMem ms;
+        const int64_t bs = 2 * al(4 * int64_t(K)) + al(4 * int64_t(K)) + 2 * al(8 * int64_t(K)) +
+                           al(8 * int64_t(K)) + al(4 * int64_t(K));
+        MH_CHECK(ctx, hipMalloc(&ms.p, bs));
+        Arena AS{static_cast<char *>(ms.p), 0, bs};
+        int *scomp = AS.take<int>(K);
+        int *sup = AS.take<int>(K);
+        unsigned *cw = AS.take<unsigned>(K);
+        auto *cx = AS.take<unsigned long long>(K);
+        auto *edge = AS.take<unsigned long long>(K);
+        auto *centry = AS.take<int64_t>(K);
+        int *sup2 = AS.take<int>(K);
+        if (!sup2) return mh::set_error(ctx, MH_ERR_ARG, __func__, "segment tree layout");
+        const unsigned nbK = mh::blocks_for(K), nbKK = mh::blocks_for(int64_t(K) * K);
+        hipLaunchKernelGGL(sm_init_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, K, scomp, cw, cx, edge);
+        for (int round = 0;; ++round) {
+            if (round >= kMaxRounds) return mh::set_error(ctx, MH_ERR_HIP, __func__, "segment tree rounds");
+            hipLaunchKernelGGL(sm_w2_kernel, dim3(nbKK), dim3(mh::kBlock), 0, s, w2min, K, scomp, cw);
+            hipLaunchKernelGGL(sm_pair_kernel, dim3(nbKK), dim3(mh::kBlock), 0, s, w2min, idxmin, K, scomp,
+                               cw, cx);
+            hipLaunchKernelGGL(sm_entry_kernel, dim3(nbKK), dim3(mh::kBlock), 0, s, w2min, idxmin, K, scomp,
+                               cw, cx, centry);
+            MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
+            hipLaunchKernelGGL(sm_hook_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, K, scomp, cw, cx, centry,
+                               sup, edge, flags);
+            MH_CHECK_LAUNCH(ctx, "segment tree round");
+            bool hooked = false;
+            int rc = any_flag(hooked);
+            if (rc != MH_OK) return rc;
+            if (!hooked) break;
+            for (int step = 0;; ++step) {
+                if (step > 34) return mh::set_error(ctx, MH_ERR_HIP, __func__, "segment tree doubling");
+                MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
+                hipLaunchKernelGGL(bv_jump_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, scomp, sup, sup2, K,
+                                   flags);
+                std::swap(sup, sup2);
+                bool changed = false;
+                rc = any_flag(changed);
+                if (rc != MH_OK) return rc;
+                if (!changed) break;
+            }
+            hipLaunchKernelGGL(sm_relabel_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, scomp, sup, K, cw, cx);
+        }
+        std::vector<unsigned long long> he(static_cast<size_t>(K));
+        MH_CHECK(ctx, hipMemcpyAsync(he.data(), edge, 8 * size_t(K), hipMemcpyDeviceToHost, s));
+        MH_CHECK(ctx, hipStreamSynchronize(s));
+        for (unsigned long long e : he)
+            if (e != kNoKey) inter.push_back({int64_t(e >> kPtBits), int64_t(e & kPtMask)});
+        if (int(inter.size()) != K - 1) return mh::set_error(ctx, MH_ERR_HIP, __func__, "segment tree");
+        return MH_OK;
     };
 
     // 5-6. The tree between the segments marked in `keep` and the walk over
@@ -926,6 +1154,10 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
             hipLaunchKernelGGL(pair_kernel<1>, dim3(nbp), dim3(mh::kBlock), 0, s, others, m, pts, labels, at,
                                seg_of_root, start, K, bigk, ed, eb, g, w2min, idxmin);
             MH_CHECK_LAUNCH(ctx, "segment pairs");
+            if (K > kHostPrimMax) {
+                const int rc2 = segment_mst_device(K, w2min, idxmin, inter);
+                if (rc2 != MH_OK) return rc2;
+            } else {
             std::vector<unsigned> hw2(static_cast<size_t>(K) * K);
             std::vector<unsigned long long> hix(static_cast<size_t>(K) * K);
             MH_CHECK(ctx, hipMemcpyAsync(hw2.data(), w2min, 4 * hw2.size(), hipMemcpyDeviceToHost, s));
@@ -960,6 +1192,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
                         bx[size_t(k)] = hix[sl];
                     }
                 }
+            }
             }
         }
 
@@ -1125,8 +1358,10 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
             nin += sure_in ? 1 : 0;
         }
         if (!decided || nin != want) {
+            const auto t_full = std::chrono::steady_clock::now();
             std::fill(keep.begin(), keep.end(), 1);
             const int rc = tree_walk();
+            stats[5] = usec_since(t_full);
             if (rc != MH_OK) return rc;
             std::vector<double> len(static_cast<size_t>(S), 0.0), acc(3 * size_t(S), 0.0);
             for (int k = 0; k < P; ++k)
@@ -1202,5 +1437,6 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         if (rc != MH_OK) return rc;
         MH_CHECK(ctx, hipStreamSynchronize(s));
     }
+    stats[6] = usec_since(t_start);
     return MH_OK;
 }

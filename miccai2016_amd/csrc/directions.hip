@@ -206,6 +206,84 @@ edt_axis2_kernel(const unsigned *__restrict__ ridx, unsigned *__restrict__ dist,
     }
 }
 
+// The same pass with one wave per line, the line in registers (E elements
+// per lane, lane l holding positions l*E .. l*E+E-1): the nearest marked
+// voxel at or left of each position is a prefix maximum of marked positions,
+// the nearest at or right of it a suffix minimum, both over the wave by
+// shuffles; the lower index wins when the left and right ones are
+// equidistant.
+template <int E>
+__global__ void __launch_bounds__(256)
+edt_axis2_wave_kernel(const unsigned *__restrict__ ridx, unsigned *__restrict__ dist,
+                      unsigned *__restrict__ best, int64_t nlines, int d2)
+{
+    const int64_t line = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (line >= nlines) return;  // a whole wave: the line is the wave's
+    const int lane = int(threadIdx.x & 63);
+    const unsigned *in = ridx + line * d2;
+    int pre[E], suf[E];
+    int run = -1;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int x = lane * E + k;
+        const bool m = x < d2 && in[x] != kNoRidge;
+        run = m ? x : run;
+        pre[k] = run;
+    }
+    // exclusive prefix maximum of the lanes' last marked positions
+    int inc = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off, 64);
+        if (lane >= off) inc = max(inc, v);
+    }
+    int exl = __shfl_up(inc, 1, 64);
+    if (lane == 0) exl = -1;
+    run = 0x7fffffff;
+#pragma unroll
+    for (int k = E - 1; k >= 0; --k) {
+        const int x = lane * E + k;
+        const bool m = x < d2 && in[x] != kNoRidge;
+        run = m ? x : run;
+        suf[k] = run;
+        pre[k] = max(pre[k], exl);
+    }
+    // exclusive suffix minimum of the lanes' first marked positions
+    int sinc = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_down(sinc, off, 64);
+        if (lane + off < 64) sinc = min(sinc, v);
+    }
+    int sexl = __shfl_down(sinc, 1, 64);
+    if (lane == 63) sexl = 0x7fffffff;
+    unsigned *od = dist + line * d2, *ob = best + line * d2;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int x = lane * E + k;
+        if (x >= d2) break;
+        const int f = pre[k], b = min(suf[k], sexl);
+        int c = f;
+        if (b < 0x7fffffff) {
+            if (f < 0) {
+                c = b;
+            } else {
+                const unsigned df = unsigned(x - f) * unsigned(x - f);
+                const unsigned db = unsigned(b - x) * unsigned(b - x);
+                if (db < df || (db == df && in[b] < in[f])) c = b;
+            }
+        }
+        if (c < 0) {
+            od[x] = kNoRidge;
+            ob[x] = kNoRidge;
+        } else {
+            const unsigned dx = unsigned(abs(x - c));
+            od[x] = dx * dx;
+            ob[x] = in[c];
+        }
+    }
+}
+
 // Pass along axis 1 or 0: scan outward from the voxel, stopping once the
 // squared offset alone reaches the best value found.
 template <int AXIS>
@@ -414,8 +492,18 @@ int mh_nearest_marked(mh_ctx *ctx, const int64_t dims[3], unsigned *work, hipStr
     unsigned *dA = ridx + n, *bA = dA + n, *dB = bA + n, *bB = dB + n;
     const int d0 = int(dims[0]), d1 = int(dims[1]), d2 = int(dims[2]);
     const unsigned nb = mh::blocks_for(n);
-    hipLaunchKernelGGL(edt_axis2_kernel, dim3(mh::blocks_for(int64_t(d0) * d1)), dim3(mh::kBlock), 0, s,
-                       ridx, dA, bA, d0, d1, d2);
+    const int64_t nlines = int64_t(d0) * d1;
+    const unsigned nbw = unsigned((nlines * 64 + 255) / 256);
+    if (d2 <= 256) {
+        hipLaunchKernelGGL(edt_axis2_wave_kernel<4>, dim3(nbw), dim3(256), 0, s, ridx, dA, bA, nlines, d2);
+    } else if (d2 <= 512) {
+        hipLaunchKernelGGL(edt_axis2_wave_kernel<8>, dim3(nbw), dim3(256), 0, s, ridx, dA, bA, nlines, d2);
+    } else if (d2 <= 1024) {
+        hipLaunchKernelGGL(edt_axis2_wave_kernel<16>, dim3(nbw), dim3(256), 0, s, ridx, dA, bA, nlines, d2);
+    } else {
+        hipLaunchKernelGGL(edt_axis2_kernel, dim3(mh::blocks_for(int64_t(d0) * d1)), dim3(mh::kBlock), 0,
+                           s, ridx, dA, bA, d0, d1, d2);
+    }
     hipLaunchKernelGGL(edt_scan_kernel<1>, dim3(nb), dim3(mh::kBlock), 0, s, dA, bA, dB, bB, d0, d1, d2);
     hipLaunchKernelGGL(edt_scan_kernel<0>, dim3(nb), dim3(mh::kBlock), 0, s, dB, bB, dA, bA, d0, d1, d2);
     MH_CHECK_LAUNCH(ctx, "separable nearest-voxel transform");

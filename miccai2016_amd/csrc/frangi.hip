@@ -62,23 +62,27 @@ __device__ __forceinline__ int mirror_index(int p, int len)
 // across the work-group and are read from the kernel arguments.
 
 constexpr int kTileX = 64;    // consecutive voxels along axis 2 per tile
-constexpr int kRunA = 16;     // outputs per thread along the pass axis (axes 0, 1)
-constexpr int kTileA = 4 * kRunA;
+constexpr int kRunA = 8;      // outputs per thread along the pass axis (axes 0, 1)
+constexpr int kTileY = 8;     // threads per tile column
+constexpr int kTileA = kTileY * kRunA;
+constexpr int kTileThreads = kTileX * kTileY;
 constexpr int kRows2 = 16;    // rows (axis 1) per tile of the axis-2 pass
 
 // Pass along axis 0 or 1. Tile: 64 voxels along axis 2 x 64 along the pass
-// axis; thread (x, y) computes the 16 consecutive outputs y*16 .. y*16+15 of
-// column x. LDS: 64 x (64 + 2R + 16) floats: the tile and its halo, then 16
-// zero rows that the window's last loads read; then the group's taps.
+// axis; thread (x, y) computes the kRunA (8) consecutive outputs y*8 .. y*8+7
+// of column x, 8 threads per column. LDS: 64 x (64 + 2R + 8) floats: the tile
+// and its halo, then 8 zero rows that the window's last loads read; then the
+// group's taps.
 //
-// The thread keeps a sliding window of the 16 column elements its outputs
+// The thread keeps a sliding window of the 8 column elements its outputs
 // need for the current tap in registers, as a ring indexed by position mod
-// 16: at tap j, output q reads element q + j from slot (q + j) mod 16, and
-// once all 16 outputs have used element j its slot takes element j + 16.
-// Taps go in unrolled blocks of 16 (the last one cut at K by a uniform
-// test), so every slot index is a constant: one LDS read per tap for the
-// column and one broadcast read per tap and output order for the weights,
-// instead of one per tap and output.
+// 8: at tap j, output q reads element q + j from slot (q + j) mod 8, and
+// once all 8 outputs have used element j its slot takes element j + 8.
+// Taps go in unrolled blocks of 8 (the last one cut at K by a uniform test),
+// so every slot index is a constant: one LDS read per tap for the column and
+// one broadcast read per tap and output order for the weights, instead of
+// one per tap and output. (16 outputs per thread read the column half as
+// often but held 127 VGPRs, four waves per SIMD; 8 outputs fit 80.)
 constexpr int kZeroRows = kRunA;
 
 template <int NOUT>
@@ -94,7 +98,7 @@ __device__ __forceinline__ void conv_tap(const float *wts, int j, int jj, float 
 #pragma unroll
         for (int k = 0; k < NOUT; ++k) acc[k][q] += w[k] * v;
     }
-    // element j is used up: its slot takes element j + 16
+    // element j is used up: its slot takes element j + kRunA
     win[jj] = col[(j + kRunA) * kTileX];
 }
 
@@ -133,7 +137,7 @@ __device__ __forceinline__ void conv_column(const float *wts, const ConvOut *out
 }
 
 template <int AXIS>
-__global__ void __launch_bounds__(256, 4)
+__global__ void __launch_bounds__(kTileThreads, 4)
 conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
 {
     extern __shared__ float lds[];
@@ -158,11 +162,11 @@ conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
     float *wts = lds + rows_alloc * kTileX;   // the group's taps, [output][j]
     for (int g = 0; g < P.ngroups; ++g) {
         const ConvGroup &G = P.grp[g];
-        for (int r = y; r < rows_alloc; r += 4)
+        for (int r = y; r < rows_alloc; r += kTileY)
             lds[r * kTileX + x] = (live && r < rows)
                                       ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa]
                                       : 0.f;
-        for (int t = threadIdx.x; t < G.nout * kTapsPad; t += 256)
+        for (int t = threadIdx.x; t < G.nout * kTapsPad; t += kTileThreads)
             wts[t] = P.ctaps[G.out[t / kTapsPad].order][t % kTapsPad];
         __syncthreads();
         if (G.nout == 3) conv_column<3>(wts, G.out, col, K, live, base, sa, a0, len, y);
@@ -304,9 +308,11 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
         const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius + kZeroRows) +
                                             3 * kTapsPad);
         if (axis == 0)
-            hipLaunchKernelGGL(conv_tile_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+            hipLaunchKernelGGL(conv_tile_kernel<0>, dim3(unsigned(blocks)), dim3(kTileThreads), shm, s, P, d0,
+                               d1, d2);
         else
-            hipLaunchKernelGGL(conv_tile_kernel<1>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+            hipLaunchKernelGGL(conv_tile_kernel<1>, dim3(unsigned(blocks)), dim3(kTileThreads), shm, s, P, d0,
+                               d1, d2);
     }
     MH_CHECK_LAUNCH(ctx, "conv pass");
     return MH_OK;

@@ -52,18 +52,24 @@ def test_device_adjustment_equals_host(dev, name, spec, scales):
     st = ad.device_stats
     print(f"{name}: device form {'ran' if ad.device_used else 'deferred'}; ridge points {st[0]}, "
           f"segments {st[1]}, kept {st[2]} with {st[3]} points, zero products {st[4]}, "
-          f"forest rounds {st[7]}", flush=True)
+          f"forest rounds {st[7]}, full-tree walk {st[5] / 1e3:.2f} ms, call {st[6] / 1e3:.2f} ms",
+          flush=True)
     assert ad.device_used
     assert np.array_equal(rd, rh)
     assert np.array_equal(dd, dh)
 
 
-def test_device_adjustment_without_cleaning(dev):
+@pytest.mark.parametrize("scales", [(4 / 1.2, 4 * 1.2, 16), (1.0, 2.0, 3)],
+                         ids=["matched-scales", "workload"])
+def test_device_adjustment_without_cleaning(dev, scales):
+    """Every segment kept: at the workload's scales the tree joins 161
+    segments, through the segment tree's Boruvka rounds on the device."""
     spec = ph.HelixSpec((64, 64, 64), 4.0, 16.0, 24.0, 1.5)
-    quat_t, ves_t, shape = _prepared(dev, spec, (4 / 1.2, 4 * 1.2, 16))
+    quat_t, ves_t, shape = _prepared(dev, spec, scales)
     (dd, rd, ad), (dh, rh, _) = _both(dev, quat_t, ves_t, shape, clean=False)
     print("without cleaning: device form", "ran" if ad.device_used else "deferred",
           ad.device_stats, flush=True)
+    assert ad.device_used
     assert np.array_equal(rd, rh)
     assert np.array_equal(dd, dh)
 

@@ -221,3 +221,27 @@ def test_converged_cut_is_the_brute_force_min_cut(case):
                                  update=4000, error_bound=0)
     assert done == 4000
     np.testing.assert_array_equal(u >= 0.5, want)
+
+
+def test_ridge_margin_decides_as_the_ridge_test(rng):
+    """ref32.ridge_margin (resp - the ridge test's sample maximum) is positive
+    exactly where a voxel above the threshold is a ridge candidate, away from
+    the voxels it marks as ties (0)."""
+    r = rng.random((12, 13, 14)).astype(np.float32)
+    q = (rng.random((12, 13, 14, 3)) * 3).astype(np.float32)
+    cand = ref32.detect_ridges(r, q, 0.01) != 0
+    m = ref32.ridge_margin(r, q)
+    decided = m != 0
+    assert decided.mean() > 0.99
+    assert np.array_equal(cand[decided], ((r > 0.01) & (m > 0))[decided])
+
+
+def test_ridge_margin_marks_the_truncation_jump(rng):
+    """Identity frames sample the in-plane neighbours at pos +- e1 +- e2: on
+    the faces i1 = 0 and i2 = 0 a sample coordinate is exactly -1, where the
+    reference's truncation toward zero jumps, so those decisions are ties;
+    interior voxels are not."""
+    r = rng.random((6, 6, 6)).astype(np.float32)
+    m = ref32.ridge_margin(r, np.zeros((6, 6, 6, 3), np.float32))
+    assert np.all(m[:, 0, :] == 0) and np.all(m[:, :, 0] == 0)
+    assert np.all(m[:, 1:, 1:] != 0)
