@@ -470,7 +470,10 @@ gather_sums_kernel(const int *__restrict__ roots, int m, const double *__restric
 __device__ __forceinline__ int kept_seg(int p, const int *pts, const int *labels, const int *at,
                                         const int *seg_of_root)
 {
-    return seg_of_root[at[labels[pts[p]]]];
+    // labels from another ridge than `ridge` would point off it: no segment
+    const int lab = labels[pts[p]];
+    const int r = lab >= 0 ? at[lab] : -1;
+    return r >= 0 ? seg_of_root[r] : -1;
 }
 
 // the largest kept segment's points marked with their indices; the other
@@ -965,6 +968,9 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         MH_CHECK(ctx, hipMemcpyAsync(hlab.data(), olab, 4 * size_t(P), hipMemcpyDeviceToHost, s));
         MH_CHECK(ctx, hipStreamSynchronize(s));
     }
+    for (int k = 0; k < P; ++k)
+        if (hlab[size_t(k)] < 0 || hlab[size_t(k)] >= n)
+            return mh::set_error(ctx, MH_ERR_ARG, __func__, "labels do not match the ridge");
     // segments: sorted distinct labels (the segments' smallest voxels)
     std::vector<int> slab(hlab);
     std::sort(slab.begin(), slab.end());
