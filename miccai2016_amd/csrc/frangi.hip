@@ -67,6 +67,10 @@ constexpr int kTileY = 8;     // threads per tile column
 constexpr int kTileA = kTileY * kRunA;
 constexpr int kTileThreads = kTileX * kTileY;
 constexpr int kRows2 = 16;    // rows (axis 1) per tile of the axis-2 pass
+// LDS row width of the axis-2 pass: the tile's 64 + 2R inputs and 8 zero
+// columns, rounded to 16 bytes; each of the row's 16 threads loads every
+// 16th entry. Rows of up to 16 kRowLoads floats (R <= 26) prefetch.
+constexpr int kRowLoads = 8;
 
 // Pass along axis 0 or 1. Tile: 64 voxels along axis 2 x 64 along the pass
 // axis; thread (x, y) computes the kRunA (8) consecutive outputs y*8 .. y*8+7
@@ -200,6 +204,7 @@ __device__ __forceinline__ void row_tap(float *a, const float4 &A, const float4 
     for (int q = 0; q < 4; ++q) a[q] += w * v[q + jj];
 }
 
+template <int NL>
 __global__ void __launch_bounds__(256)
 conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
 {
@@ -221,15 +226,37 @@ conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
     const int li1 = t1 * kRows2 + lr;
     const int64_t lrow = (int64_t(i0) * d1 + li1) * d2;
     const float *line = lds + row * width + xq;
+    // NL > 0 (rows of at most 16 NL floats): the next group's row segment is
+    // fetched into registers while the current group's taps run, so the
+    // loads' latency hides behind them
+    float nxt[NL > 0 ? NL : 1];
+    auto fetch = [&](int g) {
+        const float *in = P.grp[g].in;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int c = lc + 16 * k;
+            nxt[k] = (li1 < d1 && c < valid) ? in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
+        }
+    };
+    if (NL > 0) fetch(0);
     float res[4][6];
 #pragma unroll
     for (int g = 0; g < 6; ++g) {
         if (g >= P.ngroups) break;
         const ConvGroup &G = P.grp[g];
-        for (int c = lc; c < width; c += 16)
-            lds[lr * width + c] =
-                (li1 < d1 && c < valid) ? G.in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
+        if (NL > 0) {
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int c = lc + 16 * k;
+                if (c < width) lds[lr * width + c] = nxt[k];
+            }
+        } else {
+            for (int c = lc; c < width; c += 16)
+                lds[lr * width + c] =
+                    (li1 < d1 && c < valid) ? G.in[lrow + mirror_index(s0 - R + c, d2)] : 0.f;
+        }
         __syncthreads();
+        if (NL > 0 && g + 1 < P.ngroups) fetch(g + 1);
         float a[4] = {0.f, 0.f, 0.f, 0.f};
         const float *wts = P.ctaps[G.out[0].order];
         float4 A = lds4(line), B = lds4(line + 4);
@@ -300,7 +327,12 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
         const int64_t blocks = n2 * ((d1 + kRows2 - 1) / kRows2) * d0;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
         const size_t shm = sizeof(float) * kRows2 * ((kTileX + 2 * P.radius + 8 + 3) & ~3);
-        hipLaunchKernelGGL(conv_row_kernel, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
+        const int width = (kTileX + 2 * P.radius + 8 + 3) & ~3;
+        if (width <= 16 * kRowLoads)
+            hipLaunchKernelGGL(conv_row_kernel<kRowLoads>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0,
+                               d1, d2);
+        else
+            hipLaunchKernelGGL(conv_row_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
     } else {
         const int len = axis == 0 ? d0 : d1, other = axis == 0 ? d1 : d0;
         const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other;
