@@ -149,7 +149,12 @@ conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
     const int other = AXIS == 0 ? d1 : d0;
     const int n2 = (d2 + kTileX - 1) / kTileX;
     const int na = (len + kTileA - 1) / kTileA;
-    const unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    // one input group per work-group: the groups of a tile run side by side
+    // (each group's load latency hides behind other work-groups' taps)
+    // instead of one after another in one work-group
+    unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    const int g = int(b % unsigned(P.ngroups));
+    b /= unsigned(P.ngroups);
     const int t2 = int(b % unsigned(n2));
     const int ta = int((b / unsigned(n2)) % unsigned(na));
     const int o = int(b / (unsigned(n2) * unsigned(na)));
@@ -164,20 +169,17 @@ conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
     const bool live = i2 < d2;
     const float *col = lds + (y * kRunA) * kTileX + x;
     float *wts = lds + rows_alloc * kTileX;   // the group's taps, [output][j]
-    for (int g = 0; g < P.ngroups; ++g) {
-        const ConvGroup &G = P.grp[g];
-        for (int r = y; r < rows_alloc; r += kTileY)
-            lds[r * kTileX + x] = (live && r < rows)
-                                      ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa]
-                                      : 0.f;
-        for (int t = threadIdx.x; t < G.nout * kTapsPad; t += kTileThreads)
-            wts[t] = P.ctaps[G.out[t / kTapsPad].order][t % kTapsPad];
-        __syncthreads();
-        if (G.nout == 3) conv_column<3>(wts, G.out, col, K, live, base, sa, a0, len, y);
-        else if (G.nout == 2) conv_column<2>(wts, G.out, col, K, live, base, sa, a0, len, y);
-        else conv_column<1>(wts, G.out, col, K, live, base, sa, a0, len, y);
-        __syncthreads();
-    }
+    const ConvGroup &G = P.grp[g];
+    for (int r = y; r < rows_alloc; r += kTileY)
+        lds[r * kTileX + x] = (live && r < rows)
+                                  ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa]
+                                  : 0.f;
+    for (int t = threadIdx.x; t < G.nout * kTapsPad; t += kTileThreads)
+        wts[t] = P.ctaps[G.out[t / kTapsPad].order][t % kTapsPad];
+    __syncthreads();
+    if (G.nout == 3) conv_column<3>(wts, G.out, col, K, live, base, sa, a0, len, y);
+    else if (G.nout == 2) conv_column<2>(wts, G.out, col, K, live, base, sa, a0, len, y);
+    else conv_column<1>(wts, G.out, col, K, live, base, sa, a0, len, y);
 }
 
 // Pass along axis 2 (contiguous), one output per group. Tile: 64 outputs
@@ -335,7 +337,7 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
             hipLaunchKernelGGL(conv_row_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
     } else {
         const int len = axis == 0 ? d0 : d1, other = axis == 0 ? d1 : d0;
-        const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other;
+        const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other * P.ngroups;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
         const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius + kZeroRows) +
                                             3 * kTapsPad);
