@@ -742,11 +742,20 @@ struct Arena {
     }
 };
 
+// Scratch from the device's stream-ordered pool, released in stream order:
+// no device-wide synchronisation (hipFree's), so the preparation workers'
+// and the serving loop's other streams keep running.
 struct DevMem {
     void *p = nullptr;
+    hipStream_t s = nullptr;
+    hipError_t alloc(int64_t bytes, hipStream_t stream)
+    {
+        s = stream;
+        return hipMallocAsync(&p, size_t(bytes), stream);
+    }
     ~DevMem()
     {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, s);
     }
 };
 
@@ -813,7 +822,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     // 1. ridge points
     DevMem m0;
     const int64_t bytes0 = al(4 * n) + al(4 * int64_t(ntile)) + al(4 * 8) + al(4 * n);
-    MH_CHECK(ctx, hipMalloc(&m0.p, bytes0));
+    MH_CHECK(ctx, m0.alloc(bytes0, s));
     Arena A0{static_cast<char *>(m0.p), 0, bytes0};
     int *at = A0.take<int>(n);
     unsigned *tcount = A0.take<unsigned>(ntile);
@@ -838,7 +847,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
                            al(24 * int64_t(R)) + 2 * al(4 * int64_t(R)) + al(8 * int64_t(kCutCap)) +
                            al(4 * 8) + al(4 * int64_t(R)) + al(int64_t(R)) + 2 * al(4 * int64_t(R)) +
                            al(4 * kFlagSlots * kFlagStride) + al(4 * int64_t(rtile) + 4 * 8);
-    MH_CHECK(ctx, hipMalloc(&m1.p, bytes1));
+    MH_CHECK(ctx, m1.alloc(bytes1, s));
     Arena A1{static_cast<char *>(m1.p), 0, bytes1};
     int *comp = A1.take<int>(R);
     int *up = A1.take<int>(R);
@@ -950,7 +959,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     {
         DevMem mg;
         const int64_t bg = al(24 * int64_t(P)) + 3 * al(4 * int64_t(P)) + al(12 * int64_t(P));
-        MH_CHECK(ctx, hipMalloc(&mg.p, bg));
+        MH_CHECK(ctx, mg.alloc(bg, s));
         Arena AG{static_cast<char *>(mg.p), 0, bg};
         double *osum = AG.take<double>(3 * int64_t(P));
         unsigned *ocnt = AG.take<unsigned>(P);
@@ -996,7 +1005,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     // the nearest-voxel transform's workspace (the largest segment's, then
     // every voxel's nearest kept point)
     DevMem m2;
-    MH_CHECK(ctx, hipMalloc(&m2.p, al(5 * 4 * n)));
+    MH_CHECK(ctx, m2.alloc(al(5 * 4 * n), s));
     unsigned *edt = static_cast<unsigned *>(m2.p);
     std::vector<unsigned long long> hcuts(std::min<unsigned>(nzero, kCutCap));
     if (!hcuts.empty()) {
@@ -1015,7 +1024,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         DevMem ms;
         const int64_t bs = 2 * al(4 * int64_t(K)) + al(4 * int64_t(K)) + 2 * al(8 * int64_t(K)) +
                            al(8 * int64_t(K)) + al(4 * int64_t(K));
-        MH_CHECK(ctx, hipMalloc(&ms.p, bs));
+        MH_CHECK(ctx, ms.alloc(bs, s));
         Arena AS{static_cast<char *>(ms.p), 0, bs};
         int *scomp = AS.take<int>(K);
         int *sup = AS.take<int>(K);
@@ -1096,7 +1105,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
             for (int t = 0; t < K; ++t) labs[size_t(t)] = slab[size_t(kseg[size_t(t)])];
             std::iota(ids.begin(), ids.end(), 0);
             DevMem mkk;
-            MH_CHECK(ctx, hipMalloc(&mkk.p, 3 * al(4 * int64_t(K))));
+            MH_CHECK(ctx, mkk.alloc(3 * al(4 * int64_t(K)), s));
             Arena AKK{static_cast<char *>(mkk.p), 0, 3 * al(4 * int64_t(K))};
             int *d_labs = AKK.take<int>(K), *d_pts = AKK.take<int>(K), *d_ids = AKK.take<int>(K);
             if (!d_ids) return mh::set_error(ctx, MH_ERR_ARG, __func__, "segment root layout");
@@ -1123,7 +1132,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
             DevMem mt;
             const int64_t bt = 2 * al(4 * int64_t(K + 1)) + al(4 * m_others) + al(4 * int64_t(K) * K) +
                                al(8 * int64_t(K) * K);
-            MH_CHECK(ctx, hipMalloc(&mt.p, bt));
+            MH_CHECK(ctx, mt.alloc(bt, s));
             Arena AT{static_cast<char *>(mt.p), 0, bt};
             unsigned *start = AT.take<unsigned>(K + 1);
             unsigned *fill = AT.take<unsigned>(K + 1);
@@ -1222,7 +1231,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         {
             DevMem mq;
             const int64_t bq = 3 * al(4 * int64_t(Q)) + al(12 * int64_t(Q));
-            MH_CHECK(ctx, hipMalloc(&mq.p, bq));
+            MH_CHECK(ctx, mq.alloc(bq, s));
             Arena AQ{static_cast<char *>(mq.p), 0, bq};
             int *dl = AQ.take<int>(Q);
             unsigned *opw = AQ.take<unsigned>(Q);
@@ -1405,7 +1414,7 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         DevMem mk;
         const int64_t bk = al(4 * int64_t(NK)) + al(4 * int64_t(NK)) + al(4 * int64_t(ntile)) + al(4 * 8) +
                            al(4 * kept_pts) + al(6 * kept_pts) + al(12 * kept_pts);
-        MH_CHECK(ctx, hipMalloc(&mk.p, bk));
+        MH_CHECK(ctx, mk.alloc(bk, s));
         Arena AK{static_cast<char *>(mk.p), 0, bk};
         int *dr = AK.take<int>(NK);
         float *ds = AK.take<float>(NK);

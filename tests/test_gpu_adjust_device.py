@@ -100,3 +100,70 @@ def test_device_adjustment_small_ridge_defers(dev):
                                         shape)).reshape(shape + (3,))
     assert not adj.device_used and adj.device_stats[0] <= 256
     assert d.shape == shape + (3,) and n > 0
+
+
+def _synthetic_ridge(seed, shape=(48, 52, 56), nan_dirs=0):
+    """A ridge of straight runs and blobs whose main directions are half exact
+    axis vectors, so that many spanning-tree products are exactly zero (cuts
+    of the walk) or exactly +-1, and half random unit directions."""
+    rng = np.random.default_rng(seed)
+    cand = np.zeros(shape, np.float32)
+    for _ in range(60):   # straight runs along random axes
+        ax = rng.integers(0, 3)
+        p = [rng.integers(0, s) for s in shape]
+        ln = rng.integers(6, 40)
+        for t in range(ln):
+            q = list(p)
+            q[ax] = min(shape[ax] - 1, p[ax] + t)
+            cand[tuple(q)] = 1
+    for _ in range(25):   # small blobs
+        c = np.array([rng.integers(2, s - 2) for s in shape])
+        cand[c[0] - 1:c[0] + 2, c[1] - 1:c[1] + 1, c[2]:c[2] + 2] = 1
+    resp = (cand * (0.2 + 0.8 * rng.random(shape))).astype(np.float32)
+    axes = np.eye(3, dtype=np.float32)
+    pick = rng.integers(0, 6, shape)   # half exact axes, half random (distinct segment lengths)
+    sign = np.where(rng.random(shape) < 0.5, -1.0, 1.0).astype(np.float32)
+    rnd = rng.normal(size=shape + (3,)).astype(np.float32)
+    rnd /= np.linalg.norm(rnd, axis=-1, keepdims=True)
+    dirs = np.where((pick < 3)[..., None], axes[np.minimum(pick, 2)] * sign[..., None], rnd)
+    if nan_dirs:
+        idx = np.argwhere(cand > 0)[:nan_dirs]
+        dirs[tuple(idx.T)] = np.nan
+    return cand, resp, dirs.astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("clean", [True, False], ids=["clean", "all-segments"])
+def test_device_adjustment_with_many_zero_products(dev, seed, clean):
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    cand, resp, dirs = _synthetic_ridge(seed)
+    shape = cand.shape
+    out = []
+    for device in (True, False):
+        adj = AdvancedAdjuster((0.01, 0.3), None, None, 1.0, clean_ridge=clean, loader=dev)
+        adj.device_adjust = device
+        d = adj.execute_tensor(None, dev.upload(resp).reshape(-1), shape,
+                               candidates_t=dev.upload(cand).reshape(-1),
+                               main_dirs_t=dev.upload(dirs).reshape(-1))
+        out.append((dev.download(d).reshape(shape + (3,)), np.asarray(adj.ridge, bool), adj))
+    (dd, rd, ad), (dh, rh, _) = out
+    st = ad.device_stats
+    print(f"seed {seed} {'clean' if clean else 'all segments'}: device form "
+          f"{'ran' if ad.device_used else 'deferred'}; ridge points {st[0]}, segments {st[1]}, "
+          f"kept {st[2]}, zero products {st[4]}, full-tree walk {st[5] / 1e3:.2f} ms", flush=True)
+    assert np.array_equal(rd, rh)
+    assert np.array_equal(dd, dh)
+    if ad.device_used:
+        assert st[4] > 10   # the case exercises the walk's cuts
+
+
+def test_device_adjustment_defers_nan_products(dev):
+    from miccai2016_amd.processing.adjust_directions import AdvancedAdjuster
+    cand, resp, dirs = _synthetic_ridge(4, nan_dirs=3)
+    shape = cand.shape
+    adj = AdvancedAdjuster((0.01, 0.3), None, None, 1.0, loader=dev)
+    d = adj.execute_tensor(None, dev.upload(resp).reshape(-1), shape,
+                           candidates_t=dev.upload(cand).reshape(-1),
+                           main_dirs_t=dev.upload(dirs).reshape(-1))
+    assert not adj.device_used
+    assert dev.download(d).shape == (int(np.prod(shape)) * 3,)
