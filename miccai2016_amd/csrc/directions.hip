@@ -47,19 +47,38 @@ ridge_kernel(float *__restrict__ ridge, const float *__restrict__ resp,
             f[k] = p - float(o[k]);
         }
         float acc = 0.f;
+        // a sample whose unit cube lies inside the volume (every sample of a
+        // voxel 2 from the faces does) takes the same sums without the
+        // per-corner bounds tests
+        const bool inside8 = o[0] >= 0 && o[0] < d0 - 1 && o[1] >= 0 && o[1] < d1 - 1 && o[2] >= 0 &&
+                             o[2] < d2 - 1;
+        if (inside8) {
+            const float *r0 = resp + (o[2] + int64_t(d2) * (o[1] + int64_t(d1) * o[0]));
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cr[3] = {c >> 2, (c >> 1) & 1, c & 1};
-            int ca[3];
-            bool inside = true;
-            float w[3];
-            for (int k = 0; k < 3; ++k) {
-                ca[k] = o[k] + cr[k];
-                inside = inside && ca[k] >= 0 && ca[k] < dim[k];
-                w[k] = cr[k] ? f[k] : 1.f - f[k];
+            for (int c = 0; c < 8; ++c) {
+                const int cr[3] = {c >> 2, (c >> 1) & 1, c & 1};
+                const float w0 = cr[0] ? f[0] : 1.f - f[0];
+                const float w1 = cr[1] ? f[1] : 1.f - f[1];
+                const float w2 = cr[2] ? f[2] : 1.f - f[2];
+                const float s = r0[cr[2] + d2 * (cr[1] + d1 * cr[0])];
+                acc += s * w0 * w1 * w2;
             }
-            const float s = inside ? resp[ca[2] + int64_t(d2) * (ca[1] + int64_t(d1) * ca[0])] : -1.f;
-            acc += s * w[0] * w[1] * w[2];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int cr[3] = {c >> 2, (c >> 1) & 1, c & 1};
+                int ca[3];
+                bool inside = true;
+                float w[3];
+                for (int k = 0; k < 3; ++k) {
+                    ca[k] = o[k] + cr[k];
+                    inside = inside && ca[k] >= 0 && ca[k] < dim[k];
+                    w[k] = cr[k] ? f[k] : 1.f - f[k];
+                }
+                const float s =
+                    inside ? resp[ca[2] + int64_t(d2) * (ca[1] + int64_t(d1) * ca[0])] : -1.f;
+                acc += s * w[0] * w[1] * w[2];
+            }
         }
         best = acc > best ? acc : best;
     }
