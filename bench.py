@@ -178,8 +178,8 @@ def main():
     ap.add_argument("--no-ragged", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one GPU and this build has no "
-              "multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; the solver runs "
+              "on one device (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     run = {p: (args.only == p) if args.only else True for p in PARTS}
     if not args.only:
@@ -555,7 +555,7 @@ def main():
                                      "ones are refused at submission (OverloadError), so p99 of "
                                      "the served requests stays bounded"}
 
-    # ---- the experiment's grid at every noise level (config 3 in full)
+    # ---- the experiment's grid at every noise level (all 15 noise levels)
     if run["sigmas"]:
         from miccai2016_amd import _lib
         from miccai2016_amd.evaluation import DeviceDice, device_dice_for

@@ -1,14 +1,15 @@
 """The whole GPU pipeline against the oracle's chain, and the benchmark
 workloads at their own sizes.
 
-  config 1 (64^3)   every stage from the oracle (oracle/chain.py) against the
-                    GPU pipeline, numpy API and device path: cut Dice >= 0.999
-  config 3 (128^3)  two grid cells x 500 iterations through the batched pair
+  64^3 pipeline     every stage from the oracle (oracle/chain.py) against the
+                    GPU pipeline: end to end statistically, stage by stage
+                    exactly
+  128^3 grid        two grid cells x 500 iterations through the batched pair
                     sweep against the oracle's solve: max |u| difference <= 1e-4
-  config 2 (256^3)  the preparation (Frangi, ridges, signs, GVF, frames)
+  256^3 preparation the preparation (Frangi, ridges, signs, GVF, frames)
                     against the oracle at reduced GVF iterations
-  config 4          ragged volumes with sides drawn from U[48, 192]
-  config 5          serving requests with sides drawn from log-U[64, 320]
+  ragged set        ragged volumes with sides drawn from U[48, 192]
+  serving           serving requests with sides drawn from log-U[64, 320]
   1-D chains        the converged GPU cut equals the brute-force minimum cut
   offset limits     the two-level sweep at the largest volumes it takes
 """
@@ -45,7 +46,7 @@ def _frame_angle(qa, qb):
     return np.arccos(np.clip(np.abs((a * b).sum(-1)), 0, 1))
 
 
-# ------------------------------------------------ config 1: the whole chain --
+# ------------------------------------------- 64^3 pipeline: the whole chain --
 
 def _gpu_prepare_steps(img, sigma, cfg, dev):
     """pipeline.prepare's stages one by one, keeping the intermediates."""
@@ -210,7 +211,7 @@ def test_pipeline_matches_oracle_chain_64(dev, name, noise, scales):
     assert abs(ph.fast_dice(got_d, truth) - ph.fast_dice(want, truth)) <= 3e-3
 
 
-# ------------------------------------------------ config 3: the headline grid --
+# --------------------------------------------------------- 128^3 grid cells --
 
 GRID_CELLS = [(10 ** -1.625, 10 ** 1.5), (10 ** 0.5, 10 ** -1.0)]
 
@@ -266,7 +267,7 @@ def test_gpu_cut_is_the_brute_force_min_cut(dev, case):
     np.testing.assert_array_equal(u >= 0.5, want)
 
 
-# --------------------------------------------- config 2: 256^3 preparation --
+# ------------------------------------------------------ 256^3 preparation --
 
 def test_preparation_256_against_oracle(dev):
     """The 256^3 workload's preparation (tube 8, helix 64, pitch 96, noise
@@ -364,7 +365,7 @@ def test_preparation_256_against_oracle(dev):
     assert np.mean(_frame_angle(frames[big], fw[big]) <= 1e-3) >= 0.999
 
 
-# ------------------------------------------------------- config 4: ragged --
+# -------------------------------------------------------------- ragged set --
 
 def test_ragged_set_at_config_sizes(dev):
     """Six volumes with every side drawn from U[48, 192] and the workload's
@@ -393,7 +394,7 @@ def test_ragged_set_at_config_sizes(dev):
         assert np.array_equal(dev.download(rr.volume(v)), dev.download(u).reshape(x.shape)), v
 
 
-# ------------------------------------------------------ config 5: serving --
+# ----------------------------------------------------------------- serving --
 
 def _lone_phantom_cut(dev, r):
     """A PhantomCutRequest (centre-line frames) solved alone on the device."""
