@@ -56,8 +56,8 @@ TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 44.1, 2: 49.3, 3: 44.5, 4: 45.4, 5: 41.9}
 TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.1, 2: 56.6, 3: 52.5, 4: 46.7, 5: 46.9}
 # the batched grid's sweep, two solves per thread (the 1024-thread tile shapes 1, 4
-# and 5), at 128^3; shapes 4 and 5 unmeasured
-TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER = {1: 37.5}
+# and 5), at 128^3, with the axial p-step; shapes 4 and 5 unmeasured
+TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER = {1: 38.8}
 PAIR_TILES = (1, 4, 5)
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
