@@ -39,10 +39,10 @@ sys.path.insert(0, ROOT)
 BYTES_PER_VOXEL_ITER = 84.0      # bytes of one max-flow iteration as two stencil passes
 HBM_PEAK_GBS = 8000.0            # MI355X
 
-# Algorithmic bytes of one pass of the two-level sweep per voxel: each solve's
+# Algorithmic bytes of one pass of the multi-level sweep per voxel: each solve's
 # state (hp x3, pt, tmp, u) read once and written once (48 B) and its alpha
 # read (4 B), plus the fields a pass reads once for every solve it advances
-# (frame as 4 floats, cs, ct: 24 B). A pass applies one or two iterations.
+# (frame as 4 floats, cs, ct: 24 B). A pass applies one to L iterations.
 SWEEP_STATE_BYTES = 52.0
 SWEEP_SHARED_BYTES = 24.0
 
@@ -55,10 +55,13 @@ TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 # (the grid) and at 256^3
 TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 44.1, 2: 49.3, 3: 44.5, 4: 45.4, 5: 41.9}
 TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.1, 2: 56.6, 3: 52.5, 4: 46.7, 5: 46.9}
-# the batched grid's sweep, two solves per thread (the 1024-thread tile shapes 1, 4
-# and 5), at 128^3, with the axial p-step; shapes 4 and 5 unmeasured
+# the batched grid's sweep, two solves per thread (the 1024-thread tile shapes of
+# two and three levels: 1, 4, 5, 6, 7), at 128^3, with the axial p-step; shapes
+# 4, 5 and 6 unmeasured
 TRAFFIC_SWEEP_PAIR_PER_VOXEL_ITER = {1: 38.8}
-PAIR_TILES = (1, 4, 5)
+PAIR_TILES = (1, 4, 5, 6, 7)
+# iterations per pass of each tile shape (1-based)
+SWEEP_LEVELS = {1: 2, 2: 2, 3: 2, 4: 2, 5: 2, 6: 3, 7: 3, 8: 4, 9: 4}
 GRID_GROUP = 4                         # solves per group of the batched per-iteration kernels
 
 PARTS = ("grid", "plain", "plain64", "plain256", "ragged", "serve", "cpu", "sigmas")
@@ -81,21 +84,30 @@ def timed(torch, fn, reps=1):
     return e0.elapsed_time(e1) / 1e3
 
 
-def sweep_choice(dev, shape):
-    """The tile shape (1-based) of the two-level sweep the library tuned for
-    volumes of this shape, or 0 for the per-iteration kernels / not tuned."""
+def sweep_choice(dev, shape, batched=False):
+    """The tile shape (1-based) of the multi-level sweep the library tuned for
+    volumes of this shape (with batched, the batched solver's pair choice when
+    it made one), or 0 for the per-iteration kernels / not tuned."""
     from miccai2016_amd import _lib
     tile, zc = C.c_int(0), C.c_int(0)
+    if batched:
+        ok = dev.lib.mh_cmf_batched_sweep_choice(dev.handle, _lib.dims3(shape), C.byref(tile),
+                                                 C.byref(zc))
+        if ok == 1:
+            return tile.value
     ok = dev.lib.mh_cmf_sweep_choice(dev.handle, _lib.dims3(shape), C.byref(tile), C.byref(zc))
     return tile.value if ok == 1 else 0
 
 
-def sweep_passes(iters, update):
-    """Passes of the two-level sweep for a solve of `iters` iterations: a
-    criterion iteration always ends its pass."""
+def sweep_passes(iters, update, levels=2):
+    """Passes of an L-level sweep for a solve of `iters` iterations: a
+    criterion iteration always ends its pass, and a pass stops short of
+    the end."""
     it = passes = 0
     while it < iters:
-        lv = 2 if (it % update != 0 and it + 1 < iters) else 1
+        lv = 1
+        while lv < levels and it + lv < iters and (it + lv - 1) % update != 0:
+            lv += 1
         it += lv
         passes += 1
     return passes
@@ -280,16 +292,18 @@ def main():
                                 "best_decay": float(d_all[best])}
         log(f"grid: best fast Dice {grid_dice[best]:.4f} at w={w_all[best]:.4g}, "
             f"decay={d_all[best]:.4g}")
-        tile = sweep_choice(dev, img.shape)
+        tile = sweep_choice(dev, img.shape, batched=True)
         if args.sweep == 2 and args.batched_sweep != 1:
             tile = args.sweep_tile or 1      # forced, not tuned
         step_s = secs / args.steps
         effective = value * BYTES_PER_VOXEL_ITER / 1e9
         if tile:
-            # the groups ran one after another through the two-level sweep:
-            # one launch is one pass of one or two iterations over a pair of
-            # solves (the 1024-thread tile shapes) or over one solve
-            passes = sweep_passes(args.iters, cfg.cut_update)
+            # the groups ran one after another through the multi-level sweep:
+            # one launch is one pass of up to L iterations over a pair of
+            # solves (the 1024-thread tile shapes of up to three levels) or
+            # over one solve
+            levels = SWEEP_LEVELS.get(tile, 2)
+            passes = sweep_passes(args.iters, cfg.cut_update, levels)
             pair = tile in PAIR_TILES and n < (1 << 26)
             per_launch = 2 if pair else 1
             launches = (S // per_launch + S % per_launch) * passes
@@ -303,10 +317,10 @@ def main():
             line["roofline"] = {
                 "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None if per_vi is None else per_vi * 2 * per_launch * n,
-                "kernel": f"cmf_sweep_kernel (tile shape {tile}, {per_launch} solve(s) per "
-                          f"thread): one launch = one pass of up to two max-flow iterations over "
-                          f"{per_launch} solve(s)",
+                "traffic": None if per_vi is None else per_vi * levels * per_launch * n,
+                "kernel": f"{'cmf_sweep_kernel' if levels == 2 else 'cmf_sweepL_kernel'} (tile "
+                          f"shape {tile}, {per_launch} solve(s) per thread): one launch = one pass "
+                          f"of up to {levels} max-flow iterations over {per_launch} solve(s)",
                 "algorithmic_bytes_per_launch": pass_bytes,
                 "algorithmic_bytes": "the blocked scheme's minimum: per pass and voxel, each "
                                      "solve's state (hp x3, pt, tmp, u) read and written once and "
@@ -314,10 +328,10 @@ def main():
                                      "for the pair",
                 "launch_us": step_s / launches * 1e6,
                 "traffic_per_voxel_iteration": per_vi,
-                "traffic_note": "measured HBM bytes per launch of two iterations (rocprofv3 "
-                                "FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); at 128^3 one "
-                                "pair's working set (~250 MB) mostly stays in the 256 MB "
-                                "last-level cache",
+                "traffic_note": f"measured HBM bytes per launch of {levels} iterations "
+                                "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); at "
+                                "128^3 one pair's working set (~250 MB) mostly stays in the 256 "
+                                "MB last-level cache",
                 "effective_GBs_84B": effective,
                 "effective_frac_84B": effective / HBM_PEAK_GBS}
         else:

@@ -357,19 +357,22 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * which the 1024-thread tile shapes advance two solves per thread (group 1:
  * one per thread). Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
-/* Two-level sweep of mh_cmf_solve_batched's groups (no counterpart in the
+/* Multi-level sweep of mh_cmf_solve_batched's groups (no counterpart in the
  * reference). 0 (default): when the groups run one after another and the
  * single-solve tuner picks a sweep for these dims (it is run on a probe copy
- * of solve 0 if it has not been for these dims), solves run one at a time
- * through that sweep; otherwise groups of 4 use the per-iteration kernels,
- * which share the frame and capacities in registers. 1: never sweep. 2: sweep
- * the groups whenever eligible, with the tile shape forced by
- * mh_cmf_set_sweep(mode 2) or the tuned one.
- * Results do not depend on it, bit for bit. */
+ * of solve 0 if it has not been for these dims), the solves run through a
+ * sweep in pairs (two per thread) where the volume allows: the first group's
+ * passes time the single-solve choice and each pair-capable tile shape, and
+ * the fastest runs the rest (kept per dims, mh_cmf_batched_sweep_choice);
+ * otherwise groups of 4 use the per-iteration kernels, which share the frame
+ * and capacities in registers. 1: never sweep. 2: sweep the groups whenever
+ * eligible, with the tile shape forced by mh_cmf_set_sweep(mode 2) or the
+ * tuned one. Results do not depend on it, bit for bit. */
 int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
-/* Two-level sweep of mh_cmf_solve on this context (no counterpart in the
- * reference): unsupervised constant-eigenvalue solves may advance two
- * iterations per pass over the state (temporal blocking along axis 0). mode 0
+/* Multi-level sweep of mh_cmf_solve on this context (no counterpart in the
+ * reference): unsupervised constant-eigenvalue solves may advance two, three
+ * or four iterations per pass over the state (temporal blocking along axis
+ * 0; tile shapes 1-5 two levels, 6-7 three, 8-9 four). mode 0
  * (default) tunes: the first solve of a volume shape of >= 96 iterations times
  * the plain kernels and a few sweep tile shapes on its own iterations and the
  * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
@@ -390,6 +393,11 @@ int mh_cmf_set_graphs(mh_ctx *ctx, int on);
  * returns 1 and sets tile (0: the per-iteration kernels, else the 1-based tile
  * shape) and zchunk, or returns 0 when nothing was tuned for them. */
 int mh_cmf_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile, int *zchunk);
+/* The method mh_cmf_solve_batched's pair tuner chose on this context for
+ * volumes of these dims (no counterpart in the reference): returns 1 and sets
+ * tile (1-based tile shape) and zchunk, or returns 0 when no pairs were timed
+ * for them (the batched solver then uses the single-solve choice). */
+int mh_cmf_batched_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile, int *zchunk);
 
 /* Packed ragged batch: volume v has dims vdims_host[3v..3v+2] and its fields
  * start at element offset voff_host[v] in every packed array (u, eigs*3,

@@ -104,6 +104,7 @@ SIGNATURES = {
     "mh_cmf_set_sweep": (i32, [vp, i32, i32, i32]),
     "mh_cmf_batched_set_sweep": (i32, [vp, i32]),
     "mh_cmf_sweep_choice": (i32, [vp, I64x3, vp, vp]),
+    "mh_cmf_batched_sweep_choice": (i32, [vp, I64x3, vp, vp]),
     "mh_cmf_set_graphs": (i32, [vp, i32]),
     "mh_initial_u_work_floats": (i64, [i64]),
     "mh_initial_u": (i32, [vp, vp, i64, vp, vp, vp]),

@@ -210,14 +210,18 @@ struct SolveParams {
 // only steps 1 and 2 run and step 2 stores. The per-voxel arithmetic is the
 // shared helpers', so the result equals two launches of the plain kernels bit
 // for bit.
-// Output tiles (axis 1 x axis 2) and work-group sizes: the widened tile
-// (TY + 3) x (TX + 3) has one point per thread.
+// Output tiles (axis 1 x axis 2), work-group sizes and levels: the widened
+// tile (TY + 2L - 1) x (TX + 2L - 1) of an L-level sweep has one point per
+// thread. Passes shorter than a shape's levels (a criterion or the end cuts
+// them) run shape 0 (one or two levels) or shape 5 (three).
 struct SweepShape {
-    int ty, tx, threads;
+    int ty, tx, threads, levels;
 };
-constexpr SweepShape kSweepShapes[] = {{12, 64, 1024}, {11, 32, 512}, {4, 64, 512}, {26, 32, 1024},
-                                       {13, 61, 1024}};
-constexpr int kNumSweepShapes = 5;
+constexpr SweepShape kSweepShapes[] = {{12, 64, 1024, 2}, {11, 32, 512, 2}, {4, 64, 512, 2},
+                                       {26, 32, 1024, 2}, {13, 61, 1024, 2}, {11, 59, 1024, 3},
+                                       {27, 27, 1024, 3}, {9, 57, 1024, 4},  {25, 25, 1024, 4}};
+constexpr int kNumSweepShapes = 9;
+constexpr int kShape3 = 5;  // the three-level shape of a four-level method's three-level passes
 
 // Two solves per thread: the descriptors span both solves' fields (hp: 24 B
 // per voxel), so the volume must stay below half the single-solve limit
@@ -230,7 +234,20 @@ constexpr int64_t kSweepPairMaxVoxels = int64_t(1) << 26;
 // from three resident work-groups per CU to two, and stay one per thread.
 bool sweep_pairs(int shape_ix, int64_t n)
 {
-    return kSweepShapes[shape_ix].threads >= 1024 && n < kSweepPairMaxVoxels;
+    // (four levels for two solves would need more than 128 registers)
+    return kSweepShapes[shape_ix].threads >= 1024 && kSweepShapes[shape_ix].levels <= 3 &&
+           n < kSweepPairMaxVoxels;
+}
+
+// Levels of the sweep pass at iteration it of a shape of `levels` levels,
+// the last iteration `lim` - 1 and a criterion every `update` iterations: a
+// criterion iteration can only be a pass's last level (its u-step sums the
+// error), so passes are cut short before one, and at the end.
+int sweep_levels(int levels, int it, int lim, int update)
+{
+    int lv = 1;
+    while (lv < levels && it + lv < lim && (it + lv - 1) % update != 0) ++lv;
+    return lv;
 }
 
 // a way to run iterations: shape < 0 the plain kernels, else a two-level
@@ -787,6 +804,282 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
     }
 }
 
+// ------------------------------------------------------ L-level sweep ---
+// L iterations (levels 0 .. L-1) in one pass over the state, 1024 threads,
+// NS solves per thread (NS = 2 only in a batch: the pair shares the frames,
+// capacities, plane constants and barriers): the two-level scheme with more
+// levels. The widened tile is (TY + 2L - 1) x (TX + 2L - 1) points and the
+// output tile starts at (L, L). At step z level l works at plane z - l:
+//   its p-step on rows and columns l .. T + 2L - 2 - l, from level l-1's hp
+//     at that plane (the loaded state for level 0) and level l-1's tmp there
+//     and at the plane above (+1 in-plane neighbours from LDS),
+//   a barrier (its hp along axes 1 and 2 is in LDS),
+//   its u-step on rows and columns l + 1 .. T + 2L - 2 - l, from its hp and
+//     the -1 in-plane neighbours' (LDS), its hp along axis 0 at the plane
+//     below (carried), and level l-1's pt and u there;
+// the last level's u-step covers the output tile and stores iteration t + L.
+// Each level's tmp goes to LDS at the end of the step, for the next level's
+// p-step at the next step. Per plane the state is read and written once per
+// L iterations; each level's carried values and plane constants (alpha,
+// frame axis, cs, ct) shift by one level per step. The per-voxel arithmetic
+// is the shared helpers', so a pass equals L launches of the plain kernels
+// bit for bit.
+template <int TY, int TX, int L, bool BATCH, int NS>
+__global__ void __launch_bounds__(1024)
+cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__restrict__ alpha,
+                  const float *__restrict__ cs, const float *__restrict__ ct, const Dims D,
+                  const PParams PP, const UParams UP, int zchunk, int check, int out_set,
+                  int *__restrict__ ctrl, float *__restrict__ partials, const SweepBatch SB)
+{
+    static_assert(L >= 2 && L <= 6, "2 to 6 levels");
+    static_assert(NS == 1 || (BATCH && NS == 2), "two solves per thread only in a batch");
+    unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
+    int ks = 0;
+    if (BATCH) {
+        const unsigned units = unsigned(SB.ns / NS);
+        ks = int(b % units) * NS;
+        b /= units;
+    }
+    // a stopped solve is neither advanced nor reported
+    bool act[NS];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        act[j] = !(BATCH ? SB.stop[SB.s0 + ks + j] : ctrl[kStop]);
+        any = any || act[j];
+    }
+    if (!any) return;
+    const int64_t n = int64_t(D.d0) * D.d1 * D.d2;
+    if (BATCH) {
+        const int64_t o = n * ks;
+        B.hp += 3 * o;
+        B.tmp += o;
+        B.pt += o;
+        B.u += o;
+        B.hpo += 3 * o;
+        B.tmpo += o;
+        B.pto += o;
+        B.uo += o;
+        alpha += o;
+    }
+    constexpr int W = 2 * L - 1;  // halo width of the widened tile
+    constexpr int EY = TY + W, EX = TX + W, NE = EY * EX;
+    static_assert(NE <= 1024, "one point per thread");
+    __shared__ float sH1[L][NS][NE], sH2[L][NS][NE];  // hp of each level along axes 1, 2
+    __shared__ float sT[L - 1][NS][NE];                // tmp of levels 0 .. L-2
+
+    const int ntx = (D.d2 + TX - 1) / TX, nty = (D.d1 + TY - 1) / TY;
+    const int tx = int(b % unsigned(ntx)), ty = int((b / unsigned(ntx)) % unsigned(nty));
+    const int tz = int(b / unsigned(ntx * nty));
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * zchunk;
+    const int z1 = min(z0 + zchunk, D.d0);
+    const int64_t st0 = int64_t(D.d1) * D.d2;
+    const SweepRsrc R{rsrc_of(B.tmp, 4 * n * NS), rsrc_of(B.hp, 12 * n * NS), rsrc_of(q4, 16 * n),
+                      rsrc_of(alpha, 4 * n * NS), rsrc_of(B.pt, 4 * n * NS), rsrc_of(B.u, 4 * n * NS),
+                      rsrc_of(cs, 4 * n), rsrc_of(ct, 4 * n), rsrc_of(B.hpo, 12 * n * NS),
+                      rsrc_of(B.tmpo, 4 * n * NS), rsrc_of(B.pto, 4 * n * NS),
+                      rsrc_of(B.uo, 4 * n * NS), B.tmp, B.hp, alpha, B.pt, B.u, cs, ct, q4};
+    const unsigned nb = unsigned(4 * n), pb = unsigned(4 * st0), d2b = 4u * unsigned(D.d2);
+
+    const int t = int(threadIdx.x);
+    const int yy = t / EX, xx = t % EX;
+    const bool live = t < NE;
+    const int p = yy * EX + xx;
+    const int y = y0 - L + yy, x = x0 - L + xx;
+    const bool inv = live && y >= 0 && y < D.d1 && x >= 0 && x < D.d2;
+    // the in-plane regions of level l: p-step rows/columns l .. hi_l - 1,
+    // u-step l + 1 .. hi_l - 1, with hi_l = T + 2L - 1 - l
+    bool inP[L], inU[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        inP[l] = inv && yy >= l && yy < TY + W - l && xx >= l && xx < TX + W - l;
+        inU[l] = inv && yy >= l + 1 && yy < TY + W - l && xx >= l + 1 && xx < TX + W - l;
+    }
+    const unsigned f = (y < D.d1 - 1 ? 16u : 0u) | (x < D.d2 - 1 ? 32u : 0u) |
+                       (y > 0 ? 64u : 0u) | (x > 0 ? 128u : 0u);
+    const unsigned ob = inv ? 4u * unsigned(y * D.d2 + x) : 0u;
+    const unsigned ob1 = (inv && (f & 16u)) ? ob + d2b : ob;
+    const unsigned ob2 = (inv && (f & 32u)) ? ob + 4u : ob;
+
+    const int zs = max(z0 - L, 0), zend = z1 + L - 2;
+    // carried from the previous step, per level l (its plane there was z-1-l)
+    // and solve j: hp, tmp, pt, u; and the plane constants of plane z-1-l
+    float h0[L][NS], h1[L][NS], h2[L][NS], tp[L][NS], ptp[L][NS], up[L][NS], ac[L][NS];
+    float csc[L], ctc[L], v0[L], v1[L], v2[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) h0[l][j] = h1[l][j] = h2[l][j] = tp[l][j] = ptp[l][j] = up[l][j] = ac[l][j] = 0.f;
+        csc[l] = ctc[l] = 0.f;
+        v0[l] = 1.f;
+        v1[l] = v2[l] = 0.f;
+    }
+    float err_acc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) err_acc[j] = 0.f;
+    SweepIn Li[NS];
+    SweepInShared LC;
+    LC.q = make_float4(1.f, 0.f, 0.f, 0.f);
+    LC.cs = LC.ct = 0.f;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        Li[j].t = Li[j].t0 = Li[j].t1 = Li[j].t2 = Li[j].h0 = Li[j].h1 = Li[j].h2 = 0.f;
+        Li[j].a = Li[j].pt = Li[j].u = 0.f;
+    }
+    auto load = [&](const int z) {
+        const int zc = min(z, D.d0 - 1);
+        sweep_load_all<NS, true>(Li, LC, R, nb, pb, unsigned(zc) * pb, ob, ob1, ob2, zc < D.d0 - 1);
+    };
+    if (zs < D.d0) load(zs);
+    auto step = [&](const int z) {
+        // plane constants of plane z (level 0), from the loads
+        float vN0, vN1, vN2;
+        mh::frame_axis(LC.q, vN0, vN1, vN2);
+        float aN[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) aN[j] = Li[j].a;
+        const float csN = LC.cs, ctN = LC.ct;
+        // results of this step per level, at plane z - l
+        float nh0[L][NS], nh1[L][NS], nh2[L][NS], nt[L][NS], npt[L][NS], nu[L][NS];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int w = z - l;
+            const bool pon = inP[l] && w >= max(z0 - (L - l), 0) && w < min(z1 + (L - 1 - l), D.d0);
+            const bool uon = inU[l] && w >= max(z0 - (L - 1 - l), 0) && w < min(z1 + (L - 1 - l), D.d0);
+            const bool e0 = w < D.d0 - 1, f0 = w > 0;
+            // plane constants of plane w
+            const float c_s = l == 0 ? csN : csc[l - 1], c_t = l == 0 ? ctN : ctc[l - 1];
+            const float w0 = l == 0 ? vN0 : v0[l - 1], w1 = l == 0 ? vN1 : v1[l - 1],
+                        w2 = l == 0 ? vN2 : v2[l - 1];
+            // p-step: from level l-1's hp at plane w (carried) or the loads
+            float g0h[NS], g1h[NS], g2h[NS];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                g0h[j] = l == 0 ? Li[j].h0 : h0[l - 1][j];
+                g1h[j] = l == 0 ? Li[j].h1 : h1[l - 1][j];
+                g2h[j] = l == 0 ? Li[j].h2 : h2[l - 1][j];
+            }
+            if (pon) {
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    float g0, g1, g2;
+                    if (l == 0) {
+                        g0 = e0 ? (Li[j].t0 - Li[j].t) * PP.ss0 : 0.f;
+                        g1 = (f & 16u) ? (Li[j].t1 - Li[j].t) * PP.ss1 : 0.f;
+                        g2 = (f & 32u) ? (Li[j].t2 - Li[j].t) * PP.ss2 : 0.f;
+                    } else {
+                        // level l-1's tmp: at plane w (carried, and its +1
+                        // in-plane neighbours in LDS) and at w+1 (this step)
+                        const float tc = tp[l - 1][j];
+                        const float t1 = sT[l - 1][j][p + EX], t2 = sT[l - 1][j][p + 1];
+                        g0 = e0 ? (nt[l - 1][j] - tc) * PP.ss0 : 0.f;
+                        g1 = (f & 16u) ? (t1 - tc) * PP.ss1 : 0.f;
+                        g2 = (f & 32u) ? (t2 - tc) * PP.ss2 : 0.f;
+                    }
+                    const float a = l == 0 ? aN[j] : ac[l - 1][j];
+                    mh::cmf_p_voxel_axi(g0h[j], g1h[j], g2h[j], g0, g1, g2, w0, w1, w2, PP.A, a);
+                }
+            }
+            if (live) {
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    sH1[l][j][p] = g1h[j];
+                    sH2[l][j][p] = g2h[j];
+                }
+            }
+            __syncthreads();
+            // u-step: from level l-1's pt, u at plane w (carried) or the loads
+            float tt[NS], ptt[NS], uu[NS];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                tt[j] = l == 0 ? Li[j].t : 0.f;
+                ptt[j] = l == 0 ? Li[j].pt : ptp[l - 1][j];
+                uu[j] = l == 0 ? Li[j].u : up[l - 1][j];
+            }
+            if (uon) {
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    const float n1 = sH1[l][j][p - EX], n2 = sH2[l][j][p - 1];
+                    const float x0v = (e0 ? g0h[j] : 0.f) - (f0 ? h0[l][j] : 0.f);
+                    const float x1v = ((f & 16u) ? g1h[j] : 0.f) - ((f & 64u) ? n1 : 0.f);
+                    const float x2v = ((f & 32u) ? g2h[j] : 0.f) - ((f & 128u) ? n2 : 0.f);
+                    const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
+                    const float err = mh::cmf_u_voxel(div, ptt[j], uu[j], tt[j], c_s, c_t, UP.rcc,
+                                                      UP.inv_rcc, 1.f, 0.f);
+                    if (l == L - 1) {
+                        if (check) err_acc[j] += fabsf(err);
+                        if (NS > 1 && !act[j]) continue;
+                        const unsigned zp = unsigned(w) * pb, zj = zp + unsigned(j) * nb,
+                                       zh = zp + 3u * unsigned(j) * nb;
+                        pst(B.hpo, ob, zh, g0h[j]);
+                        pst(B.hpo, ob, zh + nb, g1h[j]);
+                        pst(B.hpo, ob, zh + 2u * nb, g2h[j]);
+                        pst(B.pto, ob, zj, ptt[j]);
+                        pst(B.uo, ob, zj, uu[j]);
+                        pst(B.tmpo, ob, zj, tt[j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                nh0[l][j] = g0h[j];
+                nh1[l][j] = g1h[j];
+                nh2[l][j] = g2h[j];
+                nt[l][j] = tt[j];
+                npt[l][j] = ptt[j];
+                nu[l][j] = uu[j];
+            }
+            // the next plane's inputs, in flight during the later levels
+            if (l == 0 && z + 1 <= zend && z + 1 < D.d0) load(z + 1);
+        }
+        // tmp of levels 0 .. L-2 for the next step's p-steps (every read of
+        // the previous ones finished before the barriers above)
+        if (live) {
+#pragma unroll
+            for (int l = 0; l < L - 1; ++l)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) sT[l][j][p] = nt[l][j];
+        }
+#pragma unroll
+        for (int l = L - 1; l >= 0; --l) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                h0[l][j] = nh0[l][j];
+                h1[l][j] = nh1[l][j];
+                h2[l][j] = nh2[l][j];
+                tp[l][j] = nt[l][j];
+                ptp[l][j] = npt[l][j];
+                up[l][j] = nu[l][j];
+                ac[l][j] = l == 0 ? aN[j] : ac[l - 1][j];
+            }
+            csc[l] = l == 0 ? csN : csc[l - 1];
+            ctc[l] = l == 0 ? ctN : ctc[l - 1];
+            v0[l] = l == 0 ? vN0 : v0[l - 1];
+            v1[l] = l == 0 ? vN1 : v1[l - 1];
+            v2[l] = l == 0 ? vN2 : v2[l - 1];
+        }
+    };
+    for (int z = zs; z <= zend; ++z) step(z);
+    if (check) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const float v = mh::block_reduce<SumOp>(err_acc[j]);
+            if (threadIdx.x == 0 && (NS == 1 || act[j])) {
+                if (BATCH) partials[int64_t(SB.s0 + ks + j) * SB.nper + b] = v;
+                else partials[blockIdx.x] = v;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (NS > 1 && !act[j]) continue;
+            if (BATCH && b == 0) SB.parity[SB.s0 + ks + j] = out_set;
+            else if (!BATCH && blockIdx.x == 0) ctrl[kParity] = out_set;
+        }
+    }
+}
+
 // batch: NS solves per thread, grid = work-groups per solve x batch->ns / NS
 template <int TY, int TX, int NT, int MAP>
 void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const float4 *q4,
@@ -808,6 +1101,24 @@ void launch_sweep_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const fl
                            partials, SweepBatch{});
 }
 
+// L levels: lv must be L (shorter passes take a shape of fewer levels)
+template <int TY, int TX, int L>
+void launch_sweepL_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const float4 *q4,
+                     const float *alpha, const float *cs, const float *ct, const Dims &D,
+                     const PParams &PP, const UParams &UP, int zchunk, int chk, int out_set, int *ctrl,
+                     float *partials, const SweepBatch *batch, int ns_thread)
+{
+    if (batch && ns_thread == 2)
+        hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, true, 2>), dim3(grid), dim3(1024), 0, s, SBf, q4,
+                           alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, *batch);
+    else if (batch)
+        hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, true, 1>), dim3(grid), dim3(1024), 0, s, SBf, q4,
+                           alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, *batch);
+    else
+        hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, false, 1>), dim3(grid), dim3(1024), 0, s, SBf, q4,
+                           alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, SweepBatch{});
+}
+
 // thread-to-point mapping of each tile shape (sweep_point)
 constexpr int kMap0 = 2, kMap1 = 1, kMap2 = 2, kMap3 = 2, kMap4 = 0;
 
@@ -817,6 +1128,12 @@ int sweep_occupancy_t(int &blocks)
     return int(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &blocks, reinterpret_cast<const void *>(&cmf_sweep_kernel<TY, TX, NT, MAP, false, 1>), NT, 0));
 }
+template <int TY, int TX, int L>
+int sweepL_occupancy_t(int &blocks)
+{
+    return int(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, reinterpret_cast<const void *>(&cmf_sweepL_kernel<TY, TX, L, false, 1>), 1024, 0));
+}
 
 // Work-groups of one sweep launch with this tile shape that the device holds
 // at once (CU count x resident work-groups per CU), cached in the context.
@@ -824,9 +1141,9 @@ int sweep_slots(mh_ctx *ctx, int shape_ix);
 
 // The plane chunk of a tile shape that a launch's cost model favours:
 // work-groups run in ceil(groups / slots) waves, each group sweeping its chunk
-// plus the two planes below it that the levels recompute, so the launch takes
-// about waves x (chunk + 2) plane steps. Chunks giving more than max_groups
-// work-groups are skipped.
+// plus the planes below and above it that the levels recompute (two for two
+// levels, four for three), so the launch takes about waves x (chunk + 2 L - 2)
+// plane steps. Chunks giving more than max_groups work-groups are skipped.
 int sweep_model_zchunk(mh_ctx *ctx, int shape_ix, const Dims &D, int64_t max_groups)
 {
     const SweepShape &S = kSweepShapes[shape_ix];
@@ -837,7 +1154,7 @@ int sweep_model_zchunk(mh_ctx *ctx, int shape_ix, const Dims &D, int64_t max_gro
     for (int z = std::min(4, D.d0); z <= std::min(64, D.d0); ++z) {
         const int64_t groups = tiles * ((D.d0 + z - 1) / z);
         if (groups > max_groups) continue;
-        const int64_t cost = ((groups + slots - 1) / slots) * (z + 2);
+        const int64_t cost = ((groups + slots - 1) / slots) * (z + 2 * S.levels - 2);
         if (best_cost < 0 || cost < best_cost || (cost == best_cost && z > best)) {
             best_cost = cost;
             best = z;
@@ -858,6 +1175,10 @@ int sweep_slots(mh_ctx *ctx, int shape_ix)
     case 2: rc = sweep_occupancy_t<4, 64, 512, kMap2>(per); break;
     case 3: rc = sweep_occupancy_t<26, 32, 1024, kMap3>(per); break;
     case 4: rc = sweep_occupancy_t<13, 61, 1024, kMap4>(per); break;
+    case 5: rc = sweepL_occupancy_t<11, 59, 3>(per); break;
+    case 6: rc = sweepL_occupancy_t<27, 27, 3>(per); break;
+    case 7: rc = sweepL_occupancy_t<9, 57, 4>(per); break;
+    case 8: rc = sweepL_occupancy_t<25, 25, 4>(per); break;
     default: rc = sweep_occupancy_t<12, 64, 1024, kMap0>(per); break;
     }
     (void)hipGetLastError();
@@ -877,6 +1198,10 @@ void launch_sweep(int shape_ix, unsigned grid, hipStream_t s, const SweepBufs &S
     case 2: launch_sweep_t<4, 64, 512, kMap2>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
     case 3: launch_sweep_t<26, 32, 1024, kMap3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
     case 4: launch_sweep_t<13, 61, 1024, kMap4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 5: launch_sweepL_t<11, 59, 3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 6: launch_sweepL_t<27, 27, 3>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 7: launch_sweepL_t<9, 57, 4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, batch, ns_thread); break;
+    case 8: launch_sweepL_t<25, 25, 4>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, batch, ns_thread); break;
     default: launch_sweep_t<12, 64, 1024, kMap0>(grid, s, SBf, q4, alpha, cs, ct, D, PP, UP, zchunk, lv, chk, out_set, ctrl, partials, batch, ns_thread); break;
     }
 }
@@ -1021,6 +1346,14 @@ struct Runner {
     }
 
     int model_zchunk(int shape_ix) const { return sweep_model_zchunk(ctx, shape_ix, D, int64_t(nb) + 64); }
+    // the method of a pass of lv levels under method m
+    SweepMethod pass_method(const SweepMethod &m, int lv) const
+    {
+        const int levels = kSweepShapes[m.shape].levels;
+        if (lv == levels || (levels == 2 && lv < 2)) return m;
+        const int k = lv <= 2 ? 0 : kShape3;
+        return SweepMethod{k, model_zchunk(k)};
+    }
 
     // one iteration (plain kernels) or one pass of up to two (sweep) that
     // does not go past `limit`; returns the iterations advanced
@@ -1055,16 +1388,17 @@ struct Runner {
             return 1;
         }
         // a criterion iteration is always the last level of its pass, so a
-        // stop never skips an iteration the pass already applied
+        // stop never skips an iteration the pass already applied; passes of
+        // fewer levels than a three-level shape takes run the first shape
         const int lim = std::min(prm->num_iter, limit);
-        const bool chk_a = (it % prm->update) == 0;
-        const int lv = (!chk_a && it + 1 < lim) ? 2 : 1;
+        const int lv = sweep_levels(kSweepShapes[m.shape].levels, it, lim, prm->update);
         const bool chk = ((it + lv - 1) % prm->update) == 0;
         SweepBufs SB;
         if (cur == 0) SB = SweepBufs{W.hp, W.tmp, W.pt, u, W.hp2, W.tmp2, W.pt2, W.u2};
         else SB = SweepBufs{W.hp2, W.tmp2, W.pt2, W.u2, W.hp, W.tmp, W.pt, u};
-        const int64_t grid = sweep_grid_of(m);
-        launch_sweep(m.shape, unsigned(grid), s, SB, W.q4, alpha, cs, ct, D, PP, UP, m.zchunk, lv,
+        const SweepMethod mm = pass_method(m, lv);
+        const int64_t grid = sweep_grid_of(mm);
+        launch_sweep(mm.shape, unsigned(grid), s, SB, W.q4, alpha, cs, ct, D, PP, UP, mm.zchunk, lv,
                      int(chk), 1 - cur, W.ctrl, W.partials);
         if (chk)
             hipLaunchKernelGGL(cmf_crit_kernel, dim3(1), dim3(1024), 0, s, W.partials, int(grid),
@@ -1109,17 +1443,17 @@ struct Runner {
         cands[0] = SweepMethod{-1, 0};
         for (int k = 0; k < kNumSweepShapes; ++k) cands[1 + k] = SweepMethod{k, model_zchunk(k)};
         // time each candidate on this solve's own iterations: one warm-up
-        // pass, then >= 8 iterations
+        // pass, then >= 12 iterations
         hipEvent_t e0, e1;
         MH_CHECK(ctx, hipEventCreate(&e0));
         MH_CHECK(ctx, hipEventCreate(&e1));
         float best = 0.f;
         for (const SweepMethod &m : cands) {
-            if (!usable(m) || it + 12 > prm->num_iter) continue;
+            if (!usable(m) || it + 16 > prm->num_iter) continue;
             step(m);
             MH_CHECK(ctx, hipEventRecord(e0, s));
             int done = 0;
-            while (done < 8) done += step(m);
+            while (done < 12) done += step(m);
             MH_CHECK(ctx, hipEventRecord(e1, s));
             MH_CHECK(ctx, hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1752,7 +2086,7 @@ int tune_on_probe(mh_ctx *ctx, const float *u, float w, float decay, const float
         Workspace W;
         layout(n, mem, &W);
         mh_cmf_params probe = *prm;
-        probe.num_iter = 96;
+        probe.num_iter = 192;  // every candidate's window
         probe.error_bound = 0.f;
         rc = solve_core(ctx, us, eigs, as, cs, ct, nullptr, D, n, &probe, W, s);
     }
@@ -1819,6 +2153,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     const bool sweep_elig = V >= prm->num_iter && n < kSweepMaxVoxels && ctx->batch_sweep != 1 &&
                             mh::axi_params(prm).on;
     bool want_sweep = sweep_elig && ctx->batch_sweep == 2;
+    bool tune_pairs = false;
     SweepMethod sm{0, 16};
     if (ctx->sweep_mode == 2) {
         // the forced single-solve method, applied to every solve in turn
@@ -1841,7 +2176,15 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
             sm = SweepMethod{f->second.first, f->second.second};
             if (sweep_elig && ctx->batch_sweep == 0 && ctx->sweep_mode == 0) {
                 want_sweep = true;
-                if (G <= 0) G = sweep_pairs(sm.shape, n) ? 2 : 1;
+                // pairs run at other relative speeds than single solves: the
+                // pair-capable shapes are timed on the first group's pair
+                // (below) unless that was done for these dims
+                auto fb2 = ctx->batch_tuned.find(key);
+                if (n < kSweepPairMaxVoxels && S >= 2) {
+                    if (fb2 != ctx->batch_tuned.end()) sm = SweepMethod{fb2->second.first, fb2->second.second};
+                    else tune_pairs = prm->num_iter >= 96;
+                }
+                if (G <= 0) G = (sweep_pairs(sm.shape, n) || tune_pairs) ? 2 : 1;
             }
         }
     }
@@ -1859,7 +2202,38 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     const SweepShape &SS = kSweepShapes[sm.shape];
     const int64_t nper = int64_t((D.d2 + SS.tx - 1) / SS.tx) * ((D.d1 + SS.ty - 1) / SS.ty) *
                          ((D.d0 + sm.zchunk - 1) / sm.zchunk);
-    const bool use_sweep = want_sweep && nper <= int64_t(nb);
+    auto groups_of = [&](const SweepMethod &m) {
+        const SweepShape &X = kSweepShapes[m.shape];
+        return int64_t((D.d2 + X.tx - 1) / X.tx) * ((D.d1 + X.ty - 1) / X.ty) *
+               ((D.d0 + m.zchunk - 1) / m.zchunk);
+    };
+    const SweepMethod fb{0, sweep_model_zchunk(ctx, 0, D, int64_t(nb))};
+    const SweepMethod fb3{kShape3, sweep_model_zchunk(ctx, kShape3, D, int64_t(nb))};
+    const int64_t nper_fb = groups_of(fb), nper_fb3 = groups_of(fb3);
+    const bool use_sweep = want_sweep && nper <= int64_t(nb) && nper_fb <= int64_t(nb) &&
+                           nper_fb3 <= int64_t(nb);
+    // Pair tuning: the first group's passes try the tuned single-solve method
+    // and each pair-capable shape (one warm-up pass, then >= 15 iterations
+    // timed with events); the fastest per iteration runs the rest. Results do
+    // not depend on the methods, bit for bit.
+    std::vector<SweepMethod> cands;
+    if (use_sweep && tune_pairs) {
+        cands.push_back(sm);
+        for (int k = 0; k < kNumSweepShapes; ++k) {
+            if (!sweep_pairs(k, n)) continue;
+            const SweepMethod m{k, sweep_model_zchunk(ctx, k, D, int64_t(nb))};
+            if (k != sm.shape && groups_of(m) <= int64_t(nb)) cands.push_back(m);
+        }
+    }
+    size_t ci = 0;
+    int tune_phase = 0, tune_it0 = 0;
+    float tune_best = 0.f;
+    SweepMethod tune_pick = sm;
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    if (!cands.empty()) {
+        MH_CHECK(ctx, hipEventCreate(&te0));
+        MH_CHECK(ctx, hipEventCreate(&te1));
+    }
     if (!use_sweep) {
         hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, S), dim3(mh::kBlock), 0, s, W.hp, W.pt,
                            W.tmp, u, cs, ct, n, prm->rcc);
@@ -1892,17 +2266,23 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                            alB);
         int cur = 0;
         for (int it = 0; it < prm->num_iter;) {
-            const bool chk_a = (it % prm->update) == 0;
-            const int lv = (!chk_a && it + 1 < prm->num_iter) ? 2 : 1;
+            const bool tuning = g == 0 && ci < cands.size();
+            const SweepMethod mth = tuning ? cands[ci] : sm;
+            const int levels = kSweepShapes[mth.shape].levels;
+            const int lv = sweep_levels(levels, it, prm->num_iter, prm->update);
             const bool chk = ((it + lv - 1) % prm->update) == 0;
+            // a shorter pass than the method's levels runs shape 0 or kShape3
+            const bool own = lv == levels || (levels == 2 && lv < 2);
+            const SweepMethod pm = own ? mth : (lv <= 2 ? fb : fb3);
+            const int64_t pn = groups_of(pm);
             SweepBufs B0{hpA, tmpA, ptA, u + o, hpB, tmpB, ptB, uB};
             SweepBufs B1{hpB, tmpB, ptB, uB, hpA, tmpA, ptA, u + o};
             // the group's solves two per thread, an odd last one alone
-            const int npair = sweep_pairs(sm.shape, n) ? ns / 2 : 0, rest = ns - 2 * npair;
+            const int npair = sweep_pairs(pm.shape, n) ? ns / 2 : 0, rest = ns - 2 * npair;
             if (npair) {
-                const SweepBatch SBp{s0, 2 * npair, int(nper), W.stop, W.parity};
-                launch_sweep(sm.shape, unsigned(nper * npair), s, cur == 0 ? B0 : B1, W.q4, alB, cs, ct, D,
-                             PP, UP, sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBp, 2);
+                const SweepBatch SBp{s0, 2 * npair, int(pn), W.stop, W.parity};
+                launch_sweep(pm.shape, unsigned(pn * npair), s, cur == 0 ? B0 : B1, W.q4, alB, cs, ct, D,
+                             PP, UP, pm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBp, 2);
             }
             if (rest) {
                 const int64_t r = n * (2 * npair);
@@ -1915,21 +2295,51 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                 Br.tmpo += r;
                 Br.pto += r;
                 Br.uo += r;
-                const SweepBatch SBr{s0 + 2 * npair, rest, int(nper), W.stop, W.parity};
-                launch_sweep(sm.shape, unsigned(nper * rest), s, Br, W.q4, alB + r, cs, ct, D, PP, UP,
-                             sm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBr, 1);
+                const SweepBatch SBr{s0 + 2 * npair, rest, int(pn), W.stop, W.parity};
+                launch_sweep(pm.shape, unsigned(pn * rest), s, Br, W.q4, alB + r, cs, ct, D, PP, UP,
+                             pm.zchunk, lv, int(chk), 1 - cur, nullptr, W.partials, &SBr, 1);
             }
             if (chk)
                 hipLaunchKernelGGL(cmf_batch_crit_kernel, dim3(ns), dim3(mh::kBlock), 0, s, W.partials,
-                                   int(nper), UP.inv_n_rcc, UP.error_bound, C, s0);
+                                   int(pn), UP.inv_n_rcc, UP.error_bound, C, s0);
             MH_CHECK_LAUNCH(ctx, "batched sweep");
             cur ^= 1;
             it += lv;
+            if (tuning) {
+                if (tune_phase == 0) {  // the warm-up pass is done
+                    MH_CHECK(ctx, hipEventRecord(te0, s));
+                    tune_phase = 1;
+                    tune_it0 = it;
+                } else if (it - tune_it0 >= 15) {
+                    MH_CHECK(ctx, hipEventRecord(te1, s));
+                    MH_CHECK(ctx, hipEventSynchronize(te1));
+                    float ms = 0.f;
+                    MH_CHECK(ctx, hipEventElapsedTime(&ms, te0, te1));
+                    const float per = ms / float(it - tune_it0);
+                    if (tune_best == 0.f || per < tune_best) {
+                        tune_best = per;
+                        tune_pick = mth;
+                    }
+                    tune_phase = 0;
+                    if (++ci == cands.size()) {
+                        sm = tune_pick;
+                        // timings of a stopped solve (empty launches) choose nothing
+                        std::vector<int> stopped(static_cast<size_t>(ns), 0);
+                        MH_CHECK(ctx, hipMemcpy(stopped.data(), W.stop, sizeof(int) * size_t(ns),
+                                                hipMemcpyDeviceToHost));
+                        if (std::all_of(stopped.begin(), stopped.end(), [](int v) { return v == 0; }))
+                            ctx->batch_tuned[key] = {sm.shape, sm.zchunk};
+                    }
+                }
+            }
         }
         hipLaunchKernelGGL(cmf_batch_sweep_final_kernel, dim3(nb, ns), dim3(mh::kBlock), 0, s, u, uB, n,
                            s0, W.parity);
         MH_CHECK_LAUNCH(ctx, "batched sweep final");
+        if (g == 0 && ci < cands.size()) sm = tune_pick;  // too few iterations to time them all
     }
+    if (te0) (void)hipEventDestroy(te0);
+    if (te1) (void)hipEventDestroy(te1);
     for (int base = 0; !use_sweep && base < prm->num_iter; base += V) {
         const int last = std::min(prm->num_iter, base + V);
         for (int g = 0; g < ngroups; ++g) {
@@ -2246,6 +2656,16 @@ MH_EXPORT int mh_cmf_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile,
     auto f = ctx->sweep_tuned.find({int(dims[0]), int(dims[1]), int(dims[2])});
     if (f == ctx->sweep_tuned.end()) return 0;
     *tile = f->second.first + 1;  // 0: the per-iteration kernels
+    *zchunk = f->second.second;
+    return 1;
+}
+
+MH_EXPORT int mh_cmf_batched_sweep_choice(mh_ctx *ctx, const int64_t dims[3], int *tile, int *zchunk)
+{
+    MH_REQUIRE(ctx, mh::dims_ok(dims) && tile && zchunk, "bad arguments");
+    auto f = ctx->batch_tuned.find({int(dims[0]), int(dims[1]), int(dims[2])});
+    if (f == ctx->batch_tuned.end()) return 0;
+    *tile = f->second.first + 1;
     *zchunk = f->second.second;
     return 1;
 }

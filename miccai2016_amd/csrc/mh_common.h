@@ -48,9 +48,12 @@ struct mh_ctx {
     // method the tuner chose per volume dims: (tile shape or -1 for the plain
     // kernels, plane chunk)
     std::map<std::array<int, 3>, std::pair<int, int>> sweep_tuned;
+    // method the batched solver's tuner chose per volume dims (tile shape,
+    // plane chunk), timed on pairs of solves
+    std::map<std::array<int, 3>, std::pair<int, int>> batch_tuned;
     // resident work-groups of the sweep kernel on the device, per tile shape
     // (CU count x occupancy; 0 until queried)
-    std::array<int, 8> sweep_slots{};
+    std::array<int, 16> sweep_slots{};
     // this device's copy of the no-allocation solver's scratch array (set by
     // mh_create; never freed, it lives as long as the process)
     void *noalloc_scratch = nullptr;

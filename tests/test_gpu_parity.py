@@ -538,7 +538,7 @@ def test_batched_solves_equal_plain_solves(dev, rng):
         assert np.array_equal(ub[s], up)
 
 
-@pytest.mark.parametrize("tile,zchunk", [(1, 0), (2, 5), (5, 0)])
+@pytest.mark.parametrize("tile,zchunk", [(1, 0), (2, 5), (5, 0), (6, 0), (7, 4), (8, 0), (9, 5)])
 def test_forced_sweep_applies_to_batched_solves(dev, rng, tile, zchunk):
     """With the single-solve sweep forced (tile shape, modelled or given plane
     chunk), the batched solver runs every solve through it; each equals the
@@ -621,10 +621,13 @@ def test_fill_and_reductions(dev, rng):
     assert (dev.download(t) == 2.5).all()
 
 
-@pytest.mark.parametrize("shape,iters,update,bound", [
+SWEEP_CASES = [
     ((16, 16, 16), 120, 30, 0.0), ((20, 17, 23), 61, 7, 0.0), ((7, 64, 5), 90, 30, 0.0),
     ((33, 9, 130), 75, 1, 0.0), ((2, 20, 20), 40, 3, 0.0), ((3, 3, 3), 17, 4, 0.0),
-    ((40, 70, 66), 3000, 7, 1e-3), ((24, 31, 140), 2, 30, 0.0)])
+    ((40, 70, 66), 3000, 7, 1e-3), ((24, 31, 140), 2, 30, 0.0)]
+
+
+@pytest.mark.parametrize("shape,iters,update,bound", SWEEP_CASES)
 def test_two_level_sweep_equals_plain_kernels(dev, rng, shape, iters, update, bound):
     """The temporally blocked sweep computes the plain kernels' iterations bit
     for bit, and stops at the same iteration."""
@@ -644,6 +647,30 @@ def test_two_level_sweep_equals_plain_kernels(dev, rng, shape, iters, update, bo
         assert out[1][1:3] == out[mode][1:3]
         # the criterion sums the same |err| in another order
         np.testing.assert_allclose(out[1][3], out[mode][3], rtol=1e-5)
+
+
+@pytest.mark.parametrize("tile", [6, 7, 8, 9])
+@pytest.mark.parametrize("shape,iters,update,bound",
+                         SWEEP_CASES + [((4, 40, 50), 30, 30, 0.0), ((5, 3, 70), 44, 2, 0.0),
+                                        ((9, 30, 31), 47, 5, 0.0)])
+def test_multi_level_sweep_equals_plain_kernels(dev, rng, tile, shape, iters, update, bound):
+    """The three- and four-level sweeps (tile shapes 6, 7 and 8, 9; the passes
+    a criterion or the end cuts short run shape 1 or 6) compute the plain
+    kernels' iterations bit for bit and stop at the same iteration."""
+    img, q, alpha, cs, ct, _ = _cut_case(rng, shape)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    out = []
+    try:
+        for mode, t in ((1, 0), (2, tile)):
+            dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, mode, t, 0))
+            c = _cutter(dev, iters, update=update, error_bound=bound)
+            u = c.execute(img, q, eigs, alpha, cs, ct, verbose=False)
+            out.append((u, c.last_result.iterations, c.last_result.stopped, c.last_criteria))
+    finally:
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1:3] == out[1][1:3]
+    np.testing.assert_allclose(out[0][3], out[1][3], rtol=1e-5)
 
 
 def _device_cut_case(dev, shape, seed=3):
@@ -666,11 +693,13 @@ def _device_cut_case(dev, shape, seed=3):
     return q, cs, ct, gms, a, u0
 
 
-@pytest.mark.parametrize("side,tile", [(128, 1), (256, 1), (256, 3)])
+@pytest.mark.parametrize("side,tile", [(128, 1), (256, 1), (256, 3), (128, 6), (256, 6), (256, 7),
+                                       (128, 9), (256, 8)])
 def test_sweep_equals_plain_kernels_at_full_size(dev, side, tile):
     """At the full volume sizes of the grid (128^3) and of the plain solve
-    (256^3) the two-level sweep computes the plain kernels' iterations bit
-    for bit (single- and two-level passes, criterion checks included)."""
+    (256^3) the two-, three- and four-level sweeps compute the plain kernels'
+    iterations bit for bit (passes of every level count, criterion checks
+    included)."""
     shape = (side,) * 3
     q, cs, ct, _, a, u0 = _device_cut_case(dev, shape)
     eigs = (1, 10 ** -1.5, 10 ** -1.5)
@@ -689,11 +718,12 @@ def test_sweep_equals_plain_kernels_at_full_size(dev, side, tile):
     assert dev.torch.equal(outs[0][0], outs[1][0])
 
 
-def test_batched_pairs_equal_lone_solves_at_full_size(dev):
+@pytest.mark.parametrize("tile", [1, 7])
+def test_batched_pairs_equal_lone_solves_at_full_size(dev, tile):
     """The grid's volume size (128^3): nine solves through the batched sweep
     (four pairs advanced two per thread and a lone ninth, each group's state
-    in the workspace's compact block) equal the same solves run one at a
-    time, bit for bit."""
+    in the workspace's compact block; two and three levels) equal the same
+    solves run one at a time, bit for bit."""
     shape = (128, 128, 128)
     n = int(np.prod(shape))
     q, cs, ct, gms, _, u0 = _device_cut_case(dev, shape, seed=5)
@@ -703,7 +733,7 @@ def test_batched_pairs_equal_lone_solves_at_full_size(dev):
     d = np.geomspace(1.0, 1000.0, S)[::-1].astype(np.float32)
     torch = dev.torch
     try:
-        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, 1, 0))
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, tile, 0))
         ub = u0.reshape(1, -1).repeat(S, 1)
         _cutter(dev, 33, update=7).execute_batched_device(ub, S, w, d, q, gms, cs, ct, shape, eigs)
         a = dev.empty(n)

@@ -456,12 +456,13 @@ def _device_cut_case(dev, shape, seed=3):
 def test_sweep_at_its_largest_volume(dev):
     """511 x 512 x 512 voxels, just below the single-solve sweep's limit of
     2^27 voxels (the frame field's byte offsets reach 2^31 - 64 MiB): the
-    forced sweep equals the per-iteration kernels bit for bit."""
+    forced two-, three- and four-level sweeps equal the per-iteration kernels
+    bit for bit."""
     shape = (511, 512, 512)
     q, cs, ct, _, a, u0 = _device_cut_case(dev, shape)
     outs = []
     try:
-        for mode, tile in ((1, 0), (2, 1)):
+        for mode, tile in ((1, 0), (2, 1), (2, 6), (2, 8)):
             dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, mode, tile, 0))
             ud = u0.clone()
             c = _cutter(dev, 21, update=7)
@@ -470,14 +471,16 @@ def test_sweep_at_its_largest_volume(dev):
     finally:
         dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
     dev.torch.cuda.synchronize()
-    assert outs[0][1] == outs[1][1] == 21
-    assert dev.torch.equal(outs[0][0], outs[1][0])
+    assert all(o[1] == 21 for o in outs)
+    for o in outs[1:]:
+        assert dev.torch.equal(outs[0][0], o[0])
 
 
-def test_pair_sweep_at_its_largest_volume(dev):
+@pytest.mark.parametrize("tile", [1, 6])
+def test_pair_sweep_at_its_largest_volume(dev, tile):
     """256 x 512 x 511 voxels, just below the pair sweep's limit of 2^26
-    voxels: two batched solves advanced two per thread equal the same solves
-    through the per-iteration kernels, bit for bit."""
+    voxels: two batched solves advanced two per thread (two and three levels)
+    equal the same solves through the per-iteration kernels, bit for bit."""
     shape = (256, 512, 511)
     n = int(np.prod(shape))
     q, cs, ct, gms, _, u0 = _device_cut_case(dev, shape, seed=4)
@@ -485,7 +488,7 @@ def test_pair_sweep_at_its_largest_volume(dev):
     d = np.array([30.0, 3.0], np.float32)
     torch = dev.torch
     try:
-        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, 1, 0))
+        dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 2, tile, 0))
         ub = u0.reshape(1, -1).repeat(2, 1)
         _cutter(dev, 21, update=7).execute_batched_device(ub, 2, w, d, q, gms, cs, ct, shape, EIGS)
         dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 1, 0, 0))
