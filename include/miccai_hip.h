@@ -353,7 +353,7 @@ int mh_cmf_solve_batched(mh_ctx *ctx, float *u_dev, int nsolves, const float *w_
  * advances `visit` iterations before the next group takes its turn (visit >=
  * num_iter runs the groups one after another). 0 selects the default: groups
  * of 4 run one after another, which keeps a 128^3 group's state within the
- * last-level cache; when the two-level sweep runs the solves, groups of 2,
+ * last-level cache; when a multi-level sweep runs the solves, groups of 2,
  * which the 1024-thread tile shapes advance two solves per thread (group 1:
  * one per thread). Results do not depend on the schedule, bit for bit. */
 int mh_cmf_batched_set_schedule(mh_ctx *ctx, int group, int visit);
@@ -467,7 +467,7 @@ int mh_dice_counts(mh_ctx *ctx, const float *u_dev, int nsolves, int64_t n, cons
  * solves run start to finish, continuous_cut_anisotropic_gpu.py:276-343): the
  * serving loop keeps large cuts resident this way and advances each by a
  * chunk of iterations per turn, with the same methods as mh_cmf_solve
- * (per-iteration kernels or two-level sweep), so the result is mh_cmf_solve's
+ * (per-iteration kernels or multi-level sweep), so the result is mh_cmf_solve's
  * bit for bit. Unsupervised. work_dev: mh_cmf_workspace_bytes(n) bytes,
  * 256-byte aligned; every pointer is a device pointer the caller owns. */
 typedef struct {
@@ -481,7 +481,7 @@ typedef struct {
     mh_cmf_params params;
     int iter;                /* iterations enqueued so far (set by the calls) */
     int cur;                 /* buffer set holding the state (set by the calls) */
-    int swept;               /* a two-level sweep has run (set by the calls) */
+    int swept;               /* a multi-level sweep has run (set by the calls) */
     int method;              /* -2 not chosen yet, -1 per-iteration kernels, else tile shape */
     int zchunk;
 } mh_cmf_resident;
@@ -491,7 +491,7 @@ int mh_cmf_resident_init(mh_ctx *ctx, mh_cmf_resident *r, void *stream);
 /* Enqueue up to `iters` more iterations (never past num_iter). The method is
  * chosen at the first step: tuned for these dims on this context, tuned now
  * on this solve's iterations when allow_tune (blocks while timing), or the
- * default (the sweep with the first tile shape for volumes of at least
+ * default (the three-level sweep with tile shape 7 for volumes of at least
  * 800,000 voxels, where it measured faster, else the per-iteration kernels). */
 int mh_cmf_resident_step(mh_ctx *ctx, mh_cmf_resident *r, int iters, int allow_tune, void *stream);
 /* Blocks; *stopped = 0 running, 1 NaN criterion, 2 converged. */

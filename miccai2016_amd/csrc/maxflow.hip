@@ -222,6 +222,7 @@ constexpr SweepShape kSweepShapes[] = {{12, 64, 1024, 2}, {11, 32, 512, 2}, {4, 
                                        {27, 27, 1024, 3}, {9, 57, 1024, 4},  {25, 25, 1024, 4}};
 constexpr int kNumSweepShapes = 9;
 constexpr int kShape3 = 5;  // the three-level shape of a four-level method's three-level passes
+constexpr int kDefaultShape = 6;  // untuned solves: the 27 x 27 three-level shape
 
 // Two solves per thread: the descriptors span both solves' fields (hp: 24 B
 // per voxel), so the volume must stay below half the single-solve limit
@@ -250,7 +251,7 @@ int sweep_levels(int levels, int it, int lim, int update)
     return lv;
 }
 
-// a way to run iterations: shape < 0 the plain kernels, else a two-level
+// a way to run iterations: shape < 0 the plain kernels, else a multi-level
 // sweep with that tile shape and plane chunk
 struct SweepMethod {
     int shape, zchunk;
@@ -1108,10 +1109,14 @@ void launch_sweepL_t(unsigned grid, hipStream_t s, const SweepBufs &SBf, const f
                      const PParams &PP, const UParams &UP, int zchunk, int chk, int out_set, int *ctrl,
                      float *partials, const SweepBatch *batch, int ns_thread)
 {
-    if (batch && ns_thread == 2)
-        hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, true, 2>), dim3(grid), dim3(1024), 0, s, SBf, q4,
-                           alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, *batch);
-    else if (batch)
+    if constexpr (L <= 3) {  // pairs only up to three levels (sweep_pairs)
+        if (batch && ns_thread == 2) {
+            hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, true, 2>), dim3(grid), dim3(1024), 0, s, SBf,
+                               q4, alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, *batch);
+            return;
+        }
+    }
+    if (batch)
         hipLaunchKernelGGL((cmf_sweepL_kernel<TY, TX, L, true, 1>), dim3(grid), dim3(1024), 0, s, SBf, q4,
                            alpha, cs, ct, D, PP, UP, zchunk, chk, out_set, ctrl, partials, *batch);
     else
@@ -1247,7 +1252,7 @@ struct Workspace {
     float *pt;
     float *tmp;
     float4 *q4;
-    // second state set of the two-level sweep (ping-pong)
+    // second state set of the multi-level sweep (ping-pong)
     float *hp2, *pt2, *tmp2, *u2;
 };
 
@@ -1289,7 +1294,7 @@ namespace {
 
 // One solve on workspace W, advanced in steps. An iteration runs either as
 // the plain kernels (in place on the current buffer set) or inside a
-// two-level sweep (current set -> the other set); both compute the same
+// multi-level sweep (current set -> the other set); both compute the same
 // values, so a solve may mix them, and the final u is in the set the last
 // sweep wrote (ctrl[kParity]).
 struct Runner {
@@ -1355,7 +1360,7 @@ struct Runner {
         return SweepMethod{k, model_zchunk(k)};
     }
 
-    // one iteration (plain kernels) or one pass of up to two (sweep) that
+    // one iteration (plain kernels) or one pass of up to L (sweep) that
     // does not go past `limit`; returns the iterations advanced
     int step(const SweepMethod &m)
     {
@@ -1389,7 +1394,7 @@ struct Runner {
         }
         // a criterion iteration is always the last level of its pass, so a
         // stop never skips an iteration the pass already applied; passes of
-        // fewer levels than a three-level shape takes run the first shape
+        // fewer levels than the shape takes run shape 0 or kShape3
         const int lim = std::min(prm->num_iter, limit);
         const int lv = sweep_levels(kSweepShapes[m.shape].levels, it, lim, prm->update);
         const bool chk = ((it + lv - 1) % prm->update) == 0;
@@ -1411,9 +1416,9 @@ struct Runner {
 
     // The method for this solve: forced by the context, tuned earlier for
     // these dims, tuned now on this solve's own iterations (allow_tune, at
-    // least 96 iterations), or the default (the sweep with the first tile
-    // shape when eligible and the volume has >= 800,000 voxels, else the
-    // plain kernels).
+    // least 96 iterations), or the default (the three-level sweep with the
+    // 27 x 27 tile shape when eligible and the volume has >= 800,000 voxels,
+    // else the plain kernels).
     int choose(bool allow_tune, SweepMethod &use)
     {
         use = SweepMethod{-1, 0};
@@ -1432,8 +1437,10 @@ struct Runner {
         }
         if (!allow_tune || prm->num_iter < 96) {
             // measured: the sweep wins from about 10^6 voxels (100^3) on, the
-            // per-iteration kernels at 80^3
-            const SweepMethod m{0, model_zchunk(0)};
+            // per-iteration kernels at 80^3; the 27 x 27 three-level shape
+            // was the fastest single solve at 128^3 and within 5 % of the
+            // fastest at 256^3
+            const SweepMethod m{kDefaultShape, model_zchunk(kDefaultShape)};
             if (usable(m) && n >= 800000) use = m;
             return MH_OK;
         }
@@ -1722,7 +1729,7 @@ MH_EXPORT int mh_cmf_resident_finish(mh_ctx *ctx, mh_cmf_resident *r, mh_cmf_res
 // =========================================================================
 // mh_cmf_solve for callers that cannot allocate device memory. The solve's
 // workspace (control block, criterion log, partial sums, both state sets of
-// the two-level sweep and the expanded frames: mh_cmf_workspace_bytes(n),
+// the multi-level sweep and the expanded frames: mh_cmf_workspace_bytes(n),
 // about 60 B per voxel) lives in `scratch`, one file-scope device array per
 // GPU. Its device address is looked up once per device when a context is
 // created on it (mh_noalloc_setup, called by mh_create with the device
@@ -1984,7 +1991,7 @@ struct BatchWorkspace {
     unsigned *count;
     float *crit_log, *partials, *hp, *pt, *tmp;
     float4 *q4;
-    // second state set of one group for the two-level sweep
+    // second state set of one group for the multi-level sweep
     float *hp2, *pt2, *tmp2, *u2, *alpha2;  // and the group's alpha fields
     int maxcheck;
 };
@@ -2143,7 +2150,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     // the last-level cache between its iterations when it fits there.
     int G = ctx->batch_group, V = ctx->batch_visit;
     if (V <= 0) V = std::max(1, prm->num_iter);
-    // Method. The two-level sweep runs a group's iterations in passes over two
+    // Method. The multi-level sweep runs a group's iterations in passes over two
     // buffer sets, so it needs the groups one after another. Automatic: solve
     // one at a time through the sweep when the tuner of single solves chose a
     // sweep for these dims (tuning now on solve 0's fields in scratch memory

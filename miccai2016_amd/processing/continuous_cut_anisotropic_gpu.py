@@ -216,7 +216,7 @@ class CmfCutQAnisotropicGpu(TreePrintable):
     def execute_ragged_device(self, u, quats, alpha, cs, ct, shapes, offsets, eigs, scaling=None,
                               sweep_min_voxels=1 << 19):
         """Volumes of at least sweep_min_voxels voxels are solved one at a
-        time with the tuned single-solve method (the two-level sweep where it
+        time with the tuned single-solve method (the multi-level sweep where it
         wins), the others together in one packed launch sequence; both give
         the same iterations bit for bit (None: all packed)."""
         dev = self.loader

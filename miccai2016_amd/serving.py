@@ -7,7 +7,7 @@ footprint fits the unreserved part of an HBM budget. An admitted cut joins
 the resident set at the next chunk boundary; every chunk, each cut advances
 ``chunk`` iterations: cuts of at least ``resident_min_voxels`` voxels as
 resident solves (``mh_cmf_resident_step``: per-iteration kernels or the
-two-level sweep, whichever the library uses for that shape), the smaller ones
+multi-level sweep, whichever the library uses for that shape), the smaller ones
 together in one packed launch sequence (``mh_cmf_jobs_step``, any mix of
 shapes). A finished cut leaves the set and releases its memory. Every cut
 computes exactly the iterations of a lone
@@ -47,7 +47,7 @@ FIELD_BYTES_PER_VOXEL = 28
 # buffers (field, result and swap SoA x3, response) and the cut's fields
 PREPARE_BYTES_PER_VOXEL = 160
 # cuts of at least this many voxels run as resident solves, which may use the
-# two-level sweep; smaller ones share the packed job launches
+# multi-level sweep; smaller ones share the packed job launches
 RESIDENT_MIN_VOXELS = 1 << 20
 
 
@@ -146,7 +146,7 @@ def job_footprint_bytes(shape, num_iter, update):
 
 def resident_footprint_bytes(shape):
     """Device bytes a cut advanced as a resident solve holds: its fields plus
-    the solve's workspace (both buffer sets of the two-level sweep)."""
+    the solve's workspace (both buffer sets of the multi-level sweep)."""
     n = int(np.prod(shape))
     w = int(_lib.lib().mh_cmf_workspace_bytes(n))
     if w < 0:

@@ -78,7 +78,7 @@ def _solve(dev, fields, shape, P, noalloc):
 
 CASES = [
     # (shape, iterations, update, pointwise eigenvalues, supervised): the
-    # first reaches the two-level sweep (>= 800,000 voxels), the second is
+    # first reaches the multi-level sweep (>= 800,000 voxels), the second is
     # smaller and runs the per-iteration kernels with pointwise eigenvalues
     # and seeds
     ((100, 100, 100), 61, 30, False, False),
