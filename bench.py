@@ -328,6 +328,10 @@ def main():
                                      "for the pair",
                 "launch_us": step_s / launches * 1e6,
                 "traffic_per_voxel_iteration": per_vi,
+                # the measured bytes over the same launches, beside the
+                # scheme's minimum that `achieved` and `frac` take
+                "achieved_traffic_GBs": None if per_vi is None else per_vi * value / 1e9,
+                "frac_traffic": None if per_vi is None else per_vi * value / 1e9 / HBM_PEAK_GBS,
                 "traffic_note": f"measured HBM bytes per launch of {levels} iterations "
                                 "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); at "
                                 "128^3 one pair's working set (~250 MB) mostly stays in the 256 "
