@@ -162,9 +162,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--iters-256", type=int, default=1000)
     ap.add_argument("--sweep", type=int, default=0,
-                    help="two-level sweep of single solves: 0 tuned, 1 off, 2 on")
+                    help="multi-level sweep of single solves: 0 tuned, 1 off, 2 on")
     ap.add_argument("--sweep-tile", type=int, default=0,
-                    help="with --sweep 2: the tile shape (1-5; 0: the first)")
+                    help="with --sweep 2: the tile shape (1-5 two levels, 6-7 three, 8-9 "
+                         "four; 0: the first)")
     ap.add_argument("--batched-sweep", type=int, default=0,
                     help="batched grid: 0 automatic, 1 per-iteration kernels of groups, 2 sweep")
     ap.add_argument("--cpu-iters", type=int, default=500)
