@@ -183,7 +183,8 @@ int mh_ridge_segments(mh_ctx *ctx, const float *cand_dev, const float *resp_dev,
  * mh_forest_pieces_host, mh_ridge_tree_from_forest_host, mh_tree_signs_host,
  * mh_nonridge_signs) bit for bit. Returns MH_DEFER, with dirs_dev unchanged,
  * where that cannot be guaranteed or the host's own rule differs: another
- * voxel size, at most 256 ridge or kept points (the dense tree), a NaN
+ * voxel size, 2^30 voxels or more or an axis longer than 16384 (30-bit point
+ * indices), at most 256 ridge or kept points (the dense tree), a NaN
  * direction product, segment lengths the bounds do not separate, more than
  * 65536 kept points outside the largest kept segment, more than 4096 kept
  * segments. Blocks until done. */

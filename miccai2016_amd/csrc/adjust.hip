@@ -57,12 +57,16 @@ struct Grid {
     int d0, d1, d2;
 };
 
-__device__ __forceinline__ unsigned ld_acq(const unsigned *p)
+// Relaxed atomics: the union-find words carry all the information a reader
+// needs (every word names an ancestor with the parity to it, at any moment),
+// and flags are read by the host after the kernel ends, so no ordering
+// against other memory is required.
+__device__ __forceinline__ unsigned ld_relaxed(const unsigned *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void st_rel(unsigned *p, unsigned v)
+__device__ __forceinline__ void st_relaxed(unsigned *p, unsigned v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -77,7 +81,7 @@ __device__ __forceinline__ void wave_flag(bool pred, unsigned *flags)
 {
     const unsigned long long m = __ballot(pred);
     if (m && __lane_id() == unsigned(__builtin_ctzll(m)))
-        st_rel(flags + (blockIdx.x & (kFlagSlots - 1)) * kFlagStride, 1u);
+        st_relaxed(flags + (blockIdx.x & (kFlagSlots - 1)) * kFlagStride, 1u);
 }
 
 // float dot product, one rounding per operation, left to right
@@ -277,19 +281,19 @@ __device__ unsigned pw_find(unsigned *pw, unsigned x, unsigned &par, int R, unsi
 {
     unsigned acc = 0;
     for (int it = 0; it <= R; ++it) {
-        const unsigned w = ld_acq(pw + x);
+        const unsigned w = ld_relaxed(pw + x);
         const unsigned px = w >> 1;
         if (px == x) {
             par = acc;
             return x;
         }
-        const unsigned wp = ld_acq(pw + px);
+        const unsigned wp = ld_relaxed(pw + px);
         const unsigned gx = wp >> 1;
         if (HALVE && gx != px) {
             // halve the path: x now points at its grandparent (an ancestor,
             // with the parity of the two links; roots are never written here)
             const unsigned nw = (gx << 1) | ((w ^ wp) & 1u);
-            st_rel(pw + x, nw);
+            st_relaxed(pw + x, nw);
             acc ^= nw & 1u;
             x = gx;
         } else {
@@ -297,7 +301,7 @@ __device__ unsigned pw_find(unsigned *pw, unsigned x, unsigned &par, int R, unsi
             x = px;
         }
     }
-    st_rel(err, 1u);
+    st_relaxed(err, 1u);
     par = acc;
     return x;
 }
@@ -340,7 +344,7 @@ pu_union_kernel(const unsigned long long *__restrict__ fkey, int R, const int *_
         const unsigned want = (lo << 1) | (pa ^ pb ^ rel);
         if (atomicCAS(pw + hi, hi << 1, want) == (hi << 1)) return;
     }
-    st_rel(err, 1u);
+    st_relaxed(err, 1u);
 }
 
 // every point linked straight to its root. The first pass (HALVE) shortens
@@ -355,7 +359,7 @@ pu_flatten_kernel(unsigned *pw, int R, unsigned *__restrict__ err)
     if (p >= R) return;
     unsigned par;
     const unsigned r = pw_find<HALVE>(pw, unsigned(p), par, R, err);
-    st_rel(pw + p, (r << 1) | par);
+    st_relaxed(pw + p, (r << 1) | par);
 }
 
 // per piece: the sum of its points' signed directions (float64) and its
@@ -441,11 +445,13 @@ gather_points_kernel(const int *__restrict__ list, int m, const unsigned *__rest
     if (k >= m) return;
     const int p = list[k];
     const int v = pts[p];
-    out_pw[k] = pw[p];
     out_lab[k] = labels[v];
-    out_dir[3 * k] = dirs[3 * int64_t(v)];
-    out_dir[3 * k + 1] = dirs[3 * int64_t(v) + 1];
-    out_dir[3 * k + 2] = dirs[3 * int64_t(v) + 2];
+    if (out_pw) out_pw[k] = pw[p];
+    if (out_dir) {
+        out_dir[3 * k] = dirs[3 * int64_t(v)];
+        out_dir[3 * k + 1] = dirs[3 * int64_t(v) + 1];
+        out_dir[3 * k + 2] = dirs[3 * int64_t(v) + 2];
+    }
 }
 
 __global__ void __launch_bounds__(mh::kBlock)
@@ -494,7 +500,7 @@ split_kept_kernel(const int *__restrict__ pts, const int *__restrict__ labels,
     } else {
         const unsigned slot = start[s] + atomicAdd(fill + s, 1u);
         if (slot < start[s + 1]) others[slot] = p;
-        else st_rel(err, 1u);
+        else st_relaxed(err, 1u);
     }
 }
 
@@ -953,24 +959,22 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     // scratch for gathers: reuse up / up2 / comp (free after the forest)
     MH_CHECK(ctx, hipMemcpyAsync(up, proot.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
     std::vector<double> hsum(3 * size_t(P));
-    std::vector<unsigned> hcnt(static_cast<size_t>(P)), hpw(static_cast<size_t>(P));
+    std::vector<unsigned> hcnt(static_cast<size_t>(P));
     std::vector<int> hlab(static_cast<size_t>(P));
-    std::vector<float> hdir(3 * size_t(P));
     {
         DevMem mg;
-        const int64_t bg = al(24 * int64_t(P)) + 3 * al(4 * int64_t(P)) + al(12 * int64_t(P));
+        const int64_t bg = al(24 * int64_t(P)) + 2 * al(4 * int64_t(P));
         MH_CHECK(ctx, mg.alloc(bg, s));
         Arena AG{static_cast<char *>(mg.p), 0, bg};
         double *osum = AG.take<double>(3 * int64_t(P));
         unsigned *ocnt = AG.take<unsigned>(P);
-        unsigned *opw = AG.take<unsigned>(P);
         int *olab = AG.take<int>(P);
-        float *odir = AG.take<float>(3 * int64_t(P));
-        if (!odir) return mh::set_error(ctx, MH_ERR_ARG, __func__, "gather layout");
+        if (!olab) return mh::set_error(ctx, MH_ERR_ARG, __func__, "gather layout");
         hipLaunchKernelGGL(gather_sums_kernel, dim3(mh::blocks_for(P)), dim3(mh::kBlock), 0, s, up, P,
                            psum, pcnt, osum, ocnt);
+        // the pieces' segment labels (their roots' voxels)
         hipLaunchKernelGGL(gather_points_kernel, dim3(mh::blocks_for(P)), dim3(mh::kBlock), 0, s, up, P,
-                           pw, pts, labels, dirs, opw, olab, odir);
+                           pw, pts, labels, dirs, (unsigned *)nullptr, olab, (float *)nullptr);
         MH_CHECK_LAUNCH(ctx, "piece gathers");
         MH_CHECK(ctx, hipMemcpyAsync(hsum.data(), osum, 24 * size_t(P), hipMemcpyDeviceToHost, s));
         MH_CHECK(ctx, hipMemcpyAsync(hcnt.data(), ocnt, 4 * size_t(P), hipMemcpyDeviceToHost, s));
