@@ -185,6 +185,18 @@ cmf_crit_kernel(const float *__restrict__ partials, int nparts, float inv_n_rcc,
 
 constexpr int kSI = 8;  // most solves per group of the batched solver
 
+// an event destroyed when it goes out of scope, error returns included
+struct ScopedEvent {
+    hipEvent_t e = nullptr;
+    ScopedEvent() = default;
+    ScopedEvent(const ScopedEvent &) = delete;
+    ScopedEvent &operator=(const ScopedEvent &) = delete;
+    ~ScopedEvent()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // per-solve constants of a group: alpha = w exp(-|grad I|^2 k), k = 1 / decay^2
 struct SolveParams {
     float w[kSI];
@@ -1451,9 +1463,10 @@ struct Runner {
         for (int k = 0; k < kNumSweepShapes; ++k) cands[1 + k] = SweepMethod{k, model_zchunk(k)};
         // time each candidate on this solve's own iterations: one warm-up
         // pass, then >= 12 iterations
-        hipEvent_t e0, e1;
-        MH_CHECK(ctx, hipEventCreate(&e0));
-        MH_CHECK(ctx, hipEventCreate(&e1));
+        ScopedEvent se0, se1;
+        MH_CHECK(ctx, hipEventCreate(&se0.e));
+        MH_CHECK(ctx, hipEventCreate(&se1.e));
+        hipEvent_t e0 = se0.e, e1 = se1.e;
         float best = 0.f;
         for (const SweepMethod &m : cands) {
             if (!usable(m) || it + 16 > prm->num_iter) continue;
@@ -1472,8 +1485,6 @@ struct Runner {
             }
         }
         MH_CHECK_LAUNCH(ctx, "cmf tuning");
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         int stopped = 0;
         MH_CHECK(ctx, hipMemcpy(&stopped, W.ctrl + kStop, sizeof(int), hipMemcpyDeviceToHost));
         // timings of a stopped solve (empty launches) choose nothing
@@ -1514,9 +1525,10 @@ struct Runner {
         if (!ctx->graph_stream)
             MH_CHECK(ctx, hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
         hipStream_t gs = ctx->graph_stream;
-        hipEvent_t before, after;
-        MH_CHECK(ctx, hipEventCreateWithFlags(&before, hipEventDisableTiming));
-        MH_CHECK(ctx, hipEventCreateWithFlags(&after, hipEventDisableTiming));
+        ScopedEvent sb, sa;
+        MH_CHECK(ctx, hipEventCreateWithFlags(&sb.e, hipEventDisableTiming));
+        MH_CHECK(ctx, hipEventCreateWithFlags(&sa.e, hipEventDisableTiming));
+        hipEvent_t before = sb.e, after = sa.e;
         MH_CHECK(ctx, hipEventRecord(before, s));
         MH_CHECK(ctx, hipStreamWaitEvent(gs, before, 0));
         const hipStream_t caller = s;
@@ -1542,8 +1554,6 @@ struct Runner {
         s = caller;
         MH_CHECK(ctx, hipEventRecord(after, gs));
         MH_CHECK(ctx, hipStreamWaitEvent(s, after, 0));
-        (void)hipEventDestroy(before);
-        (void)hipEventDestroy(after);
         MH_CHECK(ctx, e);
         return MH_OK;
     }
@@ -2236,11 +2246,12 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     int tune_phase = 0, tune_it0 = 0;
     float tune_best = 0.f;
     SweepMethod tune_pick = sm;
-    hipEvent_t te0 = nullptr, te1 = nullptr;
+    ScopedEvent ste0, ste1;
     if (!cands.empty()) {
-        MH_CHECK(ctx, hipEventCreate(&te0));
-        MH_CHECK(ctx, hipEventCreate(&te1));
+        MH_CHECK(ctx, hipEventCreate(&ste0.e));
+        MH_CHECK(ctx, hipEventCreate(&ste1.e));
     }
+    hipEvent_t te0 = ste0.e, te1 = ste1.e;
     if (!use_sweep) {
         hipLaunchKernelGGL(cmf_batch_init_kernel, dim3(nb, S), dim3(mh::kBlock), 0, s, W.hp, W.pt,
                            W.tmp, u, cs, ct, n, prm->rcc);
@@ -2345,8 +2356,6 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
         MH_CHECK_LAUNCH(ctx, "batched sweep final");
         if (g == 0 && ci < cands.size()) sm = tune_pick;  // too few iterations to time them all
     }
-    if (te0) (void)hipEventDestroy(te0);
-    if (te1) (void)hipEventDestroy(te1);
     for (int base = 0; !use_sweep && base < prm->num_iter; base += V) {
         const int last = std::min(prm->num_iter, base + V);
         for (int g = 0; g < ngroups; ++g) {
