@@ -749,6 +749,40 @@ def test_batched_pairs_equal_lone_solves_at_full_size(dev, tile):
         dev.check(dev.lib.mh_cmf_set_sweep(dev.handle, 0, 0, 0))
 
 
+def test_pair_tuner_choice_equals_lone_solves(dev):
+    """Default settings near the grid's volume size (dims no other test
+    tunes): the batched solver times the pair-capable tile shapes on its first
+    pair's own iterations and keeps the fastest (mh_cmf_batched_sweep_choice
+    reports it); five solves through it equal the same solves run one at a
+    time, bit for bit."""
+    import ctypes
+    from miccai2016_amd import _lib
+    shape = (120, 128, 128)
+    n = int(np.prod(shape))
+    q, cs, ct, gms, _, u0 = _device_cut_case(dev, shape, seed=8)
+    eigs = (1, 10 ** -1.5, 10 ** -1.5)
+    S = 5
+    w = np.geomspace(0.02, 2.0, S).astype(np.float32)
+    d = np.geomspace(3.0, 300.0, S).astype(np.float32)
+    torch = dev.torch
+    ub = u0.reshape(1, -1).repeat(S, 1)
+    _cutter(dev, 140, update=30).execute_batched_device(ub, S, w, d, q, gms, cs, ct, shape, eigs)
+    tile, zc = ctypes.c_int(0), ctypes.c_int(0)
+    assert dev.lib.mh_cmf_batched_sweep_choice(dev.handle, _lib.dims3(shape), ctypes.byref(tile),
+                                               ctypes.byref(zc)) == 1
+    print(f"pair tuner chose tile shape {tile.value}, {zc.value} planes per work-group", flush=True)
+    assert 1 <= tile.value <= 9 and zc.value >= 1
+    a = dev.empty(n)
+    for k in range(S):
+        dev.check(dev.lib.mh_nonterminal_weight(dev.handle, dev.ptr(gms), n, float(w[k]),
+                                                float(d[k]), dev.ptr(a), dev.stream))
+        ud = u0.clone()
+        _cutter(dev, 140, update=30).execute_device(ud, q, a, cs, ct,
+                                                    const_sqevals=np.sqrt(eigs), shape=shape)
+        torch.cuda.synchronize()
+        assert torch.equal(ub[k], ud.reshape(-1)), k
+
+
 @pytest.mark.parametrize("shape,iters,update,bound,labelled", [
     ((20, 17, 23), 200, 7, 0.0, False), ((16, 16, 16), 3000, 7, 1e-3, False),
     ((12, 14, 16), 95, 30, 0.0, True), ((5, 6, 7), 64, 1, 0.0, False)])
