@@ -52,6 +52,7 @@ constexpr int kMaxRounds = 64;
 constexpr int kMaxKept = 4096;              // kept segments (their pair table is K x K)
 constexpr int kHostPrimMax = 64;            // larger segment trees by Boruvka on the device
 constexpr int kDensePoints = 256;           // the host's dense spanning tree up to this many
+constexpr int kJumpBatch = 4;               // pointer-doubling steps per host check
 
 struct Grid {
     int d0, d1, d2;
@@ -884,6 +885,25 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
     };
     const int *pts = pts_all;
     const unsigned nbR = mh::blocks_for(R);
+    // Pointer doubling over hook trees until no pointer moves (at most
+    // log2(m) + 1 steps), in batches of kJumpBatch steps per host check: only
+    // the batch's last step raises the flag, and once a step moves nothing no
+    // later one does. `up` ends as the settled pointers.
+    auto jump_until_settled = [&](const int *cmp, int *&up_a, int *&up_b, int m, unsigned nb) -> int {
+        for (int step = 0;;) {
+            if (step > 40) return mh::set_error(ctx, MH_ERR_HIP, __func__, "hook doubling");
+            for (int k = 0; k < kJumpBatch; ++k, ++step) {
+                if (k == kJumpBatch - 1)
+                    MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
+                hipLaunchKernelGGL(bv_jump_kernel, dim3(nb), dim3(mh::kBlock), 0, s, cmp, up_a, up_b, m, flags);
+                std::swap(up_a, up_b);
+            }
+            bool changed = false;
+            const int rc = any_flag(changed);
+            if (rc != MH_OK) return rc;
+            if (!changed) return MH_OK;
+        }
+    };
 
     // 2. Boruvka rounds
     MH_CHECK(ctx, hipMemsetAsync(cnt, 0, 8 * sizeof(unsigned), s));
@@ -912,16 +932,8 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
         ncur = int(nnext);
         hipLaunchKernelGGL(bv_hook_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, best, R, up, fkey);
         // doubling until no pointer moves (at most log2(R) + 1 steps)
-        for (int step = 0;; ++step) {
-            if (step > 34) return mh::set_error(ctx, MH_ERR_HIP, __func__, "hook doubling");
-            MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
-            hipLaunchKernelGGL(bv_jump_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, up, up2, R, flags);
-            std::swap(up, up2);
-            bool changed = false;
-            const int rc = any_flag(changed);
-            if (rc != MH_OK) return rc;
-            if (!changed) break;
-        }
+        const int rc = jump_until_settled(comp, up, up2, R, nbR);
+        if (rc != MH_OK) return rc;
         hipLaunchKernelGGL(bv_relabel_kernel, dim3(nbR), dim3(mh::kBlock), 0, s, comp, up, R, best);
         MH_CHECK_LAUNCH(ctx, "forest rounds");
     }
@@ -1055,17 +1067,8 @@ MH_EXPORT int mh_adjust_ridge_device(mh_ctx *ctx, float *dirs, const float *ridg
             int rc = any_flag(hooked);
             if (rc != MH_OK) return rc;
             if (!hooked) break;
-            for (int step = 0;; ++step) {
-                if (step > 34) return mh::set_error(ctx, MH_ERR_HIP, __func__, "segment tree doubling");
-                MH_CHECK(ctx, hipMemsetAsync(flags, 0, 4 * kFlagSlots * kFlagStride, s));
-                hipLaunchKernelGGL(bv_jump_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, scomp, sup, sup2, K,
-                                   flags);
-                std::swap(sup, sup2);
-                bool changed = false;
-                rc = any_flag(changed);
-                if (rc != MH_OK) return rc;
-                if (!changed) break;
-            }
+            rc = jump_until_settled(scomp, sup, sup2, K, nbK);
+            if (rc != MH_OK) return rc;
             hipLaunchKernelGGL(sm_relabel_kernel, dim3(nbK), dim3(mh::kBlock), 0, s, scomp, sup, K, cw, cx);
         }
         std::vector<unsigned long long> he(static_cast<size_t>(K));
