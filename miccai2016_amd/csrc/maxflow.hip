@@ -819,8 +819,8 @@ cmf_sweep_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__rest
 
 // ------------------------------------------------------ L-level sweep ---
 // L iterations (levels 0 .. L-1) in one pass over the state, 1024 threads,
-// NS solves per thread (NS = 2 only in a batch: the pair shares the frames,
-// capacities, plane constants and barriers): the two-level scheme with more
+// NS solves per thread (NS = 2 only in a batch: the pair shares the frame,
+// cs, ct, their loads and the barriers; alpha is per solve): the two-level scheme with more
 // levels. The widened tile is (TY + 2L - 1) x (TX + 2L - 1) points and the
 // output tile starts at (L, L). At step z level l works at plane z - l:
 //   its p-step on rows and columns l .. T + 2L - 2 - l, from level l-1's hp
