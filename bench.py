@@ -370,7 +370,8 @@ def main():
                 ud.copy_(u0.reshape(-1))
                 plain_cut.execute_device(ud, q_d, a_d, cs_d, ct_d, const_sqevals=np.sqrt(eigs),
                                          shape=img.shape)
-            one()          # the first solve of a shape also tunes the method
+            one()          # the first solve of a shape runs the default method,
+            one()          # the second tunes it (unless the grid's probe did)
             t_dev += timed(torch, one)
             if ub is not None:
                 dices.append(ph.dice_between(ub[j], ud.detach().cpu().numpy()))
@@ -417,7 +418,8 @@ def main():
             us.copy_(u0s)
             c64.execute_device(us, qs, als, css, cts, const_sqevals=np.sqrt(eigs),
                                shape=spec.shape)
-        solve64()
+        solve64()          # the first solve of a shape runs the default method,
+        solve64()          # the second tunes it
         t64 = timed(torch, solve64)
         n64 = u0s.numel()
         try:
@@ -455,7 +457,8 @@ def main():
             ub_.copy_(u0b)
             c256.execute_device(ub_, qb, al_b, cs_b, ct_b, const_sqevals=np.sqrt(eigs),
                                 shape=spec.shape)
-        solve256()         # tunes the method for this shape
+        solve256()         # the first solve of a shape runs the default method,
+        solve256()         # the second tunes it
         t256 = timed(torch, solve256)
         rate256 = nb * it256 / t256
         tile256 = sweep_choice(dev, spec.shape)

@@ -374,9 +374,11 @@ int mh_cmf_batched_set_sweep(mh_ctx *ctx, int mode);
  * reference): unsupervised constant-eigenvalue solves may advance two, three
  * or four iterations per pass over the state (temporal blocking along axis
  * 0; tile shapes 1-5 two levels, 6-7 three, 8-9 four). mode 0
- * (default) tunes: the first solve of a volume shape of >= 96 iterations times
- * the plain kernels and a few sweep tile shapes on its own iterations and the
- * context keeps the fastest for that shape; mode 1 never sweeps; mode 2 always
+ * (default) tunes: the second mh_cmf_solve of a volume shape on this context
+ * (>= 96 iterations; the first runs the untuned default, so shapes met once
+ * pay nothing for timing) times the plain kernels and the sweep tile shapes
+ * on its own iterations and the context keeps the fastest for that shape;
+ * mh_cmf_solve_batched tunes on a probe at once; mode 1 never sweeps; mode 2 always
  * sweeps when the solve is eligible, with tile (1-based index of the built-in
  * tile shapes, 0 the first) and zchunk (planes per work-group, 0: the plane
  * chunk a cost model of the launch favours for that tile shape).

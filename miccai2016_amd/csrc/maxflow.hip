@@ -235,6 +235,9 @@ constexpr SweepShape kSweepShapes[] = {{12, 64, 1024, 2}, {11, 32, 512, 2}, {4, 
 constexpr int kNumSweepShapes = 9;
 constexpr int kShape3 = 5;  // the three-level shape of a four-level method's three-level passes
 constexpr int kDefaultShape = 6;  // untuned solves: the 27 x 27 three-level shape
+// iterations timed per candidate method (after a warm-up pass): shorter
+// windows let launch-to-launch noise of a few per cent pick a slower shape
+constexpr int kTuneIters = 24;
 
 // Two solves per thread: the descriptors span both solves' fields (hp: 24 B
 // per voxel), so the volume must stay below half the single-solve limit
@@ -1427,11 +1430,13 @@ struct Runner {
     }
 
     // The method for this solve: forced by the context, tuned earlier for
-    // these dims, tuned now on this solve's own iterations (allow_tune, at
-    // least 96 iterations), or the default (the three-level sweep with the
-    // 27 x 27 tile shape when eligible and the volume has >= 800,000 voxels,
-    // else the plain kernels).
-    int choose(bool allow_tune, SweepMethod &use)
+    // these dims, tuned now on this solve's own iterations (at least 96
+    // iterations; tune 2: always, tune 1: when these dims were solved on this
+    // context before, so that a shape met once, as in a ragged set, does not
+    // pay for timing every candidate), or the default (the three-level sweep
+    // with the 27 x 27 tile shape when eligible and the volume has >= 800,000
+    // voxels, else the plain kernels).
+    int choose(int tune, SweepMethod &use)
     {
         use = SweepMethod{-1, 0};
         const std::array<int, 3> key{D.d0, D.d1, D.d2};
@@ -1447,6 +1452,8 @@ struct Runner {
             use = SweepMethod{f->second.first, f->second.second};
             return MH_OK;
         }
+        bool allow_tune = tune == 2;
+        if (tune == 1) allow_tune = !ctx->sweep_seen.insert(key).second;
         if (!allow_tune || prm->num_iter < 96) {
             // measured: the sweep wins from about 10^6 voxels (100^3) on, the
             // per-iteration kernels at 80^3; the 27 x 27 three-level shape
@@ -1462,18 +1469,18 @@ struct Runner {
         cands[0] = SweepMethod{-1, 0};
         for (int k = 0; k < kNumSweepShapes; ++k) cands[1 + k] = SweepMethod{k, model_zchunk(k)};
         // time each candidate on this solve's own iterations: one warm-up
-        // pass, then >= 12 iterations
+        // pass, then >= kTuneIters iterations
         ScopedEvent se0, se1;
         MH_CHECK(ctx, hipEventCreate(&se0.e));
         MH_CHECK(ctx, hipEventCreate(&se1.e));
         hipEvent_t e0 = se0.e, e1 = se1.e;
         float best = 0.f;
         for (const SweepMethod &m : cands) {
-            if (!usable(m) || it + 16 > prm->num_iter) continue;
+            if (!usable(m) || it + kTuneIters + 4 > prm->num_iter) continue;
             step(m);
             MH_CHECK(ctx, hipEventRecord(e0, s));
             int done = 0;
-            while (done < 12) done += step(m);
+            while (done < kTuneIters) done += step(m);
             MH_CHECK(ctx, hipEventRecord(e1, s));
             MH_CHECK(ctx, hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1570,18 +1577,18 @@ struct Runner {
 };
 
 // The iterations of one solve on workspace W: initialisation, the method
-// chosen for these dims (tuned on the first solve of a shape), the final copy
-// of u when a sweep left it in the second buffer set. Enqueues only, except
-// while tuning.
+// chosen for these dims (tune as in Runner::choose: the default for a shape
+// met the first time, tuned on the second), the final copy of u when a sweep
+// left it in the second buffer set. Enqueues only, except while tuning.
 int solve_core(mh_ctx *ctx, float *u, const float *eigs, const float *alpha, const float *cs,
                const float *ct, const uint8_t *labels, const Dims D, int64_t n,
-               const mh_cmf_params *prm, const Workspace &W, hipStream_t s)
+               const mh_cmf_params *prm, const Workspace &W, hipStream_t s, int tune)
 {
     Runner R(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
     int rc = R.init();
     if (rc) return rc;
     SweepMethod use;
-    rc = R.choose(true, use);
+    rc = R.choose(tune, use);
     if (rc) return rc;
     rc = R.run_until(use, prm->num_iter);
     if (rc) return rc;
@@ -1618,7 +1625,7 @@ MH_EXPORT int mh_cmf_solve(mh_ctx *ctx, float *u, const float *eigs, const float
     hipStream_t s = (hipStream_t)stream;
     const Dims D{int(dims[0]), int(dims[1]), int(dims[2])};
 
-    const int rc_core = solve_core(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s);
+    const int rc_core = solve_core(ctx, u, eigs, alpha, cs, ct, labels, D, n, prm, W, s, 1);
     if (rc_core) return rc_core;
     return read_result(ctx, W, prm, res, crit_host, s);
 }
@@ -1697,7 +1704,7 @@ MH_EXPORT int mh_cmf_resident_step(mh_ctx *ctx, mh_cmf_resident *r, int iters, i
     Runner R = runner_of(ctx, r, W, (hipStream_t)stream);
     SweepMethod m{r->method, r->zchunk};
     if (r->method == -2) {
-        rc = R.choose(allow_tune != 0, m);
+        rc = R.choose(allow_tune != 0 ? 2 : 0, m);
         if (rc) return rc;
         r->method = m.shape;
         r->zchunk = m.zchunk;
@@ -1813,7 +1820,7 @@ MH_EXPORT int mh_cmf_solve_noalloc(mh_ctx *ctx, float *u, const float *eigs, con
     int rc = R.init();
     if (rc) return rc;
     SweepMethod use;
-    rc = R.choose(false, use);
+    rc = R.choose(0, use);
     if (rc) return rc;
     rc = R.run_until(use, prm->num_iter);
     if (rc) return rc;
@@ -2103,9 +2110,9 @@ int tune_on_probe(mh_ctx *ctx, const float *u, float w, float decay, const float
         Workspace W;
         layout(n, mem, &W);
         mh_cmf_params probe = *prm;
-        probe.num_iter = 192;  // every candidate's window
+        probe.num_iter = 10 * (kTuneIters + 4) + 32;  // every candidate's window
         probe.error_bound = 0.f;
-        rc = solve_core(ctx, us, eigs, as, cs, ct, nullptr, D, n, &probe, W, s);
+        rc = solve_core(ctx, us, eigs, as, cs, ct, nullptr, D, n, &probe, W, s, 2);
     }
     const hipError_t e = hipStreamSynchronize(s);
     (void)hipFree(mem);
@@ -2235,12 +2242,16 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
     // not depend on the methods, bit for bit.
     std::vector<SweepMethod> cands;
     if (use_sweep && tune_pairs) {
+        // the single-solve choice first, then the three-level shapes (the
+        // fastest pairs measured), so that a solve too short to time every
+        // candidate still times the likely ones
         cands.push_back(sm);
-        for (int k = 0; k < kNumSweepShapes; ++k) {
-            if (!sweep_pairs(k, n)) continue;
-            const SweepMethod m{k, sweep_model_zchunk(ctx, k, D, int64_t(nb))};
-            if (k != sm.shape && groups_of(m) <= int64_t(nb)) cands.push_back(m);
-        }
+        for (int lv = 3; lv >= 2; --lv)
+            for (int k = 0; k < kNumSweepShapes; ++k) {
+                if (!sweep_pairs(k, n) || kSweepShapes[k].levels != lv) continue;
+                const SweepMethod m{k, sweep_model_zchunk(ctx, k, D, int64_t(nb))};
+                if (k != sm.shape && groups_of(m) <= int64_t(nb)) cands.push_back(m);
+            }
     }
     size_t ci = 0;
     int tune_phase = 0, tune_it0 = 0;
@@ -2328,7 +2339,7 @@ MH_EXPORT int mh_cmf_solve_batched(mh_ctx *ctx, float *u, int S, const float *w_
                     MH_CHECK(ctx, hipEventRecord(te0, s));
                     tune_phase = 1;
                     tune_it0 = it;
-                } else if (it - tune_it0 >= 15) {
+                } else if (it - tune_it0 >= kTuneIters) {
                     MH_CHECK(ctx, hipEventRecord(te1, s));
                     MH_CHECK(ctx, hipEventSynchronize(te1));
                     float ms = 0.f;

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <set>
 #include <utility>
 
 #include "../../include/miccai_hip.h"
@@ -48,6 +49,9 @@ struct mh_ctx {
     // method the tuner chose per volume dims: (tile shape or -1 for the plain
     // kernels, plane chunk)
     std::map<std::array<int, 3>, std::pair<int, int>> sweep_tuned;
+    // volume dims mh_cmf_solve has met on this context (it tunes a shape
+    // the second time it meets it)
+    std::set<std::array<int, 3>> sweep_seen;
     // method the batched solver's tuner chose per volume dims (tile shape,
     // plane chunk), timed on pairs of solves
     std::map<std::array<int, 3>, std::pair<int, int>> batch_tuned;
