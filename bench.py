@@ -20,8 +20,8 @@ with the HBM traffic rocprofv3 measured for the same kernels; and the CPU
 oracle timed on a bounded sample on this host.
 
 ``--only PART`` runs one part (grid, plain, plain64, plain256, ragged, serve,
-cpu; sigmas: the grid at all 15 noise levels of the experiment, not part of
-the default run; or copy: a 512 MiB device copy used to calibrate the
+cpu; sigmas: the ATV and the TV grid at all 15 noise levels of the experiment,
+not part of the default run; or copy: a 512 MiB device copy used to calibrate the
 rocprofv3 byte counters).
 """
 import argparse
@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--ragged-volumes", type=int, default=64)
     ap.add_argument("--sigmas", type=int, default=15,
                     help="noise levels of --only sigmas (the first of linspace(0.1, 1.5, 15))")
+    ap.add_argument("--sigmas-no-tv", action="store_true",
+                    help="--only sigmas runs the ATV grid only, not the TV grid beside it")
     ap.add_argument("--ragged-workers", type=int, default=8,
                     help="threads preparing the ragged volumes (own context and stream each)")
     ap.add_argument("--no-ragged", action="store_true")
@@ -604,29 +606,43 @@ def main():
             dev.synchronize()
             t_prep = time.perf_counter() - t0
 
-            def grid_step():
-                u.copy_(u0.reshape(1, -1).expand(S, n))
-                cutter.execute_batched_device(u, S, w, d, prep.quats, prep.gms, prep.cs, prep.ct,
-                                              shape, eigs)
-            t_grid = timed(torch, grid_step)
-            fd = DeviceDice(truth, dev)(u, S)
-            best = int(np.nanargmax(fd))
-            try:
-                true_dice = float(device_dice_for(u[best], truth, shape, dev)[0])
-            except ValueError:
-                true_dice = float("nan")
-            rows.append({"sigma": float(sigma), "prepare_s": t_prep, "grid_s": t_grid,
-                         "voxel_iters_per_s": S * n * args.iters / t_grid,
-                         "ridge_voxels": prep.ridge_voxels, "best_fast_dice": float(fd[best]),
-                         "dice_of_largest_component": true_dice,
-                         "best_weighting": float(w_all[best]), "best_decay": float(d_all[best])})
-            log(f"sigma {sigma:.2f}: prepare {t_prep:.2f} s, grid {t_grid:.2f} s, best fast Dice "
-                f"{fd[best]:.4f} (largest component {true_dice:.4f}) at w={w_all[best]:.4g}, "
-                f"decay={d_all[best]:.4g}")
+            def grid(eg):
+                def grid_step():
+                    u.copy_(u0.reshape(1, -1).expand(S, n))
+                    cutter.execute_batched_device(u, S, w, d, prep.quats, prep.gms, prep.cs,
+                                                  prep.ct, shape, eg)
+                t_grid = timed(torch, grid_step)
+                fd = DeviceDice(truth, dev)(u, S)
+                best = int(np.nanargmax(fd))
+                try:
+                    true_dice = float(device_dice_for(u[best], truth, shape, dev)[0])
+                except ValueError:
+                    true_dice = float("nan")
+                return {"grid_s": t_grid, "voxel_iters_per_s": S * n * args.iters / t_grid,
+                        "best_fast_dice": float(fd[best]), "dice_of_largest_component": true_dice,
+                        "best_weighting": float(w_all[best]), "best_decay": float(d_all[best])}
+
+            row = {"sigma": float(sigma), "prepare_s": t_prep, "ridge_voxels": prep.ridge_voxels}
+            row.update(grid(eigs))
+            msg = (f"sigma {sigma:.2f}: prepare {t_prep:.2f} s, ATV grid {row['grid_s']:.2f} s, "
+                   f"best fast Dice {row['best_fast_dice']:.4f} (largest component "
+                   f"{row['dice_of_largest_component']:.4f}) at w={row['best_weighting']:.4g}, "
+                   f"decay={row['best_decay']:.4g}")
+            if not args.sigmas_no_tv:
+                # the experiment's second regularisation: isotropic TV, eigenvalues (1, 1, 1)
+                # (phantom_experiment.py:365-368)
+                row["tv"] = grid((1.0, 1.0, 1.0))
+                tv = row["tv"]
+                msg += (f"; TV grid {tv['grid_s']:.2f} s, best fast Dice {tv['best_fast_dice']:.4f}"
+                        f" (largest component {tv['dice_of_largest_component']:.4f}) at "
+                        f"w={tv['best_weighting']:.4g}, decay={tv['best_decay']:.4g}")
+            rows.append(row)
+            log(msg)
             del prep, img, truth, u0
         line["grid_sigmas"] = {
             "levels": rows, "total_s": time.perf_counter() - t_all,
-            "workload": f"{S} ATV solves x {args.iters} iterations per noise level on the 128^3 helix "
+            "workload": f"{S} ATV solves{'' if args.sigmas_no_tv else f' and {S} TV solves'} x "
+                        f"{args.iters} iterations per noise level on the 128^3 helix "
                         "(tube 6, radius 32, pitch 48), noise linspace(0.1, 1.5, 15); Frangi, "
                         "adjustment and GVF once per level on the device"}
         del u
