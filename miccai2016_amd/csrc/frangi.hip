@@ -55,62 +55,52 @@ __device__ __forceinline__ int mirror_index(int p, int len)
     return t < len ? t : period - t;
 }
 
-// Both pass kernels stage a tile of the input line segments plus a halo of
-// R taps on each side in LDS (each element loaded once, coalesced along axis
-// 2); every output then sums its K taps in the order j = 0 .. K-1 of the true
+// The axis-2 pass stages a tile of the input line segments plus a halo of R
+// taps on each side in LDS (each element loaded once, coalesced along axis
+// 2); the axis-0/1 passes read their columns straight from global memory.
+// Every output sums its K taps in the order j = 0 .. K-1 of the true
 // convolution (tap K-1-j against element p - R + j). The taps are uniform
-// across the work-group and are read from the kernel arguments.
+// across the work-group and are staged from the kernel arguments into LDS.
 
 constexpr int kTileX = 64;    // consecutive voxels along axis 2 per tile
-constexpr int kRunA = 8;      // outputs per thread along the pass axis (axes 0, 1)
-constexpr int kTileY = 8;     // threads per tile column
-constexpr int kTileA = kTileY * kRunA;
-constexpr int kTileThreads = kTileX * kTileY;
+constexpr int kRunA = 8;      // outputs per lane along the pass axis (axes 0, 1)
+constexpr int kColWaves = 4;  // waves (runs) per work-group of the axis-0/1 pass
 constexpr int kRows2 = 16;    // rows (axis 1) per tile of the axis-2 pass
 // LDS row width of the axis-2 pass: the tile's 64 + 2R inputs and 8 zero
 // columns, rounded to 16 bytes; each of the row's 16 threads loads every
 // 16th entry. Rows of up to 16 kRowLoads floats (R <= 26) prefetch.
 constexpr int kRowLoads = 8;
 
-// Pass along axis 0 or 1. Tile: 64 voxels along axis 2 x 64 along the pass
-// axis; thread (x, y) computes the kRunA (8) consecutive outputs y*8 .. y*8+7
-// of column x, 8 threads per column. LDS: 64 x (64 + 2R + 8) floats: the tile
-// and its halo, then 8 zero rows that the window's last loads read; then the
-// group's taps.
-//
-// The thread keeps a sliding window of the 8 column elements its outputs
-// need for the current tap in registers, as a ring indexed by position mod
-// 8: at tap j, output q reads element q + j from slot (q + j) mod 8, and
-// once all 8 outputs have used element j its slot takes element j + 8.
-// Taps go in unrolled blocks of 8 (the last one cut at K by a uniform test),
-// so every slot index is a constant: one LDS read per tap for the column and
-// one broadcast read per tap and output order for the weights, instead of
-// one per tap and output. (16 outputs per thread read the column half as
-// often but held 127 VGPRs, four waves per SIMD; 8 outputs fit 80.)
-constexpr int kZeroRows = kRunA;
-
-template <int NOUT>
-__device__ __forceinline__ void conv_tap(const float *wts, int j, int jj, float *win,
-                                         float (*acc)[kRunA], const float *col)
+// Pass along axis 0 or 1. A wave owns 64 consecutive voxels along axis 2
+// (one per lane) and a run of kRunA (8) consecutive outputs along the pass
+// axis; a work-group holds kColWaves runs, one above the other, and reads
+// one input group. Each lane slides a window of the 8 column elements its
+// outputs need for the current tap along its column, as a ring indexed by
+// position mod 8: at tap j, output q reads element q + j from slot
+// (q + j) mod 8, and once all 8 outputs have used element j its slot takes
+// element j + 8, loaded from global memory. Taps go in unrolled blocks of 8
+// (the last one cut at K by a uniform test), so every slot index is a
+// constant. The column's rows are a uniform address plus the lane's voxel,
+// so a load costs one address add; the runs' overlapping reads (K + 7 rows
+// for 8 outputs) hit in the caches, and HBM sees each element about once.
+// There is no barrier after the taps are staged: each wave's loads, taps and
+// stores go on by themselves, which keeps more loads in flight than a tile
+// staged in LDS, whose load, compute and store phases alternate (144 against
+// 184 us for the axis-1 pass at 256^3, bit for bit the same outputs).
+template <int NOUT, bool INTERIOR>
+__device__ __forceinline__ void col_run(const float *wts, const float *in, const ConvOut *out,
+                                        int R, int K, bool live, int64_t cbase, int xs,
+                                        int64_t sa, int a0, int len)
 {
-    float w[NOUT];
-#pragma unroll
-    for (int k = 0; k < NOUT; ++k) w[k] = wts[k * kTapsPad + j];
-#pragma unroll
-    for (int q = 0; q < kRunA; ++q) {
-        const float v = win[(jj + q) & (kRunA - 1)];
-#pragma unroll
-        for (int k = 0; k < NOUT; ++k) acc[k][q] += w[k] * v;
-    }
-    // element j is used up: its slot takes element j + kRunA
-    win[jj] = col[(j + kRunA) * kTileX];
-}
-
-template <int NOUT>
-__device__ __forceinline__ void conv_column(const float *wts, const ConvOut *out,
-                                            const float *col, int K, bool live, int64_t base,
-                                            int64_t sa, int a0, int len, int y)
-{
+    // element m of the column is in[cbase + m * sa + xs]; cbase is uniform,
+    // xs the lane's voxel (a valid one for lanes past the volume, which
+    // store nothing); INTERIOR: every element of the run's window, a0 - R
+    // .. a0 + R + 8, lies inside the axis
+    const int lo = a0 - R;
+    auto fetch = [&](int e) -> float {
+        const int m = INTERIOR ? lo + e : mirror_index(lo + e, len);
+        return in[cbase + int64_t(m) * sa + xs];
+    };
     float acc[NOUT][kRunA];
 #pragma unroll
     for (int q = 0; q < kRunA; ++q)
@@ -118,22 +108,35 @@ __device__ __forceinline__ void conv_column(const float *wts, const ConvOut *out
         for (int k = 0; k < NOUT; ++k) acc[k][q] = 0.f;
     float win[kRunA];
 #pragma unroll
-    for (int q = 0; q < kRunA; ++q) win[q] = col[q * kTileX];
+    for (int q = 0; q < kRunA; ++q) win[q] = fetch(q);
+    auto tap = [&](int j, int jj) {
+        float w[NOUT];
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) w[k] = wts[k * kTapsPad + j];
+#pragma unroll
+        for (int q = 0; q < kRunA; ++q) {
+            const float v = win[(jj + q) & (kRunA - 1)];
+#pragma unroll
+            for (int k = 0; k < NOUT; ++k) acc[k][q] += w[k] * v;
+        }
+        // element j is used up: its slot takes element j + kRunA
+        win[jj] = fetch(j + kRunA);
+    };
     int j0 = 0;
     for (; j0 + kRunA <= K; j0 += kRunA) {
 #pragma unroll
-        for (int jj = 0; jj < kRunA; ++jj) conv_tap<NOUT>(wts, j0 + jj, jj, win, acc, col);
+        for (int jj = 0; jj < kRunA; ++jj) tap(j0 + jj, jj);
     }
     const int rem = K - j0;
 #pragma unroll
     for (int jj = 0; jj < kRunA; ++jj)
-        if (jj < rem) conv_tap<NOUT>(wts, j0 + jj, jj, win, acc, col);
+        if (jj < rem) tap(j0 + jj, jj);
     if (!live) return;
 #pragma unroll
     for (int q = 0; q < kRunA; ++q) {
-        const int a = a0 + y * kRunA + q;
+        const int a = a0 + q;
         if (a < len) {
-            const int64_t i = base + int64_t(a) * sa;
+            const int64_t i = cbase + int64_t(a) * sa + xs;
 #pragma unroll
             for (int k = 0; k < NOUT; ++k) out[k].ptr[i * out[k].stride + out[k].offset] = acc[k][q];
         }
@@ -141,45 +144,47 @@ __device__ __forceinline__ void conv_column(const float *wts, const ConvOut *out
 }
 
 template <int AXIS>
-__global__ void __launch_bounds__(kTileThreads, 4)
-conv_tile_kernel(const ConvPass P, int d0, int d1, int d2)
+__global__ void __launch_bounds__(64 * kColWaves)
+conv_col_kernel(const ConvPass P, int d0, int d1, int d2)
 {
-    extern __shared__ float lds[];
     const int len = AXIS == 0 ? d0 : d1;
     const int other = AXIS == 0 ? d1 : d0;
     const int n2 = (d2 + kTileX - 1) / kTileX;
-    const int na = (len + kTileA - 1) / kTileA;
-    // one input group per work-group: the groups of a tile run side by side
-    // (each group's load latency hides behind other work-groups' taps)
-    // instead of one after another in one work-group
+    const int na = (len + kRunA * kColWaves - 1) / (kRunA * kColWaves);
+    // one input group per work-group, the groups of a run side by side
     unsigned b = mh::xcd_block(blockIdx.x, gridDim.x);
     const int g = int(b % unsigned(P.ngroups));
     b /= unsigned(P.ngroups);
     const int t2 = int(b % unsigned(n2));
     const int ta = int((b / unsigned(n2)) % unsigned(na));
     const int o = int(b / (unsigned(n2) * unsigned(na)));
-    if (o >= other) return;
-    const int x = threadIdx.x & (kTileX - 1), y = threadIdx.x / kTileX;
-    const int i2 = t2 * kTileX + x;
-    const int a0 = ta * kTileA;
-    const int R = P.radius, K = 2 * R + 1, rows = kTileA + 2 * R;
-    const int rows_alloc = rows + kZeroRows;
-    const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : int64_t(d2);
-    const int64_t base = AXIS == 0 ? int64_t(o) * d2 + i2 : int64_t(o) * d1 * d2 + i2;
-    const bool live = i2 < d2;
-    const float *col = lds + (y * kRunA) * kTileX + x;
-    float *wts = lds + rows_alloc * kTileX;   // the group's taps, [output][j]
+    if (o >= other) return;   // uniform per work-group
+    const int x = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const ConvGroup &G = P.grp[g];
-    for (int r = y; r < rows_alloc; r += kTileY)
-        lds[r * kTileX + x] = (live && r < rows)
-                                  ? G.in[base + int64_t(mirror_index(a0 - R + r, len)) * sa]
-                                  : 0.f;
-    for (int t = threadIdx.x; t < G.nout * kTapsPad; t += kTileThreads)
+    __shared__ float wts[3 * kTapsPad];   // the group's taps, [output][j]
+    for (int t = threadIdx.x; t < G.nout * kTapsPad; t += 64 * kColWaves)
         wts[t] = P.ctaps[G.out[t / kTapsPad].order][t % kTapsPad];
     __syncthreads();
-    if (G.nout == 3) conv_column<3>(wts, G.out, col, K, live, base, sa, a0, len, y);
-    else if (G.nout == 2) conv_column<2>(wts, G.out, col, K, live, base, sa, a0, len, y);
-    else conv_column<1>(wts, G.out, col, K, live, base, sa, a0, len, y);
+    const int a0 = (ta * kColWaves + wv) * kRunA;
+    if (a0 >= len) return;    // uniform per wave; no barrier follows
+    const int R = P.radius, K = 2 * R + 1;
+    const int64_t sa = AXIS == 0 ? int64_t(d1) * d2 : int64_t(d2);
+    const int64_t cbase = (AXIS == 0 ? int64_t(o) * d2 : int64_t(o) * d1 * d2) + int64_t(t2) * kTileX;
+    const bool live = t2 * kTileX + x < d2;
+    const int xs = live ? x : d2 - 1 - t2 * kTileX;
+    const bool interior = a0 - R >= 0 && a0 + R + kRunA < len;
+    const float *in = G.in;
+    const ConvOut *out = G.out;
+    if (interior) {
+        if (G.nout == 3) col_run<3, true>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+        else if (G.nout == 2) col_run<2, true>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+        else col_run<1, true>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+    } else {
+        if (G.nout == 3) col_run<3, false>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+        else if (G.nout == 2) col_run<2, false>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+        else col_run<1, false>(wts, in, out, R, K, live, cbase, xs, sa, a0, len);
+    }
 }
 
 // Pass along axis 2 (contiguous), one output per group. Tile: 64 outputs
@@ -337,16 +342,15 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
             hipLaunchKernelGGL(conv_row_kernel<0>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0, d1, d2);
     } else {
         const int len = axis == 0 ? d0 : d1, other = axis == 0 ? d1 : d0;
-        const int64_t blocks = n2 * ((len + kTileA - 1) / kTileA) * other * P.ngroups;
+        const int64_t blocks = n2 * ((len + kRunA * kColWaves - 1) / (kRunA * kColWaves)) * other *
+                               P.ngroups;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
-        const size_t shm = sizeof(float) * (kTileX * (kTileA + 2 * P.radius + kZeroRows) +
-                                            3 * kTapsPad);
         if (axis == 0)
-            hipLaunchKernelGGL(conv_tile_kernel<0>, dim3(unsigned(blocks)), dim3(kTileThreads), shm, s, P, d0,
-                               d1, d2);
+            hipLaunchKernelGGL(conv_col_kernel<0>, dim3(unsigned(blocks)), dim3(64 * kColWaves), 0, s,
+                               P, d0, d1, d2);
         else
-            hipLaunchKernelGGL(conv_tile_kernel<1>, dim3(unsigned(blocks)), dim3(kTileThreads), shm, s, P, d0,
-                               d1, d2);
+            hipLaunchKernelGGL(conv_col_kernel<1>, dim3(unsigned(blocks)), dim3(64 * kColWaves), 0, s,
+                               P, d0, d1, d2);
     }
     MH_CHECK_LAUNCH(ctx, "conv pass");
     return MH_OK;
