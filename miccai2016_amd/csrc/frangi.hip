@@ -197,8 +197,7 @@ conv_col_kernel(const ConvPass P, int d0, int d1, int d2)
 // LDS read: taps go in blocks of 4 (the last one cut at K by a uniform
 // test), where output q at tap j0 + jj reads element q + jj of A:B, then A
 // takes B and B the next 4 elements. The results of all groups stay in
-// registers and are stored together, so a voxel's AoS components leave in
-// one burst.
+// registers and are stored together at the end.
 __device__ __forceinline__ float4 lds4(const float *p)
 {
     return *reinterpret_cast<const float4 *>(p);
@@ -282,28 +281,49 @@ conv_row_kernel(const ConvPass P, int d0, int d1, int d2)
         for (int q = 0; q < 4; ++q) res[q][g] = a[q];
         __syncthreads();
     }
-    const int i1 = t1 * kRows2 + row;
-    if (i1 >= d1) return;
-    const int64_t i = (int64_t(i0) * d1 + i1) * d2 + s0 + xq;
-    // the Hessian's AoS x6 output: the thread's 4 voxels are 24 consecutive
-    // floats, stored as 6 aligned 16-byte writes
-    bool aos6 = P.ngroups == 6 && s0 + xq + 3 < d2;
+    // the Hessian's AoS x6 output (all six groups into one 16-byte aligned
+    // buffer, component g at offset g): the tile's block goes through LDS
+    // (free after the last group's barrier) and leaves row by row in 16-byte
+    // pieces in lane order, so a wave's store covers 1 KiB of consecutive
+    // output; a thread storing its own 4 voxels' 24 floats scatters each
+    // store over 6 KiB (222 against 199 us at 256^3)
+    bool aos6 = P.ngroups == 6;
 #pragma unroll
     for (int g = 0; g < 6; ++g)
         aos6 = aos6 && P.grp[g].out[0].ptr == P.grp[0].out[0].ptr &&
                P.grp[g].out[0].stride == 6 && P.grp[g].out[0].offset == g;
     aos6 = aos6 && (reinterpret_cast<uintptr_t>(P.grp[0].out[0].ptr) & 15) == 0;
-    if (aos6) {
-        float4 *o = reinterpret_cast<float4 *>(P.grp[0].out[0].ptr + i * 6);
+    if (aos6) {   // uniform per work-group
+        constexpr int kRowFloats = 6 * kTileX, kRowPieces = kRowFloats / 4;
+        float4 *sp = reinterpret_cast<float4 *>(lds + row * kRowFloats + xq * 6);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             float v[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[e] = res[(4 * k + e) / 6][(4 * k + e) % 6];
-            o[k] = make_float4(v[0], v[1], v[2], v[3]);
+            sp[k] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        __syncthreads();
+        const int nf = 6 * min(kTileX, d2 - s0);   // valid floats per row
+        float *ob = P.grp[0].out[0].ptr;
+        for (int f = threadIdx.x; f < kRows2 * kRowPieces; f += 256) {
+            const int r = f / kRowPieces, e0 = 4 * (f % kRowPieces);
+            const int ri = t1 * kRows2 + r;
+            if (ri >= d1 || e0 >= nf) continue;
+            const float4 v = *reinterpret_cast<const float4 *>(lds + r * kRowFloats + e0);
+            float *dst = ob + ((int64_t(i0) * d1 + ri) * d2 + s0) * 6 + e0;
+            if (e0 + 4 <= nf) {
+                *reinterpret_cast<float4 *>(dst) = v;
+            } else {
+                const float vv[4] = {v.x, v.y, v.z, v.w};
+                for (int e = 0; e < 4 && e0 + e < nf; ++e) dst[e] = vv[e];
+            }
         }
         return;
     }
+    const int i1 = t1 * kRows2 + row;
+    if (i1 >= d1) return;
+    const int64_t i = (int64_t(i0) * d1 + i1) * d2 + s0 + xq;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s0 + xq + q >= d2) continue;
@@ -333,7 +353,8 @@ int launch_pass(mh_ctx *ctx, int axis, const ConvPass &P, const int64_t *dims, h
     if (axis == 2) {
         const int64_t blocks = n2 * ((d1 + kRows2 - 1) / kRows2) * d0;
         MH_REQUIRE(ctx, blocks < (int64_t(1) << 31), "volume too large");
-        const size_t shm = sizeof(float) * kRows2 * ((kTileX + 2 * P.radius + 8 + 3) & ~3);
+        // the input tile, or the AoS x6 block of the results
+        const size_t shm = sizeof(float) * kRows2 * std::max((kTileX + 2 * P.radius + 8 + 3) & ~3, 6 * kTileX);
         const int width = (kTileX + 2 * P.radius + 8 + 3) & ~3;
         if (width <= 16 * kRowLoads)
             hipLaunchKernelGGL(conv_row_kernel<kRowLoads>, dim3(unsigned(blocks)), dim3(256), shm, s, P, d0,
