@@ -918,14 +918,26 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
     const unsigned ob2 = (inv && (f & 32u)) ? ob + 4u : ob;
 
     const int zs = max(z0 - L, 0), zend = z1 + L - 2;
-    // carried from the previous step, per level l (its plane there was z-1-l)
-    // and solve j: hp, tmp, pt, u; and the plane constants of plane z-1-l
-    float h0[L][NS], h1[L][NS], h2[L][NS], tp[L][NS], ptp[L][NS], up[L][NS], ac[L][NS];
-    float csc[L], ctc[L], v0[L], v1[L], v2[L];
+    // The loop over steps is unrolled U = lcm(2, L) times and every carried
+    // value sits in a ring indexed by the step's phase PH (compile-time), so
+    // nothing is copied from step to step:
+    //   per level l and solve j, hp, tmp, pt, u at plane z - l in slot PH & 1
+    //   (written this step) and at plane z - 1 - l in slot (PH + 1) & 1 (the
+    //   previous step's);
+    //   the plane constants (alpha per solve, cs, ct, frame axis) of plane p
+    //   in slot p mod L: plane z - l in slot (PH - l) mod L.
+    constexpr int U = L % 2 == 0 ? L : 2 * L;
+    float h0[2][L][NS], h1[2][L][NS], h2[2][L][NS], tp[2][L][NS], ptp[2][L][NS], up[2][L][NS];
+    float ac[L][NS], csc[L], ctc[L], v0[L], v1[L], v2[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
 #pragma unroll
-        for (int j = 0; j < NS; ++j) h0[l][j] = h1[l][j] = h2[l][j] = tp[l][j] = ptp[l][j] = up[l][j] = ac[l][j] = 0.f;
+        for (int j = 0; j < NS; ++j) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                h0[k][l][j] = h1[k][l][j] = h2[k][l][j] = tp[k][l][j] = ptp[k][l][j] = up[k][l][j] = 0.f;
+            ac[l][j] = 0.f;
+        }
         csc[l] = ctc[l] = 0.f;
         v0[l] = 1.f;
         v1[l] = v2[l] = 0.f;
@@ -947,33 +959,32 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
         sweep_load_all<NS, true>(Li, LC, R, nb, pb, unsigned(zc) * pb, ob, ob1, ob2, zc < D.d0 - 1);
     };
     if (zs < D.d0) load(zs);
-    auto step = [&](const int z) {
-        // plane constants of plane z (level 0), from the loads
-        float vN0, vN1, vN2;
-        mh::frame_axis(LC.q, vN0, vN1, vN2);
-        float aN[NS];
+    auto step = [&](auto phase, const int z) {
+        constexpr int PH = decltype(phase)::value;
+        constexpr int C = PH & 1, P = C ^ 1;   // this step's and the previous step's slots
+        // plane constants of plane z (level 0), from the loads, into slot PH mod L
+        // (its last reader, level L-1 at plane z - L, ran the step before)
+        mh::frame_axis(LC.q, v0[PH % L], v1[PH % L], v2[PH % L]);
 #pragma unroll
-        for (int j = 0; j < NS; ++j) aN[j] = Li[j].a;
-        const float csN = LC.cs, ctN = LC.ct;
-        // results of this step per level, at plane z - l
-        float nh0[L][NS], nh1[L][NS], nh2[L][NS], nt[L][NS], npt[L][NS], nu[L][NS];
+        for (int j = 0; j < NS; ++j) ac[PH % L][j] = Li[j].a;
+        csc[PH % L] = LC.cs;
+        ctc[PH % L] = LC.ct;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             const int w = z - l;
+            const int cl = (PH - l + L * U) % L;   // plane w's constants
             const bool pon = inP[l] && w >= max(z0 - (L - l), 0) && w < min(z1 + (L - 1 - l), D.d0);
             const bool uon = inU[l] && w >= max(z0 - (L - 1 - l), 0) && w < min(z1 + (L - 1 - l), D.d0);
             const bool e0 = w < D.d0 - 1, f0 = w > 0;
-            // plane constants of plane w
-            const float c_s = l == 0 ? csN : csc[l - 1], c_t = l == 0 ? ctN : ctc[l - 1];
-            const float w0 = l == 0 ? vN0 : v0[l - 1], w1 = l == 0 ? vN1 : v1[l - 1],
-                        w2 = l == 0 ? vN2 : v2[l - 1];
-            // p-step: from level l-1's hp at plane w (carried) or the loads
+            const float c_s = csc[cl], c_t = ctc[cl];
+            const float w0 = v0[cl], w1 = v1[cl], w2 = v2[cl];
+            // p-step: from level l-1's hp at plane w (previous step) or the loads
             float g0h[NS], g1h[NS], g2h[NS];
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
-                g0h[j] = l == 0 ? Li[j].h0 : h0[l - 1][j];
-                g1h[j] = l == 0 ? Li[j].h1 : h1[l - 1][j];
-                g2h[j] = l == 0 ? Li[j].h2 : h2[l - 1][j];
+                g0h[j] = l == 0 ? Li[j].h0 : h0[P][l - 1][j];
+                g1h[j] = l == 0 ? Li[j].h1 : h1[P][l - 1][j];
+                g2h[j] = l == 0 ? Li[j].h2 : h2[P][l - 1][j];
             }
             if (pon) {
 #pragma unroll
@@ -984,16 +995,15 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
                         g1 = (f & 16u) ? (Li[j].t1 - Li[j].t) * PP.ss1 : 0.f;
                         g2 = (f & 32u) ? (Li[j].t2 - Li[j].t) * PP.ss2 : 0.f;
                     } else {
-                        // level l-1's tmp: at plane w (carried, and its +1
+                        // level l-1's tmp: at plane w (previous step, and its +1
                         // in-plane neighbours in LDS) and at w+1 (this step)
-                        const float tc = tp[l - 1][j];
+                        const float tc = tp[P][l - 1][j];
                         const float t1 = sT[l - 1][j][p + EX], t2 = sT[l - 1][j][p + 1];
-                        g0 = e0 ? (nt[l - 1][j] - tc) * PP.ss0 : 0.f;
+                        g0 = e0 ? (tp[C][l - 1][j] - tc) * PP.ss0 : 0.f;
                         g1 = (f & 16u) ? (t1 - tc) * PP.ss1 : 0.f;
                         g2 = (f & 32u) ? (t2 - tc) * PP.ss2 : 0.f;
                     }
-                    const float a = l == 0 ? aN[j] : ac[l - 1][j];
-                    mh::cmf_p_voxel_axi(g0h[j], g1h[j], g2h[j], g0, g1, g2, w0, w1, w2, PP.A, a);
+                    mh::cmf_p_voxel_axi(g0h[j], g1h[j], g2h[j], g0, g1, g2, w0, w1, w2, PP.A, ac[cl][j]);
                 }
             }
             if (live) {
@@ -1004,19 +1014,19 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
                 }
             }
             __syncthreads();
-            // u-step: from level l-1's pt, u at plane w (carried) or the loads
+            // u-step: from level l-1's pt, u at plane w (previous step) or the loads
             float tt[NS], ptt[NS], uu[NS];
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
                 tt[j] = l == 0 ? Li[j].t : 0.f;
-                ptt[j] = l == 0 ? Li[j].pt : ptp[l - 1][j];
-                uu[j] = l == 0 ? Li[j].u : up[l - 1][j];
+                ptt[j] = l == 0 ? Li[j].pt : ptp[P][l - 1][j];
+                uu[j] = l == 0 ? Li[j].u : up[P][l - 1][j];
             }
             if (uon) {
 #pragma unroll
                 for (int j = 0; j < NS; ++j) {
                     const float n1 = sH1[l][j][p - EX], n2 = sH2[l][j][p - 1];
-                    const float x0v = (e0 ? g0h[j] : 0.f) - (f0 ? h0[l][j] : 0.f);
+                    const float x0v = (e0 ? g0h[j] : 0.f) - (f0 ? h0[P][l][j] : 0.f);
                     const float x1v = ((f & 16u) ? g1h[j] : 0.f) - ((f & 64u) ? n1 : 0.f);
                     const float x2v = ((f & 32u) ? g2h[j] : 0.f) - ((f & 128u) ? n2 : 0.f);
                     const float div = mh::cmf_div(x0v, x1v, x2v, UP.rs0, UP.rs1, UP.rs2);
@@ -1038,12 +1048,12 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
             }
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
-                nh0[l][j] = g0h[j];
-                nh1[l][j] = g1h[j];
-                nh2[l][j] = g2h[j];
-                nt[l][j] = tt[j];
-                npt[l][j] = ptt[j];
-                nu[l][j] = uu[j];
+                h0[C][l][j] = g0h[j];
+                h1[C][l][j] = g1h[j];
+                h2[C][l][j] = g2h[j];
+                tp[C][l][j] = tt[j];
+                ptp[C][l][j] = ptt[j];
+                up[C][l][j] = uu[j];
             }
             // the next plane's inputs, in flight during the later levels
             if (l == 0 && z + 1 <= zend && z + 1 < D.d0) load(z + 1);
@@ -1054,28 +1064,19 @@ cmf_sweepL_kernel(SweepBufs B, const float4 *__restrict__ q4, const float *__res
 #pragma unroll
             for (int l = 0; l < L - 1; ++l)
 #pragma unroll
-                for (int j = 0; j < NS; ++j) sT[l][j][p] = nt[l][j];
-        }
-#pragma unroll
-        for (int l = L - 1; l >= 0; --l) {
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                h0[l][j] = nh0[l][j];
-                h1[l][j] = nh1[l][j];
-                h2[l][j] = nh2[l][j];
-                tp[l][j] = nt[l][j];
-                ptp[l][j] = npt[l][j];
-                up[l][j] = nu[l][j];
-                ac[l][j] = l == 0 ? aN[j] : ac[l - 1][j];
-            }
-            csc[l] = l == 0 ? csN : csc[l - 1];
-            ctc[l] = l == 0 ? ctN : ctc[l - 1];
-            v0[l] = l == 0 ? vN0 : v0[l - 1];
-            v1[l] = l == 0 ? vN1 : v1[l - 1];
-            v2[l] = l == 0 ? vN2 : v2[l - 1];
+                for (int j = 0; j < NS; ++j) sT[l][j][p] = tp[C][l][j];
         }
     };
-    for (int z = zs; z <= zend; ++z) step(z);
+    for (int z = zs; z <= zend; z += U) {
+        // U steps with compile-time phases; z - zs is a multiple of U here
+        const bool go = mh::unrolled_while<U>([&](auto ph) {
+            constexpr int K = decltype(ph)::value;
+            if (z + K > zend) return false;   // uniform: zs and zend are the work-group's
+            step(ph, z + K);
+            return true;
+        });
+        if (!go) break;
+    }
     if (check) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {

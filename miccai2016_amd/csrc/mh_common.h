@@ -15,6 +15,7 @@
 #include <array>
 #include <map>
 #include <set>
+#include <type_traits>
 #include <utility>
 
 #include "../../include/miccai_hip.h"
@@ -77,6 +78,20 @@ int mh_nearest_marked(mh_ctx *ctx, const int64_t dims[3], unsigned *work, hipStr
 namespace mh {
 
 constexpr int kBlock = 256;  // threads per work-group for 1-D kernels (4 waves)
+
+// Calls f(std::integral_constant<int, K>{}) for K = 0 .. N-1, in order, while
+// it returns true (a loop body unrolled with its index as a compile-time
+// constant); returns false if f stopped it.
+template <int N, int K = 0, typename F>
+__device__ __forceinline__ bool unrolled_while(F &&f)
+{
+    if constexpr (K < N) {
+        if (!f(std::integral_constant<int, K>{})) return false;
+        return unrolled_while<N, K + 1>(f);
+    } else {
+        return true;
+    }
+}
 
 inline int set_error(mh_ctx *ctx, int code, const char *where, const char *what)
 {
