@@ -52,10 +52,10 @@ SWEEP_SHARED_BYTES = 24.0
 TRAFFIC_GRID_PER_VOXEL_ITER = 67.6     # batched p + u kernels, groups of 4 solves, 128^3
 TRAFFIC_PLAIN_PER_VOXEL_ITER = 88.2    # per-iteration p + u kernels, 256^3
 # the sweep by tile shape (1-based, as mh_cmf_sweep_choice reports it), one solve per
-# thread, at 128^3 (two-level shapes) and at 256^3 (two levels, the three-level 6 and
-# the four-level 9)
+# thread, at 128^3 (two-level shapes) and at 256^3 (two, three and four levels)
 TRAFFIC_SWEEP_PER_VOXEL_ITER = {1: 44.1, 2: 49.3, 3: 44.5, 4: 45.4, 5: 41.9}
-TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.1, 2: 56.6, 3: 52.5, 4: 46.7, 5: 46.9, 6: 32.4, 9: 28.0}
+TRAFFIC_SWEEP_256_PER_VOXEL_ITER = {1: 43.1, 2: 56.6, 3: 52.5, 4: 46.7, 5: 46.9, 6: 32.4, 7: 32.9,
+                                    8: 27.6, 9: 28.0}
 # the batched grid's sweep, two solves per thread (the 1024-thread tile shapes of
 # two and three levels: 1, 4, 5, 6, 7), at 128^3, with the axial p-step: the two-level
 # shape 1 and the three-level shape 7; shapes 4, 5 and 6 unmeasured
